@@ -1,0 +1,169 @@
+/*
+ * md_api.h -- C ABI of the MI355X MD step engine (libmdgpu.so).
+ *
+ * Plain C types only: host pointers and sizes, no HIP or torch types.  Every
+ * call is synchronous at return unless documented otherwise, returns MD_OK (0)
+ * or a negative MD_E* code, and never exits the process.  The message of the
+ * last failure on the calling thread is md_last_error().  One engine handle is
+ * not thread-safe; separate handles are independent.
+ *
+ * Units are the reference's: A, fs, kJ/mol, e, g/mol (src/simulation.cu:177,
+ * src/forces.cu:47).  Boxes are orthorhombic and centred at the origin
+ * (src/nve_leapfrog.cu:50-55).
+ *
+ * The entry points replace the single-GPU step path of the reference:
+ *   md_set_forcefield   <- forces_init constant uploads      src/forces.cu:75-111
+ *   md_set_topology     <- per-atom bond/angle lists          src/forces.cu:113-179
+ *                          and the exclusion table            src/neighborlist.cu:64-99
+ *   md_set_system /
+ *   md_upload_state     <- config_init / config_HostToDevice  src/config.cu:22-40,56-62
+ *   md_download_state   <- config_finish / _DeviceToHost      src/config.cu:42-77
+ *   md_compute_forces   <- forces() + energy()                src/forces.cu:189-231
+ *   md_run              <- SimulationRun_CUDA time loop       src/runGPU.cu:69-104
+ *                          (neighborlist -> forces -> integrate, src/neighborlist.cu:109-146,
+ *                           src/integrator.cu:131-221)
+ *   md_get_neighbors    <- nnlist / nnlist_max readback        src/neighborlist.cu:111-115
+ *   md_set_integrator   <- integrate_init ensemble/dt/T0      src/integrator.cu:47-97
+ * The force-matching and batching entry points have no counterpart in the
+ * reference (they extend the same step path).
+ */
+#ifndef MDGPU_MD_API_H
+#define MDGPU_MD_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MD_OK 0
+#define MD_E_INVALID (-1)      /* bad argument */
+#define MD_E_HIP (-2)          /* HIP runtime error */
+#define MD_E_STATE (-3)        /* call out of order (e.g. run before upload) */
+#define MD_E_CAPACITY (-4)     /* a size limit was exceeded */
+#define MD_E_UNSTABLE (-5)     /* non-finite coordinates or an atom left the box */
+
+#define MD_MAX_TYPES 8          /* NUM_PARTICLE_TYPES, src/simulation.h:47 */
+#define MD_MAX_BOND_TYPES 8     /* NUM_BOND_TYPES, src/simulation.h:48 */
+
+/* Coulomb kinds = the reference's lkspace (src/simulation.cu:82-95) + none */
+enum { MD_COUL_RF = 0, MD_COUL_WOLF = 1, MD_COUL_DAMPED = 2, MD_COUL_NONE = 3 };
+enum { MD_LEAPFROG = 0, MD_VVERLET = 1 };
+enum { MD_TH_NONE = 0, MD_TH_BERENDSEN = 1, MD_TH_NOSEHOOVER = 2 };
+
+typedef struct {
+    int   ntype;                         /* 1..MD_MAX_TYPES */
+    float lj1[MD_MAX_TYPES * MD_MAX_TYPES];   /* [ti*ntype+tj] = 48 eps sig^12, src/simulation.cu:191 */
+    float lj2[MD_MAX_TYPES * MD_MAX_TYPES];   /* [ti*ntype+tj] = 24 eps sig^6,  src/simulation.cu:194 */
+    int   coulomb;                       /* MD_COUL_* */
+    float rc;                            /* cutoff, A */
+    float alpha;                         /* Wolf / damped alpha, 1/A */
+    float eps_rf;                        /* reaction-field dielectric */
+    float skin;                          /* Verlet skin, A */
+    float qfac;                          /* 1389.35427611722 (0 selects that default) */
+    int   nbond_types;
+    float bond_k2[MD_MAX_BOND_TYPES];    /* bnd1 = 2k (E = k (r-r0)^2), src/simulation.cu:227 */
+    float bond_r0[MD_MAX_BOND_TYPES];
+    int   nangle_types;
+    float angle_k2[MD_MAX_BOND_TYPES];   /* ang1 = 2k, src/simulation.cu:242 */
+    float angle_t0[MD_MAX_BOND_TYPES];   /* radians, src/simulation.cu:243 */
+} md_forcefield;
+
+typedef struct {
+    int   integrator;   /* MD_LEAPFROG (vel = v(t-dt/2)) or MD_VVERLET (vel = v(t)) */
+    int   thermostat;   /* MD_TH_*; leapfrog supports NONE and BERENDSEN */
+    float dt;           /* fs */
+    float tau;          /* fs; Berendsen time constant (reference tco = 0.01/fs -> 100 fs) or NH period */
+} md_integrator;
+
+/* Per-step, per-replica statistics written by md_run (FP64 throughout). */
+typedef struct {
+    double ke;          /* kinetic energy, kJ/mol */
+    double temp;        /* K, N_f = 3N-3, k_B = 8.314e-3 (src/integrator.cu:155) */
+    double pe_lj;       /* LJ energy (shifted to 0 at rc), valid when has_energy */
+    double pe_coul;     /* Coulomb pair energy, valid when has_energy */
+    double pe_bond;
+    double pe_angle;
+    double virial;      /* sum over pairs and bonds of r.F, valid when has_energy */
+    double conserved;   /* ke + pe (+ Nose-Hoover terms), valid when has_energy */
+    double xi, vxi;     /* Nose-Hoover thermostat position / velocity */
+    int    has_energy;
+    int    pad;
+} md_step_stats;
+
+typedef struct md_engine md_engine;
+
+const char* md_last_error(void);
+/* library build string (arch, compiler) */
+const char* md_version(void);
+
+int md_create(md_engine** out, int device);
+int md_destroy(md_engine* e);
+
+int md_set_forcefield(md_engine* e, const md_forcefield* ff);
+int md_set_integrator(md_engine* e, const md_integrator* in);
+
+/* nrep independent replicas sharing one force field.  Atom arrays are the
+ * replicas' atoms concatenated; natom[r], box[3r..3r+2], T0[r] per replica.
+ * type: 0-based, charge: e, mass: g/mol per atom. */
+int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, const float* T0,
+                  const int* type, const float* charge, const float* mass);
+/* bonds: nb x (i, j, type); angles: na x (i, center j, k, type); global 0-based
+ * atom indices.  1-2 bonded pairs are excluded from pair forces (<= 4 per atom). */
+int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* angles);
+
+/* pos (N x 3, required), vel (N x 3 or NULL = zero).  Builds the neighbor lists. */
+int md_upload_state(md_engine* e, const float* pos, const float* vel);
+/* Any pointer may be NULL.  Arrays are N x 3 in the uploaded atom order. */
+int md_download_state(md_engine* e, float* pos, float* vel, float* force);
+
+/* Forces at the current positions (N x 3, may be NULL); energy per replica
+ * nrep x 4 (lj, coul, bond, angle) and virial per replica (may be NULL). */
+int md_compute_forces(md_engine* e, float* force, double* energy, double* virial);
+
+/* Advance nsteps.  energy_every > 0 computes energies at steps with
+ * (step index % energy_every == 0); stats (nsteps x nrep, may be NULL).
+ * Work is queued as hipGraph segments; the call returns when done. */
+int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats);
+
+/* Full neighbor lists in uploaded atom indices: counts[N], nbrs[N x cap]
+ * (unsorted within a row).  Returns MD_E_CAPACITY if cap is too small
+ * (counts still complete). */
+int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs);
+
+/* Canonical FP32 thresholds the kernels use (for the parity checker):
+ * rc2 = rc*rc, rl2 = (float)((skin + sqrt(rc2))^2), skin2q = skin*skin/4. */
+int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q);
+
+/* Engine counters: neighbor-list rebuilds per replica summed, steps run,
+ * list capacity per atom, current lanes-per-atom and grid of the step kernel. */
+int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid);
+
+/* Launch configuration of the step kernel: lanes cooperating on one atom
+ * (4, 8, 16, 32), persistent grid size in workgroups (0 = automatic),
+ * cell edge divisor (1..3: cell edge >= r_list / div).  Rebuilds lists. */
+int md_set_launch(md_engine* e, int lanes, int grid, int cell_div);
+/* Time candidate launch configurations on the current system (nsteps each)
+ * and keep the fastest.  best_ms_per_step may be NULL. */
+int md_autotune(md_engine* e, int nsteps, double* best_ms_per_step);
+
+/* Kernel timing with HIP events recorded on the engine's stream around every
+ * launch of the pair/step kernel during md_run.  enable resets the counters. */
+int md_timing_enable(md_engine* e, int enable);
+int md_timing_get(md_engine* e, double* step_kernel_ms, long long* step_kernel_launches,
+                  double* rebuild_ms, long long* rebuild_count);
+/* Plain device copy of nbytes (calibration of HBM counters). Returns ms. */
+int md_copy_bench(md_engine* e, long long nbytes, int reps, double* ms);
+
+/* ---- force matching (no counterpart in the reference) ----
+ * Frames share the engine's system (type, charge, box of replica 0).
+ * md_fm_set_frames uploads nframes x N x 3 positions and target forces once. */
+int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* target_force);
+/* loss = mean over frames*atoms*3 of (F - F*)^2 with charges scaled by qscale
+ * and LJ tables lj1/lj2 (ntype x ntype, ordered).  Gradients per ordered table
+ * entry and for qscale, FP64.  Any gradient pointer may be NULL. */
+int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qscale,
+                    double* loss, double* g_lj1, double* g_lj2, double* g_qscale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1124 @@
+// engine.hip -- host side of the MD step engine and its C ABI (md_api.h).
+//
+// One engine = one HIP stream on one device, holding R >= 1 replicas that
+// share a force field.  A single system is R = 1.  The reference's per-step
+// host round trips (skin check flag, KE partials: src/neighborlist.cu:127-129,
+// src/integrator.cu:152) are replaced by device-side flags and reductions, so
+// a run is queued as hipGraph segments of up to kSegment steps and the host
+// synchronises once per segment.
+//
+// Per step (leapfrog):          [rebuild, gated by flag]  k_step(LF)
+// Per step (velocity Verlet):   k_vv_kick  [rebuild, gated]  k_step(VV)
+// A rebuild is skipped on the device (the five kernels exit at once) unless
+// the previous position update moved some atom of that replica by at least
+// skin/2 since its last build (src/neighborlist.cu:36,157-176).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "md_device.h"
+#include "mdgpu/md_api.h"
+
+extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int* tmpid, hipStream_t st);
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int energy, int grid, hipStream_t st);
+extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
+extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
+                                           int mode, hipStream_t st);
+extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* gacc, int lanes, int coul, int nt, int grid,
+                                           hipStream_t st);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) return fail(MD_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x,     \
+                                          hipGetErrorString(e_));                                \
+    } while (0)
+
+#define CHECK(x)                 \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_ != MD_OK) return rc_; \
+    } while (0)
+
+const int kSegment = 100;        // steps per graph segment
+const double kPi = 3.14159265358979323846;
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MD_E_HIP, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        e = hipMemset(p, 0, count * sizeof(T));
+        if (e != hipSuccess) return fail(MD_E_HIP, "hipMemset: %s", hipGetErrorString(e));
+        return MD_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Skilling's transpose form of the Hilbert index ("Programming the Hilbert
+// curve", AIP Conf. Proc. 707, 381 (2004)), 3 axes of `bits` bits each.
+uint64_t hilbert3(unsigned x, unsigned y, unsigned z, int bits) {
+    unsigned X[3] = {x, y, z};
+    unsigned M = 1u << (bits - 1);
+    for (unsigned Q = M; Q > 1; Q >>= 1) {
+        unsigned P = Q - 1;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                unsigned t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+    unsigned t = 0;
+    for (unsigned Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; i++) X[i] ^= t;
+    uint64_t h = 0;
+    for (int b = bits - 1; b >= 0; b--)
+        for (int i = 0; i < 3; i++) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
+}
+
+double as_poly(double t) {   // A&S 7.1.26 polynomial, src/simulation.h:39-43
+    return ((((1.061405429 * t - 1.453152027) * t + 1.421413741) * t - 0.284496736) * t + 0.254829592) * t;
+}
+
+}  // namespace
+
+struct md_engine {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int ncu = 256;
+    std::string arch;
+
+    bool has_ff = false, has_integ = false, has_sys = false, has_state = false;
+    md_forcefield ff{};
+    md_integrator in{};
+    float rc2 = 0, rl2 = 0, skin2q = 0;
+    float palfa = 0, spiialfa = 0, alfasq = 0, wolf = 0, reacf = 0, ecut = 0, log2rc = 0;
+    int coul_kernel = MD_COUL_NONE;
+
+    int R = 0, N = 0, S = 0, ncell_total = 0;
+    std::vector<int> natom, atom_off, slot_off;
+    std::vector<float> box, T0;
+    std::vector<int> type_h;
+    std::vector<float> charge_h, mass_h;
+    std::vector<RepParam> rep_h;
+    std::vector<int> tile_rep_h;
+    std::vector<int> bonds_h, angles_h;
+    int nb = 0, na = 0;
+    bool bonded = false, has_excl = false;
+
+    DBuf<float4> pos[2], vel, force, pos_build, pos_s, vel_s, force_s, target;
+    DBuf<int> ids, typ, ids_s, typ_s, slot_of, key, rank, tmp, tmpid;
+    DBuf<int> cell_count, cell_start, cell_rank;
+    DBuf<uint32_t> nlist;
+    DBuf<int> ncount, maxcount;
+    DBuf<int4> excl, ang_list;
+    DBuf<uint32_t> bond_list;
+    DBuf<int> nbond, nang;
+    DBuf<RepParam> rep;
+    DBuf<int> tile_rep;
+    DBuf<ThState> th;
+    DBuf<double> acc, fm_acc;
+    DBuf<int> units_done, flags, err;
+    DBuf<unsigned long long> reb_count;
+    DBuf<StatRec> stats;
+    DBuf<float4> ljtab;
+    // snapshot for overflow recovery
+    DBuf<float4> snap_pos, snap_vel, snap_force;
+    DBuf<int> snap_ids, snap_typ, snap_slot_of;
+    DBuf<ThState> snap_th;
+    int cap = 0;
+
+    long long step = 0;
+    long long rebuilds = 0;
+    int lanes = 16, grid = 0, cell_div = 2;
+    int fm_nframes = 0;
+    std::map<std::string, hipGraphExec_t> graphs;
+
+    bool timing = false;
+    double t_step_ms = 0, t_reb_ms = 0;
+    long long n_step = 0, n_reb = 0;
+    std::vector<hipEvent_t> ev_pool;
+
+    ~md_engine() {
+        drop_graphs();
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    int apb() const { return 256 / lanes; }
+    int nunits() const { return S / apb(); }
+    int grid_blocks() const {
+        int g = grid > 0 ? grid : ncu * 4;
+        return std::max(1, std::min(g, nunits()));
+    }
+};
+
+namespace {
+
+int check_ready(md_engine* e, bool need_state) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (!e->has_ff) return fail(MD_E_STATE, "force field not set");
+    if (!e->has_integ) return fail(MD_E_STATE, "integrator not set");
+    if (!e->has_sys) return fail(MD_E_STATE, "system not set");
+    if (need_state && !e->has_state) return fail(MD_E_STATE, "state not uploaded");
+    return MD_OK;
+}
+
+// ---- derived constants -------------------------------------------------
+void derive_ff(md_engine* e) {
+    const md_forcefield& f = e->ff;
+    float rc = f.rc;
+    e->rc2 = rc * rc;                                                  // src/simulation.cu:96
+    double rl = (double)f.skin + std::sqrt((double)e->rc2);
+    e->rl2 = (float)(rl * rl);                                         // src/neighborlist.cu:35
+    e->skin2q = f.skin * f.skin / 4.0f;                                // src/neighborlist.cu:36
+    double a = f.alpha, rcd = f.rc;
+    e->palfa = (float)(a * 0.3275911);                                 // src/forces.cu:98
+    e->spiialfa = (float)(2.0 / std::sqrt(kPi) * a);                   // :100
+    e->alfasq = (float)(-a * a / std::log(2.0));                       // :102
+    e->wolf = 0.f;
+    e->ecut = 0.f;
+    e->reacf = 0.f;
+    e->log2rc = (float)std::log2(rcd);
+    if (f.coulomb == MD_COUL_WOLF) {                                   // :104-109 (in FP64 here)
+        double t = 1.0 / (1.0 + a * 0.3275911 * rcd);
+        double ex = std::exp(-a * a * rcd * rcd);
+        e->wolf = (float)((2.0 / std::sqrt(kPi) * a * rcd + as_poly(t)) * ex / rcd);
+        e->ecut = (float)(as_poly(t) * ex / rcd);
+    } else if (f.coulomb == MD_COUL_RF) {                              // src/simulation.cu:86
+        double kr = 2.0 * ((double)f.eps_rf - 1.0) / (2.0 * (double)f.eps_rf + 1.0) / (rcd * rcd * rcd);
+        e->reacf = (float)kr;
+        e->ecut = (float)(1.0 / rcd + 0.5 * kr * rcd * rcd);
+    }
+}
+
+int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
+    const int nt = e->ff.ntype;
+    std::vector<float4> t(64, make_float4(0, 0, 0, 0));
+    double rc2 = (double)e->ff.rc * e->ff.rc;
+    for (int i = 0; i < nt; i++)
+        for (int j = 0; j < nt; j++) {
+            double l1 = lj1[i * nt + j], l2 = lj2[i * nt + j];
+            double ri6 = 1.0 / (rc2 * rc2 * rc2);
+            double shift = l1 / 12.0 * ri6 * ri6 - l2 / 6.0 * ri6;
+            t[i * 8 + j] = make_float4((float)l1, (float)l2, (float)shift, 0.f);
+        }
+    HIPCHECK(hipMemcpyAsync(e->ljtab.p, t.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
+    return MD_OK;
+}
+
+// ---- cells --------------------------------------------------------------
+int setup_cells(md_engine* e) {
+    double rl = std::sqrt((double)e->rl2);
+    std::vector<int> ranks;
+    int off = 0;
+    for (int r = 0; r < e->R; r++) {
+        RepParam& p = e->rep_h[r];
+        int nmax = 1;
+        for (int d = 0; d < 3; d++) {
+            double L = p.L[d];
+            if (rl >= 0.5 * L) return fail(MD_E_INVALID, "replica %d: list radius %.4f must be below half the box (%.4f)", r, rl, L);
+            int nc = (int)std::floor(L * e->cell_div / rl);
+            nc = std::max(1, std::min(nc, 1024));
+            p.nc[d] = nc;
+            p.cedge[d] = (float)(L / nc);
+            p.cinv[d] = (float)(nc / L);
+            nmax = std::max(nmax, nc);
+        }
+        int bits = 1;
+        while ((1 << bits) < nmax) bits++;
+        int ncell = p.nc[0] * p.nc[1] * p.nc[2];
+        std::vector<std::pair<uint64_t, int>> h(ncell);
+        for (int cz = 0; cz < p.nc[2]; cz++)
+            for (int cy = 0; cy < p.nc[1]; cy++)
+                for (int cx = 0; cx < p.nc[0]; cx++) {
+                    int lin = cx + p.nc[0] * (cy + p.nc[1] * cz);
+                    h[lin] = {hilbert3(cx, cy, cz, bits), lin};
+                }
+        std::sort(h.begin(), h.end());
+        ranks.resize(off + ncell);
+        for (int k = 0; k < ncell; k++) ranks[off + h[k].second] = k;
+        p.cell_off = off;
+        p.cs_off = off + r;
+        off += ncell;
+    }
+    e->ncell_total = off;
+    CHECK(e->cell_rank.alloc(off));
+    CHECK(e->cell_count.alloc(off));
+    CHECK(e->cell_start.alloc(off + e->R));
+    HIPCHECK(hipMemcpy(e->cell_rank.p, ranks.data(), off * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->rep.p, e->rep_h.data(), e->R * sizeof(RepParam), hipMemcpyHostToDevice));
+    return MD_OK;
+}
+
+int estimate_cap(md_engine* e) {
+    double rl = std::sqrt((double)e->rl2);
+    double dens = 0;
+    for (int r = 0; r < e->R; r++) {
+        double V = (double)e->box[3 * r] * e->box[3 * r + 1] * e->box[3 * r + 2];
+        dens = std::max(dens, e->natom[r] / V);
+    }
+    double k = dens * 4.0 / 3.0 * kPi * rl * rl * rl;
+    int c = (int)(k * 1.4 + 64);
+    c = std::min(c, e->N);
+    return std::max(32, (c + 31) / 32 * 32);
+}
+
+int alloc_list(md_engine* e, int cap) {
+    e->cap = cap;
+    CHECK(e->nlist.alloc((size_t)e->S * cap));
+    e->drop_graphs();
+    return MD_OK;
+}
+
+BuildArgs build_args(md_engine* e, float4* P, int* flag) {
+    BuildArgs b{};
+    b.pos = P; b.vel = e->vel.p; b.force = e->force.p; b.ids = e->ids.p; b.typ = e->typ.p;
+    b.pos_s = e->pos_s.p; b.vel_s = e->vel_s.p; b.force_s = e->force_s.p; b.ids_s = e->ids_s.p; b.typ_s = e->typ_s.p;
+    b.pos_dst = P; b.vel_dst = e->vel.p; b.force_dst = e->force.p; b.ids_dst = e->ids.p; b.typ_dst = e->typ.p;
+    b.pos_build = e->pos_build.p;
+    b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p;
+    b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
+    b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p; b.maxcount = e->maxcount.p;
+    b.excl = e->excl.p; b.flag = flag; b.rebuild_count = e->reb_count.p; b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
+    b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
+    return b;
+}
+
+StepArgs step_args(md_engine* e, const float4* pin, float4* pout, int mode, int* fset, int* fclear, int stat_index) {
+    StepArgs a{};
+    a.pos_in = pin; a.pos_out = pout; a.vel = e->vel.p; a.force = e->force.p;
+    a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
+    a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.th = e->th.p; a.acc = e->acc.p; a.units_done = e->units_done.p;
+    a.flag_set = fset; a.flag_clear = fclear; a.err = e->err.p; a.stats = e->stats.p;
+    a.ljtab = reinterpret_cast<const float2*>(e->ljtab.p);
+    a.slot_of = e->slot_of.p; a.bond_list = e->bond_list.p; a.nbond = e->nbond.p; a.ang_list = e->ang_list.p; a.nang = e->nang.p;
+    a.target = e->target.p;
+    a.cap = e->cap; a.ntype = e->ff.ntype; a.nrep = e->R; a.nunits = e->nunits(); a.mode = mode;
+    a.thermostat = e->in.thermostat; a.bonded = e->bonded ? 1 : 0; a.stat_index = stat_index;
+    a.rc2 = e->rc2; a.qfac = e->ff.qfac; a.palfa = e->palfa; a.spiialfa = e->spiialfa; a.alfasq = e->alfasq;
+    a.wolf = e->wolf; a.reacf = e->reacf; a.ecut = e->ecut; a.log2rc = e->log2rc; a.qq_scale = 1.0f;
+    a.dt = e->in.dt; a.tau = e->in.tau; a.skin2q = e->skin2q;
+    for (int k = 0; k < 8; k++) {
+        a.bk2[k] = e->ff.bond_k2[k]; a.br0[k] = e->ff.bond_r0[k];
+        a.ak2[k] = e->ff.angle_k2[k]; a.at0[k] = e->ff.angle_t0[k];
+    }
+    return a;
+}
+
+hipEvent_t ev_get(md_engine* e, size_t i) {
+    while (e->ev_pool.size() <= i) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+        e->ev_pool.push_back(ev);
+    }
+    return e->ev_pool[i];
+}
+
+// timing bookkeeping for direct (non-graph) launches
+struct TimedSpan {
+    md_engine* e;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> steps, rebs;
+    size_t used = 0;
+    std::pair<hipEvent_t, hipEvent_t> take() {
+        hipEvent_t a = ev_get(e, used++), b = ev_get(e, used++);
+        return {a, b};
+    }
+};
+
+int enqueue_rebuild(md_engine* e, float4* P, int parity, TimedSpan* ts) {
+    BuildArgs b = build_args(e, P, e->flags.p + parity * e->R);
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
+    HIPCHECK(md_launch_rebuild(b, e->tmpid.p, e->st));
+    if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->rebs.push_back(ev); }
+    return MD_OK;
+}
+
+int enqueue_stepkernel(md_engine* e, const StepArgs& a, bool energy, TimedSpan* ts) {
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
+    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, energy ? 1 : 0, e->grid_blocks(), e->st));
+    if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->steps.push_back(ev); }
+    return MD_OK;
+}
+
+// Queue step n (absolute index), statistics record k of the current segment.
+int enqueue_step(md_engine* e, long long n, int k, bool energy, TimedSpan* ts) {
+    const int p = (int)(n & 1), q = p ^ 1;
+    int* fl = e->flags.p;
+    if (e->in.integrator == MD_LEAPFROG) {
+        CHECK(enqueue_rebuild(e, e->pos[p].p, p, ts));
+        StepArgs a = step_args(e, e->pos[p].p, e->pos[q].p, MODE_LF, fl + q * e->R, fl + p * e->R, k);
+        CHECK(enqueue_stepkernel(e, a, energy, ts));
+    } else {
+        KickArgs ka{};
+        ka.pos_in = e->pos[p].p; ka.pos_out = e->pos[q].p; ka.vel = e->vel.p; ka.force = e->force.p;
+        ka.ids = e->ids.p; ka.pos_build = e->pos_build.p; ka.tile_rep = e->tile_rep.p; ka.rep = e->rep.p;
+        ka.th = e->th.p; ka.flag_set = fl + q * e->R; ka.flag_clear = fl + p * e->R; ka.err = e->err.p;
+        ka.nslot = e->S; ka.nrep = e->R; ka.dt = e->in.dt; ka.skin2q = e->skin2q; ka.apply_drift = 1;
+        HIPCHECK(md_launch_kick(ka, e->st));
+        CHECK(enqueue_rebuild(e, e->pos[q].p, q, ts));
+        StepArgs a = step_args(e, e->pos[q].p, e->pos[q].p, MODE_VV, nullptr, nullptr, k);
+        CHECK(enqueue_stepkernel(e, a, energy, ts));
+    }
+    return MD_OK;
+}
+
+int snapshot(md_engine* e, bool restore) {
+    const int p = (int)(e->step & 1);
+    struct Pair { void* a; void* b; size_t n; };
+    Pair list[] = {
+        {e->snap_pos.p, e->pos[p].p, e->pos[p].bytes()}, {e->snap_vel.p, e->vel.p, e->vel.bytes()},
+        {e->snap_force.p, e->force.p, e->force.bytes()}, {e->snap_ids.p, e->ids.p, e->ids.bytes()},
+        {e->snap_typ.p, e->typ.p, e->typ.bytes()}, {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()},
+        {e->snap_th.p, e->th.p, e->th.bytes()}};
+    for (auto& x : list) {
+        if (restore) HIPCHECK(hipMemcpyAsync(x.b, x.a, x.n, hipMemcpyDeviceToDevice, e->st));
+        else HIPCHECK(hipMemcpyAsync(x.a, x.b, x.n, hipMemcpyDeviceToDevice, e->st));
+    }
+    return MD_OK;
+}
+
+// Rebuild all replicas now at the current positions (host synchronous).
+int full_rebuild(md_engine* e) {
+    const int p = (int)(e->step & 1);
+    std::vector<int> one(e->R, 1);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        HIPCHECK(hipMemcpyAsync(e->flags.p + p * e->R, one.data(), e->R * sizeof(int), hipMemcpyHostToDevice, e->st));
+        HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+        CHECK(enqueue_rebuild(e, e->pos[p].p, p, nullptr));
+        std::vector<int> mc(e->R);
+        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipStreamSynchronize(e->st));
+        int m = *std::max_element(mc.begin(), mc.end());
+        if (m <= e->cap) break;
+        CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
+    }
+    HIPCHECK(hipMemsetAsync(e->flags.p + p * e->R, 0, e->R * sizeof(int), e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    return MD_OK;
+}
+
+int ensure_lists_current(md_engine* e) {
+    if (e->in.integrator != MD_LEAPFROG) return MD_OK;
+    const int p = (int)(e->step & 1);
+    CHECK(enqueue_rebuild(e, e->pos[p].p, p, nullptr));
+    HIPCHECK(hipMemsetAsync(e->flags.p + p * e->R, 0, e->R * sizeof(int), e->st));
+    return MD_OK;
+}
+
+int ensure_stats(md_engine* e, int nrec) {
+    if (e->stats.n < (size_t)nrec * e->R) CHECK(e->stats.alloc((size_t)nrec * e->R));
+    return MD_OK;
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char* md_last_error(void) { return g_err.c_str(); }
+
+const char* md_version(void) {
+    return "mdgpu 0.1 (gfx950, hipcc " __clang_version__ ")";
+}
+
+int md_create(md_engine** out, int device) {
+    if (!out) return fail(MD_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(MD_E_INVALID, "device %d not present (%d devices)", device, ndev);
+    HIPCHECK(hipSetDevice(device));
+    md_engine* e = new md_engine();
+    e->dev = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        e->ncu = prop.multiProcessorCount;
+        e->arch = prop.gcnArchName;
+    }
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return fail(MD_E_HIP, "hipStreamCreate failed");
+    }
+    if (e->ljtab.alloc(64) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
+        delete e;
+        return MD_E_HIP;
+    }
+    *out = e;
+    return MD_OK;
+}
+
+int md_destroy(md_engine* e) {
+    if (!e) return MD_OK;
+    (void)hipSetDevice(e->dev);
+    (void)hipStreamSynchronize(e->st);
+    delete e;
+    return MD_OK;
+}
+
+int md_set_forcefield(md_engine* e, const md_forcefield* f) {
+    if (!e || !f) return fail(MD_E_INVALID, "null argument");
+    if (f->ntype < 1 || f->ntype > MD_MAX_TYPES) return fail(MD_E_INVALID, "ntype %d outside 1..%d", f->ntype, MD_MAX_TYPES);
+    if (!(f->rc > 0) || !(f->skin >= 0)) return fail(MD_E_INVALID, "rc must be > 0 and skin >= 0");
+    if (f->coulomb < 0 || f->coulomb > 3) return fail(MD_E_INVALID, "unknown coulomb kind %d", f->coulomb);
+    if (f->nbond_types < 0 || f->nbond_types > 8 || f->nangle_types < 0 || f->nangle_types > 8)
+        return fail(MD_E_INVALID, "bond/angle types must be 0..8");
+    HIPCHECK(hipSetDevice(e->dev));
+    e->ff = *f;
+    if (e->ff.qfac == 0.0f) e->ff.qfac = 1389.35427611722f;
+    derive_ff(e);
+    CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    e->has_ff = true;
+    e->has_sys = false;   // cells depend on r_list: set the system again
+    e->has_state = false;
+    e->drop_graphs();
+    return MD_OK;
+}
+
+int md_set_integrator(md_engine* e, const md_integrator* in) {
+    if (!e || !in) return fail(MD_E_INVALID, "null argument");
+    if (in->integrator != MD_LEAPFROG && in->integrator != MD_VVERLET) return fail(MD_E_INVALID, "unknown integrator");
+    if (in->thermostat < 0 || in->thermostat > 2) return fail(MD_E_INVALID, "unknown thermostat");
+    if (in->integrator == MD_LEAPFROG && in->thermostat == MD_TH_NOSEHOOVER)
+        return fail(MD_E_INVALID, "Nose-Hoover needs the velocity-Verlet integrator");
+    if (!(in->dt > 0)) return fail(MD_E_INVALID, "dt must be > 0");
+    if (in->thermostat != MD_TH_NONE && !(in->tau > 0)) return fail(MD_E_INVALID, "tau must be > 0");
+    e->in = *in;
+    e->has_integ = true;
+    e->drop_graphs();
+    if (e->has_sys) {
+        for (int r = 0; r < e->R; r++) {
+            RepParam& p = e->rep_h[r];
+            p.Q = (double)p.nf * p.kT0 * (double)in->tau * in->tau;
+        }
+        HIPCHECK(hipMemcpy(e->rep.p, e->rep_h.data(), e->R * sizeof(RepParam), hipMemcpyHostToDevice));
+    }
+    return MD_OK;
+}
+
+int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, const float* T0,
+                  const int* type, const float* charge, const float* mass) {
+    if (!e || !natom || !box || !type || !charge || !mass) return fail(MD_E_INVALID, "null argument");
+    if (!e->has_ff) return fail(MD_E_STATE, "set the force field first");
+    if (nrep < 1) return fail(MD_E_INVALID, "nrep must be >= 1");
+    HIPCHECK(hipSetDevice(e->dev));
+    e->R = nrep;
+    e->natom.assign(natom, natom + nrep);
+    e->box.assign(box, box + 3 * nrep);
+    e->T0.assign(nrep, 0.f);
+    if (T0) e->T0.assign(T0, T0 + nrep);
+    e->atom_off.resize(nrep);
+    e->slot_off.resize(nrep);
+    long long N = 0, S = 0;
+    for (int r = 0; r < nrep; r++) {
+        if (natom[r] < 2) return fail(MD_E_INVALID, "replica %d has %d atoms", r, natom[r]);
+        for (int d = 0; d < 3; d++)
+            if (!(box[3 * r + d] > 0)) return fail(MD_E_INVALID, "replica %d: box must be positive", r);
+        e->atom_off[r] = (int)N;
+        e->slot_off[r] = (int)S;
+        N += natom[r];
+        S += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
+    }
+    if (S >= (1LL << MD_JBITS)) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << MD_JBITS);
+    e->N = (int)N;
+    e->S = (int)S;
+    e->type_h.assign(type, type + N);
+    e->charge_h.assign(charge, charge + N);
+    e->mass_h.assign(mass, mass + N);
+    bool charged = false;
+    for (long long i = 0; i < N; i++) {
+        if (type[i] < 0 || type[i] >= e->ff.ntype) return fail(MD_E_INVALID, "atom %lld: type %d outside 0..%d", i, type[i], e->ff.ntype - 1);
+        if (!(mass[i] > 0)) return fail(MD_E_INVALID, "atom %lld: mass must be > 0", i);
+        if (charge[i] != 0.0f) charged = true;
+    }
+    e->coul_kernel = charged ? e->ff.coulomb : MD_COUL_NONE;
+    // replica parameters
+    e->rep_h.assign(nrep, RepParam{});
+    e->tile_rep_h.assign(S / MD_TILE, 0);
+    for (int r = 0; r < nrep; r++) {
+        RepParam& p = e->rep_h[r];
+        for (int d = 0; d < 3; d++) {
+            p.L[d] = box[3 * r + d];
+            p.Linv[d] = (float)(1.0 / (double)box[3 * r + d]);         // src/simulation.cu:290-292
+            p.halfL[d] = 0.5f * box[3 * r + d];
+        }
+        p.slot_off = e->slot_off[r];
+        p.nslot = (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
+        p.natom = natom[r];
+        p.atom_off = e->atom_off[r];
+        p.T0 = e->T0[r];
+        p.nf = (float)(3.0 * natom[r] - 3.0);                           // src/integrator.cu:155
+        p.kT0 = MD_KB_REF * (double)p.T0;
+        p.Q = (double)p.nf * p.kT0 * (double)e->in.tau * e->in.tau;
+        for (int t = p.slot_off / MD_TILE; t < (p.slot_off + p.nslot) / MD_TILE; t++) e->tile_rep_h[t] = r;
+    }
+    // device buffers
+    e->drop_graphs();
+    const size_t Sz = S;
+    for (auto* b : {&e->pos[0], &e->pos[1], &e->vel, &e->force, &e->pos_build, &e->pos_s, &e->vel_s, &e->force_s,
+                    &e->target, &e->snap_pos, &e->snap_vel, &e->snap_force})
+        CHECK(b->alloc(Sz));
+    for (auto* b : {&e->ids, &e->typ, &e->ids_s, &e->typ_s, &e->key, &e->rank, &e->tmp, &e->tmpid, &e->ncount,
+                    &e->snap_ids, &e->snap_typ})
+        CHECK(b->alloc(Sz));
+    CHECK(e->slot_of.alloc(N));
+    CHECK(e->snap_slot_of.alloc(N));
+    CHECK(e->rep.alloc(nrep));
+    CHECK(e->tile_rep.alloc(S / MD_TILE));
+    CHECK(e->th.alloc(nrep));
+    CHECK(e->snap_th.alloc(nrep));
+    CHECK(e->acc.alloc((size_t)nrep * MD_NSTAT));
+    CHECK(e->units_done.alloc(nrep));
+    CHECK(e->flags.alloc(2 * (size_t)nrep));
+    CHECK(e->maxcount.alloc(nrep));
+    CHECK(ensure_stats(e, kSegment + 1));
+    std::vector<int> neg(S, -1);
+    HIPCHECK(hipMemcpy(e->ids.p, neg.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->ids_s.p, neg.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->tile_rep.p, e->tile_rep_h.data(), (S / MD_TILE) * sizeof(int), hipMemcpyHostToDevice));
+    CHECK(setup_cells(e));
+    CHECK(alloc_list(e, estimate_cap(e)));
+    // topology defaults: none
+    e->nb = e->na = 0;
+    e->bonded = e->has_excl = false;
+    CHECK(e->excl.alloc(N));
+    CHECK(e->bond_list.alloc(4 * (size_t)N));
+    CHECK(e->ang_list.alloc(4 * (size_t)N));
+    CHECK(e->nbond.alloc(N));
+    CHECK(e->nang.alloc(N));
+    e->has_sys = true;
+    e->has_state = false;
+    e->fm_nframes = 0;
+    return MD_OK;
+}
+
+int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* angles) {
+    if (!e || !e->has_sys) return fail(MD_E_STATE, "set the system first");
+    if (nb < 0 || na < 0 || (nb && !bonds) || (na && !angles)) return fail(MD_E_INVALID, "bad topology arrays");
+    const int N = e->N;
+    std::vector<int4> ex(N, make_int4(-1, -1, -1, -1));
+    std::vector<uint32_t> bl(4 * (size_t)N, 0);
+    std::vector<int> nbo(N, 0), nan_(N, 0);
+    std::vector<int4> al(4 * (size_t)N, make_int4(0, 0, 0, 0));
+    std::vector<int> nex(N, 0);
+    for (int b = 0; b < nb; b++) {
+        int i = bonds[3 * b], j = bonds[3 * b + 1], t = bonds[3 * b + 2];
+        if (i < 0 || i >= N || j < 0 || j >= N || i == j) return fail(MD_E_INVALID, "bond %d: bad atoms %d %d", b, i, j);
+        if (t < 0 || t >= e->ff.nbond_types) return fail(MD_E_INVALID, "bond %d: type %d not defined", b, t);
+        for (int s = 0; s < 2; s++) {
+            int a = s ? j : i, c = s ? i : j;
+            if (nbo[a] >= 4) return fail(MD_E_CAPACITY, "atom %d has more than 4 bonds (src/forces.cu:118)", a);
+            bl[(size_t)a * 4 + nbo[a]++] = (uint32_t)c | ((uint32_t)t << MD_JBITS);
+            int* sl = &ex[a].x;
+            sl[nex[a]++] = c;   // 1-2 exclusion (src/neighborlist.cu:74-97, slot bug not copied)
+        }
+    }
+    for (int q = 0; q < na; q++) {
+        int i = angles[4 * q], j = angles[4 * q + 1], k = angles[4 * q + 2], t = angles[4 * q + 3];
+        if (i < 0 || i >= N || j < 0 || j >= N || k < 0 || k >= N) return fail(MD_E_INVALID, "angle %d: bad atoms", q);
+        if (t < 0 || t >= e->ff.nangle_types) return fail(MD_E_INVALID, "angle %d: type %d not defined", q, t);
+        int who[3] = {i, j, k};
+        int4 ent[3] = {make_int4(0 | (t << 8), j, k, 0), make_int4(1 | (t << 8), i, k, 0), make_int4(0 | (t << 8), j, i, 0)};
+        for (int s = 0; s < 3; s++) {   // per-atom angle list, src/forces.cu:154-173
+            int a = who[s];
+            if (nan_[a] >= 4) return fail(MD_E_CAPACITY, "atom %d is in more than 4 angles (src/forces.cu:147)", a);
+            al[(size_t)a * 4 + nan_[a]++] = ent[s];
+        }
+    }
+    HIPCHECK(hipMemcpy(e->excl.p, ex.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->bond_list.p, bl.data(), bl.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->nbond.p, nbo.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->ang_list.p, al.data(), al.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->nang.p, nan_.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    e->nb = nb;
+    e->na = na;
+    e->bonded = nb > 0 || na > 0;
+    e->has_excl = nb > 0;
+    e->drop_graphs();
+    if (e->has_state) CHECK(full_rebuild(e));
+    return MD_OK;
+}
+
+int md_upload_state(md_engine* e, const float* pos, const float* vel) {
+    CHECK(check_ready(e, false));
+    if (!pos) return fail(MD_E_INVALID, "null positions");
+    HIPCHECK(hipSetDevice(e->dev));
+    const int S = e->S;
+    std::vector<float4> P(S, make_float4(0, 0, 0, 0)), V(S, make_float4(0, 0, 0, 0));
+    std::vector<int> ids(S, -1), typ(S, 0), slot_of(e->N);
+    std::vector<ThState> th(e->R);
+    for (int r = 0; r < e->R; r++) {
+        double ke2 = 0;
+        for (int k = 0; k < e->natom[r]; k++) {
+            int i = e->atom_off[r] + k, s = e->slot_off[r] + k;
+            float x[3];
+            for (int d = 0; d < 3; d++) {
+                float L = e->box[3 * r + d];
+                float v = pos[3 * (size_t)i + d];
+                if (!std::isfinite(v)) return fail(MD_E_INVALID, "atom %d: non-finite position", i);
+                v -= L * std::floor(v / L + 0.5f);   // into the centred box
+                if (v >= 0.5f * L) v -= L;
+                x[d] = v;
+            }
+            P[s] = make_float4(x[0], x[1], x[2], e->charge_h[i]);
+            float invm = (float)(1.0 / (1e4 * (double)e->mass_h[i]));   // src/simulation.cu:177
+            float v3[3] = {0, 0, 0};
+            if (vel)
+                for (int d = 0; d < 3; d++) v3[d] = vel[3 * (size_t)i + d];
+            V[s] = make_float4(v3[0], v3[1], v3[2], invm);
+            ke2 += (double)(v3[0] * (double)v3[0] + v3[1] * (double)v3[1] + v3[2] * (double)v3[2]) / (double)invm;
+            ids[s] = i;
+            typ[s] = e->type_h[i];
+            slot_of[i] = s;
+        }
+        th[r] = ThState{0, 0, 1.0, 1.0, 0, 0, ke2, 0};
+    }
+    const int p = (int)(e->step & 1);
+    HIPCHECK(hipMemcpy(e->pos[p].p, P.data(), S * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->pos[p ^ 1].p, P.data(), S * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->vel.p, V.data(), S * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(e->force.p, 0, S * sizeof(float4)));
+    HIPCHECK(hipMemcpy(e->ids.p, ids.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->typ.p, typ.data(), S * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->slot_of.p, slot_of.data(), e->N * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->th.p, th.data(), e->R * sizeof(ThState), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemset(e->acc.p, 0, e->acc.bytes()));
+    HIPCHECK(hipMemset(e->units_done.p, 0, e->units_done.bytes()));
+    HIPCHECK(hipMemset(e->flags.p, 0, e->flags.bytes()));
+    HIPCHECK(hipMemset(e->err.p, 0, sizeof(int)));
+    e->has_state = true;
+    CHECK(full_rebuild(e));
+    if (e->in.integrator == MD_VVERLET) {
+        // velocity Verlet needs F(x0) before the first half kick
+        StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, -1);
+        CHECK(enqueue_stepkernel(e, a, false, nullptr));
+        HIPCHECK(hipStreamSynchronize(e->st));
+    }
+    return MD_OK;
+}
+
+int md_download_state(md_engine* e, float* pos, float* vel, float* force) {
+    CHECK(check_ready(e, true));
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    const int S = e->S, p = (int)(e->step & 1);
+    std::vector<int> ids(S);
+    HIPCHECK(hipMemcpy(ids.data(), e->ids.p, S * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<float4> buf(S);
+    float* outs[3] = {pos, vel, force};
+    float4* srcs[3] = {e->pos[p].p, e->vel.p, e->force.p};
+    for (int k = 0; k < 3; k++) {
+        if (!outs[k]) continue;
+        HIPCHECK(hipMemcpy(buf.data(), srcs[k], S * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int s = 0; s < S; s++) {
+            int i = ids[s];
+            if (i < 0) continue;
+            outs[k][3 * (size_t)i] = buf[s].x;
+            outs[k][3 * (size_t)i + 1] = buf[s].y;
+            outs[k][3 * (size_t)i + 2] = buf[s].z;
+        }
+    }
+    return MD_OK;
+}
+
+int md_compute_forces(md_engine* e, float* force, double* energy, double* virial) {
+    CHECK(check_ready(e, true));
+    HIPCHECK(hipSetDevice(e->dev));
+    CHECK(ensure_lists_current(e));
+    const int p = (int)(e->step & 1);
+    StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, 0);
+    CHECK(enqueue_stepkernel(e, a, true, nullptr));
+    std::vector<StatRec> rec(e->R);
+    HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, e->R * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    for (int r = 0; r < e->R; r++) {
+        if (energy) {
+            energy[4 * r] = rec[r].pe_lj;
+            energy[4 * r + 1] = rec[r].pe_coul;
+            energy[4 * r + 2] = rec[r].pe_bond;
+            energy[4 * r + 3] = rec[r].pe_angle;
+        }
+        if (virial) virial[r] = rec[r].virial;
+    }
+    if (force) CHECK(md_download_state(e, nullptr, nullptr, force));
+    return MD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Run one segment of K steps starting at e->step.
+int run_segment(md_engine* e, int K, int energy_every, TimedSpan* ts) {
+    const long long n0 = e->step;
+    auto is_energy = [&](long long n) { return energy_every > 0 && (n % energy_every) == 0; };
+    if (ts) {   // direct launches with events (timing mode)
+        for (int k = 0; k < K; k++) CHECK(enqueue_step(e, n0 + k, k, is_energy(n0 + k), ts));
+        return MD_OK;
+    }
+    char keyb[128];
+    snprintf(keyb, sizeof keyb, "%d:%lld:%d:%lld", K, n0 & 1, energy_every, energy_every > 0 ? n0 % energy_every : 0LL);
+    std::string key(keyb);
+    auto it = e->graphs.find(key);
+    if (it == e->graphs.end()) {
+        hipGraph_t g;
+        HIPCHECK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+        int rc = MD_OK;
+        for (int k = 0; k < K && rc == MD_OK; k++) rc = enqueue_step(e, n0 + k, k, is_energy(n0 + k), nullptr);
+        hipError_t ce = hipStreamEndCapture(e->st, &g);
+        if (rc != MD_OK) return rc;
+        if (ce != hipSuccess) return fail(MD_E_HIP, "graph capture: %s", hipGetErrorString(ce));
+        hipGraphExec_t ex;
+        hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return fail(MD_E_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+        it = e->graphs.emplace(key, ex).first;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, e->st));
+    return MD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
+    CHECK(check_ready(e, true));
+    if (nsteps < 0) return fail(MD_E_INVALID, "nsteps < 0");
+    if (nsteps == 0) return MD_OK;
+    HIPCHECK(hipSetDevice(e->dev));
+    const bool vv = e->in.integrator == MD_VVERLET;
+    TimedSpan span{e, {}, {}, 0};
+    TimedSpan* ts = e->timing ? &span : nullptr;
+    if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
+    int done = 0, retries = 0;
+    std::vector<StatRec> rec;
+    while (done < nsteps) {
+        const int K = std::min(kSegment, nsteps - done);
+        CHECK(snapshot(e, false));
+        HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+        span.used = 0;
+        span.steps.clear();
+        span.rebs.clear();
+        CHECK(run_segment(e, K, energy_every, ts));
+        std::vector<int> mc(e->R);
+        int errf = 0;
+        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
+        if (stats) {
+            rec.resize((size_t)K * e->R);
+            HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, rec.size() * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
+        }
+        HIPCHECK(hipStreamSynchronize(e->st));
+        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box near step %lld", e->step);
+        int m = *std::max_element(mc.begin(), mc.end());
+        if (m > e->cap) {   // a list row overflowed: roll the segment back with a larger capacity
+            if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
+            CHECK(snapshot(e, true));
+            CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
+            CHECK(full_rebuild(e));
+            continue;
+        }
+        retries = 0;
+        if (ts) {
+            for (auto& pr : span.steps) {
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+                e->t_step_ms += ms;
+                e->n_step++;
+            }
+            for (auto& pr : span.rebs) {
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+                e->t_reb_ms += ms;
+                e->n_reb++;
+            }
+        }
+        if (stats) {
+            for (int k = 0; k < K; k++)
+                for (int r = 0; r < e->R; r++) {
+                    const StatRec& s = rec[(size_t)k * e->R + r];
+                    md_step_stats& o = stats[(size_t)(done + k) * e->R + r];
+                    o.ke = s.ke; o.temp = s.temp; o.pe_lj = s.pe_lj; o.pe_coul = s.pe_coul; o.pe_bond = s.pe_bond;
+                    o.pe_angle = s.pe_angle; o.virial = s.virial; o.conserved = s.conserved; o.xi = s.xi;
+                    o.vxi = s.vxi; o.has_energy = s.has_energy; o.pad = 0;
+                }
+        }
+        e->step += K;
+        done += K;
+    }
+    if (vv) {
+        KickArgs ka{};
+        const int p = (int)(e->step & 1);
+        ka.pos_in = e->pos[p].p; ka.pos_out = e->pos[p].p; ka.vel = e->vel.p; ka.force = e->force.p; ka.ids = e->ids.p;
+        ka.pos_build = e->pos_build.p; ka.tile_rep = e->tile_rep.p; ka.rep = e->rep.p; ka.th = e->th.p;
+        ka.err = e->err.p; ka.nslot = e->S; ka.nrep = e->R; ka.dt = e->in.dt; ka.skin2q = e->skin2q; ka.apply_drift = 0;
+        HIPCHECK(md_launch_kick(ka, e->st));
+        HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 1, e->st));
+    }
+    HIPCHECK(hipStreamSynchronize(e->st));
+    return MD_OK;
+}
+
+int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
+    CHECK(check_ready(e, true));
+    if (!counts || cap < 0 || (cap > 0 && !nbrs)) return fail(MD_E_INVALID, "bad output arrays");
+    HIPCHECK(hipSetDevice(e->dev));
+    CHECK(ensure_lists_current(e));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    const int S = e->S;
+    std::vector<int> ids(S), nc(S);
+    std::vector<uint32_t> L((size_t)S * e->cap);
+    HIPCHECK(hipMemcpy(ids.data(), e->ids.p, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(nc.data(), e->ncount.p, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(L.data(), e->nlist.p, L.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    int worst = 0;
+    for (int s = 0; s < S; s++) {
+        int i = ids[s];
+        if (i < 0) continue;
+        counts[i] = nc[s];
+        worst = std::max(worst, nc[s]);
+        int c = std::min(std::min(nc[s], cap), e->cap);
+        for (int k = 0; k < c; k++) nbrs[(size_t)i * cap + k] = ids[L[(size_t)s * e->cap + k] & MD_JMASK];
+    }
+    if (worst > cap) return fail(MD_E_CAPACITY, "rows need %d entries, cap is %d", worst, cap);
+    if (worst > e->cap) return fail(MD_E_CAPACITY, "device list overflowed (%d > %d)", worst, e->cap);
+    return MD_OK;
+}
+
+int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q) {
+    if (!e || !e->has_ff) return fail(MD_E_STATE, "force field not set");
+    if (rc2) *rc2 = e->rc2;
+    if (rl2) *rl2 = e->rl2;
+    if (skin2q) *skin2q = e->skin2q;
+    return MD_OK;
+}
+
+int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (rebuilds) {
+        unsigned long long c = 0;
+        HIPCHECK(hipSetDevice(e->dev));
+        HIPCHECK(hipStreamSynchronize(e->st));
+        HIPCHECK(hipMemcpy(&c, e->reb_count.p, sizeof c, hipMemcpyDeviceToHost));
+        *rebuilds = (long long)c;
+    }
+    if (steps) *steps = e->step;
+    if (cap) *cap = e->cap;
+    if (lanes) *lanes = e->lanes;
+    if (grid) *grid = e->has_sys ? e->grid_blocks() : e->grid;
+    return MD_OK;
+}
+
+int md_set_launch(md_engine* e, int lanes, int grid, int cell_div) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) return fail(MD_E_INVALID, "lanes must be 4, 8, 16 or 32");
+    if (grid < 0) return fail(MD_E_INVALID, "grid must be >= 0");
+    if (cell_div < 1 || cell_div > 3) return fail(MD_E_INVALID, "cell_div must be 1..3");
+    HIPCHECK(hipSetDevice(e->dev));
+    e->lanes = lanes;
+    e->grid = grid;
+    e->drop_graphs();
+    if (cell_div != e->cell_div) {
+        e->cell_div = cell_div;
+        if (e->has_sys) {
+            CHECK(setup_cells(e));
+            if (e->has_state) CHECK(full_rebuild(e));
+        }
+    }
+    return MD_OK;
+}
+
+int md_timing_enable(md_engine* e, int enable) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    e->timing = enable != 0;
+    e->t_step_ms = e->t_reb_ms = 0;
+    e->n_step = e->n_reb = 0;
+    return MD_OK;
+}
+
+int md_timing_get(md_engine* e, double* step_ms, long long* step_n, double* reb_ms, long long* reb_n) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (step_ms) *step_ms = e->t_step_ms;
+    if (step_n) *step_n = e->n_step;
+    if (reb_ms) *reb_ms = e->t_reb_ms;
+    if (reb_n) *reb_n = e->n_reb;
+    return MD_OK;
+}
+
+int md_autotune(md_engine* e, int nsteps, double* best_ms) {
+    CHECK(check_ready(e, true));
+    if (nsteps < 1) return fail(MD_E_INVALID, "nsteps must be >= 1");
+    HIPCHECK(hipSetDevice(e->dev));
+    // candidate grid: lanes x cell divisor x workgroups per CU
+    const int lanes_c[4] = {4, 8, 16, 32};
+    const int div_c[2] = {1, 2};
+    const int wpc_c[3] = {2, 4, 8};
+    const bool saved_timing = e->timing;
+    e->timing = false;
+    double best = 1e30;
+    int bl = e->lanes, bg = e->grid, bd = e->cell_div;
+    hipEvent_t t0 = ev_get(e, 0), t1 = ev_get(e, 1);
+    // keep the physical state: snapshot in host memory
+    const int N = e->N;
+    std::vector<float> P(3 * (size_t)N), V(3 * (size_t)N);
+    const long long step0 = e->step;
+    CHECK(md_download_state(e, P.data(), V.data(), nullptr));
+    for (int d : div_c)
+        for (int l : lanes_c)
+            for (int w : wpc_c) {
+                CHECK(md_set_launch(e, l, e->ncu * w, d));
+                e->step = step0;
+                CHECK(md_upload_state(e, P.data(), V.data()));
+                CHECK(md_run(e, std::min(nsteps, 5), 0, nullptr));   // warm up (graph capture)
+                HIPCHECK(hipEventRecord(t0, e->st));
+                CHECK(md_run(e, nsteps, 0, nullptr));
+                HIPCHECK(hipEventRecord(t1, e->st));
+                HIPCHECK(hipEventSynchronize(t1));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+                double per = ms / nsteps;
+                if (per < best) { best = per; bl = l; bg = e->ncu * w; bd = d; }
+            }
+    CHECK(md_set_launch(e, bl, bg, bd));
+    e->step = step0;
+    CHECK(md_upload_state(e, P.data(), V.data()));
+    e->timing = saved_timing;
+    if (best_ms) *best_ms = best;
+    return MD_OK;
+}
+
+int md_copy_bench(md_engine* e, long long nbytes, int reps, double* ms) {
+    if (!e || nbytes <= 0 || reps < 1) return fail(MD_E_INVALID, "bad arguments");
+    HIPCHECK(hipSetDevice(e->dev));
+    void *a = nullptr, *b = nullptr;
+    HIPCHECK(hipMalloc(&a, nbytes));
+    HIPCHECK(hipMalloc(&b, nbytes));
+    HIPCHECK(hipMemsetAsync(a, 1, nbytes, e->st));
+    hipEvent_t t0 = ev_get(e, 0), t1 = ev_get(e, 1);
+    HIPCHECK(hipMemcpyAsync(b, a, nbytes, hipMemcpyDeviceToDevice, e->st));
+    HIPCHECK(hipEventRecord(t0, e->st));
+    for (int k = 0; k < reps; k++) HIPCHECK(hipMemcpyAsync(b, a, nbytes, hipMemcpyDeviceToDevice, e->st));
+    HIPCHECK(hipEventRecord(t1, e->st));
+    HIPCHECK(hipEventSynchronize(t1));
+    float t = 0;
+    HIPCHECK(hipEventElapsedTime(&t, t0, t1));
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (ms) *ms = t / reps;
+    return MD_OK;
+}
+
+// ---------------------------------------------------------------- force matching
+int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* target_force) {
+    CHECK(check_ready(e, false));
+    if (nframes < 1 || !pos || !target_force) return fail(MD_E_INVALID, "bad frames");
+    if (e->fm_nframes == 0 && e->R != 1) return fail(MD_E_STATE, "force matching needs a single-replica system");
+    const int n = e->fm_nframes ? e->natom[0] : e->N;
+    // frames become replicas of the base system (replica 0)
+    std::vector<int> na(nframes, n), ty((size_t)nframes * n);
+    std::vector<float> bx(3 * (size_t)nframes), t0(nframes, e->T0[0]), q((size_t)nframes * n), m((size_t)nframes * n);
+    std::vector<int> base_t(e->type_h.begin(), e->type_h.begin() + n);
+    std::vector<float> base_q(e->charge_h.begin(), e->charge_h.begin() + n), base_m(e->mass_h.begin(), e->mass_h.begin() + n);
+    float b0[3] = {e->box[0], e->box[1], e->box[2]};
+    for (int f = 0; f < nframes; f++) {
+        for (int d = 0; d < 3; d++) bx[3 * f + d] = b0[d];
+        for (int i = 0; i < n; i++) {
+            ty[(size_t)f * n + i] = base_t[i];
+            q[(size_t)f * n + i] = base_q[i];
+            m[(size_t)f * n + i] = base_m[i];
+        }
+    }
+    CHECK(md_set_system(e, nframes, na.data(), bx.data(), t0.data(), ty.data(), q.data(), m.data()));
+    e->step = 0;
+    CHECK(md_upload_state(e, pos, nullptr));
+    // targets in slot order
+    std::vector<int> so(e->N);
+    HIPCHECK(hipMemcpy(so.data(), e->slot_of.p, e->N * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<float4> T(e->S, make_float4(0, 0, 0, 0));
+    for (int i = 0; i < e->N; i++)
+        T[so[i]] = make_float4(target_force[3 * (size_t)i], target_force[3 * (size_t)i + 1], target_force[3 * (size_t)i + 2], 0.f);
+    HIPCHECK(hipMemcpy(e->target.p, T.data(), e->S * sizeof(float4), hipMemcpyHostToDevice));
+    e->fm_nframes = nframes;
+    return MD_OK;
+}
+
+int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qscale, double* loss, double* g_lj1,
+                    double* g_lj2, double* g_qscale) {
+    CHECK(check_ready(e, true));
+    if (!e->fm_nframes) return fail(MD_E_STATE, "call md_fm_set_frames first");
+    if (!lj1 || !lj2) return fail(MD_E_INVALID, "null LJ tables");
+    const int nt = e->ff.ntype;
+    if (nt > 4) return fail(MD_E_CAPACITY, "force matching supports at most 4 atom types");
+    const int ntk = nt == 1 ? 1 : (nt == 2 ? 2 : 4);
+    HIPCHECK(hipSetDevice(e->dev));
+    CHECK(upload_ljtab(e, lj1, lj2));
+    const int p = (int)(e->step & 1);
+    StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, -1);
+    a.qq_scale = qscale * qscale;
+    const int coul = e->coul_kernel;
+    HIPCHECK(md_launch_step(a, e->lanes, coul, 0, e->grid_blocks(), e->st));
+    HIPCHECK(hipMemsetAsync(e->fm_acc.p, 0, e->fm_acc.bytes(), e->st));
+    a.qq_scale = qscale;
+    HIPCHECK(md_launch_fm_adjoint(a, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+    std::vector<double> g(2 + 2 * ntk * ntk);
+    HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
+    CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    const double M = 3.0 * (double)e->N;
+    if (loss) *loss = g[0] / M;
+    if (g_qscale) *g_qscale = 2.0 / M * g[1];
+    for (int i = 0; i < nt; i++)
+        for (int j = 0; j < nt; j++) {
+            if (g_lj1) g_lj1[i * nt + j] = 2.0 / M * g[2 + i * ntk + j];
+            if (g_lj2) g_lj2[i * nt + j] = 2.0 / M * g[2 + ntk * ntk + i * ntk + j];
+        }
+    return MD_OK;
+}
+
+}  // extern "C"

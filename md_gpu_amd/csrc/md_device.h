@@ -1,0 +1,162 @@
+// md_device.h -- data layout and kernel argument records shared by the HIP
+// kernels and the host engine.
+//
+// Layout in HBM (all arrays sorted along the space-filling curve; "slot" =
+// position in that order):
+//   pos[2][S]   float4 (x, y, z, q)   double-buffered: a step reads one, writes the other
+//   vel[S]      float4 (vx, vy, vz, 1/m_int)
+//   force[S]    float4 (fx, fy, fz, 0)
+//   ids[S]      int    original atom index (-1 = padding slot)
+//   typ[S]      int    atom type
+//   nlist[S*cap] uint32 (j_slot | type_j << 29), row-major per atom, ncount[S]
+// Replicas occupy 64-aligned slot ranges, so a wavefront never straddles two.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MD_TILE 64
+#define MD_JBITS 29
+#define MD_JMASK 0x1FFFFFFFu
+#define MD_KB_REF 8.314e-3      // src/integrator.cu:155 (1000/8.314)
+#define MD_NSTAT 8              // accumulators per replica
+
+enum { ACC_KE2 = 0, ACC_LJ, ACC_COUL, ACC_BOND, ACC_ANGLE, ACC_VIR };
+enum { MODE_FORCE = 0, MODE_LF = 1, MODE_VV = 2 };
+
+struct RepParam {
+    float L[3], Linv[3], halfL[3];
+    float cinv[3];     // nc / L
+    float cedge[3];    // L / nc
+    int nc[3];
+    int cell_off;      // offset of this replica's cells in the cell tables
+    int cs_off;        // offset in cell_start (= cell_off + replica index)
+    int slot_off, nslot, natom, atom_off;
+    float T0;
+    float nf;          // 3N - 3
+    double kT0, Q;     // Nose-Hoover: kB T0, N_f kB T0 tau^2
+};
+
+// Thermostat state per replica (FP64).  S is the velocity scale the next
+// position-writing kernel applies; *_end keep the state after the end-of-step
+// half so a run can stop there.
+struct ThState {
+    double xi, vxi;
+    double S;          // pending scale (LF: lambda for next step; VV: s_end * s_begin)
+    double S_end;      // VV: end-of-step half scale only
+    double xi_end, vxi_end;
+    double ke2_cur;    // VV: 2 KE of the velocities after the end-of-step half
+    double pad;
+};
+
+// Step statistics record (mirrors md_step_stats in md_api.h)
+struct StatRec {
+    double ke, temp, pe_lj, pe_coul, pe_bond, pe_angle, virial, conserved, xi, vxi;
+    int has_energy, pad;
+};
+
+struct StepArgs {
+    const float4* pos_in;
+    float4* pos_out;
+    float4* vel;
+    float4* force;
+    const int* ids;
+    const int* typ;
+    const uint32_t* nlist;
+    const int* ncount;
+    const float4* pos_build;
+    const int* tile_rep;
+    const RepParam* rep;
+    ThState* th;
+    double* acc;          // [R][MD_NSTAT]
+    int* units_done;      // [R]
+    int* flag_set;        // [R] rebuild flag for the positions this kernel writes
+    int* flag_clear;      // [R] flag consumed before this kernel
+    int* err;             // [1] non-finite coordinates
+    StatRec* stats;       // [nrec][R] or null
+    const float2* ljtab;  // [ntype*ntype] (lj1, lj2)
+    const int* slot_of;   // [N] original index -> slot
+    const uint32_t* bond_list;  // [N*4] (partner | type << 29)
+    const int* nbond;           // [N]
+    const int4* ang_list;       // [N*4] (role | type << 8, a, b, 0)
+    const int* nang;            // [N]
+    // force-matching residual pass (null otherwise)
+    const float4* target;       // [S] target forces in slot order
+    int cap;
+    int ntype;
+    int nrep;
+    int nunits;           // S / atoms-per-unit
+    int mode;             // MODE_*
+    int thermostat;
+    int bonded;
+    int stat_index;       // -1: do not write stats
+    float rc2, qfac, palfa, spiialfa, alfasq, wolf, reacf;
+    float ecut;           // Coulomb energy shift: Wolf E(rc)/rc, RF c_rf, else 0
+    float log2rc;         // log2(rc) for the Wolf energy's ln(r/rc) term
+    float qq_scale;       // multiplies q_i q_j (force matching)
+    float dt, tau;
+    float skin2q;
+    float bk2[8], br0[8], ak2[8], at0[8];
+};
+
+// Streaming kernel of velocity Verlet (begin of step): v = S v + F dt/2m; x += v dt
+struct KickArgs {
+    const float4* pos_in;
+    float4* pos_out;
+    float4* vel;
+    const float4* force;
+    const int* ids;
+    const float4* pos_build;
+    const int* tile_rep;
+    const RepParam* rep;
+    const ThState* th;
+    int* flag_set;
+    int* flag_clear;
+    int* err;
+    int nslot;
+    int nrep;
+    float dt, skin2q;
+    int apply_drift;      // 0: finalize (v *= S_end only)
+};
+
+struct BuildArgs {
+    // old order (read by bin / scatter / permute)
+    const float4* pos;
+    const float4* vel;
+    const float4* force;
+    const int* ids;
+    const int* typ;
+    // scratch, new order
+    float4* pos_s;
+    float4* vel_s;
+    float4* force_s;
+    int* ids_s;
+    int* typ_s;
+    // copy-back targets (new order)
+    float4* pos_dst;
+    float4* vel_dst;
+    float4* force_dst;
+    int* ids_dst;
+    int* typ_dst;
+    float4* pos_build;
+    // cells
+    int* key;             // [S] global cell index of each (old) slot
+    int* rank;            // [S] arrival rank in its cell
+    int* tmp;             // [S] new slot -> old slot (cell order, arrival order)
+    int* cell_count;      // [C]
+    int* cell_start;      // [C + R]
+    const int* cell_rank; // [C] linear cell -> curve rank (per replica block)
+    int* slot_of;         // [N]
+    uint32_t* nlist;
+    int* ncount;
+    int* maxcount;        // [R]
+    const int4* excl;     // [N] 1-2 partners (-1 empty)
+    const int* flag;      // [R]
+    unsigned long long* rebuild_count;  // replicas rebuilt (counter)
+    const int* tile_rep;
+    const RepParam* rep;
+    int nslot;
+    int nrep;
+    int cap;
+    float rl2;
+    int has_excl;
+};

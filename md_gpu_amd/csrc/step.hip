@@ -1,0 +1,438 @@
+// step.hip -- the fused MD step: pair forces (LJ + Wolf / reaction-field /
+// damped Coulomb), harmonic bonds and angles, and the integrator epilogue,
+// in one persistent launch per step.
+//
+// Reference arithmetic restated (FP32, same operation order):
+//   pair loop          src/pair_forces.cu:151-200 (Wolf), :60-126 (RF)
+//   bonds / angles     src/bonded_forces.cu:50-91, :119-188
+//   leapfrog NVE/NVT   src/nve_leapfrog.cu:37-55, src/nvt_leapfrog_ber.cu:35-55
+//   Berendsen lambda   src/integrator.cu:146-155 (computed here on the device,
+//                      from the previous step's temperature, as the reference does)
+//   velocity Verlet    textbook half kicks (the reference's vverlet_nve_pre
+//                      applies the half kick twice, src/nve_vverlet.cu:49-54)
+//   Nose-Hoover        MTK chain-of-one Trotter half steps (Martyna, Tuckerman,
+//                      Klein, Mol. Phys. 87 (1996) 1117), no reference code exists
+//
+// Work decomposition: G lanes cooperate on one atom (each strides over its
+// neighbor row), 64/G atoms per wavefront, 4 wavefronts per workgroup, and a
+// persistent grid walks contiguous ranges of 256/G-atom units.  Per-atom sums
+// are reduced across the G lanes with cross-lane shuffles; KE / energies /
+// virial are reduced per workgroup and added with FP64 atomics; the last
+// workgroup to finish a replica (atomic ticket) computes that replica's
+// temperature, thermostat update and statistics record.
+//
+// Algorithmic work per ordered pair (FMA = 2, add/mul/rsqrt/rcp/exp2/rint = 1):
+// LJ + Wolf 65 flop, LJ + RF 45, LJ 36, Wolf only 60 (src/pair_forces.cu:151-200).
+#include "md_device.h"
+
+namespace {
+
+__device__ __forceinline__ float minimg(float d, float L, float Linv) {
+    return fmaf(-L, rintf(d * Linv), d);
+}
+
+__device__ __forceinline__ double atomic_load_f64(double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// MTK chain-of-one half step: returns the velocity scale, updates xi, vxi, K.
+__device__ double nh_half(double& xi, double& vxi, double& K, double nf, double kT0, double Q, double dt) {
+    double G = (2.0 * K - nf * kT0) / Q;
+    vxi += 0.25 * dt * G;
+    double s = exp(-0.5 * dt * vxi);
+    K *= s * s;
+    xi += 0.5 * dt * vxi;
+    G = (2.0 * K - nf * kT0) / Q;
+    vxi += 0.25 * dt * G;
+    return s;
+}
+
+__device__ void finalize_replica(const StepArgs& a, int r) {
+    const RepParam& p = a.rep[r];
+    double* acc = a.acc + (size_t)r * MD_NSTAT;
+    double v[MD_NSTAT];
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = atomic_load_f64(acc + k);
+    ThState& th = a.th[r];
+    StatRec rec;
+    rec.ke = rec.temp = rec.pe_lj = rec.pe_coul = rec.pe_bond = rec.pe_angle = rec.virial = rec.conserved = 0.0;
+    rec.xi = th.xi;
+    rec.vxi = th.vxi;
+    rec.has_energy = 0;
+    rec.pad = 0;
+    const double dt = a.dt, tau = a.tau;
+    double nh_terms = 0.0;
+    if (a.mode == MODE_LF) {
+        double T = v[ACC_KE2] / ((double)p.nf * MD_KB_REF);
+        double lam = 1.0;
+        if (a.thermostat == 1) lam = sqrt(1.0 + dt / tau * ((double)p.T0 / T - 1.0));   // src/integrator.cu:146-147
+        th.S = lam;
+        rec.ke = 0.5 * v[ACC_KE2];
+        rec.temp = T;
+    } else if (a.mode == MODE_VV) {
+        double K = 0.5 * v[ACC_KE2];
+        double xi = th.xi, vxi = th.vxi, s1 = 1.0;
+        if (a.thermostat == 2) {
+            s1 = nh_half(xi, vxi, K, p.nf, p.kT0, p.Q, dt);
+        } else if (a.thermostat == 1) {
+            double T = 2.0 * K / ((double)p.nf * MD_KB_REF);
+            s1 = sqrt(1.0 + dt / tau * ((double)p.T0 / T - 1.0));
+            K *= s1 * s1;
+        }
+        th.S_end = s1;
+        th.xi_end = xi;
+        th.vxi_end = vxi;
+        th.ke2_cur = 2.0 * K;
+        rec.ke = K;
+        rec.temp = 2.0 * K / ((double)p.nf * MD_KB_REF);
+        rec.xi = xi;
+        rec.vxi = vxi;
+        if (a.thermostat == 2) nh_terms = 0.5 * p.Q * vxi * vxi + (double)p.nf * p.kT0 * xi;
+        double s2 = 1.0;
+        if (a.thermostat == 2) s2 = nh_half(xi, vxi, K, p.nf, p.kT0, p.Q, dt);
+        th.xi = xi;
+        th.vxi = vxi;
+        th.S = s1 * s2;
+    }
+    if (a.stat_index >= 0 && a.stats) {
+        rec.pe_lj = 0.5 * v[ACC_LJ];
+        rec.pe_coul = 0.5 * v[ACC_COUL];
+        rec.pe_bond = v[ACC_BOND];
+        rec.pe_angle = v[ACC_ANGLE];
+        rec.virial = 0.5 * v[ACC_VIR];
+        rec.has_energy = v[MD_NSTAT - 1] > 0.0 ? 1 : 0;
+        rec.conserved = rec.ke + rec.pe_lj + rec.pe_coul + rec.pe_bond + rec.pe_angle + nh_terms;
+        a.stats[(size_t)a.stat_index * a.nrep + r] = rec;
+    }
+    for (int k = 0; k < MD_NSTAT; k++) __hip_atomic_exchange(acc + k, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_exchange(a.units_done + r, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace
+
+template <int G, int COUL, bool EN>
+__global__ __launch_bounds__(256) void k_step(const StepArgs a) {
+    __shared__ float4 s_lj[64];
+    __shared__ double s_red[4][MD_NSTAT];
+    __shared__ int s_last;
+    constexpr int APW = 64 / G;        // atoms per wavefront
+    constexpr int APB = 4 * APW;       // atoms per unit (workgroup pass)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sub = lane & (G - 1), grp = lane / G;
+    s_lj[tid & 63] = reinterpret_cast<const float4*>(a.ljtab)[tid & 63];
+    if (blockIdx.x == 0 && a.flag_clear)
+        for (int r = tid; r < a.nrep; r += 256) a.flag_clear[r] = 0;
+    __syncthreads();
+
+    const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
+    const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
+    int cur_r = -1, nunits_r = 0;
+    float L0 = 0, L1 = 0, L2 = 0, Li0 = 0, Li1 = 0, Li2 = 0, lam = 1.0f;
+    double dke = 0.0, dlj = 0.0, dco = 0.0, dbo = 0.0, dan = 0.0, dvi = 0.0;
+
+    auto flush = [&]() {
+        // workgroup reduction of the FP64 partials, one atomic per quantity
+        double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, EN ? 1.0 : 0.0};
+        for (int k = 0; k < MD_NSTAT - 1; k++)
+            for (int o = 32; o >= 1; o >>= 1) vals[k] += __shfl_xor(vals[k], o, 64);
+        if (lane == 0)
+            for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = vals[k];
+        __syncthreads();
+        if (tid == 0) {
+            double* acc = a.acc + (size_t)cur_r * MD_NSTAT;
+            double chk = 0.0;
+            for (int k = 0; k < MD_NSTAT; k++) {
+                double t = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+                if (k == MD_NSTAT - 1) t = EN ? 1.0 : 0.0;
+                if (t != 0.0) chk += atomicAdd(acc + k, t);   // returned value orders the ticket below
+            }
+            int add = nunits_r + (chk != chk ? 1 : 0);
+            int old = atomicAdd(a.units_done + cur_r, add);
+            const RepParam& p = a.rep[cur_r];
+            s_last = (old + add == p.nslot / APB) ? 1 : 0;
+        }
+        __syncthreads();
+        if (s_last && tid == 0) finalize_replica(a, cur_r);
+        __syncthreads();
+        dke = dlj = dco = dbo = dan = dvi = 0.0;
+    };
+
+    for (int u = u0; u < u1; u++) {
+        const int slot0 = u * APB;
+        const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
+        if (r != cur_r) {
+            if (cur_r >= 0) flush();
+            cur_r = r;
+            nunits_r = 0;
+            const RepParam& p = a.rep[r];
+            L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
+            Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
+            lam = (a.mode == MODE_LF) ? (float)a.th[r].S : 1.0f;
+        }
+        nunits_r++;
+        const int slot = slot0 + wave * APW + grp;
+        const int id = a.ids[slot];
+        const bool active = id >= 0;
+        const float4 xi = a.pos_in[slot];
+        const int ti = active ? a.typ[slot] : 0;
+        const int nn = active ? min(a.ncount[slot], a.cap) : 0;
+        const float qi = xi.w * a.qfac * a.qq_scale;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        double elj = 0.0, eco = 0.0, evi = 0.0;
+        const uint32_t* row = a.nlist + (size_t)slot * a.cap;
+        const float4* ljrow = s_lj + ti * 8;
+        for (int k = sub; k < nn; k += G) {
+            const uint32_t e = row[k];
+            const int j = (int)(e & MD_JMASK);
+            const int tj = (int)(e >> MD_JBITS);
+            const float4 pj = a.pos_in[j];
+            const float4 lj = ljrow[tj];
+            const float dx = minimg(xi.x - pj.x, L0, Li0);
+            const float dy = minimg(xi.y - pj.y, L1, Li1);
+            const float dz = minimg(xi.z - pj.z, L2, Li2);
+            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float ri = __builtin_amdgcn_rsqf(rsq);
+            float ff = 0.0f;
+            float ucoul = 0.0f;
+            if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
+                const float qq = qi * pj.w;
+                ff = qq * (ri - a.reacf * rsq);
+                if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
+            } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:178-183
+                const float qq = qi * pj.w;
+                const float r = rsq * ri;
+                const float t = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
+                const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+                                    0.254829592f) * t;
+                const float ex = __builtin_amdgcn_exp2f(a.alfasq * rsq);
+                float c = fmaf(a.spiialfa, r, poly) * ex * ri;
+                if (COUL == 1) c -= a.wolf;
+                ff = c * qq;
+                if (EN) {
+                    float u = poly * ex * ri - a.ecut;
+                    if (COUL == 1) u += a.wolf * (__builtin_amdgcn_logf(r) - a.log2rc) * 0.69314718056f;
+                    ucoul = qq * u;
+                }
+            }
+            const float ri2 = ri * ri;
+            ff *= ri2;
+            const float ri8 = (ri2 * ri2) * (ri2 * ri2);
+            ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
+            const bool inside = !(rsq > a.rc2);                   // src/pair_forces.cu:195
+            ff = inside ? ff : 0.0f;
+            fx = fmaf(ff, dx, fx);
+            fy = fmaf(ff, dy, fy);
+            fz = fmaf(ff, dz, fz);
+            if (EN && inside) {
+                const float ri6 = ri2 * ri2 * ri2;
+                const float ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
+                elj += (double)ulj;
+                eco += (double)ucoul;
+                evi += (double)(ff * rsq);
+            }
+        }
+        // bonded terms (lanes of the atom's group share them)
+        double ebo = 0.0, ean = 0.0, bvi = 0.0;
+        if (a.bonded && active) {
+            const int nb = a.nbond[id];
+            for (int b = sub; b < nb; b += G) {          // src/bonded_forces.cu:50-84
+                const uint32_t e = a.bond_list[(size_t)id * 4 + b];
+                const int js = a.slot_of[e & MD_JMASK];
+                const int bt = (int)(e >> MD_JBITS);
+                const float4 pj = a.pos_in[js];
+                const float dx = minimg(xi.x - pj.x, L0, Li0);
+                const float dy = minimg(xi.y - pj.y, L1, Li1);
+                const float dz = minimg(xi.z - pj.z, L2, Li2);
+                const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float kk = a.bk2[bt], r0 = a.br0[bt];
+                const float fa = kk * (r0 * __builtin_amdgcn_rsqf(rsq) - 1.0f);
+                fx = fmaf(fa, dx, fx);
+                fy = fmaf(fa, dy, fy);
+                fz = fmaf(fa, dz, fz);
+                if (EN) {
+                    const float dr = sqrtf(rsq) - r0;
+                    ebo += 0.25 * (double)kk * (double)dr * (double)dr;   // each bond is seen from both atoms
+                    bvi += 0.5 * (double)(fa * rsq);
+                }
+            }
+            const int na = a.nang[id];
+            for (int q = sub; q < na; q += G) {          // src/bonded_forces.cu:119-188
+                const int4 e = a.ang_list[(size_t)id * 4 + q];
+                const int role = e.x & 0xff, at = e.x >> 8;
+                const float4 pA = a.pos_in[a.slot_of[e.y]];
+                const float4 pk = a.pos_in[a.slot_of[e.z]];
+                float4 pi = xi, pc = pA;
+                if (role == 1) { pi = pA; pc = xi; }
+                float ax = minimg(pi.x - pc.x, L0, Li0), ay = minimg(pi.y - pc.y, L1, Li1), az = minimg(pi.z - pc.z, L2, Li2);
+                float ai = __builtin_amdgcn_rsqf(fmaf(az, az, fmaf(ay, ay, ax * ax)));
+                float bx = minimg(pk.x - pc.x, L0, Li0), by = minimg(pk.y - pc.y, L1, Li1), bz = minimg(pk.z - pc.z, L2, Li2);
+                float bi = __builtin_amdgcn_rsqf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+                const float ab = fmaf(az, bz, fmaf(ay, by, ax * bx));
+                float cang = ab * ai * bi;
+                cang = fminf(fmaxf(cang, -0.9999999f), 0.9999999f);
+                const float th = acosf(cang);
+                const float fk = a.ak2[at] * (th - a.at0[at]) * __builtin_amdgcn_rsqf(1.0f - cang * cang) * ai * bi;
+                const float ca = ab * ai * ai;
+                float gx = bx - ca * ax, gy = by - ca * ay, gz = bz - ca * az;
+                if (role == 1) {
+                    const float cb = ab * bi * bi;
+                    gx = cb * bx - ax - gx;
+                    gy = cb * by - ay - gy;
+                    gz = cb * bz - az - gz;
+                }
+                fx = fmaf(fk, gx, fx);
+                fy = fmaf(fk, gy, fy);
+                fz = fmaf(fk, gz, fz);
+                if (EN) {
+                    const double d = (double)th - (double)a.at0[at];
+                    ean += (0.5 / 3.0) * (double)a.ak2[at] * d * d;   // each angle is seen from 3 atoms
+                }
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o >= 1; o >>= 1) {
+            fx += __shfl_xor(fx, o, 64);
+            fy += __shfl_xor(fy, o, 64);
+            fz += __shfl_xor(fz, o, 64);
+        }
+        if (EN) {
+            dlj += elj;
+            dco += eco;
+            dvi += evi + 2.0 * bvi;
+            dbo += ebo;
+            dan += ean;
+        }
+        if (active && sub == 0) {
+            const float4 f4 = make_float4(fx, fy, fz, 0.0f);
+            a.force[slot] = f4;
+            if (a.mode == MODE_LF) {
+                float4 v = a.vel[slot];
+                const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
+                v.x = fmaf(fx, massif, v.x) * lam;
+                v.y = fmaf(fy, massif, v.y) * lam;
+                v.z = fmaf(fz, massif, v.z) * lam;
+                a.vel[slot] = v;
+                float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
+                x = fmaf(-L0, truncf(x * 2.0f * Li0), x);   // src/nve_leapfrog.cu:50-55
+                y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
+                z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
+                a.pos_out[slot] = make_float4(x, y, z, xi.w);
+                dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+                const float4 pb = a.pos_build[slot];
+                const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
+                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) a.flag_set[cur_r] = 1;   // src/neighborlist.cu:174
+                if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
+            } else if (a.mode == MODE_VV) {
+                float4 v = a.vel[slot];
+                const float h = 0.5f * a.dt * v.w;
+                v.x = fmaf(fx, h, v.x);
+                v.y = fmaf(fy, h, v.y);
+                v.z = fmaf(fz, h, v.z);
+                a.vel[slot] = v;
+                dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+            }
+        }
+    }
+    if (cur_r >= 0) flush();
+}
+
+// Velocity-Verlet begin-of-step kernel: v = S v + F dt/2m; x += v dt; wrap;
+// skin check.  With apply_drift = 0 it only applies the end-of-step
+// thermostat scale (run end).
+__global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && a.flag_clear)
+        for (int r = threadIdx.x; r < a.nrep; r += blockDim.x) a.flag_clear[r] = 0;
+    if (s >= a.nslot) return;
+    if (a.ids[s] < 0) return;
+    const int r = a.tile_rep[s >> 6];
+    const RepParam& p = a.rep[r];
+    float4 v = a.vel[s];
+    if (!a.apply_drift) {
+        const float S = (float)a.th[r].S_end;
+        v.x *= S; v.y *= S; v.z *= S;
+        a.vel[s] = v;
+        return;
+    }
+    const float S = (float)a.th[r].S;
+    const float4 f = a.force[s];
+    const float h = 0.5f * a.dt * v.w;
+    v.x = fmaf(f.x, h, v.x * S);
+    v.y = fmaf(f.y, h, v.y * S);
+    v.z = fmaf(f.z, h, v.z * S);
+    a.vel[s] = v;
+    const float4 xi = a.pos_in[s];
+    float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
+    x = fmaf(-p.L[0], truncf(x * 2.0f * p.Linv[0]), x);
+    y = fmaf(-p.L[1], truncf(y * 2.0f * p.Linv[1]), y);
+    z = fmaf(-p.L[2], truncf(z * 2.0f * p.Linv[2]), z);
+    a.pos_out[s] = make_float4(x, y, z, xi.w);
+    const float4 pb = a.pos_build[s];
+    const float ddx = minimg(x - pb.x, p.L[0], p.Linv[0]);
+    const float ddy = minimg(y - pb.y, p.L[1], p.Linv[1]);
+    const float ddz = minimg(z - pb.z, p.L[2], p.Linv[2]);
+    if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) a.flag_set[r] = 1;
+    if (!(fabsf(x) <= p.L[0] && fabsf(y) <= p.L[1] && fabsf(z) <= p.L[2])) *a.err = 1;
+}
+
+// Thermostat bookkeeping at run boundaries (one workgroup).
+//   mode 0 (velocity-Verlet run start): begin-of-step half from ke2_cur -> S
+//   mode 1 (velocity-Verlet run end):   restore the state after the end-of-step half
+__global__ void k_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt, int mode) {
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+        ThState& t = th[r];
+        if (mode == 0) {
+            if (thermostat == 2) {
+                double K = 0.5 * t.ke2_cur, xi = t.xi, vxi = t.vxi;
+                double s = nh_half(xi, vxi, K, rep[r].nf, rep[r].kT0, rep[r].Q, dt);
+                t.xi = xi;
+                t.vxi = vxi;
+                t.S = s;
+            } else {
+                t.S = 1.0;
+            }
+        } else {
+            t.xi = t.xi_end;
+            t.vxi = t.vxi_end;
+            t.S = 1.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int G>
+static void launch_g(const StepArgs& a, int coul, bool en, int grid, hipStream_t st) {
+#define MD_STEP_CASE(C)                                                                   \
+    if (coul == C) {                                                                      \
+        if (en) hipLaunchKernelGGL((k_step<G, C, true>), dim3(grid), dim3(256), 0, st, a); \
+        else hipLaunchKernelGGL((k_step<G, C, false>), dim3(grid), dim3(256), 0, st, a);   \
+        return;                                                                           \
+    }
+    MD_STEP_CASE(0)
+    MD_STEP_CASE(1)
+    MD_STEP_CASE(2)
+    MD_STEP_CASE(3)
+#undef MD_STEP_CASE
+}
+
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int energy, int grid, hipStream_t st) {
+    switch (lanes) {
+    case 4: launch_g<4>(a, coul, energy != 0, grid, st); break;
+    case 8: launch_g<8>(a, coul, energy != 0, grid, st); break;
+    case 16: launch_g<16>(a, coul, energy != 0, grid, st); break;
+    case 32: launch_g<32>(a, coul, energy != 0, grid, st); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_vv_kick, dim3((a.nslot + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
+                                           int mode, hipStream_t st) {
+    hipLaunchKernelGGL(k_thermo_fix, dim3(1), dim3(256), 0, st, th, rep, nrep, thermostat, dt, mode);
+    return hipGetLastError();
+}

@@ -1,0 +1,217 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product package
+md_gpu_amd never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+MAXT = 8
+
+
+class OrcFF(C.Structure):
+    _fields_ = [("ntype", C.c_int), ("lj1", C.c_double * 64), ("lj2", C.c_double * 64),
+                ("coulomb", C.c_int), ("rc", C.c_double), ("alpha", C.c_double), ("eps_rf", C.c_double),
+                ("qfac", C.c_double), ("erfc_exact", C.c_int),
+                ("nbtype", C.c_int), ("bk2", C.c_double * 8), ("br0", C.c_double * 8),
+                ("natype", C.c_int), ("ak2", C.c_double * 8), ("at0", C.c_double * 8)]
+
+
+class OrcMD(C.Structure):
+    _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_double), ("T0", C.c_double),
+                ("tau", C.c_double), ("skin", C.c_double)]
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P = C.c_void_p
+        L.orc_canonical_thresholds.argtypes = [C.c_float, C.c_float, P, P, P, P, P]
+        L.orc_rsq32.argtypes = [P, P, P, P]
+        L.orc_rsq32.restype = C.c_float
+        for f in (L.orc_nlist_brute, L.orc_nlist_cells):
+            f.argtypes = [C.c_int, P, P, C.c_float, P, C.c_int, P, P]
+            f.restype = C.c_int
+        L.orc_pair_forces.argtypes = [C.c_int, P, P, P, P, P, C.c_float, C.c_int, P, P, P, P, P]
+        L.orc_bond_forces.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P, P]
+        L.orc_angle_forces.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P]
+        L.orc_md_run.argtypes = [C.c_int, P, P, P, P, P, P, P, P, C.c_int, P, C.c_int, P, P, C.c_int,
+                                 P, P, P, P, P]
+        L.orc_md_run.restype = C.c_int
+        L.orc_fm_loss_grad.argtypes = [C.c_int, C.c_int, P, P, P, P, P, C.c_double, P, P, P, P]
+        L.orc_fm_loss_grad.restype = C.c_double
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def make_ff(ff, erfc_exact: int = 0, lj1=None, lj2=None) -> OrcFF:
+    """OrcFF from an md_gpu_amd.systems.ForceField (double precision copy)."""
+    o = OrcFF()
+    nt = ff.ntype
+    o.ntype = nt
+    l1 = np.zeros(64)
+    l2 = np.zeros(64)
+    l1[:nt * nt] = np.asarray(ff.lj1 if lj1 is None else lj1, np.float64).reshape(-1)
+    l2[:nt * nt] = np.asarray(ff.lj2 if lj2 is None else lj2, np.float64).reshape(-1)
+    o.lj1[:] = l1.tolist()
+    o.lj2[:] = l2.tolist()
+    o.coulomb = int(ff.coulomb)
+    o.rc = float(ff.rc)
+    o.alpha = float(ff.alpha)
+    o.eps_rf = float(ff.eps_rf)
+    o.qfac = float(ff.qfac)
+    o.erfc_exact = int(erfc_exact)
+    bk = np.zeros(8); br = np.zeros(8); ak = np.zeros(8); at = np.zeros(8)
+    nb = len(ff.bond_k); na = len(ff.angle_k)
+    bk[:nb] = 2.0 * np.asarray(ff.bond_k); br[:nb] = ff.bond_r0
+    ak[:na] = 2.0 * np.asarray(ff.angle_k); at[:na] = ff.angle_t0
+    o.nbtype, o.natype = nb, na
+    o.bk2[:] = bk.tolist(); o.br0[:] = br.tolist(); o.ak2[:] = ak.tolist(); o.at0[:] = at.tolist()
+    return o
+
+
+def thresholds(rc: float, skin: float, box):
+    box = np.ascontiguousarray(box, np.float32)
+    rc2 = np.zeros(1, np.float32); rl2 = np.zeros(1, np.float32); s2 = np.zeros(1, np.float32)
+    linv = np.zeros(3, np.float32)
+    lib().orc_canonical_thresholds(C.c_float(rc), C.c_float(skin), _p(box), _p(rc2), _p(rl2), _p(s2), _p(linv))
+    return float(rc2[0]), float(rl2[0]), float(s2[0]), linv
+
+
+def rsq32(xi, xj, box):
+    box = np.ascontiguousarray(box, np.float32)
+    linv = (1.0 / box.astype(np.float64)).astype(np.float32)
+    xi = np.ascontiguousarray(xi, np.float32); xj = np.ascontiguousarray(xj, np.float32)
+    return lib().orc_rsq32(_p(xi), _p(xj), _p(box), _p(linv))
+
+
+def nlist(pos, box, rl2, excl=None, cap=None, method="cells"):
+    """Full neighbor lists as a list of sorted int arrays."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    n = pos.shape[0]
+    ex = None if excl is None else np.ascontiguousarray(excl, np.int32)
+    f = lib().orc_nlist_cells if method == "cells" else lib().orc_nlist_brute
+    counts = np.zeros(n, np.int32)
+    if cap is None:
+        dummy = np.zeros(n, np.int32)
+        cap = max(1, f(n, _p(pos), _p(box), C.c_float(rl2), _p(ex), 1, _p(counts), _p(dummy)))
+    nb = np.zeros((n, cap), np.int32)
+    f(n, _p(pos), _p(box), C.c_float(rl2), _p(ex), cap, _p(counts), _p(nb))
+    return counts, nb
+
+
+def pair_forces(sys_, counts, nbrs, rc2=None, erfc_exact=0, want_energy=True, charge=None, ff=None):
+    """FP64 pair forces (n,3), energies [lj, coul], virial."""
+    ff = ff or sys_.ff
+    pos = np.ascontiguousarray(sys_.pos, np.float32)
+    box = np.ascontiguousarray(sys_.box, np.float32)
+    typ = np.ascontiguousarray(sys_.type, np.int32)
+    q = np.ascontiguousarray(sys_.charge if charge is None else charge, np.float64)
+    if rc2 is None:
+        rc2 = thresholds(ff.rc, ff.skin, box)[0]
+    o = make_ff(ff, erfc_exact)
+    F = np.zeros((sys_.n, 3))
+    e = np.zeros(2)
+    w = np.zeros(1)
+    counts = np.ascontiguousarray(counts, np.int32)
+    nbrs = np.ascontiguousarray(nbrs, np.int32)
+    lib().orc_pair_forces(sys_.n, _p(pos), _p(typ), _p(q), _p(box), C.byref(o), C.c_float(rc2), nbrs.shape[1],
+                          _p(counts), _p(nbrs), _p(F), _p(e) if want_energy else None,
+                          _p(w) if want_energy else None)
+    return F, e, float(w[0])
+
+
+def bonded_forces(sys_, pos=None):
+    """FP64 bond + angle forces (n,3) and energies [bond, angle], bond virial."""
+    p = np.ascontiguousarray(sys_.pos if pos is None else pos, np.float32)
+    box = np.ascontiguousarray(sys_.box, np.float32)
+    o = make_ff(sys_.ff)
+    F = np.zeros((sys_.n, 3))
+    eb = np.zeros(1); ea = np.zeros(1); wb = np.zeros(1)
+    b = np.ascontiguousarray(sys_.bonds, np.int32)
+    a = np.ascontiguousarray(sys_.angles, np.int32)
+    lib().orc_bond_forces(sys_.n, _p(p), _p(box), C.byref(o), b.shape[0], _p(b), _p(F), _p(eb), _p(wb))
+    lib().orc_angle_forces(sys_.n, _p(p), _p(box), C.byref(o), a.shape[0], _p(a), _p(F), _p(ea))
+    return F, np.array([eb[0], ea[0]]), float(wb[0])
+
+
+def forces(sys_, erfc_exact=0, cap=None):
+    """Total FP64 force with canonical FP32 list/cutoff predicates.
+    Returns F (n,3), energies [lj, coul, bond, angle], virial (pair + bond)."""
+    rc2, rl2, _, _ = thresholds(sys_.ff.rc, sys_.ff.skin, sys_.box)
+    counts, nb = nlist(sys_.pos, sys_.box, rl2, sys_.exclusions(), cap)
+    Fp, ep, wp = pair_forces(sys_, counts, nb, rc2, erfc_exact)
+    Fb, eb, wb = bonded_forces(sys_)
+    return Fp + Fb, np.concatenate([ep, eb]), wp + wb
+
+
+def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
+    """FP64 trajectory.  Returns dict of per-step arrays and the final state."""
+    n = sys_.n
+    p = np.ascontiguousarray(sys_.pos if pos is None else pos, np.float64).copy()
+    v = np.ascontiguousarray(sys_.vel if vel is None else vel, np.float64).copy()
+    typ = np.ascontiguousarray(sys_.type, np.int32)
+    q = np.ascontiguousarray(sys_.charge, np.float64)
+    m = np.ascontiguousarray(sys_.mass, np.float64)
+    box = np.ascontiguousarray(sys_.box, np.float32)
+    o = make_ff(sys_.ff)
+    md = OrcMD()
+    md.integrator = {"leapfrog": 0, "vverlet": 1}[sys_.integrator]
+    md.thermostat = {"none": 0, "berendsen": 1, "nosehoover": 2}[sys_.thermostat]
+    md.dt, md.T0, md.tau = float(sys_.dt), float(sys_.T0), float(sys_.tau)
+    md.skin = float(sys_.ff.skin if skin is None else skin)
+    b = np.ascontiguousarray(sys_.bonds, np.int32)
+    a = np.ascontiguousarray(sys_.angles, np.int32)
+    ex = np.ascontiguousarray(sys_.exclusions(), np.int32)
+    ke = np.zeros(nsteps); pe = np.zeros(nsteps); T = np.zeros(nsteps); H = np.zeros(nsteps)
+    ts = np.zeros(2) if thermo_state is None else np.ascontiguousarray(thermo_state, np.float64).copy()
+    nreb = lib().orc_md_run(n, _p(p), _p(v), _p(typ), _p(q), _p(m), _p(box), C.byref(o), C.byref(md),
+                            b.shape[0], _p(b), a.shape[0], _p(a), _p(ex), nsteps, _p(ke), _p(pe), _p(T), _p(H),
+                            _p(ts))
+    return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb)
+
+
+def fm_loss_grad(frames, type_, charge, box, ff, qscale, targets, lj1=None, lj2=None):
+    """FP64 force-matching loss and gradient (g_lj1 (nt,nt), g_lj2 (nt,nt), g_qscale)."""
+    frames = np.ascontiguousarray(frames, np.float32)
+    nf, n = frames.shape[0], frames.shape[1]
+    o = make_ff(ff, 0, lj1, lj2)
+    nt = ff.ntype
+    g1 = np.zeros(nt * nt); g2 = np.zeros(nt * nt); gq = np.zeros(1)
+    typ = np.ascontiguousarray(type_, np.int32)
+    q = np.ascontiguousarray(charge, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    t = np.ascontiguousarray(targets, np.float64)
+    loss = lib().orc_fm_loss_grad(nf, n, _p(frames), _p(typ), _p(q), _p(b), C.byref(o), C.c_double(qscale), _p(t),
+                                  _p(g1), _p(g2), _p(gq))
+    return loss, g1.reshape(nt, nt), g2.reshape(nt, nt), float(gq[0])
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())

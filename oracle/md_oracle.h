@@ -1,0 +1,134 @@
+/*
+ * md_oracle.h -- FP64 CPU restatement of the md_gpu reference MD step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle and the
+ * timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  The product
+ * (md_gpu_amd/, libmdgpu.so) never links, loads or calls anything here.
+ *
+ * What it restates (reference files, read as text):
+ *   pair forces      src/pair_forces.cu:34-204   (RF + LJ, Wolf + LJ)
+ *   force constants  src/forces.cu:47-111, src/simulation.cu:184-205, :86, :94
+ *   neighbor list    src/neighborlist.cu:35-36, :64-99, :157-176, :231-297
+ *   bonds / angles   src/bonded_forces.cu:34-192, src/simulation.cu:225-244
+ *   integrators      src/nve_leapfrog.cu:31-59, src/nvt_leapfrog_ber.cu:29-80,
+ *                    src/integrator.cu:131-156, src/nve_vverlet.cu:31-78
+ *   wrap / units     src/nve_leapfrog.cu:50-55, src/simulation.cu:177
+ * Parts with no reference arithmetic (energies, Nose-Hoover, force-matching
+ * gradient) follow the textbook forms cited at each function.
+ *
+ * Pinning status of every function is listed in the header comment of
+ * md_oracle.cpp; tests/test_oracle_*.py hold the checks.
+ */
+#ifndef MD_ORACLE_H
+#define MD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_TYPES 8
+
+/* Coulomb kinds, matching the reference's lkspace switch (src/simulation.cu:82-95)
+ * plus an explicit "none" used for uncharged systems. */
+enum { ORC_COUL_RF = 0, ORC_COUL_WOLF = 1, ORC_COUL_DAMPED = 2, ORC_COUL_NONE = 3 };
+
+typedef struct {
+    int    ntype;
+    double lj1[ORC_MAX_TYPES * ORC_MAX_TYPES];   /* 48 eps sig^12 (or 12 A), src/simulation.cu:191,200 */
+    double lj2[ORC_MAX_TYPES * ORC_MAX_TYPES];   /* 24 eps sig^6  (or 6 B),  src/simulation.cu:194,202 */
+    int    coulomb;                               /* ORC_COUL_* */
+    double rc;                                    /* cutoff (A) */
+    double alpha;                                 /* Wolf / damped alpha (1/A) */
+    double eps_rf;                                /* reaction-field dielectric */
+    double qfac;                                  /* 1389.35427611722, src/forces.cu:47 */
+    int    erfc_exact;                            /* 0: A&S 7.1.26 polynomial (reference); 1: libm erfc */
+    int    nbtype;
+    double bk2[8], br0[8];                        /* bnd1 = 2k, bnd2 = r0 (src/simulation.cu:226-227) */
+    int    natype;
+    double ak2[8], at0[8];                        /* ang1 = 2k, ang2 = theta0 rad (src/simulation.cu:241-243) */
+} orc_ff;
+
+/* Canonical FP32 thresholds (shared definition with the HIP engine):
+ *   rc2   = (float)rc * (float)rc                               src/simulation.cu:96
+ *   rl2   = (float)((skin + sqrt((double)rc2))^2)               src/neighborlist.cu:35
+ *   skin2q= skin*skin/4 in float                                src/neighborlist.cu:36
+ *   Linv  = (float)(1.0/(double)L)                              src/simulation.cu:290-292 */
+void orc_canonical_thresholds(float rc, float skin, const float* box3,
+                              float* rc2, float* rl2, float* skin2q, float* linv3);
+
+/* Canonical FP32 squared minimum-image distance between two float positions
+ * (the predicate the neighbor list and the cutoff test use on both sides). */
+float orc_rsq32(const float* xi, const float* xj, const float* box3, const float* linv3);
+
+/* Full neighbor lists, src/neighborlist.cu:231-297 semantics:
+ *   j in N(i)  <=>  j != i, j not in excl(i), rsq32(i,j) < rl2.
+ * excl: n*4 ints, -1 = empty slot (1-2 bonded partners, src/neighborlist.cu:64-99).
+ * Lists are written sorted ascending into nbrs[i*cap + k]; counts[i] is the
+ * full count even if it exceeds cap.  Returns max count. */
+int orc_nlist_brute(int n, const float* pos, const float* box3, float rl2,
+                    const int* excl, int cap, int* counts, int* nbrs);
+/* Same sets via a cell list (O(N)); must equal orc_nlist_brute. */
+int orc_nlist_cells(int n, const float* pos, const float* box3, float rl2,
+                    const int* excl, int cap, int* counts, int* nbrs);
+
+/* Pair forces over full lists (src/pair_forces.cu:133-204 / :34-131), FP64.
+ * Pair included iff rsq32 <= rc2 (src/pair_forces.cu:95,195; the reference
+ * zeroes ff when rsq > rcutsq).  F[i*3..] = sum_j ff*d_ij.
+ * energy[0] = LJ (shifted to 0 at rc), energy[1] = Coulomb, virial = sum ff r^2
+ * over unordered pairs (both halves counted once).  Pass energy/virial NULL to skip. */
+void orc_pair_forces(int n, const float* pos, const int* type, const double* charge,
+                     const float* box3, const orc_ff* ff, float rc2,
+                     int cap, const int* counts, const int* nbrs,
+                     double* F, double* energy, double* virial);
+
+/* Harmonic bonds (src/bonded_forces.cu:34-95), E = k (r-r0)^2 with bnd1 = 2k.
+ * bonds: nb*3 ints (i, j, type), 0-based.  Adds into F.  energy/virial may be NULL. */
+void orc_bond_forces(int n, const float* pos, const float* box3, const orc_ff* ff,
+                     int nb, const int* bonds, double* F, double* energy, double* virial);
+
+/* Harmonic angles (src/bonded_forces.cu:98-192), E = k (theta-theta0)^2 with ang1 = 2k.
+ * angles: na*4 ints (i, center j, k, type), 0-based.  Adds into F. */
+void orc_angle_forces(int n, const float* pos, const float* box3, const orc_ff* ff,
+                      int na, const int* angles, double* F, double* energy);
+
+/* ---- MD trajectories in FP64 (positions and velocities double) ---- */
+enum { ORC_INT_LEAPFROG = 0, ORC_INT_VVERLET = 1 };
+enum { ORC_TH_NONE = 0, ORC_TH_BERENDSEN = 1, ORC_TH_NOSEHOOVER = 2 };
+
+typedef struct {
+    int    integrator;      /* ORC_INT_* */
+    int    thermostat;      /* ORC_TH_* */
+    double dt;              /* fs */
+    double T0;              /* K */
+    double tau;             /* fs (Berendsen tau = 1/tco, src/integrator.cu:146; NH period) */
+    double skin;            /* A */
+} orc_md;
+
+/* Stats per step (length nsteps): ke, pe (lj+coul+bond+angle), temperature,
+ * conserved quantity (H for NVE; H + NH terms for Nose-Hoover).
+ * masses: per atom (g/mol).  State arrays are updated in place.
+ * Leapfrog: vel holds v(t-dt/2) on entry and exit (reference convention);
+ * velocity Verlet: vel holds full-step v(t). */
+int orc_md_run(int n, double* pos, double* vel, const int* type, const double* charge,
+               const double* mass, const float* box3, const orc_ff* ff, const orc_md* md,
+               int nb, const int* bonds, int na, const int* angles, const int* excl,
+               int nsteps, double* ke, double* pe, double* temp, double* conserved,
+               double* thermo_state /* [xi, v_xi] in/out, may be NULL */);
+
+/* ---- force matching (config: NaCl, 7 parameters) ----
+ * loss = mean over frames*atoms*3 of (F(theta) - Ftarget)^2 with charges
+ * q_i = qscale * charge[i].  Gradient w.r.t. lj1[a*nt+b], lj2[a*nt+b] (each
+ * ordered entry separately) and qscale.  FP64 analytic adjoint. */
+double orc_fm_loss_grad(int nframes, int n, const float* pos /* frames*n*3 */,
+                        const int* type, const double* charge, const float* box3,
+                        const orc_ff* ff, double qscale, const double* target /* frames*n*3 */,
+                        double* g_lj1, double* g_lj2, double* g_qscale);
+
+/* OpenMP threads the oracle uses (omp_get_max_threads). */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 oracle.
+
+Bars (north star): neighbor sets bit-exact; per-atom forces within 1e-5
+relative RMS; energies and virial within 1e-5 relative; integrator
+trajectories within FP32 round-off of the FP64 oracle over short runs.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from md_gpu_amd import systems as S
+from conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+EPS_MAX_ARXE = 221.0 * S.KB_KJ
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+def perturbed(sys_, amp, seed=11):
+    s = sys_.copy()
+    rng = np.random.default_rng(seed)
+    p = s.pos.astype(np.float64) + rng.normal(0.0, amp, s.pos.shape)
+    s.pos = S.wrap_centered(p, s.box.astype(np.float64)).astype(np.float32)
+    return s
+
+
+def water_geometry_perturbed(sys_, amp, seed=12):
+    return perturbed(sys_, amp, seed)
+
+
+def make_cases():
+    ar = perturbed(S.argon(), 0.25)
+    na = perturbed(S.nacl(nsites=16), 0.2)
+    na_rf = dataclasses.replace(na.copy(), ff=dataclasses.replace(S.nacl_ff(), coulomb=S.COUL_RF, eps_rf=78.0))
+    na_d = dataclasses.replace(na.copy(), ff=dataclasses.replace(S.nacl_ff(), coulomb=S.COUL_DAMPED, alpha=0.25))
+    wa = perturbed(S.water(nmol=512), 0.05)
+    return {"argon864": ar, "nacl4096_wolf": na, "nacl4096_rf": na_rf, "nacl4096_damped": na_d, "water1536": wa}
+
+
+CASES = make_cases()
+
+
+@pytest.fixture(scope="module")
+def E():
+    from md_gpu_amd import engine
+    return engine
+
+
+def oracle_forces(orc, s):
+    return orc.forces(s)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_neighbor_sets_bit_exact(E, orc, name):
+    s = CASES[name]
+    say("neighbors", name, s.n)
+    e = E.Engine.from_system(s)
+    pos, _, _ = e.download()
+    rc2, rl2, sk = e.thresholds()
+    orc_rc2, orc_rl2, orc_sk, _ = orc.thresholds(s.ff.rc, s.ff.skin, s.box)
+    assert (rc2, rl2, sk) == (orc_rc2, orc_rl2, orc_sk)
+    counts, nbrs = e.neighbors()
+    oc, on = orc.nlist(pos, s.box, rl2, s.exclusions())
+    assert np.array_equal(counts, oc)
+    for i in range(s.n):
+        got = np.sort(nbrs[i, :counts[i]])
+        assert np.array_equal(got, on[i, :oc[i]]), f"atom {i}"
+    e.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_forces_energy_virial(E, orc, name):
+    s = CASES[name]
+    say("forces", name, s.n)
+    e = E.Engine.from_system(s)
+    pos, _, _ = e.download()
+    so = s.copy()
+    so.pos = pos
+    F, En, W = e.compute_forces()
+    Fo, Eo, Wo = orc.forces(so)
+    err = rel_rms(F, Fo)
+    say("  force rel rms", err, "E gpu", En[0], "E orc", Eo, "W", W[0], Wo)
+    assert err < FORCE_TOL
+    scale = max(np.abs(Eo).sum(), 1.0)
+    assert np.all(np.abs(En[0] - Eo) <= 1e-5 * scale)
+    assert abs(W[0] - Wo) <= 1e-5 * max(abs(Wo), np.abs(Fo).sum() * 1e-3, 1.0)
+    # momentum: sum of forces vanishes (full lists, Newton III in aggregate)
+    assert np.abs(F.astype(np.float64).sum(0)).max() <= 1e-4 * np.abs(F).sum(0).max() + 1e-3
+    e.close()
+
+
+def test_forces_full_size_nacl(E, orc):
+    s = perturbed(S.nacl(), 0.2)
+    say("forces nacl32768")
+    e = E.Engine.from_system(s)
+    pos, _, _ = e.download()
+    so = s.copy()
+    so.pos = pos
+    F, En, W = e.compute_forces()
+    Fo, Eo, Wo = orc.forces(so)
+    err = rel_rms(F, Fo)
+    say("  force rel rms", err)
+    assert err < FORCE_TOL
+    e.close()
+
+
+INTEG_CASES = [
+    ("argon_vv_nve", lambda: perturbed(S.argon(), 0.1), 20),
+    ("nacl_lf_berendsen", lambda: perturbed(S.nacl(nsites=16), 0.1), 20),
+    ("nacl_lf_nve", lambda: dataclasses.replace(perturbed(S.nacl(nsites=16), 0.1), thermostat="none"), 20),
+    ("water_vv_nh", lambda: perturbed(S.water(nmol=512), 0.02), 20),
+    ("nacl_vv_berendsen", lambda: dataclasses.replace(perturbed(S.nacl(nsites=16), 0.1), integrator="vverlet"), 20),
+]
+
+
+@pytest.mark.parametrize("name,mk,nsteps", INTEG_CASES)
+def test_integrator_short_run(E, orc, name, mk, nsteps):
+    s = mk()
+    say("integrate", name, s.n, nsteps)
+    e = E.Engine.from_system(s)
+    pos0, vel0, _ = e.download()
+    so = s.copy()
+    so.pos, so.vel = pos0, vel0
+    st = e.run(nsteps, energy_every=1)
+    pos, vel, _ = e.download()
+    r = orc.md_run(so, nsteps)
+    dpos = pos.astype(np.float64) - r["pos"]
+    dpos -= s.box * np.rint(dpos / s.box)
+    say("  max |dx|", np.abs(dpos).max(), "T gpu", st["temp"][-1, 0], "T orc", r["temp"][-1])
+    assert np.abs(dpos).max() < 2e-4
+    assert rel_rms(vel, r["vel"]) < 1e-4
+    assert np.allclose(st["temp"][:, 0], r["temp"], rtol=1e-4)
+    assert np.allclose(st["ke"][:, 0], r["ke"], rtol=1e-4)
+    pe = st["pe_lj"][:, 0] + st["pe_coul"][:, 0] + st["pe_bond"][:, 0] + st["pe_angle"][:, 0]
+    assert np.allclose(pe, r["pe"], rtol=1e-4, atol=1e-4 * np.abs(r["pe"]).max())
+    e.close()
+
+
+def test_nve_energy_drift(E, orc):
+    """Argon, NVE velocity Verlet dt 2 fs, 1000 steps: GPU drift no worse than the oracle's."""
+    s = S.argon()
+    say("nve drift argon 1000")
+    e = E.Engine.from_system(s)
+    st = e.run(1000, energy_every=1)
+    H = st["conserved"][:, 0]
+    r = orc.md_run(s, 1000)
+    Ho = r["conserved"]
+    t = np.arange(1000)
+    drift_g = abs(np.polyfit(t, H, 1)[0]) * 1000
+    drift_o = abs(np.polyfit(t, Ho, 1)[0]) * 1000
+    fluct_o = np.std(Ho)
+    say("  drift gpu", drift_g, "drift orc", drift_o, "fluct", fluct_o, "E0", H[0], Ho[0])
+    assert drift_g <= max(2.0 * drift_o, 0.5 * fluct_o, 1e-6 * abs(Ho[0]))
+    e.close()
+
+
+def test_batched_replicas_forces(E, orc):
+    reps = [perturbed(x, 0.1, seed=20 + k) for k, x in enumerate(S.arxe_replicas(nrep=12, seed=5, nmax=2048))]
+    say("batch forces", [x.n for x in reps])
+    e = E.Engine.from_system(reps)
+    pos, _, _ = e.download()
+    F, En, W = e.compute_forces()
+    off = 0
+    for k, x in enumerate(reps):
+        so = x.copy()
+        so.pos = pos[off:off + x.n]
+        Fo, Eo, Wo = orc.forces(so)
+        assert rel_rms(F[off:off + x.n], Fo) < FORCE_TOL, k
+        # FP32 pair energies summed in FP64: bound per atom in units of the largest epsilon
+        assert abs(En[k, 0] - Eo[0]) <= 1e-5 * abs(Eo[0]) + 2e-6 * x.n * EPS_MAX_ARXE
+        off += x.n
+    counts, nbrs = e.neighbors()
+    off = 0
+    for k, x in enumerate(reps):
+        rc2, rl2, _, _ = orc.thresholds(x.ff.rc, x.ff.skin, x.box)
+        oc, on = orc.nlist(pos[off:off + x.n], x.box, rl2)
+        assert np.array_equal(counts[off:off + x.n], oc)
+        for i in range(x.n):
+            assert np.array_equal(np.sort(nbrs[off + i, :counts[off + i]]) - off, on[i, :oc[i]])
+        off += x.n
+    e.close()
+
+
+def test_batched_replicas_run(E, orc):
+    """Per-replica Berendsen temperatures and trajectories match independent oracle runs."""
+    reps = S.arxe_replicas(nrep=6, seed=6, nmax=1024)
+    say("batch run", [x.n for x in reps])
+    e = E.Engine.from_system(reps)
+    st = e.run(20, energy_every=5)
+    pos, vel, _ = e.download()
+    off = 0
+    for k, x in enumerate(reps):
+        r = orc.md_run(x, 20)
+        assert np.allclose(st["temp"][:, k], r["temp"], rtol=2e-4), k
+        d = pos[off:off + x.n].astype(np.float64) - r["pos"]
+        d -= x.box * np.rint(d / x.box)
+        assert np.abs(d).max() < 2e-4
+        off += x.n
+    e.close()
+
+
+def test_force_matching_gradient(E, orc):
+    base = S.nacl(nsites=16)
+    rng = np.random.default_rng(3)
+    nf = 4
+    frames = np.stack([S.wrap_centered(base.pos + rng.normal(0, 0.25, base.pos.shape), base.box)
+                       for _ in range(nf)]).astype(np.float32)
+    tgt_ff, tgt_q = S.perturb_nacl_ff(base.ff, seed=9)
+    targets = []
+    for f in range(nf):
+        s = base.copy()
+        s.pos = frames[f]
+        s.charge = base.charge * tgt_q
+        Ft, _, _ = orc.forces(dataclasses.replace(s, ff=tgt_ff))
+        targets.append(Ft)
+    targets = np.stack(targets)
+    say("fm", nf, base.n)
+    e = E.Engine.from_system(base)
+    e.fm_set_frames(frames, targets)
+    loss, g1, g2, gq = e.fm_loss_grad(base.ff.lj1, base.ff.lj2, 1.0)
+    lo, o1, o2, oq = orc.fm_loss_grad(frames, base.type, base.charge, base.box, base.ff, 1.0, targets)
+    say("  loss", loss, lo, "gq", gq, oq)
+    assert abs(loss - lo) <= 1e-4 * lo
+    assert abs(gq - oq) <= 1e-4 * abs(oq)
+    assert np.allclose(g1, o1, rtol=1e-4, atol=1e-4 * np.abs(o1).max())
+    assert np.allclose(g2, o2, rtol=1e-4, atol=1e-4 * np.abs(o2).max())
+    e.close()

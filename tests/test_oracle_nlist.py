@@ -1,0 +1,124 @@
+"""Pin the oracle's canonical FP32 geometry and neighbor lists."""
+import numpy as np
+import pytest
+from scipy.spatial import cKDTree
+
+from md_gpu_amd import systems as S
+
+
+def np_minimg(d, L, Linv):
+    """numpy float32 restatement of the canonical minimum image:
+    s = rint(d * Linv) (round half even), d - L*s (L*s exact)."""
+    s = np.rint((d * Linv).astype(np.float32))
+    return (d - (L * s).astype(np.float32)).astype(np.float32)
+
+
+def np_rsq(dx, dy, dz):
+    """fmaf(dz, dz, fmaf(dy, dy, dx*dx)) with exact products (float64 holds a
+    float32 product exactly), each fma rounded once to float32 through longdouble."""
+    ld = np.longdouble
+    t = (dx.astype(np.float32) * dx.astype(np.float32)).astype(np.float32)
+    t = (dy.astype(ld) * dy.astype(ld) + t.astype(ld)).astype(np.float32)
+    t = (dz.astype(ld) * dz.astype(ld) + t.astype(ld)).astype(np.float32)
+    return t
+
+
+def test_thresholds_match_float32_formulas(orc):
+    box = np.array([34.31415, 90.24, 125.157], np.float32)
+    rc2, rl2, s2, linv = orc.thresholds(8.5125, 1.0215, box)
+    assert rc2 == np.float32(np.float32(8.5125) * np.float32(8.5125))
+    assert rl2 == np.float32((np.float64(np.float32(1.0215)) + np.sqrt(np.float64(np.float32(rc2)))) ** 2)
+    assert s2 == np.float32(np.float32(1.0215) * np.float32(1.0215) / np.float32(4.0))
+    assert np.array_equal(linv, (1.0 / box.astype(np.float64)).astype(np.float32))
+
+
+def test_rsq32_matches_numpy_restatement(orc):
+    rng = np.random.default_rng(0)
+    box = np.array([30.0, 31.5, 29.25], np.float32)
+    linv = (1.0 / box.astype(np.float64)).astype(np.float32)
+    xi = rng.uniform(-box / 2, box / 2, (4000, 3)).astype(np.float32)
+    xj = rng.uniform(-box / 2, box / 2, (4000, 3)).astype(np.float32)
+    # include separations at exactly +-L/2 (half-even rounding decides the image)
+    xj[:50] = xi[:50]
+    xj[:50, 0] = (xi[:50, 0] + box[0] / 2).astype(np.float32)
+    d = (xi - xj).astype(np.float32)
+    m = np.stack([np_minimg(d[:, k], box[k], linv[k]) for k in range(3)], 1)
+    ref = np_rsq(m[:, 0], m[:, 1], m[:, 2])
+    got = np.array([orc.rsq32(a, b, box) for a, b in zip(xi, xj)], np.float32)
+    assert np.array_equal(got, ref)
+
+
+def random_gas(n, L, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-L / 2, L / 2, (n, 3)).astype(np.float32), np.full(3, L, np.float32)
+
+
+@pytest.mark.parametrize("n,L,rl", [(700, 30.0, 9.0), (300, 20.0, 9.5), (1500, 40.0, 6.0)])
+def test_brute_equals_cells(orc, n, L, rl):
+    pos, box = random_gas(n, L, n)
+    rl2 = float(np.float32(rl * rl))
+    c1, n1 = orc.nlist(pos, box, rl2, method="cells")
+    c2, n2 = orc.nlist(pos, box, rl2, method="brute")
+    assert np.array_equal(c1, c2)
+    for i in range(n):
+        assert np.array_equal(n1[i, :c1[i]], n2[i, :c2[i]])
+
+
+def test_lists_symmetric_no_self_no_exclusions(orc):
+    s = S.water(nmol=216)
+    rc2, rl2, _, _ = orc.thresholds(s.ff.rc, s.ff.skin, s.box)
+    ex = s.exclusions()
+    c, nb = orc.nlist(s.pos, s.box, rl2, ex)
+    sets = [set(nb[i, :c[i]].tolist()) for i in range(s.n)]
+    for i in range(s.n):
+        assert i not in sets[i]
+        for j in ex[i]:
+            if j >= 0:
+                assert j not in sets[i]
+        for j in sets[i]:
+            assert i in sets[j]
+    # 1-3 pairs (H-H of one molecule) are not excluded (reference semantics)
+    assert 2 in sets[1]
+
+
+def test_lists_match_kdtree_away_from_threshold(orc):
+    pos, box = random_gas(2000, 36.0, 7)
+    rl = 8.0
+    rl2 = float(np.float32(rl * rl))
+    c, nb = orc.nlist(pos, box, rl2)
+    p = pos.astype(np.float64) + box / 2
+    p = np.mod(p, box)
+    tree = cKDTree(p, boxsize=box.astype(np.float64))
+    pairs = tree.query_pairs(rl + 1e-3, output_type="ndarray")
+    d = p[pairs[:, 0]] - p[pairs[:, 1]]
+    d -= box * np.rint(d / box)
+    r = np.sqrt((d ** 2).sum(1))
+    sets = [set(nb[i, :c[i]].tolist()) for i in range(len(pos))]
+    for (i, j), rr in zip(pairs, r):
+        if rr < rl - 1e-4:
+            assert j in sets[i] and i in sets[j]
+        elif rr > rl + 1e-4:
+            assert j not in sets[i]
+    total = sum(c)
+    assert total == 2 * int((r < rl).sum()) or abs(total - 2 * int((r < rl).sum())) <= 4
+
+
+def test_generators_match_workload_definitions():
+    ar = S.argon()
+    assert ar.n == 864 and ar.box[0] == pytest.approx(34.3142, abs=2e-4)
+    assert ar.temperature() == pytest.approx(0.722 * 119.8, rel=1e-6)
+    na = S.nacl()
+    assert na.n == 32768 and na.box[0] == pytest.approx(90.24, abs=1e-4)
+    assert na.charge.sum() == 0 and set(np.unique(na.charge)) == {-1.0, 1.0}
+    assert na.temperature() == pytest.approx(1200.0, rel=1e-6)
+    wa = S.water(nmol=65536)
+    assert wa.n == 196608 and wa.box[0] == pytest.approx(125.157, abs=2e-3)
+    assert len(wa.bonds) == 2 * 65536 and len(wa.angles) == 65536
+    reps = S.arxe_replicas()
+    sizes = np.array([r.n for r in reps])
+    assert len(reps) == 512 and sizes.min() >= 256 and sizes.max() <= 8192
+    assert 0.8e6 < sizes.sum() < 1.6e6
+    for r in reps[:20]:
+        assert np.mean(r.type == 1) == pytest.approx(0.3, abs=0.01)
+        rl = r.ff.rc + r.ff.skin
+        assert rl < r.box[0] / 2
