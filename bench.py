@@ -1,0 +1,340 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X MD step engine.
+
+One JSON line on stdout.  The headline `value` is timesteps/s of the molten
+NaCl workload (32,768 ions, Wolf alpha 0.2 rc 12 A, NVT leapfrog + Berendsen
+tau 0.1 ps, dt 2 fs, FP32 forces), inputs resident in HBM, hipGraph segments.
+The other workloads (argon 864 NVE-VV, water 196,608 atoms NVT-VV Nose-Hoover,
+512 batched Ar-Xe replicas, NaCl force matching) are reported under
+"workloads" with the same fields.
+
+Roofline of the dominant kernel (the fused step kernel): algorithmic FP32
+flops per launch = f_pair * P (P = unordered pairs within rc at the timed
+configuration, f_pair counted from src/pair_forces.cu:151-200: LJ+Wolf 65,
+Wolf only 60, LJ 36) divided by its mean launch time measured with HIP events
+on the engine's stream, against 157.3 TFLOP/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3            # TFLOP/s, MI355X
+PEAK_HBM = 8000.0            # GB/s
+METRIC = "timesteps/s (ns/day) + force-grad evals/s at ≤1e-5 force err, 1 MI355X"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def count_pairs(pos, box, rc, types=None, tmask=None):
+    """Unordered pairs with r < rc (periodic), optionally split by a type predicate."""
+    from scipy.spatial import cKDTree
+    p = np.mod(pos.astype(np.float64) + box / 2, box)
+    p = np.minimum(p, np.nextafter(box, 0))
+    tree = cKDTree(p, boxsize=box.astype(np.float64))
+    pairs = tree.query_pairs(rc, output_type="ndarray")
+    if types is None:
+        return len(pairs), None
+    t = types[pairs[:, 0]], types[pairs[:, 1]]
+    return len(pairs), int(np.sum(tmask(*t)))
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def time_run(e, nsteps, energy_every=0):
+    t0 = time.perf_counter()
+    e.run(nsteps, energy_every=energy_every, stats=False)
+    return time.perf_counter() - t0
+
+
+def kernel_time(e, nsteps):
+    """Mean duration of the fused step kernel and of a rebuild, HIP events on the engine stream."""
+    e.timing(True)
+    e.run(nsteps, energy_every=0, stats=False)
+    t = e.timing_get()
+    e.timing(False)
+    return t["step_ms"] / max(t["step_launches"], 1), t["rebuild_ms"] / max(t["rebuilds"], 1), t
+
+
+def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=None, force_check=None):
+    from md_gpu_amd import engine as E
+    log(f"{name}: setting up")
+    e = E.Engine.from_system(systems)
+    tuned = None
+    if autotune:
+        tuned = e.autotune(192)
+        log(f"{name}: autotuned {e.get_launch()} ({tuned:.4f} ms/step)")
+    e.run(warmup, stats=False)
+    pos0, _, _ = e.download()
+    flops0 = f_pair_fn(pos0)
+    reb0 = e.info()["rebuilds"]
+    wall = time_run(e, steps)
+    reb1 = e.info()["rebuilds"]
+    pos1, _, _ = e.download()
+    flops1 = f_pair_fn(pos1)
+    flops = 0.5 * (flops0 + flops1)
+    kstep_ms, kreb_ms, traw = kernel_time(e, min(steps, 2000))
+    sps = steps / wall
+    info = e.info()
+    out = {
+        "value": sps, "unit": "timesteps/s", "ms_per_step": 1e3 * wall / steps,
+        "ns_per_day": sps * dt * 0.0864, "steps": steps, "warmup": warmup,
+        "rebuilds_per_step": (reb1 - reb0) / steps / max(len(systems) if isinstance(systems, list) else 1, 1),
+        "launch": dict(e.get_launch(), list_cap=info["cap"]),
+        "step_kernel_ms": kstep_ms, "rebuild_chain_ms_avg": kreb_ms,
+        "roofline": {"bound": "valu", "achieved": flops / (kstep_ms * 1e-3) / 1e12, "peak": PEAK_FP32,
+                     "unit": "TFLOP/s", "frac": flops / (kstep_ms * 1e-3) / 1e12 / PEAK_FP32,
+                     "traffic": None, "algorithmic_flop_per_launch": flops},
+        "step_roofline_frac": flops / (1e-3 * 1e3 * wall / steps) / 1e12 / PEAK_FP32,
+    }
+    if force_check is not None:
+        out["force_rel_rms"] = force_check(e, systems)
+    if cpu_fn is not None:
+        out["cpu_baseline"] = cpu_fn(e, systems)
+    e.close()
+    return out
+
+
+def oracle_cpu_steps(s, budget_s, min_steps=2):
+    """FP64 oracle trajectory on host cores: steps/s on a bounded sample."""
+    import oracle
+    t0 = time.perf_counter()
+    oracle.md_run(s, 1)
+    one = time.perf_counter() - t0
+    n = max(min_steps, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    oracle.md_run(s, n)
+    dt_ = time.perf_counter() - t0
+    return n, n / dt_, oracle.num_threads()
+
+
+def force_error(e, s):
+    import oracle
+    pos, vel, _ = e.download()
+    F, _, _ = e.compute_forces()
+    t = s.copy()
+    t.pos = pos
+    Fo, _, _ = oracle.forces(t)
+    return float(np.sqrt(((F - Fo) ** 2).sum() / (Fo ** 2).sum()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported by this build "
+              "(the step path shards by spatial domains; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from md_gpu_amd import engine as E, systems as S
+    log("library", E.version())
+    want_cpu = not args.no_cpu
+    tune = not args.no_autotune
+
+    # ---- headline: NaCl 32,768 ions, NVT leapfrog Berendsen ----
+    nacl = S.nacl()
+
+    def f_nacl(pos):
+        return 65.0 * count_pairs(pos, nacl.box, nacl.ff.rc)[0]
+
+    def cpu_nacl(e, s):
+        if not want_cpu:
+            return None
+        pos, vel, _ = e.download()
+        t = s.copy()
+        t.pos, t.vel = pos, vel
+        n, sps, cores = oracle_cpu_steps(t, 12.0)
+        return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
+                "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
+
+    res = step_workload("nacl32768", nacl, f_nacl, args.steps, args.warmup, tune, nacl.dt, cpu_nacl,
+                        (lambda e, s: force_error(e, s)) if want_cpu else None)
+    line = {
+        "metric": METRIC, "value": res["value"], "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "nacl32768_wolf_nvt_leapfrog_berendsen", "atoms": nacl.n, "box_A": float(nacl.box[0]),
+                   "rc_A": nacl.ff.rc, "alpha": nacl.ff.alpha, "skin_A": nacl.ff.skin, "dt_fs": nacl.dt,
+                   "T_K": nacl.T0, "tau_fs": nacl.tau},
+        "ns_per_day": res["ns_per_day"], "force_rel_rms": res.get("force_rel_rms"),
+        "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline"),
+        "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_avg": res["rebuild_chain_ms_avg"],
+        "rebuilds_per_step": res["rebuilds_per_step"], "launch": res["launch"],
+    }
+    traffic = os.path.join(ROOT, "profiles", "step_kernel_nacl32768_traffic.json")
+    if os.path.exists(traffic):
+        with open(traffic) as f:
+            tr = json.load(f)
+        line["roofline"]["traffic"] = tr.get("bytes_per_launch")
+        line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
+    work = {}
+    if not args.quick:
+        work = extra_workloads(args, want_cpu, tune)
+    line["workloads"] = work
+    print(json.dumps(line), flush=True)
+
+
+def extra_workloads(args, want_cpu, tune):
+    from md_gpu_amd import systems as S
+    out = {}
+    # argon 864, NVE velocity Verlet, 1000 steps
+    ar = S.argon()
+    out["argon864_nve_vv"] = step_workload(
+        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 1000, 100, False, ar.dt,
+        (lambda e, s: cpu_small(e, s, 5.0)) if want_cpu else None)
+    # water 196,608 atoms, NVT velocity Verlet + Nose-Hoover
+    wa = S.water()
+    isO = (wa.type == 0)
+
+    def f_water(p):
+        n, noo = count_pairs(p, wa.box, wa.ff.rc, wa.type, lambda a, b: (a == 0) & (b == 0))
+        return 60.0 * n + 5.0 * noo
+    out["water196608_nvt_vv_nh"] = step_workload(
+        "water196608", wa, f_water, 1000, 200, tune, wa.dt,
+        (lambda e, s: cpu_small(e, s, 10.0)) if want_cpu else None)
+    del isO
+    # 512 batched Ar-Xe replicas, one launch per step
+    reps = S.arxe_replicas()
+
+    def f_batch(p):
+        off, tot = 0, 0
+        for r in reps:
+            tot += count_pairs(p[off:off + r.n], r.box, r.ff.rc)[0]
+            off += r.n
+        return 36.0 * tot
+    out["arxe512_batched_nvt_lf"] = step_workload("arxe512", reps, f_batch, 500, 50, tune, reps[0].dt,
+                                                  (lambda e, s: cpu_batch(e, s, 10.0)) if want_cpu else None)
+    out["arxe512_batched_nvt_lf"]["atoms"] = int(sum(r.n for r in reps))
+    out["nacl4096x64_force_matching"] = fm_workload(want_cpu)
+    return out
+
+
+def cpu_small(e, s, budget):
+    pos, vel, _ = e.download()
+    t = s.copy()
+    t.pos, t.vel = pos, vel
+    n, sps, cores = oracle_cpu_steps(t, budget)
+    return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} steps of the same state, FP64 oracle"}
+
+
+def cpu_batch(e, reps, budget):
+    """Oracle on a seeded subset of replicas; steps/s of the whole batch
+    extrapolated by atom count (the oracle cost is linear in atoms)."""
+    import oracle
+    rng = np.random.default_rng(0)
+    pick = rng.choice(len(reps), 16, replace=False)
+    t0 = time.perf_counter()
+    atoms = 0
+    nst = 5
+    for k in pick:
+        oracle.md_run(reps[k], nst)
+        atoms += reps[k].n
+    el = time.perf_counter() - t0
+    total = sum(r.n for r in reps)
+    sps = nst / (el * total / atoms)
+    return {"value": sps, "unit": "timesteps/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{nst} steps of 16 seeded replicas ({atoms} atoms), scaled to {total} atoms"}
+
+
+def fm_workload(want_cpu):
+    """Force matching on NaCl 4096 ions x 64 frames: loss + gradient over 7 parameters."""
+    import dataclasses
+    from md_gpu_amd import engine as E, systems as S
+    from md_gpu_amd.fit import ForceMatch
+    base = S.nacl(nsites=16)
+    log("fm: generating 64 frames by GPU MD")
+    e = E.Engine.from_system(base)
+    e.run(200, stats=False)
+    frames = []
+    for k in range(64):
+        e.run(10, stats=False)
+        frames.append(e.download()[0].copy())
+    frames = np.stack(frames)
+    tgt_ff, tgt_q = S.perturb_nacl_ff(base.ff, seed=9)
+    # targets from the perturbed model (GPU, force-only kernel)
+    tgt = []
+    t_sys = dataclasses.replace(base.copy(), ff=tgt_ff)
+    t_sys.charge = base.charge * tgt_q
+    e2 = E.Engine.from_system(t_sys)
+    for f in frames:
+        e2.upload(f)
+        tgt.append(e2.compute_forces()[0])
+    e2.close()
+    tgt = np.stack(tgt)
+    fm = ForceMatch(e, base, frames, tgt)
+    theta = fm.theta0()
+    fm.loss_grad(theta)
+    n = 50
+    t0 = time.perf_counter()
+    for _ in range(n):
+        loss, g = fm.loss_grad(theta)
+    el = time.perf_counter() - t0
+    # algorithmic flops per eval: (65 forward + 45 adjoint) per ordered... per unordered pair x 2 directions
+    P = np.mean([count_pairs(f, base.box, base.ff.rc)[0] for f in frames[:4]])
+    flops = 64 * P * (65 + 45)
+    out = {"value": n / el, "unit": "force-grad evals/s", "ms_per_eval": 1e3 * el / n, "loss": loss,
+           "roofline": {"bound": "valu", "achieved": flops / (el / n) / 1e12, "peak": PEAK_FP32, "unit": "TFLOP/s",
+                        "frac": flops / (el / n) / 1e12 / PEAK_FP32, "traffic": None,
+                        "algorithmic_flop_per_launch": flops}}
+    if want_cpu:
+        out["cpu_baseline"] = cpu_fd_baseline(fm, base, frames, tgt, theta, nframes=4)
+    e.close()
+    return out
+
+
+def cpu_fd_baseline(fm, base, frames, tgt, theta, nframes=4, rel_h=1e-4):
+    """FP64 oracle central finite differences over the 7 parameters (14 loss
+    evaluations + 1), on a subset of frames, scaled to 64 frames."""
+    import oracle
+    fr = frames[:nframes]
+    tg = tgt[:nframes].astype(np.float64)
+
+    def loss(th):
+        lj1, lj2, q = fm.tables(th)
+        return oracle.fm_loss_grad(fr, base.type, base.charge, base.box, base.ff, q, tg, lj1, lj2)[0]
+    t0 = time.perf_counter()
+    loss(theta)
+    for k in range(len(theta)):
+        h = rel_h * abs(theta[k])
+        tp = theta.copy(); tp[k] += h
+        tm = theta.copy(); tm[k] -= h
+        loss(tp); loss(tm)
+    el = time.perf_counter() - t0
+    scale = frames.shape[0] / nframes
+    return {"value": 1.0 / (el * scale), "unit": "force-grad evals/s", "cores": oracle.num_threads(),
+            "kind": "port", "sample": f"central FD (15 FP64 loss evals) on {nframes} of 64 frames, scaled"}
+
+
+if __name__ == "__main__":
+    main()

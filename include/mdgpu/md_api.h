@@ -137,10 +137,19 @@ int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q);
  * list capacity per atom, current lanes-per-atom and grid of the step kernel. */
 int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid);
 
-/* Launch configuration of the step kernel: lanes cooperating on one atom
- * (4, 8, 16, 32), persistent grid size in workgroups (0 = automatic),
- * cell edge divisor (1..3: cell edge >= r_list / div).  Rebuilds lists. */
-int md_set_launch(md_engine* e, int lanes, int grid, int cell_div);
+/* Launch configuration (no counterpart in the reference, whose launch shape
+ * is fixed by src/gpu_properties.cu:31-67).  Setting it rebuilds the lists
+ * when the cell grid changes. */
+typedef struct md_launch_config {
+    int lanes;         /* lanes cooperating on one atom in the step kernel: 4, 8, 16, 32 */
+    int grid;          /* persistent grid of the step kernel in workgroups (0 = automatic) */
+    int cell_div;      /* cell edge >= r_list / cell_div, 1..3 */
+    int check_every;   /* steps queued between two (device-gated) list-rebuild chains */
+    int build_lanes;   /* lanes per atom in the list build: 4, 8, 16 */
+    int submit;        /* 0: replayed HIP graph segments, 1: direct kernel launches */
+} md_launch_config;
+int md_set_launch(md_engine* e, const md_launch_config* cfg);
+int md_get_launch(md_engine* e, md_launch_config* cfg);
 /* Time candidate launch configurations on the current system (nsteps each)
  * and keep the fastest.  best_ms_per_step may be NULL. */
 int md_autotune(md_engine* e, int nsteps, double* best_ms_per_step);

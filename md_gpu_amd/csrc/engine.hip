@@ -4,14 +4,17 @@
 // share a force field.  A single system is R = 1.  The reference's per-step
 // host round trips (skin check flag, KE partials: src/neighborlist.cu:127-129,
 // src/integrator.cu:152) are replaced by device-side flags and reductions, so
-// a run is queued as hipGraph segments of up to kSegment steps and the host
-// synchronises once per segment.
+// a run is queued as replays of one captured hipGraph segment and the host
+// synchronises only to confirm that every replica reached its step target.
 //
-// Per step (leapfrog):          [rebuild, gated by flag]  k_step(LF)
-// Per step (velocity Verlet):   k_vv_kick  [rebuild, gated]  k_step(VV)
-// A rebuild is skipped on the device (the five kernels exit at once) unless
-// the previous position update moved some atom of that replica by at least
-// skin/2 since its last build (src/neighborlist.cu:36,157-176).
+// A graph segment is `blocks` x ([rebuild chain] + check_every x [step]), with
+//   step (leapfrog)         = k_step(LF)
+//   step (velocity Verlet)  = k_vv_kick + k_step(VV)
+// Every kernel decides per replica on the device whether it runs (RepState:
+// list valid, velocity-Verlet phase, step target), so one captured segment is
+// replayed until every replica reaches its target; a replica whose update
+// tripped the skin criterion (src/neighborlist.cu:36,157-176) idles until the
+// next rebuild chain in the segment.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -24,8 +27,10 @@
 #include "md_device.h"
 #include "mdgpu/md_api.h"
 
-extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int* tmpid, hipStream_t st);
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int energy, int grid, hipStream_t st);
+extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st);
+extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st);
+extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st);
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st);
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st);
@@ -59,7 +64,7 @@ int fail(int code, const char* fmt, ...) {
         if (rc_ != MD_OK) return rc_; \
     } while (0)
 
-const int kSegment = 100;        // steps per graph segment
+const int kSegment = 96;         // steps queued per graph segment (blocks x check_every)
 const double kPi = 3.14159265358979323846;
 
 template <class T>
@@ -141,7 +146,6 @@ struct md_engine {
     std::vector<float> charge_h, mass_h;
     std::vector<RepParam> rep_h;
     std::vector<int> tile_rep_h;
-    std::vector<int> bonds_h, angles_h;
     int nb = 0, na = 0;
     bool bonded = false, has_excl = false;
 
@@ -155,23 +159,31 @@ struct md_engine {
     DBuf<int> nbond, nang;
     DBuf<RepParam> rep;
     DBuf<int> tile_rep;
+    DBuf<RepState> rst;
+    DBuf<Gate> gate;
     DBuf<ThState> th;
     DBuf<double> acc, fm_acc;
-    DBuf<int> units_done, flags, err;
+    DBuf<int> err;
     DBuf<unsigned long long> reb_count;
     DBuf<StatRec> stats;
+    int stat_cap = 0;
     DBuf<float4> ljtab;
     // snapshot for overflow recovery
-    DBuf<float4> snap_pos, snap_vel, snap_force;
+    DBuf<float4> snap_pos0, snap_pos1, snap_vel, snap_force;
     DBuf<int> snap_ids, snap_typ, snap_slot_of;
     DBuf<ThState> snap_th;
+    DBuf<RepState> snap_rst;
+    DBuf<Gate> snap_gate;
     int cap = 0;
 
-    long long step = 0;
-    long long rebuilds = 0;
-    int lanes = 16, grid = 0, cell_div = 2;
+    long long step = 0;          // steps completed (all replicas advance together between calls)
+    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 16, submit = 0;
+    double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
-    std::map<std::string, hipGraphExec_t> graphs;
+    // segment graphs by key; two executable copies each, launched alternately, so
+    // the next replay can be submitted while the previous one still runs
+    struct SegGraph { hipGraphExec_t ex[2]; int next; };
+    std::map<std::string, SegGraph> graphs;
 
     bool timing = false;
     double t_step_ms = 0, t_reb_ms = 0;
@@ -184,7 +196,8 @@ struct md_engine {
         if (st) (void)hipStreamDestroy(st);
     }
     void drop_graphs() {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto& kv : graphs)
+            for (auto x : kv.second.ex) (void)hipGraphExecDestroy(x);
         graphs.clear();
     }
     int apb() const { return 256 / lanes; }
@@ -246,6 +259,7 @@ int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
             t[i * 8 + j] = make_float4((float)l1, (float)l2, (float)shift, 0.f);
         }
     HIPCHECK(hipMemcpyAsync(e->ljtab.p, t.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
 }
 
@@ -290,6 +304,7 @@ int setup_cells(md_engine* e) {
     CHECK(e->cell_start.alloc(off + e->R));
     HIPCHECK(hipMemcpy(e->cell_rank.p, ranks.data(), off * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->rep.p, e->rep_h.data(), e->R * sizeof(RepParam), hipMemcpyHostToDevice));
+    e->drop_graphs();
     return MD_OK;
 }
 
@@ -313,31 +328,44 @@ int alloc_list(md_engine* e, int cap) {
     return MD_OK;
 }
 
-BuildArgs build_args(md_engine* e, float4* P, int* flag) {
+int ensure_stats(md_engine* e, int nrec) {
+    nrec = std::max(nrec, 1);
+    if (e->stat_cap < nrec) {
+        CHECK(e->stats.alloc((size_t)nrec * e->R));
+        e->stat_cap = nrec;
+        e->drop_graphs();
+    }
+    return MD_OK;
+}
+
+BuildArgs build_args(md_engine* e) {
     BuildArgs b{};
-    b.pos = P; b.vel = e->vel.p; b.force = e->force.p; b.ids = e->ids.p; b.typ = e->typ.p;
+    b.pos[0] = e->pos[0].p; b.pos[1] = e->pos[1].p; b.vel = e->vel.p; b.force = e->force.p;
+    b.ids = e->ids.p; b.typ = e->typ.p;
     b.pos_s = e->pos_s.p; b.vel_s = e->vel_s.p; b.force_s = e->force_s.p; b.ids_s = e->ids_s.p; b.typ_s = e->typ_s.p;
-    b.pos_dst = P; b.vel_dst = e->vel.p; b.force_dst = e->force.p; b.ids_dst = e->ids.p; b.typ_dst = e->typ.p;
     b.pos_build = e->pos_build.p;
-    b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p;
+    b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p; b.tmpid = e->tmpid.p;
     b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
     b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p; b.maxcount = e->maxcount.p;
-    b.excl = e->excl.p; b.flag = flag; b.rebuild_count = e->reb_count.p; b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
+    b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p; b.rebuild_count = e->reb_count.p;
+    b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
+    b.build_lanes = e->build_lanes;
     return b;
 }
 
-StepArgs step_args(md_engine* e, const float4* pin, float4* pout, int mode, int* fset, int* fclear, int stat_index) {
+StepArgs step_args(md_engine* e, int mode, int energy_every) {
     StepArgs a{};
-    a.pos_in = pin; a.pos_out = pout; a.vel = e->vel.p; a.force = e->force.p;
+    a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
     a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
-    a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.th = e->th.p; a.acc = e->acc.p; a.units_done = e->units_done.p;
-    a.flag_set = fset; a.flag_clear = fclear; a.err = e->err.p; a.stats = e->stats.p;
-    a.ljtab = reinterpret_cast<const float2*>(e->ljtab.p);
+    a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.acc = e->acc.p;
+    a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
+    a.ljtab = e->ljtab.p;
     a.slot_of = e->slot_of.p; a.bond_list = e->bond_list.p; a.nbond = e->nbond.p; a.ang_list = e->ang_list.p; a.nang = e->nang.p;
     a.target = e->target.p;
     a.cap = e->cap; a.ntype = e->ff.ntype; a.nrep = e->R; a.nunits = e->nunits(); a.mode = mode;
-    a.thermostat = e->in.thermostat; a.bonded = e->bonded ? 1 : 0; a.stat_index = stat_index;
+    a.thermostat = e->in.thermostat; a.bonded = e->bonded ? 1 : 0;
+    a.energy_every = energy_every; a.force_energy = 0;
     a.rc2 = e->rc2; a.qfac = e->ff.qfac; a.palfa = e->palfa; a.spiialfa = e->spiialfa; a.alfasq = e->alfasq;
     a.wolf = e->wolf; a.reacf = e->reacf; a.ecut = e->ecut; a.log2rc = e->log2rc; a.qq_scale = 1.0f;
     a.dt = e->in.dt; a.tau = e->in.tau; a.skin2q = e->skin2q;
@@ -346,6 +374,14 @@ StepArgs step_args(md_engine* e, const float4* pin, float4* pout, int mode, int*
         a.ak2[k] = e->ff.angle_k2[k]; a.at0[k] = e->ff.angle_t0[k];
     }
     return a;
+}
+
+KickArgs kick_args(md_engine* e, int apply_drift) {
+    KickArgs k{};
+    k.pos[0] = e->pos[0].p; k.pos[1] = e->pos[1].p; k.vel = e->vel.p; k.force = e->force.p; k.ids = e->ids.p;
+    k.pos_build = e->pos_build.p; k.tile_rep = e->tile_rep.p; k.rep = e->rep.p; k.st = e->rst.p; k.gate = e->gate.p; k.th = e->th.p;
+    k.err = e->err.p; k.nslot = e->S; k.nrep = e->R; k.dt = e->in.dt; k.skin2q = e->skin2q; k.apply_drift = apply_drift;
+    return k;
 }
 
 hipEvent_t ev_get(md_engine* e, size_t i) {
@@ -357,64 +393,56 @@ hipEvent_t ev_get(md_engine* e, size_t i) {
     return e->ev_pool[i];
 }
 
-// timing bookkeeping for direct (non-graph) launches
+// HIP-event timing of direct (non-graph) launches
 struct TimedSpan {
     md_engine* e;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> steps, rebs;
-    size_t used = 0;
+    size_t used = 2;   // events 0 and 1 are reserved for whole-run timing
     std::pair<hipEvent_t, hipEvent_t> take() {
         hipEvent_t a = ev_get(e, used++), b = ev_get(e, used++);
         return {a, b};
     }
 };
 
-int enqueue_rebuild(md_engine* e, float4* P, int parity, TimedSpan* ts) {
-    BuildArgs b = build_args(e, P, e->flags.p + parity * e->R);
+int enqueue_rebuild(md_engine* e, TimedSpan* ts) {
+    BuildArgs b = build_args(e);
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
-    HIPCHECK(md_launch_rebuild(b, e->tmpid.p, e->st));
+    HIPCHECK(md_launch_rebuild(b, e->st));
     if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->rebs.push_back(ev); }
     return MD_OK;
 }
 
-int enqueue_stepkernel(md_engine* e, const StepArgs& a, bool energy, TimedSpan* ts) {
+int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
-    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, energy ? 1 : 0, e->grid_blocks(), e->st));
+    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st));
     if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->steps.push_back(ev); }
     return MD_OK;
 }
 
-// Queue step n (absolute index), statistics record k of the current segment.
-int enqueue_step(md_engine* e, long long n, int k, bool energy, TimedSpan* ts) {
-    const int p = (int)(n & 1), q = p ^ 1;
-    int* fl = e->flags.p;
+// One step launch (the kernels skip replicas that cannot advance).  With
+// chain_mid a gated rebuild chain sits between the velocity-Verlet kick and
+// the force kernel, so the step always completes (timed launches).
+int enqueue_step(md_engine* e, int energy_every, TimedSpan* ts, bool chain_mid = false) {
     if (e->in.integrator == MD_LEAPFROG) {
-        CHECK(enqueue_rebuild(e, e->pos[p].p, p, ts));
-        StepArgs a = step_args(e, e->pos[p].p, e->pos[q].p, MODE_LF, fl + q * e->R, fl + p * e->R, k);
-        CHECK(enqueue_stepkernel(e, a, energy, ts));
+        CHECK(enqueue_stepkernel(e, step_args(e, MODE_LF, energy_every), ts));
     } else {
-        KickArgs ka{};
-        ka.pos_in = e->pos[p].p; ka.pos_out = e->pos[q].p; ka.vel = e->vel.p; ka.force = e->force.p;
-        ka.ids = e->ids.p; ka.pos_build = e->pos_build.p; ka.tile_rep = e->tile_rep.p; ka.rep = e->rep.p;
-        ka.th = e->th.p; ka.flag_set = fl + q * e->R; ka.flag_clear = fl + p * e->R; ka.err = e->err.p;
-        ka.nslot = e->S; ka.nrep = e->R; ka.dt = e->in.dt; ka.skin2q = e->skin2q; ka.apply_drift = 1;
-        HIPCHECK(md_launch_kick(ka, e->st));
-        CHECK(enqueue_rebuild(e, e->pos[q].p, q, ts));
-        StepArgs a = step_args(e, e->pos[q].p, e->pos[q].p, MODE_VV, nullptr, nullptr, k);
-        CHECK(enqueue_stepkernel(e, a, energy, ts));
+        HIPCHECK(md_launch_kick(kick_args(e, 1), e->st));
+        if (chain_mid) CHECK(enqueue_rebuild(e, ts));
+        CHECK(enqueue_stepkernel(e, step_args(e, MODE_VV, energy_every), ts));
     }
     return MD_OK;
 }
 
 int snapshot(md_engine* e, bool restore) {
-    const int p = (int)(e->step & 1);
     struct Pair { void* a; void* b; size_t n; };
     Pair list[] = {
-        {e->snap_pos.p, e->pos[p].p, e->pos[p].bytes()}, {e->snap_vel.p, e->vel.p, e->vel.bytes()},
-        {e->snap_force.p, e->force.p, e->force.bytes()}, {e->snap_ids.p, e->ids.p, e->ids.bytes()},
-        {e->snap_typ.p, e->typ.p, e->typ.bytes()}, {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()},
-        {e->snap_th.p, e->th.p, e->th.bytes()}};
+        {e->snap_pos0.p, e->pos[0].p, e->pos[0].bytes()}, {e->snap_pos1.p, e->pos[1].p, e->pos[1].bytes()},
+        {e->snap_vel.p, e->vel.p, e->vel.bytes()}, {e->snap_force.p, e->force.p, e->force.bytes()},
+        {e->snap_ids.p, e->ids.p, e->ids.bytes()}, {e->snap_typ.p, e->typ.p, e->typ.bytes()},
+        {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()}, {e->snap_th.p, e->th.p, e->th.bytes()},
+        {e->snap_rst.p, e->rst.p, e->rst.bytes()}, {e->snap_gate.p, e->gate.p, e->gate.bytes()}};
     for (auto& x : list) {
         if (restore) HIPCHECK(hipMemcpyAsync(x.b, x.a, x.n, hipMemcpyDeviceToDevice, e->st));
         else HIPCHECK(hipMemcpyAsync(x.a, x.b, x.n, hipMemcpyDeviceToDevice, e->st));
@@ -422,36 +450,38 @@ int snapshot(md_engine* e, bool restore) {
     return MD_OK;
 }
 
-// Rebuild all replicas now at the current positions (host synchronous).
+// Rebuild all replicas now at the current positions (host synchronous),
+// growing the list capacity until every row fits.
 int full_rebuild(md_engine* e) {
-    const int p = (int)(e->step & 1);
-    std::vector<int> one(e->R, 1);
-    for (int attempt = 0; attempt < 4; attempt++) {
-        HIPCHECK(hipMemcpyAsync(e->flags.p + p * e->R, one.data(), e->R * sizeof(int), hipMemcpyHostToDevice, e->st));
+    for (int attempt = 0; attempt < 6; attempt++) {
+        HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
         HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
-        CHECK(enqueue_rebuild(e, e->pos[p].p, p, nullptr));
+        CHECK(enqueue_rebuild(e, nullptr));
         std::vector<int> mc(e->R);
         HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
         HIPCHECK(hipStreamSynchronize(e->st));
         int m = *std::max_element(mc.begin(), mc.end());
-        if (m <= e->cap) break;
+        if (m <= e->cap) return MD_OK;
         CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
     }
-    HIPCHECK(hipMemsetAsync(e->flags.p + p * e->R, 0, e->R * sizeof(int), e->st));
+    return fail(MD_E_CAPACITY, "neighbor list capacity could not be sized");
+}
+
+// Queue a rebuild chain (it runs only for replicas whose list is stale).
+int refresh_lists(md_engine* e) {
+    HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+    CHECK(enqueue_rebuild(e, nullptr));
+    std::vector<int> mc(e->R);
+    HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
+    if (*std::max_element(mc.begin(), mc.end()) > e->cap) CHECK(full_rebuild(e));
     return MD_OK;
 }
 
-int ensure_lists_current(md_engine* e) {
-    if (e->in.integrator != MD_LEAPFROG) return MD_OK;
-    const int p = (int)(e->step & 1);
-    CHECK(enqueue_rebuild(e, e->pos[p].p, p, nullptr));
-    HIPCHECK(hipMemsetAsync(e->flags.p + p * e->R, 0, e->R * sizeof(int), e->st));
-    return MD_OK;
-}
-
-int ensure_stats(md_engine* e, int nrec) {
-    if (e->stats.n < (size_t)nrec * e->R) CHECK(e->stats.alloc((size_t)nrec * e->R));
+int read_states(md_engine* e, std::vector<RepState>& s) {
+    s.resize(e->R);
+    HIPCHECK(hipMemcpyAsync(s.data(), e->rst.p, e->R * sizeof(RepState), hipMemcpyDeviceToHost, e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
 }
 
@@ -463,7 +493,7 @@ extern "C" {
 const char* md_last_error(void) { return g_err.c_str(); }
 
 const char* md_version(void) {
-    return "mdgpu 0.1 (gfx950, hipcc " __clang_version__ ")";
+    return "mdgpu 0.2 (gfx950, hipcc " __clang_version__ ")";
 }
 
 int md_create(md_engine** out, int device) {
@@ -512,7 +542,6 @@ int md_set_forcefield(md_engine* e, const md_forcefield* f) {
     if (e->ff.qfac == 0.0f) e->ff.qfac = 1389.35427611722f;
     derive_ff(e);
     CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
-    HIPCHECK(hipStreamSynchronize(e->st));
     e->has_ff = true;
     e->has_sys = false;   // cells depend on r_list: set the system again
     e->has_state = false;
@@ -528,6 +557,8 @@ int md_set_integrator(md_engine* e, const md_integrator* in) {
         return fail(MD_E_INVALID, "Nose-Hoover needs the velocity-Verlet integrator");
     if (!(in->dt > 0)) return fail(MD_E_INVALID, "dt must be > 0");
     if (in->thermostat != MD_TH_NONE && !(in->tau > 0)) return fail(MD_E_INVALID, "tau must be > 0");
+    if (e->has_state && e->in.integrator != in->integrator)
+        return fail(MD_E_STATE, "switching the integrator needs a new upload (velocity convention differs)");
     e->in = *in;
     e->has_integ = true;
     e->drop_graphs();
@@ -547,6 +578,20 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     if (!e->has_ff) return fail(MD_E_STATE, "set the force field first");
     if (nrep < 1) return fail(MD_E_INVALID, "nrep must be >= 1");
     HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    long long N = 0, S = 0;
+    for (int r = 0; r < nrep; r++) {
+        if (natom[r] < 2) return fail(MD_E_INVALID, "replica %d has %d atoms", r, natom[r]);
+        for (int d = 0; d < 3; d++)
+            if (!(box[3 * r + d] > 0)) return fail(MD_E_INVALID, "replica %d: box must be positive", r);
+        N += natom[r];
+        S += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
+    }
+    if (S >= (1LL << MD_JBITS)) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << MD_JBITS);
+    for (long long i = 0; i < N; i++) {
+        if (type[i] < 0 || type[i] >= e->ff.ntype) return fail(MD_E_INVALID, "atom %lld: type %d outside 0..%d", i, type[i], e->ff.ntype - 1);
+        if (!(mass[i] > 0)) return fail(MD_E_INVALID, "atom %lld: mass must be > 0", i);
+    }
     e->R = nrep;
     e->natom.assign(natom, natom + nrep);
     e->box.assign(box, box + 3 * nrep);
@@ -554,30 +599,21 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     if (T0) e->T0.assign(T0, T0 + nrep);
     e->atom_off.resize(nrep);
     e->slot_off.resize(nrep);
-    long long N = 0, S = 0;
-    for (int r = 0; r < nrep; r++) {
-        if (natom[r] < 2) return fail(MD_E_INVALID, "replica %d has %d atoms", r, natom[r]);
-        for (int d = 0; d < 3; d++)
-            if (!(box[3 * r + d] > 0)) return fail(MD_E_INVALID, "replica %d: box must be positive", r);
-        e->atom_off[r] = (int)N;
-        e->slot_off[r] = (int)S;
-        N += natom[r];
-        S += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
+    for (int r = 0, n = 0, s = 0; r < nrep; r++) {
+        e->atom_off[r] = n;
+        e->slot_off[r] = s;
+        n += natom[r];
+        s += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
     }
-    if (S >= (1LL << MD_JBITS)) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << MD_JBITS);
     e->N = (int)N;
     e->S = (int)S;
     e->type_h.assign(type, type + N);
     e->charge_h.assign(charge, charge + N);
     e->mass_h.assign(mass, mass + N);
     bool charged = false;
-    for (long long i = 0; i < N; i++) {
-        if (type[i] < 0 || type[i] >= e->ff.ntype) return fail(MD_E_INVALID, "atom %lld: type %d outside 0..%d", i, type[i], e->ff.ntype - 1);
-        if (!(mass[i] > 0)) return fail(MD_E_INVALID, "atom %lld: mass must be > 0", i);
+    for (long long i = 0; i < N; i++)
         if (charge[i] != 0.0f) charged = true;
-    }
     e->coul_kernel = charged ? e->ff.coulomb : MD_COUL_NONE;
-    // replica parameters
     e->rep_h.assign(nrep, RepParam{});
     e->tile_rep_h.assign(S / MD_TILE, 0);
     for (int r = 0; r < nrep; r++) {
@@ -597,11 +633,10 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
         p.Q = (double)p.nf * p.kT0 * (double)e->in.tau * e->in.tau;
         for (int t = p.slot_off / MD_TILE; t < (p.slot_off + p.nslot) / MD_TILE; t++) e->tile_rep_h[t] = r;
     }
-    // device buffers
     e->drop_graphs();
     const size_t Sz = S;
     for (auto* b : {&e->pos[0], &e->pos[1], &e->vel, &e->force, &e->pos_build, &e->pos_s, &e->vel_s, &e->force_s,
-                    &e->target, &e->snap_pos, &e->snap_vel, &e->snap_force})
+                    &e->target, &e->snap_pos0, &e->snap_pos1, &e->snap_vel, &e->snap_force})
         CHECK(b->alloc(Sz));
     for (auto* b : {&e->ids, &e->typ, &e->ids_s, &e->typ_s, &e->key, &e->rank, &e->tmp, &e->tmpid, &e->ncount,
                     &e->snap_ids, &e->snap_typ})
@@ -610,20 +645,22 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->snap_slot_of.alloc(N));
     CHECK(e->rep.alloc(nrep));
     CHECK(e->tile_rep.alloc(S / MD_TILE));
+    CHECK(e->rst.alloc(nrep));
+    CHECK(e->snap_rst.alloc(nrep));
+    CHECK(e->gate.alloc(1));
+    CHECK(e->snap_gate.alloc(1));
     CHECK(e->th.alloc(nrep));
     CHECK(e->snap_th.alloc(nrep));
     CHECK(e->acc.alloc((size_t)nrep * MD_NSTAT));
-    CHECK(e->units_done.alloc(nrep));
-    CHECK(e->flags.alloc(2 * (size_t)nrep));
     CHECK(e->maxcount.alloc(nrep));
-    CHECK(ensure_stats(e, kSegment + 1));
+    e->stat_cap = 0;
+    CHECK(ensure_stats(e, 1));
     std::vector<int> neg(S, -1);
     HIPCHECK(hipMemcpy(e->ids.p, neg.data(), S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->ids_s.p, neg.data(), S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->tile_rep.p, e->tile_rep_h.data(), (S / MD_TILE) * sizeof(int), hipMemcpyHostToDevice));
     CHECK(setup_cells(e));
     CHECK(alloc_list(e, estimate_cap(e)));
-    // topology defaults: none
     e->nb = e->na = 0;
     e->bonded = e->has_excl = false;
     CHECK(e->excl.alloc(N));
@@ -640,12 +677,12 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
 int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* angles) {
     if (!e || !e->has_sys) return fail(MD_E_STATE, "set the system first");
     if (nb < 0 || na < 0 || (nb && !bonds) || (na && !angles)) return fail(MD_E_INVALID, "bad topology arrays");
+    HIPCHECK(hipSetDevice(e->dev));
     const int N = e->N;
     std::vector<int4> ex(N, make_int4(-1, -1, -1, -1));
     std::vector<uint32_t> bl(4 * (size_t)N, 0);
-    std::vector<int> nbo(N, 0), nan_(N, 0);
+    std::vector<int> nbo(N, 0), nan_(N, 0), nex(N, 0);
     std::vector<int4> al(4 * (size_t)N, make_int4(0, 0, 0, 0));
-    std::vector<int> nex(N, 0);
     for (int b = 0; b < nb; b++) {
         int i = bonds[3 * b], j = bonds[3 * b + 1], t = bonds[3 * b + 2];
         if (i < 0 || i >= N || j < 0 || j >= N || i == j) return fail(MD_E_INVALID, "bond %d: bad atoms %d %d", b, i, j);
@@ -688,10 +725,12 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
     CHECK(check_ready(e, false));
     if (!pos) return fail(MD_E_INVALID, "null positions");
     HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
     const int S = e->S;
     std::vector<float4> P(S, make_float4(0, 0, 0, 0)), V(S, make_float4(0, 0, 0, 0));
     std::vector<int> ids(S, -1), typ(S, 0), slot_of(e->N);
     std::vector<ThState> th(e->R);
+    std::vector<RepState> rs(e->R);
     for (int r = 0; r < e->R; r++) {
         double ke2 = 0;
         for (int k = 0; k < e->natom[r]; k++) {
@@ -717,26 +756,29 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
             slot_of[i] = s;
         }
         th[r] = ThState{0, 0, 1.0, 1.0, 0, 0, ke2, 0};
+        RepState z;
+        memset(&z, 0, sizeof z);
+        z.step = z.target = z.start = e->step;
+        z.trip[0] = 1;   // no list yet
+        rs[r] = z;
     }
-    const int p = (int)(e->step & 1);
-    HIPCHECK(hipMemcpy(e->pos[p].p, P.data(), S * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(e->pos[p ^ 1].p, P.data(), S * sizeof(float4), hipMemcpyHostToDevice));
+    for (int b = 0; b < 2; b++) HIPCHECK(hipMemcpy(e->pos[b].p, P.data(), S * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->vel.p, V.data(), S * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(e->force.p, 0, S * sizeof(float4)));
     HIPCHECK(hipMemcpy(e->ids.p, ids.data(), S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->typ.p, typ.data(), S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->slot_of.p, slot_of.data(), e->N * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->th.p, th.data(), e->R * sizeof(ThState), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(e->rst.p, rs.data(), e->R * sizeof(RepState), hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(e->acc.p, 0, e->acc.bytes()));
-    HIPCHECK(hipMemset(e->units_done.p, 0, e->units_done.bytes()));
-    HIPCHECK(hipMemset(e->flags.p, 0, e->flags.bytes()));
     HIPCHECK(hipMemset(e->err.p, 0, sizeof(int)));
     e->has_state = true;
     CHECK(full_rebuild(e));
     if (e->in.integrator == MD_VVERLET) {
         // velocity Verlet needs F(x0) before the first half kick
-        StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, -1);
-        CHECK(enqueue_stepkernel(e, a, false, nullptr));
+        StepArgs a = step_args(e, MODE_FORCE, 0);
+        a.stats = nullptr;
+        CHECK(enqueue_stepkernel(e, a, nullptr));
         HIPCHECK(hipStreamSynchronize(e->st));
     }
     return MD_OK;
@@ -745,22 +787,29 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
 int md_download_state(md_engine* e, float* pos, float* vel, float* force) {
     CHECK(check_ready(e, true));
     HIPCHECK(hipSetDevice(e->dev));
-    HIPCHECK(hipStreamSynchronize(e->st));
-    const int S = e->S, p = (int)(e->step & 1);
+    std::vector<RepState> rs;
+    CHECK(read_states(e, rs));
+    const int S = e->S;
     std::vector<int> ids(S);
     HIPCHECK(hipMemcpy(ids.data(), e->ids.p, S * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<float4> buf(S);
+    std::vector<float4> buf(S), buf2;
     float* outs[3] = {pos, vel, force};
-    float4* srcs[3] = {e->pos[p].p, e->vel.p, e->force.p};
     for (int k = 0; k < 3; k++) {
         if (!outs[k]) continue;
-        HIPCHECK(hipMemcpy(buf.data(), srcs[k], S * sizeof(float4), hipMemcpyDeviceToHost));
+        if (k == 0) {   // positions: each replica's current buffer
+            buf2.resize(S);
+            HIPCHECK(hipMemcpy(buf.data(), e->pos[0].p, S * sizeof(float4), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(buf2.data(), e->pos[1].p, S * sizeof(float4), hipMemcpyDeviceToHost));
+        } else {
+            HIPCHECK(hipMemcpy(buf.data(), k == 1 ? e->vel.p : e->force.p, S * sizeof(float4), hipMemcpyDeviceToHost));
+        }
         for (int s = 0; s < S; s++) {
             int i = ids[s];
             if (i < 0) continue;
-            outs[k][3 * (size_t)i] = buf[s].x;
-            outs[k][3 * (size_t)i + 1] = buf[s].y;
-            outs[k][3 * (size_t)i + 2] = buf[s].z;
+            const float4& v = (k == 0 && cur_buf(rs[e->tile_rep_h[s / MD_TILE]]) == 1) ? buf2[s] : buf[s];
+            outs[k][3 * (size_t)i] = v.x;
+            outs[k][3 * (size_t)i + 1] = v.y;
+            outs[k][3 * (size_t)i + 2] = v.z;
         }
     }
     return MD_OK;
@@ -769,10 +818,10 @@ int md_download_state(md_engine* e, float* pos, float* vel, float* force) {
 int md_compute_forces(md_engine* e, float* force, double* energy, double* virial) {
     CHECK(check_ready(e, true));
     HIPCHECK(hipSetDevice(e->dev));
-    CHECK(ensure_lists_current(e));
-    const int p = (int)(e->step & 1);
-    StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, 0);
-    CHECK(enqueue_stepkernel(e, a, true, nullptr));
+    CHECK(refresh_lists(e));
+    StepArgs a = step_args(e, MODE_FORCE, 0);
+    a.force_energy = 1;
+    CHECK(enqueue_stepkernel(e, a, nullptr));
     std::vector<StatRec> rec(e->R);
     HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, e->R * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
@@ -793,34 +842,57 @@ int md_compute_forces(md_engine* e, float* force, double* energy, double* virial
 
 namespace {
 
-// Run one segment of K steps starting at e->step.
-int run_segment(md_engine* e, int K, int energy_every, TimedSpan* ts) {
-    const long long n0 = e->step;
-    auto is_energy = [&](long long n) { return energy_every > 0 && (n % energy_every) == 0; };
-    if (ts) {   // direct launches with events (timing mode)
-        for (int k = 0; k < K; k++) CHECK(enqueue_step(e, n0 + k, k, is_energy(n0 + k), ts));
+// One segment: `blocks` x ([gated rebuild chain] + check_every steps).
+int enqueue_segment(md_engine* e, int energy_every) {
+    const int ce = std::max(1, e->check_every);
+    const int blocks = std::max(1, kSegment / ce);
+    for (int b = 0; b < blocks; b++) {
+        CHECK(enqueue_rebuild(e, nullptr));
+        for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
+    }
+    return MD_OK;
+}
+
+// Queue `nseg` segments: replays of a captured graph, or direct launches.
+int launch_segments(md_engine* e, int energy_every, int nseg) {
+    if (e->submit == 1) {
+        for (int s = 0; s < nseg; s++) CHECK(enqueue_segment(e, energy_every));
         return MD_OK;
     }
-    char keyb[128];
-    snprintf(keyb, sizeof keyb, "%d:%lld:%d:%lld", K, n0 & 1, energy_every, energy_every > 0 ? n0 % energy_every : 0LL);
+    const int ce = std::max(1, e->check_every);
+    char keyb[96];
+    snprintf(keyb, sizeof keyb, "%d:%d", ce, energy_every);
     std::string key(keyb);
     auto it = e->graphs.find(key);
     if (it == e->graphs.end()) {
         hipGraph_t g;
         HIPCHECK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
-        int rc = MD_OK;
-        for (int k = 0; k < K && rc == MD_OK; k++) rc = enqueue_step(e, n0 + k, k, is_energy(n0 + k), nullptr);
-        hipError_t ce = hipStreamEndCapture(e->st, &g);
+        const int rc = enqueue_segment(e, energy_every);
+        hipError_t cerr = hipStreamEndCapture(e->st, &g);
         if (rc != MD_OK) return rc;
-        if (ce != hipSuccess) return fail(MD_E_HIP, "graph capture: %s", hipGetErrorString(ce));
-        hipGraphExec_t ex;
-        hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (cerr != hipSuccess) return fail(MD_E_HIP, "graph capture: %s", hipGetErrorString(cerr));
+        md_engine::SegGraph sg{{nullptr, nullptr}, 0};
+        hipError_t ie = hipGraphInstantiate(&sg.ex[0], g, nullptr, nullptr, 0);
+        if (ie == hipSuccess) ie = hipGraphInstantiate(&sg.ex[1], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) return fail(MD_E_HIP, "graph instantiate: %s", hipGetErrorString(ie));
-        it = e->graphs.emplace(key, ex).first;
+        if (ie != hipSuccess) {
+            for (auto x : sg.ex)
+                if (x) (void)hipGraphExecDestroy(x);
+            return fail(MD_E_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+        }
+        it = e->graphs.emplace(key, sg).first;
     }
-    HIPCHECK(hipGraphLaunch(it->second, e->st));
+    for (int s = 0; s < nseg; s++) {
+        md_engine::SegGraph& sg = it->second;
+        HIPCHECK(hipGraphLaunch(sg.ex[sg.next], e->st));
+        sg.next ^= 1;
+    }
     return MD_OK;
+}
+
+int segment_steps(md_engine* e) {
+    const int ce = std::max(1, e->check_every);
+    return std::max(1, kSegment / ce) * ce;
 }
 
 }  // namespace
@@ -831,41 +903,28 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     CHECK(check_ready(e, true));
     if (nsteps < 0) return fail(MD_E_INVALID, "nsteps < 0");
     if (nsteps == 0) return MD_OK;
+    if (energy_every < 0) return fail(MD_E_INVALID, "energy_every < 0");
     HIPCHECK(hipSetDevice(e->dev));
     const bool vv = e->in.integrator == MD_VVERLET;
-    TimedSpan span{e, {}, {}, 0};
-    TimedSpan* ts = e->timing ? &span : nullptr;
-    if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
-    int done = 0, retries = 0;
-    std::vector<StatRec> rec;
-    while (done < nsteps) {
-        const int K = std::min(kSegment, nsteps - done);
+    if (stats) CHECK(ensure_stats(e, nsteps));
+    const long long target = e->step + nsteps;
+    int retries = 0;
+    for (;;) {   // one attempt; repeated only after a list-capacity overflow
         CHECK(snapshot(e, false));
         HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
-        span.used = 0;
-        span.steps.clear();
-        span.rebs.clear();
-        CHECK(run_segment(e, K, energy_every, ts));
-        std::vector<int> mc(e->R);
-        int errf = 0;
-        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
-        HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
-        if (stats) {
-            rec.resize((size_t)K * e->R);
-            HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, rec.size() * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
-        }
-        HIPCHECK(hipStreamSynchronize(e->st));
-        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box near step %lld", e->step);
-        int m = *std::max_element(mc.begin(), mc.end());
-        if (m > e->cap) {   // a list row overflowed: roll the segment back with a larger capacity
-            if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
-            CHECK(snapshot(e, true));
-            CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
-            CHECK(full_rebuild(e));
-            continue;
-        }
-        retries = 0;
-        if (ts) {
+        HIPCHECK(md_launch_set_target(e->rst.p, e->gate.p, e->R, nsteps, e->st));
+        if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
+        long long launched = 0;
+        std::vector<RepState> rs;
+        if (e->timing) {
+            // direct launches, one rebuild chain per step, HIP events around each kernel
+            TimedSpan span{e, {}, {}, 2};
+            for (int k = 0; k < nsteps; k++) {
+                CHECK(enqueue_rebuild(e, &span));
+                CHECK(enqueue_step(e, energy_every, &span, true));
+            }
+            launched = nsteps;
+            CHECK(read_states(e, rs));
             for (auto& pr : span.steps) {
                 float ms = 0;
                 HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
@@ -878,30 +937,60 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
                 e->t_reb_ms += ms;
                 e->n_reb++;
             }
+        } else {
+            const int per = segment_steps(e);
+            long long want = (long long)std::ceil(nsteps * e->queue_ratio);
+            for (int round = 0; round < 64; round++) {
+                int nseg = (int)((want + per - 1) / per);
+                CHECK(launch_segments(e, energy_every, std::max(nseg, 1)));
+                launched += (long long)std::max(nseg, 1) * per;
+                CHECK(read_states(e, rs));
+                long long lag = 0;
+                for (auto& s : rs) lag = std::max(lag, target - s.step);
+                if (lag <= 0) break;
+                want = (long long)std::ceil(lag * std::max(e->queue_ratio, 1.0) * 1.1) + 1;
+            }
         }
-        if (stats) {
-            for (int k = 0; k < K; k++)
-                for (int r = 0; r < e->R; r++) {
-                    const StatRec& s = rec[(size_t)k * e->R + r];
-                    md_step_stats& o = stats[(size_t)(done + k) * e->R + r];
-                    o.ke = s.ke; o.temp = s.temp; o.pe_lj = s.pe_lj; o.pe_coul = s.pe_coul; o.pe_bond = s.pe_bond;
-                    o.pe_angle = s.pe_angle; o.virial = s.virial; o.conserved = s.conserved; o.xi = s.xi;
-                    o.vxi = s.vxi; o.has_energy = s.has_energy; o.pad = 0;
-                }
+        int errf = 0;
+        std::vector<int> mc(e->R);
+        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipStreamSynchronize(e->st));
+        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
+        int m = *std::max_element(mc.begin(), mc.end());
+        if (m > e->cap) {   // a list row overflowed: roll the run back with a larger capacity
+            if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
+            CHECK(snapshot(e, true));
+            CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
+            CHECK(full_rebuild(e));
+            continue;
         }
-        e->step += K;
-        done += K;
+        for (auto& s : rs)
+            if (s.step != target) return fail(MD_E_STATE, "replica stopped at step %lld of %lld", s.step, target);
+        if (!e->timing) {   // learn how many launches a step costs (stalls until the next rebuild chain)
+            double ratio = (double)launched / nsteps;
+            e->queue_ratio = 0.7 * e->queue_ratio + 0.3 * std::max(1.0, ratio * 0.98);
+        }
+        break;
     }
     if (vv) {
-        KickArgs ka{};
-        const int p = (int)(e->step & 1);
-        ka.pos_in = e->pos[p].p; ka.pos_out = e->pos[p].p; ka.vel = e->vel.p; ka.force = e->force.p; ka.ids = e->ids.p;
-        ka.pos_build = e->pos_build.p; ka.tile_rep = e->tile_rep.p; ka.rep = e->rep.p; ka.th = e->th.p;
-        ka.err = e->err.p; ka.nslot = e->S; ka.nrep = e->R; ka.dt = e->in.dt; ka.skin2q = e->skin2q; ka.apply_drift = 0;
-        HIPCHECK(md_launch_kick(ka, e->st));
+        HIPCHECK(md_launch_kick(kick_args(e, 0), e->st));
         HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 1, e->st));
     }
+    if (stats) {
+        std::vector<StatRec> rec((size_t)nsteps * e->R);
+        HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, rec.size() * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipStreamSynchronize(e->st));
+        for (size_t k = 0; k < rec.size(); k++) {
+            const StatRec& s = rec[k];
+            md_step_stats& o = stats[k];
+            o.ke = s.ke; o.temp = s.temp; o.pe_lj = s.pe_lj; o.pe_coul = s.pe_coul; o.pe_bond = s.pe_bond;
+            o.pe_angle = s.pe_angle; o.virial = s.virial; o.conserved = s.conserved; o.xi = s.xi;
+            o.vxi = s.vxi; o.has_energy = s.has_energy; o.pad = 0;
+        }
+    }
     HIPCHECK(hipStreamSynchronize(e->st));
+    e->step = target;
     return MD_OK;
 }
 
@@ -909,8 +998,7 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
     CHECK(check_ready(e, true));
     if (!counts || cap < 0 || (cap > 0 && !nbrs)) return fail(MD_E_INVALID, "bad output arrays");
     HIPCHECK(hipSetDevice(e->dev));
-    CHECK(ensure_lists_current(e));
-    HIPCHECK(hipStreamSynchronize(e->st));
+    CHECK(refresh_lists(e));
     const int S = e->S;
     std::vector<int> ids(S), nc(S);
     std::vector<uint32_t> L((size_t)S * e->cap);
@@ -955,22 +1043,43 @@ int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, i
     return MD_OK;
 }
 
-int md_set_launch(md_engine* e, int lanes, int grid, int cell_div) {
-    if (!e) return fail(MD_E_INVALID, "null engine");
-    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) return fail(MD_E_INVALID, "lanes must be 4, 8, 16 or 32");
-    if (grid < 0) return fail(MD_E_INVALID, "grid must be >= 0");
-    if (cell_div < 1 || cell_div > 3) return fail(MD_E_INVALID, "cell_div must be 1..3");
+int md_set_launch(md_engine* e, const md_launch_config* c) {
+    if (!e || !c) return fail(MD_E_INVALID, "null argument");
+    if (c->lanes != 4 && c->lanes != 8 && c->lanes != 16 && c->lanes != 32)
+        return fail(MD_E_INVALID, "lanes must be 4, 8, 16 or 32");
+    if (c->grid < 0) return fail(MD_E_INVALID, "grid must be >= 0");
+    if (c->cell_div < 1 || c->cell_div > 3) return fail(MD_E_INVALID, "cell_div must be 1..3");
+    if (c->check_every < 1 || c->check_every > kSegment)
+        return fail(MD_E_INVALID, "check_every must be 1..%d", kSegment);
+    if (c->build_lanes != 4 && c->build_lanes != 8 && c->build_lanes != 16)
+        return fail(MD_E_INVALID, "build_lanes must be 4, 8 or 16");
+    if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     HIPCHECK(hipSetDevice(e->dev));
-    e->lanes = lanes;
-    e->grid = grid;
+    HIPCHECK(hipStreamSynchronize(e->st));
+    e->lanes = c->lanes;
+    e->grid = c->grid;
+    e->check_every = c->check_every;
+    e->build_lanes = c->build_lanes;
+    e->submit = c->submit;
     e->drop_graphs();
-    if (cell_div != e->cell_div) {
-        e->cell_div = cell_div;
+    if (c->cell_div != e->cell_div) {
+        e->cell_div = c->cell_div;
         if (e->has_sys) {
             CHECK(setup_cells(e));
             if (e->has_state) CHECK(full_rebuild(e));
         }
     }
+    return MD_OK;
+}
+
+int md_get_launch(md_engine* e, md_launch_config* c) {
+    if (!e || !c) return fail(MD_E_INVALID, "null argument");
+    c->lanes = e->lanes;
+    c->grid = e->grid;
+    c->cell_div = e->cell_div;
+    c->check_every = e->check_every;
+    c->build_lanes = e->build_lanes;
+    c->submit = e->submit;
     return MD_OK;
 }
 
@@ -995,41 +1104,89 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     CHECK(check_ready(e, true));
     if (nsteps < 1) return fail(MD_E_INVALID, "nsteps must be >= 1");
     HIPCHECK(hipSetDevice(e->dev));
-    // candidate grid: lanes x cell divisor x workgroups per CU
-    const int lanes_c[4] = {4, 8, 16, 32};
-    const int div_c[2] = {1, 2};
-    const int wpc_c[3] = {2, 4, 8};
     const bool saved_timing = e->timing;
-    e->timing = false;
-    double best = 1e30;
-    int bl = e->lanes, bg = e->grid, bd = e->cell_div;
+    const double saved_t[2] = {e->t_step_ms, e->t_reb_ms};
+    const long long saved_n[2] = {e->n_step, e->n_reb};
     hipEvent_t t0 = ev_get(e, 0), t1 = ev_get(e, 1);
-    // keep the physical state: snapshot in host memory
+    // the physical state is kept: every trial starts from it
     const int N = e->N;
     std::vector<float> P(3 * (size_t)N), V(3 * (size_t)N);
+    std::vector<ThState> th(e->R);
     const long long step0 = e->step;
     CHECK(md_download_state(e, P.data(), V.data(), nullptr));
-    for (int d : div_c)
-        for (int l : lanes_c)
-            for (int w : wpc_c) {
-                CHECK(md_set_launch(e, l, e->ncu * w, d));
-                e->step = step0;
-                CHECK(md_upload_state(e, P.data(), V.data()));
-                CHECK(md_run(e, std::min(nsteps, 5), 0, nullptr));   // warm up (graph capture)
-                HIPCHECK(hipEventRecord(t0, e->st));
-                CHECK(md_run(e, nsteps, 0, nullptr));
-                HIPCHECK(hipEventRecord(t1, e->st));
-                HIPCHECK(hipEventSynchronize(t1));
-                float ms = 0;
-                HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-                double per = ms / nsteps;
-                if (per < best) { best = per; bl = l; bg = e->ncu * w; bd = d; }
+    HIPCHECK(hipMemcpy(th.data(), e->th.p, e->R * sizeof(ThState), hipMemcpyDeviceToHost));
+    auto restore = [&]() -> int {
+        e->step = step0;
+        CHECK(md_upload_state(e, P.data(), V.data()));
+        HIPCHECK(hipMemcpy(e->th.p, th.data(), e->R * sizeof(ThState), hipMemcpyHostToDevice));
+        return MD_OK;
+    };
+    md_launch_config best;
+    CHECK(md_get_launch(e, &best));
+    // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
+    double best_build = 1e30;
+    for (int d = 1; d <= 3; d++)
+        for (int bl : {8, 16}) {
+            md_launch_config c = best;
+            c.cell_div = d;
+            c.build_lanes = bl;
+            CHECK(md_set_launch(e, &c));
+            CHECK(restore());
+            HIPCHECK(hipEventRecord(t0, e->st));
+            for (int k = 0; k < 3; k++) {
+                HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
+                CHECK(enqueue_rebuild(e, nullptr));
             }
-    CHECK(md_set_launch(e, bl, bg, bd));
-    e->step = step0;
-    CHECK(md_upload_state(e, P.data(), V.data()));
+            HIPCHECK(hipEventRecord(t1, e->st));
+            HIPCHECK(hipEventSynchronize(t1));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+            if (ms < best_build) { best_build = ms; best.cell_div = d; best.build_lanes = bl; }
+        }
+    // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events
+    double best_step = 1e30;
+    const int ks = std::max(8, std::min(nsteps, 64));
+    for (int l : {4, 8, 16, 32})
+        for (int w : {2, 4, 8}) {
+            md_launch_config c = best;
+            c.lanes = l;
+            c.grid = e->ncu * w;
+            CHECK(md_set_launch(e, &c));
+            CHECK(restore());
+            CHECK(md_timing_enable(e, 1));
+            CHECK(md_run(e, ks, 0, nullptr));
+            const double per = e->t_step_ms / std::max(1LL, e->n_step);
+            CHECK(md_timing_enable(e, 0));
+            if (per < best_step) { best_step = per; best.lanes = l; best.grid = c.grid; }
+        }
+    // stage 3, whole runs: rebuild-check interval x submission mode
+    double best_run = 1e30;
+    const int kr = std::max(nsteps, 2 * kSegment);
+    for (int sub : {0, 1})
+        for (int ce : {2, 4, 8, 16}) {
+            md_launch_config c = best;
+            c.check_every = ce;
+            c.submit = sub;
+            CHECK(md_set_launch(e, &c));
+            CHECK(restore());
+            CHECK(md_run(e, std::min(kr, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
+            HIPCHECK(hipEventRecord(t0, e->st));
+            CHECK(md_run(e, kr, 0, nullptr));
+            HIPCHECK(hipEventRecord(t1, e->st));
+            HIPCHECK(hipEventSynchronize(t1));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+            const double per = ms / kr;
+            if (per < best_run) { best_run = per; best.check_every = ce; best.submit = sub; }
+        }
+    CHECK(md_set_launch(e, &best));
+    CHECK(restore());
     e->timing = saved_timing;
-    if (best_ms) *best_ms = best;
+    e->t_step_ms = saved_t[0];
+    e->t_reb_ms = saved_t[1];
+    e->n_step = saved_n[0];
+    e->n_reb = saved_n[1];
+    if (best_ms) *best_ms = best_run;
     return MD_OK;
 }
 
@@ -1059,7 +1216,7 @@ int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* t
     CHECK(check_ready(e, false));
     if (nframes < 1 || !pos || !target_force) return fail(MD_E_INVALID, "bad frames");
     if (e->fm_nframes == 0 && e->R != 1) return fail(MD_E_STATE, "force matching needs a single-replica system");
-    const int n = e->fm_nframes ? e->natom[0] : e->N;
+    const int n = e->natom[0];
     // frames become replicas of the base system (replica 0)
     std::vector<int> na(nframes, n), ty((size_t)nframes * n);
     std::vector<float> bx(3 * (size_t)nframes), t0(nframes, e->T0[0]), q((size_t)nframes * n), m((size_t)nframes * n);
@@ -1098,18 +1255,17 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     const int ntk = nt == 1 ? 1 : (nt == 2 ? 2 : 4);
     HIPCHECK(hipSetDevice(e->dev));
     CHECK(upload_ljtab(e, lj1, lj2));
-    const int p = (int)(e->step & 1);
-    StepArgs a = step_args(e, e->pos[p].p, e->pos[p].p, MODE_FORCE, nullptr, nullptr, -1);
+    StepArgs a = step_args(e, MODE_FORCE, 0);
+    a.stats = nullptr;
     a.qq_scale = qscale * qscale;
     const int coul = e->coul_kernel;
-    HIPCHECK(md_launch_step(a, e->lanes, coul, 0, e->grid_blocks(), e->st));
+    HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st));
     HIPCHECK(hipMemsetAsync(e->fm_acc.p, 0, e->fm_acc.bytes(), e->st));
     a.qq_scale = qscale;
     HIPCHECK(md_launch_fm_adjoint(a, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
     std::vector<double> g(2 + 2 * ntk * ntk);
     HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
     CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
-    HIPCHECK(hipStreamSynchronize(e->st));
     const double M = 3.0 * (double)e->N;
     if (loss) *loss = g[0] / M;
     if (g_qscale) *g_qscale = 2.0 / M * g[1];

@@ -35,12 +35,13 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* ga
         const int slot0 = u * APB;
         const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
         const RepParam& p = a.rep[r];
+        const float4* __restrict__ pin = a.pos[cur_buf(a.st[r])];
         const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2];
         const float Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
         const int slot = slot0 + wave * APW + grp;
         const bool active = a.ids[slot] >= 0;
         if (!active) continue;
-        const float4 xi = a.pos_in[slot];
+        const float4 xi = pin[slot];
         const int ti = a.typ[slot];
         const int nn = min(a.ncount[slot], a.cap);
         const float4 F = a.force[slot], T = a.target[slot];
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* ga
             const uint32_t e = row[k];
             const int j = (int)(e & MD_JMASK);
             const int tj = (int)(e >> MD_JBITS);
-            const float4 pj = a.pos_in[j];
+            const float4 pj = pin[j];
             const float dx = minimg(xi.x - pj.x, L0, Li0);
             const float dy = minimg(xi.y - pj.y, L1, Li1);
             const float dz = minimg(xi.z - pj.z, L2, Li2);

@@ -10,6 +10,11 @@
 //   typ[S]      int    atom type
 //   nlist[S*cap] uint32 (j_slot | type_j << 29), row-major per atom, ncount[S]
 // Replicas occupy 64-aligned slot ranges, so a wavefront never straddles two.
+//
+// Control: every replica carries a RepState on the device.  Kernels decide
+// per replica whether they run (list valid, phase, step target), so a graph
+// segment can be replayed without host decisions; the host only checks the
+// step counters when the queued work should be done.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,6 +41,36 @@ struct RepParam {
     double kT0, Q;     // Nose-Hoover: kB T0, N_f kB T0 tau^2
 };
 
+// Per-replica control state (device).
+struct RepState {
+    long long step;    // steps completed (absolute)
+    long long target;  // step count at which the current run stops
+    long long start;   // first step of the current run (statistics index base)
+    int phase;         // velocity Verlet: 0 = between steps, 1 = kicked/drifted, forces pending
+    int nb;            // completed neighbor-list builds
+    int trip[2];       // trip[nb & 1] != 0: the list is stale for the current positions
+    int trip_new;      // raised (atomically) during a position update
+    int units_done;    // ticket of the step kernel
+    int tiles_done;    // ticket of the kick kernel
+    int build_done;    // ticket of the list build
+};
+
+// Device-wide counters that let a launch with nothing to do exit on its first
+// load.  They only gate early exits: every kernel still checks each replica.
+struct Gate {
+    int n_stale;       // replicas whose list is stale
+    int n_run;         // leapfrog: replicas that can step; velocity Verlet: replicas that can kick
+    int n_force;       // velocity Verlet: replicas whose force half is pending (list valid)
+    int pad;
+};
+
+// current position buffer of a replica
+__host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step + s.phase) & 1); }
+
+// the list of the current epoch is stale (a select, not an indexed load, so a
+// register copy of the state never has to live in scratch)
+__host__ __device__ inline bool stale(const RepState& s) { return ((s.nb & 1) ? s.trip[1] : s.trip[0]) != 0; }
+
 // Thermostat state per replica (FP64).  S is the velocity scale the next
 // position-writing kernel applies; *_end keep the state after the end-of-step
 // half so a run can stop there.
@@ -55,8 +90,7 @@ struct StatRec {
 };
 
 struct StepArgs {
-    const float4* pos_in;
-    float4* pos_out;
+    float4* pos[2];
     float4* vel;
     float4* force;
     const int* ids;
@@ -66,21 +100,19 @@ struct StepArgs {
     const float4* pos_build;
     const int* tile_rep;
     const RepParam* rep;
+    RepState* st;
+    Gate* gate;
     ThState* th;
     double* acc;          // [R][MD_NSTAT]
-    int* units_done;      // [R]
-    int* flag_set;        // [R] rebuild flag for the positions this kernel writes
-    int* flag_clear;      // [R] flag consumed before this kernel
     int* err;             // [1] non-finite coordinates
-    StatRec* stats;       // [nrec][R] or null
-    const float2* ljtab;  // [ntype*ntype] (lj1, lj2)
+    StatRec* stats;       // [stat_cap][R] or null
+    const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)
     const int* slot_of;   // [N] original index -> slot
     const uint32_t* bond_list;  // [N*4] (partner | type << 29)
     const int* nbond;           // [N]
     const int4* ang_list;       // [N*4] (role | type << 8, a, b, 0)
     const int* nang;            // [N]
-    // force-matching residual pass (null otherwise)
-    const float4* target;       // [S] target forces in slot order
+    const float4* target;       // [S] target forces in slot order (force matching)
     int cap;
     int ntype;
     int nrep;
@@ -88,7 +120,9 @@ struct StepArgs {
     int mode;             // MODE_*
     int thermostat;
     int bonded;
-    int stat_index;       // -1: do not write stats
+    int energy_every;     // energies at steps with step % energy_every == 0 (0: never)
+    int force_energy;     // MODE_FORCE: compute energies
+    int stat_cap;         // records per replica in stats
     float rc2, qfac, palfa, spiialfa, alfasq, wolf, reacf;
     float ecut;           // Coulomb energy shift: Wolf E(rc)/rc, RF c_rf, else 0
     float log2rc;         // log2(rc) for the Wolf energy's ln(r/rc) term
@@ -100,48 +134,41 @@ struct StepArgs {
 
 // Streaming kernel of velocity Verlet (begin of step): v = S v + F dt/2m; x += v dt
 struct KickArgs {
-    const float4* pos_in;
-    float4* pos_out;
+    float4* pos[2];
     float4* vel;
     const float4* force;
     const int* ids;
     const float4* pos_build;
     const int* tile_rep;
     const RepParam* rep;
+    RepState* st;
+    Gate* gate;
     const ThState* th;
-    int* flag_set;
-    int* flag_clear;
     int* err;
     int nslot;
     int nrep;
     float dt, skin2q;
-    int apply_drift;      // 0: finalize (v *= S_end only)
+    int apply_drift;      // 0: run end (v *= S_end only)
 };
 
 struct BuildArgs {
-    // old order (read by bin / scatter / permute)
-    const float4* pos;
-    const float4* vel;
-    const float4* force;
-    const int* ids;
-    const int* typ;
+    float4* pos[2];       // current buffer per replica: pos[cur_buf(st[r])]
+    float4* vel;
+    float4* force;
+    int* ids;
+    int* typ;
     // scratch, new order
     float4* pos_s;
     float4* vel_s;
     float4* force_s;
     int* ids_s;
     int* typ_s;
-    // copy-back targets (new order)
-    float4* pos_dst;
-    float4* vel_dst;
-    float4* force_dst;
-    int* ids_dst;
-    int* typ_dst;
     float4* pos_build;
     // cells
     int* key;             // [S] global cell index of each (old) slot
     int* rank;            // [S] arrival rank in its cell
     int* tmp;             // [S] new slot -> old slot (cell order, arrival order)
+    int* tmpid;           // [S] new slot -> original index
     int* cell_count;      // [C]
     int* cell_start;      // [C + R]
     const int* cell_rank; // [C] linear cell -> curve rank (per replica block)
@@ -150,7 +177,8 @@ struct BuildArgs {
     int* ncount;
     int* maxcount;        // [R]
     const int4* excl;     // [N] 1-2 partners (-1 empty)
-    const int* flag;      // [R]
+    RepState* st;
+    Gate* gate;
     unsigned long long* rebuild_count;  // replicas rebuilt (counter)
     const int* tile_rep;
     const RepParam* rep;
@@ -159,4 +187,5 @@ struct BuildArgs {
     int cap;
     float rl2;
     int has_excl;
+    int build_lanes;      // lanes per atom in k_nl_build (4, 8, 16)
 };

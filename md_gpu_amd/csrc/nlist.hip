@@ -4,23 +4,31 @@
 // which never compiled; the list semantics are those of the O(N^2) build it
 // replaces, src/neighborlist.cu:231-297).
 //
-// A rebuild is five kernels, each gated per replica by the rebuild flag that
-// the previous position update raised (skin criterion, src/neighborlist.cu:157-176):
+// A rebuild is five kernels.  Each runs only for replicas whose list is stale
+// (RepState.trip, raised by the position update when some atom moved by at
+// least skin/2 since the last build: the skin criterion of
+// src/neighborlist.cu:157-176), so the chain can sit in every graph segment:
 //   k_nl_bin      cell of every atom (cells are ranked along a Hilbert curve),
 //                 arrival rank in the cell by atomic count
 //   k_nl_scan     exclusive scan of cell counts -> cell_start (one block per replica)
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
 //   k_nl_permute  deterministic order inside a cell (by original index);
 //                 gathers pos/vel/force/id/type into scratch in the new order
-//   k_nl_build    one wavefront per 64-slot tile: tile bounding box, walk the
-//                 image cells within r_list of it, test every member with the
-//                 canonical FP32 predicate (wave-uniform j -> scalar loads),
-//                 append (j | type_j << 29); copy the tile's scratch back.
+//   k_nl_build    one workgroup per 64-slot tile, LPA (4, 8, 16) lanes per atom:
+//                 the image cells within r_list of the tile's bounding box are
+//                 listed in LDS, their members streamed through LDS in
+//                 double-buffered chunks of 64*LPA, each tested with the
+//                 canonical FP32 predicate; hits are appended in candidate
+//                 order (ballot + popcount inside the atom's LPA lanes), so
+//                 rows are deterministic.  The last workgroup of a replica bumps its
+//                 build epoch, which marks the list valid.
 // Bytes per atom per rebuild (algorithmic): read pos/vel/force/id/type 56 B,
 // write them twice (scratch + copy back) 112 B, plus the list 4*K B.
 #include "md_device.h"
 
 namespace {
+
+constexpr int kTab = 1024;    // region cells per table batch
 
 __device__ __forceinline__ int cell_coord(float x, float halfL, float cinv, int nc) {
     int k = (int)floorf((x + halfL) * cinv);
@@ -32,17 +40,72 @@ __device__ __forceinline__ int floordiv(int a, int b) {  // b > 0
     return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
 }
 
+// One axis of the image-cell region around a tile's bounding box [bmin, bmax].
+struct Axis {
+    int lo, nr, nc;
+    float cedge, halfL, bmin, bmax;
+    // image index k, wrapped cell w of region coordinate c; returns the gap between
+    // the cell slab and the box along this axis (0 when they overlap)
+    __device__ float gap(int c, int& k, int& w) const {
+        k = floordiv(c, nc);
+        w = c - k * nc;
+        const float cl = c * cedge - halfL, ch = cl + cedge;
+        return fmaxf(0.0f, fmaxf(cl - bmax, bmin - ch));
+    }
+};
+
+__device__ __forceinline__ Axis make_axis(float bmin, float bmax, float rlp, float halfL, float cinv,
+                                          float cedge, int nc) {
+    Axis x;
+    const int l = max((int)floorf((bmin + halfL - rlp) * cinv), -2 * nc);
+    const int h = min((int)floorf((bmax + halfL + rlp) * cinv), 3 * nc);
+    x.lo = l;
+    x.nr = max(h - l + 1, 0);
+    x.nc = nc;
+    x.cedge = cedge;
+    x.halfL = halfL;
+    x.bmin = bmin;
+    x.bmax = bmax;
+    return x;
+}
+
+// exclusive scan over an NT-thread block; returns this thread's offset, sets total
+template <int NT>
+__device__ int block_scan(int v, int* s_w, int& total) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    int off = 0;
+    total = 0;
+    for (int k = 0; k < NW; k++) {
+        const int t = s_w[k];
+        if (k < w) off += t;
+        total += t;
+    }
+    __syncthreads();
+    return off + inc - v;
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
+    if (a.gate->n_stale == 0) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
     int r = a.tile_rep[s >> 6];
-    if (!a.flag[r]) return;
+    const RepState st = a.st[r];
+    if (!stale(st)) return;
+    const RepParam& p = a.rep[r];
+    if (s == p.slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
     int id = a.ids[s];
     if (id < 0) return;
-    const RepParam& p = a.rep[r];
-    float4 x = a.pos[s];
+    float4 x = a.pos[cur_buf(st)][s];
     int cx = cell_coord(x.x, p.halfL[0], p.cinv[0], p.nc[0]);
     int cy = cell_coord(x.y, p.halfL[1], p.cinv[1], p.nc[1]);
     int cz = cell_coord(x.z, p.halfL[2], p.cinv[2], p.nc[2]);
@@ -54,15 +117,15 @@ __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
 
 __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
     __shared__ int wsum[16];
+    if (a.gate->n_stale == 0) return;
     const int r = blockIdx.x;
-    if (!a.flag[r]) return;
+    if (!stale(a.st[r])) return;
     const RepParam& p = a.rep[r];
     const int nc = p.nc[0] * p.nc[1] * p.nc[2];
     const int per = (nc + 1023) / 1024;
     const int b = min(nc, (int)threadIdx.x * per), e = min(nc, b + per);
     int sum = 0;
     for (int k = b; k < e; k++) sum += a.cell_count[p.cell_off + k];
-    // block exclusive scan of per-thread sums
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = sum;
     for (int o = 1; o < 64; o <<= 1) {
@@ -89,34 +152,37 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a, int* tmpid) {
+__global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
+    if (a.gate->n_stale == 0) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
     int r = a.tile_rep[s >> 6];
-    if (!a.flag[r]) return;
+    if (!stale(a.st[r])) return;
     int id = a.ids[s];
     if (id < 0) return;
     int gc = a.key[s];
     int t = a.cell_start[gc + r] + a.rank[s];
     a.tmp[t] = s;
-    tmpid[t] = id;
+    a.tmpid[t] = id;
 }
 
-__global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a, const int* tmpid) {
+__global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
+    if (a.gate->n_stale == 0) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nslot) return;
     int r = a.tile_rep[t >> 6];
-    if (!a.flag[r]) return;
+    const RepState st = a.st[r];
+    if (!stale(st)) return;
     const RepParam& p = a.rep[r];
     if (t >= p.slot_off + p.natom) return;
     int o = a.tmp[t];
-    int myid = tmpid[t];
+    int myid = a.tmpid[t];
     int gc = a.key[o];
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
-    for (int u = cs; u < ce; u++) cnt += (tmpid[u] < myid);
+    for (int u = cs; u < ce; u++) cnt += (a.tmpid[u] < myid);
     int f = cs + cnt;
-    a.pos_s[f] = a.pos[o];
+    a.pos_s[f] = a.pos[cur_buf(st)][o];
     a.vel_s[f] = a.vel[o];
     a.force_s[f] = a.force[o];
     a.ids_s[f] = myid;
@@ -124,113 +190,232 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a, const int* tmpi
     a.slot_of[myid] = f;
 }
 
-template <bool EXCL>
-__global__ __launch_bounds__(256) void k_nl_build(BuildArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile * 64 >= a.nslot) return;
+// One workgroup per 64-slot tile, LPA lanes per atom (LPA * 64 threads).
+template <bool EXCL, int LPA>
+__global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
+    constexpr int NT = 64 * LPA;          // threads = candidates staged per chunk
+    constexpr int NW = NT / 64;           // waves
+    constexpr int APW = 64 / LPA;         // atoms per wave
+    constexpr int PER = kTab / NT;        // region cells probed per thread and batch
+    __shared__ float4 s_pos[2][NT];       // (xj, yj, zj, bits of j | tj << 29)
+    __shared__ float4 s_sh[2][NT];        // (-kx L, -ky L, -kz L, bits of jid): image shift of the candidate
+    __shared__ int s_tstart[kTab];
+    __shared__ int s_tpref[kTab + 1];
+    __shared__ int s_tshift[kTab];
+    __shared__ int s_w[NW];
+    __shared__ float s_bb[NW][6];
+    if (a.gate->n_stale == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tile = blockIdx.x;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
-    if (!a.flag[r]) return;
+    const RepState st = a.st[r];
+    if (!stale(st)) return;
     const RepParam& p = a.rep[r];
-    const int slot = tile * 64 + lane;
+    const int cb = cur_buf(st);
+    const int ai = wave * APW + lane / LPA;     // atom of this lane within the tile
+    const int sub = lane % LPA;
+    const int slot = tile * 64 + ai;
     const int id = a.ids_s[slot];
     const bool active = id >= 0;
     const float4 xi = a.pos_s[slot];
-
-    // tile bounding box over active lanes
-    float bmin[3], bmax[3];
-    float xv[3] = {xi.x, xi.y, xi.z};
-    for (int d = 0; d < 3; d++) {
-        bmin[d] = active ? xv[d] : 3.0e38f;
-        bmax[d] = active ? xv[d] : -3.0e38f;
-    }
-    for (int o = 32; o >= 1; o >>= 1)
-        for (int d = 0; d < 3; d++) {
-            bmin[d] = fminf(bmin[d], __shfl_xor(bmin[d], o, 64));
-            bmax[d] = fmaxf(bmax[d], __shfl_xor(bmax[d], o, 64));
-        }
-    int count = 0;
-    const bool any = bmin[0] <= bmax[0];
     int4 ex = make_int4(-1, -1, -1, -1);
     if (EXCL && active) ex = a.excl[id];
-    if (any) {
+
+    // tile bounding box over active atoms
+    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
+    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
+    for (int o = 32; o >= LPA; o >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
+        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+    }
+    if (lane == 0) {
+        s_bb[wave][0] = lx; s_bb[wave][1] = ly; s_bb[wave][2] = lz;
+        s_bb[wave][3] = hx; s_bb[wave][4] = hy; s_bb[wave][5] = hz;
+    }
+    __syncthreads();
+    for (int w = 0; w < NW; w++) {
+        lx = fminf(lx, s_bb[w][0]); ly = fminf(ly, s_bb[w][1]); lz = fminf(lz, s_bb[w][2]);
+        hx = fmaxf(hx, s_bb[w][3]); hy = fmaxf(hy, s_bb[w][4]); hz = fmaxf(hz, s_bb[w][5]);
+    }
+    int cnt = 0;
+    uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
+    if (lx <= hx) {
         const float rlp = sqrtf(a.rl2) * 1.0001f + 1e-4f;
         const float rlp2 = rlp * rlp;
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; d++) {
-            lo[d] = (int)floorf((bmin[d] + p.halfL[d] - rlp) * p.cinv[d]);
-            hi[d] = (int)floorf((bmax[d] + p.halfL[d] + rlp) * p.cinv[d]);
-            // coordinates are wrapped into the box, so the span never needs more than
-            // two box lengths of cells; the bound keeps the walk finite for any input
-            lo[d] = max(lo[d], -2 * p.nc[d]);
-            hi[d] = min(hi[d], 3 * p.nc[d]);
-        }
-        const size_t row = (size_t)slot * a.cap;
-        for (int cz = lo[2]; cz <= hi[2]; cz++) {
-            const int kz = floordiv(cz, p.nc[2]), wz = cz - kz * p.nc[2];
-            const float zl = cz * p.cedge[2] - p.halfL[2], zh = zl + p.cedge[2];
-            const float dz0 = fmaxf(0.0f, fmaxf(zl - bmax[2], bmin[2] - zh));
-            if (dz0 * dz0 >= rlp2) continue;
-            for (int cy = lo[1]; cy <= hi[1]; cy++) {
-                const int ky = floordiv(cy, p.nc[1]), wy = cy - ky * p.nc[1];
-                const float yl = cy * p.cedge[1] - p.halfL[1], yh = yl + p.cedge[1];
-                const float dy0 = fmaxf(0.0f, fmaxf(yl - bmax[1], bmin[1] - yh));
-                if (dz0 * dz0 + dy0 * dy0 >= rlp2) continue;
-                for (int cx = lo[0]; cx <= hi[0]; cx++) {
-                    const int kx = floordiv(cx, p.nc[0]), wx = cx - kx * p.nc[0];
-                    const float xl = cx * p.cedge[0] - p.halfL[0], xh = xl + p.cedge[0];
-                    const float dx0 = fmaxf(0.0f, fmaxf(xl - bmax[0], bmin[0] - xh));
-                    if (dz0 * dz0 + dy0 * dy0 + dx0 * dx0 >= rlp2) continue;
-                    const int lin = wx + p.nc[0] * (wy + p.nc[1] * wz);
-                    const int rank = a.cell_rank[p.cell_off + lin];
-                    const int cs = __builtin_amdgcn_readfirstlane(a.cell_start[p.cs_off + rank]);
-                    const int ce = __builtin_amdgcn_readfirstlane(a.cell_start[p.cs_off + rank + 1]);
-                    // image shift of this cell copy: displacement = (xi - xj) - k L (canonical form)
-                    const float skx = (float)kx, sky = (float)ky, skz = (float)kz;
-                    for (int j = cs; j < ce; j++) {
-                        const float4 pj = a.pos_s[j];
-                        const int jid = a.ids_s[j];
-                        const int tj = a.typ_s[j];
-                        const float dx = fmaf(-p.L[0], skx, xi.x - pj.x);
-                        const float dy = fmaf(-p.L[1], sky, xi.y - pj.y);
-                        const float dz = fmaf(-p.L[2], skz, xi.z - pj.z);
+        // image-cell region covering the tile box grown by r_list.  Coordinates lie in
+        // the box, so a span of more than five box lengths never occurs; the bounds
+        // keep the walk finite for any input.
+        const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
+        const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
+        const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
+        const int nreg = ax.nr * ay.nr * az.nr;
+        const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
+        // region cell c -> (cell start, member count, packed image shift); count 0 when pruned or empty
+        auto probe = [=](int c, int& cs, int& shift) -> int {
+            int kx, ky, kz, wx, wy, wz;
+            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
+            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
+            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
+            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
+            cs = a.cell_start[cs_off + rank];
+            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
+            return a.cell_start[cs_off + rank + 1] - cs;
+        };
+        for (int b0 = 0; b0 < nreg; b0 += kTab) {
+            // ---- table of non-empty image cells within r_list of the box (region order);
+            //      pass 1 counts, pass 2 writes (no per-thread arrays)
+            const int c0 = b0 + tid * PER, c1 = min(min(c0 + PER, nreg), b0 + kTab);
+            int n_keep = 0, csum = 0;
+            for (int c = c0; c < c1; c++) {
+                int cs, sh;
+                const int n = probe(c, cs, sh);
+                n_keep += n > 0;
+                csum += n;
+            }
+            int ntab, T;
+            const int off = block_scan<NT>(n_keep, s_w, ntab);
+            const int coff = block_scan<NT>(csum, s_w, T);
+            for (int c = c0, k = off, run = coff; c < c1; c++) {
+                int cs, sh;
+                const int n = probe(c, cs, sh);
+                if (n > 0) {
+                    s_tstart[k] = cs;
+                    s_tpref[k] = run;
+                    s_tshift[k] = sh;
+                    k++;
+                    run += n;
+                }
+            }
+            if (tid == 0) s_tpref[ntab] = T;
+            __syncthreads();
+            // ---- stream the candidates through LDS
+            auto fetch = [&](int base, float4& rp, float4& rs) {
+                const int q = base + tid;
+                if (q < T) {
+                    int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
+                    while (lo_ < hi_) {
+                        const int mid = (lo_ + hi_ + 1) >> 1;
+                        if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
+                    }
+                    const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
+                    const int sh = s_tshift[lo_];
+                    rp = a.pos_s[j];
+                    rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
+                    // -k L is exact for the |k| <= 1 images that can pass the predicate
+                    rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
+                                     -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
+                }
+            };
+            const int nch = (T + NT - 1) / NT;
+            float4 rp = make_float4(0, 0, 0, 0), rs = make_float4(0, 0, 0, 0);
+            if (nch > 0) {
+                fetch(0, rp, rs);
+                s_pos[0][tid] = rp;
+                s_sh[0][tid] = rs;
+            }
+            __syncthreads();
+            for (int ch = 0; ch < nch; ch++) {
+                const int buf = ch & 1;
+                const bool more = ch + 1 < nch;
+                if (more) fetch((ch + 1) * NT, rp, rs);
+                const int n = min(NT, T - ch * NT);
+                for (int u = 0; u < n; u += LPA) {
+                    const int q = u + sub;
+                    bool ok = false;
+                    uint32_t ent = 0;
+                    if (q < n) {
+                        const float4 pj = s_pos[buf][q];
+                        const float4 sj = s_sh[buf][q];
+                        // (xi - xj) - k L: the canonical minimum-image displacement for this image
+                        const float dx = (xi.x - pj.x) + sj.x;
+                        const float dy = (xi.y - pj.y) + sj.y;
+                        const float dz = (xi.z - pj.z) + sj.z;
                         const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        bool ok = active && rsq < a.rl2 && j != slot;
-                        if (EXCL) ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
-                        if (ok) {
-                            if (count < a.cap) a.nlist[row + count] = (uint32_t)j | ((uint32_t)tj << MD_JBITS);
-                            count++;
+                        ent = (uint32_t)__float_as_int(pj.w);
+                        ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
+                        if (EXCL) {
+                            const int jid = __float_as_int(sj.w);
+                            ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
                         }
                     }
+                    // append in candidate order: rank among this atom's LPA lanes
+                    const unsigned long long m = __ballot(ok);
+                    const uint32_t gb = (uint32_t)(m >> (lane & ~(LPA - 1))) & (uint32_t)((1ull << LPA) - 1);
+                    const int before = __popc(gb & ((1u << sub) - 1u));
+                    if (ok && cnt + before < a.cap) rowp[cnt + before] = ent;
+                    cnt += __popc(gb);
                 }
+                __syncthreads();
+                if (more) {
+                    s_pos[buf ^ 1][tid] = rp;
+                    s_sh[buf ^ 1][tid] = rs;
+                }
+                __syncthreads();
             }
         }
     }
-    a.ncount[slot] = active ? count : 0;
-    int m = count;
+    if (sub == 0) a.ncount[slot] = active ? cnt : 0;
+    int m = sub == 0 ? cnt : 0;
     for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
     if (lane == 0) atomicMax(&a.maxcount[r], m);
     // copy the tile back from scratch into the primary arrays (new order)
-    a.pos_dst[slot] = xi;
-    a.pos_build[slot] = xi;
-    a.vel_dst[slot] = a.vel_s[slot];
-    a.force_dst[slot] = a.force_s[slot];
-    a.ids_dst[slot] = id;
-    a.typ_dst[slot] = a.typ_s[slot];
+    if (tid < 64) {
+        const int s = tile * 64 + tid;
+        const float4 x = a.pos_s[s];
+        a.pos[cb][s] = x;
+        a.pos_build[s] = x;
+        a.vel[s] = a.vel_s[s];
+        a.force[s] = a.force_s[s];
+        a.ids[s] = a.ids_s[s];
+        a.typ[s] = a.typ_s[s];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // the last tile of the replica opens the new list epoch
+        const int old = atomicAdd(&a.st[r].build_done, 1);
+        if (old == p.nslot / 64 - 1) {
+            a.st[r].build_done = 0;
+            a.st[r].nb = st.nb + 1;
+            atomicSub(&a.gate->n_stale, 1);
+            if (st.phase == 0 && st.step < st.target) atomicAdd(&a.gate->n_run, 1);
+            if (st.phase == 1) atomicAdd(&a.gate->n_force, 1);
+        }
+    }
+}
+
+// Mark the lists of all replicas stale (host-requested full rebuild).
+__global__ void k_force_trip(RepState* st, Gate* g, int nrep) {
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) st[r].trip[st[r].nb & 1] = 1;
+    if (threadIdx.x == 0) {
+        g->n_stale = nrep;
+        g->n_run = 0;
+        g->n_force = 0;
+    }
 }
 
 // ---------------------------------------------------------------- launchers
 
-extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int* tmpid, hipStream_t st) {
+extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
     const int nb = (a.nslot + 255) / 256;
     hipLaunchKernelGGL(k_nl_bin, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_scan, dim3(a.nrep), dim3(1024), 0, st, a);
-    hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a, tmpid);
-    hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a, (const int*)tmpid);
-    const int ntiles = a.nslot / 64;
-    if (a.has_excl)
-        hipLaunchKernelGGL(k_nl_build<true>, dim3((ntiles + 3) / 4), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL(k_nl_build<false>, dim3((ntiles + 3) / 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
+    const dim3 ntiles(a.nslot / 64);
+    switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
+    case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
+    case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;
+    case 16:  hipLaunchKernelGGL((k_nl_build<false, 16>), ntiles, dim3(1024), 0, st, a); break;
+    case 104: hipLaunchKernelGGL((k_nl_build<true, 4>), ntiles, dim3(256), 0, st, a); break;
+    case 108: hipLaunchKernelGGL((k_nl_build<true, 8>), ntiles, dim3(512), 0, st, a); break;
+    case 116: hipLaunchKernelGGL((k_nl_build<true, 16>), ntiles, dim3(1024), 0, st, a); break;
+    default:  return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st) {
+    hipLaunchKernelGGL(k_force_trip, dim3(1), dim3(256), 0, st, s, g, nrep);
     return hipGetLastError();
 }

@@ -19,7 +19,10 @@
 // are reduced across the G lanes with cross-lane shuffles; KE / energies /
 // virial are reduced per workgroup and added with FP64 atomics; the last
 // workgroup to finish a replica (atomic ticket) computes that replica's
-// temperature, thermostat update and statistics record.
+// temperature, thermostat update and statistics record, folds the skin-check
+// flag into the list state and advances the replica's step counter.
+// A replica is skipped when its list is stale or its run target is reached,
+// so a captured segment can be replayed with no host decision.
 //
 // Algorithmic work per ordered pair (FMA = 2, add/mul/rsqrt/rcp/exp2/rint = 1):
 // LJ + Wolf 65 flop, LJ + RF 45, LJ 36, Wolf only 60 (src/pair_forces.cu:151-200).
@@ -35,6 +38,7 @@ __device__ __forceinline__ double atomic_load_f64(double* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+
 // MTK chain-of-one half step: returns the velocity scale, updates xi, vxi, K.
 __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double kT0, double Q, double dt) {
     double G = (2.0 * K - nf * kT0) / Q;
@@ -47,8 +51,10 @@ __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double 
     return s;
 }
 
-__device__ void finalize_replica(const StepArgs& a, int r) {
+// Runs once per replica and kernel launch, by thread 0 of the last workgroup.
+__device__ void finalize_replica(const StepArgs& a, int r, long long kstat, bool en) {
     const RepParam& p = a.rep[r];
+    RepState& st = a.st[r];
     double* acc = a.acc + (size_t)r * MD_NSTAT;
     double v[MD_NSTAT];
     for (int k = 0; k < MD_NSTAT; k++) v[k] = atomic_load_f64(acc + k);
@@ -57,7 +63,7 @@ __device__ void finalize_replica(const StepArgs& a, int r) {
     rec.ke = rec.temp = rec.pe_lj = rec.pe_coul = rec.pe_bond = rec.pe_angle = rec.virial = rec.conserved = 0.0;
     rec.xi = th.xi;
     rec.vxi = th.vxi;
-    rec.has_energy = 0;
+    rec.has_energy = en ? 1 : 0;
     rec.pad = 0;
     const double dt = a.dt, tau = a.tau;
     double nh_terms = 0.0;
@@ -93,143 +99,192 @@ __device__ void finalize_replica(const StepArgs& a, int r) {
         th.vxi = vxi;
         th.S = s1 * s2;
     }
-    if (a.stat_index >= 0 && a.stats) {
+    if (a.stats && kstat >= 0 && kstat < a.stat_cap) {
         rec.pe_lj = 0.5 * v[ACC_LJ];
         rec.pe_coul = 0.5 * v[ACC_COUL];
         rec.pe_bond = v[ACC_BOND];
         rec.pe_angle = v[ACC_ANGLE];
         rec.virial = 0.5 * v[ACC_VIR];
-        rec.has_energy = v[MD_NSTAT - 1] > 0.0 ? 1 : 0;
         rec.conserved = rec.ke + rec.pe_lj + rec.pe_coul + rec.pe_bond + rec.pe_angle + nh_terms;
-        a.stats[(size_t)a.stat_index * a.nrep + r] = rec;
+        a.stats[(size_t)kstat * a.nrep + r] = rec;
     }
     for (int k = 0; k < MD_NSTAT; k++) __hip_atomic_exchange(acc + k, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_exchange(a.units_done + r, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_exchange(&st.units_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.mode == MODE_LF) {
+        // skin check of the positions just written: the list is stale for them
+        const int tn = __hip_atomic_exchange(&st.trip_new, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tn) st.trip[st.nb & 1] = 1;
+        st.step = st.step + 1;
+        if (tn) atomicAdd(&a.gate->n_stale, 1);
+        if (tn || st.step >= st.target) atomicSub(&a.gate->n_run, 1);
+    } else if (a.mode == MODE_VV) {
+        st.phase = 0;
+        st.step = st.step + 1;
+        atomicSub(&a.gate->n_force, 1);
+        if (st.step < st.target) atomicAdd(&a.gate->n_run, 1);
+    }
+}
+
+// Pair loop of one atom over its neighbor row (lanes sub, sub+G, ...).
+template <int G, int COUL, bool EN>
+__device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
+                                         const uint32_t* __restrict__ row, int nn, int sub, const float4 xi,
+                                         float qi, float L0, float L1, float L2, float Li0, float Li1, float Li2,
+                                         float& fx, float& fy, float& fz, double& elj, double& eco, double& evi) {
+    for (int k = sub; k < nn; k += G) {
+        const uint32_t e = row[k];
+        const int j = (int)(e & MD_JMASK);
+        const int tj = (int)(e >> MD_JBITS);
+        const float4 pj = pin[j];
+        const float4 lj = s_lj[tj];
+        const float dx = minimg(xi.x - pj.x, L0, Li0);
+        const float dy = minimg(xi.y - pj.y, L1, Li1);
+        const float dz = minimg(xi.z - pj.z, L2, Li2);
+        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ri = __builtin_amdgcn_rsqf(rsq);
+        float ff = 0.0f;
+        float ucoul = 0.0f;
+        if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
+            const float qq = qi * pj.w;
+            ff = qq * (ri - a.reacf * rsq);
+            if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
+        } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:178-183
+            const float qq = qi * pj.w;
+            const float r = rsq * ri;
+            const float t = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
+            const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+                                0.254829592f) * t;
+            const float ex = __builtin_amdgcn_exp2f(a.alfasq * rsq);
+            float c = fmaf(a.spiialfa, r, poly) * ex * ri;
+            if (COUL == 1) c -= a.wolf;
+            ff = c * qq;
+            if (EN) {
+                float u = poly * ex * ri - a.ecut;
+                if (COUL == 1) u += a.wolf * (__builtin_amdgcn_logf(r) - a.log2rc) * 0.69314718056f;
+                ucoul = qq * u;
+            }
+        }
+        const float ri2 = ri * ri;
+        ff *= ri2;
+        const float ri8 = (ri2 * ri2) * (ri2 * ri2);
+        ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
+        const bool inside = !(rsq > a.rc2);                   // src/pair_forces.cu:195
+        ff = inside ? ff : 0.0f;
+        fx = fmaf(ff, dx, fx);
+        fy = fmaf(ff, dy, fy);
+        fz = fmaf(ff, dz, fz);
+        if (EN && inside) {
+            const float ri6 = ri2 * ri2 * ri2;
+            const float ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
+            elj += (double)ulj;
+            eco += (double)ucoul;
+            evi += (double)(ff * rsq);
+        }
+    }
 }
 
 }  // namespace
 
-template <int G, int COUL, bool EN>
-__global__ __launch_bounds__(256) void k_step(const StepArgs a) {
+// ENMODE 0: no energies (compiled out); 1: energies where requested (runtime, per replica)
+template <int G, int COUL, int ENMODE>
+__global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_lj[64];
     __shared__ double s_red[4][MD_NSTAT];
     __shared__ int s_last;
+    if (a.mode == MODE_LF && a.gate->n_run == 0) return;     // nothing can step: idle launch
+    if (a.mode == MODE_VV && a.gate->n_force == 0) return;
     constexpr int APW = 64 / G;        // atoms per wavefront
     constexpr int APB = 4 * APW;       // atoms per unit (workgroup pass)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
-    s_lj[tid & 63] = reinterpret_cast<const float4*>(a.ljtab)[tid & 63];
-    if (blockIdx.x == 0 && a.flag_clear)
-        for (int r = tid; r < a.nrep; r += 256) a.flag_clear[r] = 0;
+    if (tid < 64) s_lj[tid] = a.ljtab[tid];
     __syncthreads();
 
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
     int cur_r = -1, nunits_r = 0;
+    bool run_r = false, en = false;
+    long long kstat = -1;
+    const float4* pin = nullptr;
+    float4* pout = nullptr;
     float L0 = 0, L1 = 0, L2 = 0, Li0 = 0, Li1 = 0, Li2 = 0, lam = 1.0f;
     double dke = 0.0, dlj = 0.0, dco = 0.0, dbo = 0.0, dan = 0.0, dvi = 0.0;
+    int trip = 0;
 
     auto flush = [&]() {
         // workgroup reduction of the FP64 partials, one atomic per quantity
-        double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, EN ? 1.0 : 0.0};
-        for (int k = 0; k < MD_NSTAT - 1; k++)
+        double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, 0.0};
+        for (int k = 0; k < MD_NSTAT - 2; k++)
             for (int o = 32; o >= 1; o >>= 1) vals[k] += __shfl_xor(vals[k], o, 64);
-        if (lane == 0)
+        const unsigned long long tb = __ballot(trip != 0);
+        if (lane == 0) {
             for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = vals[k];
+            s_red[wave][MD_NSTAT - 1] = tb ? 1.0 : 0.0;
+        }
         __syncthreads();
         if (tid == 0) {
             double* acc = a.acc + (size_t)cur_r * MD_NSTAT;
             double chk = 0.0;
-            for (int k = 0; k < MD_NSTAT; k++) {
+            for (int k = 0; k < MD_NSTAT - 2; k++) {
                 double t = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
-                if (k == MD_NSTAT - 1) t = EN ? 1.0 : 0.0;
-                if (t != 0.0) chk += atomicAdd(acc + k, t);   // returned value orders the ticket below
+                if (t != 0.0) chk += atomicAdd(acc + k, t);   // the returned value orders the ticket below
             }
-            int add = nunits_r + (chk != chk ? 1 : 0);
-            int old = atomicAdd(a.units_done + cur_r, add);
+            const bool tr = (s_red[0][MD_NSTAT - 1] + s_red[1][MD_NSTAT - 1] + s_red[2][MD_NSTAT - 1] +
+                             s_red[3][MD_NSTAT - 1]) != 0.0;
+            int tchk = tr ? atomicOr(&a.st[cur_r].trip_new, 1) : 0;
+            int add = nunits_r + (chk != chk ? 1 : 0) + (tchk < 0 ? 1 : 0);
+            int old = atomicAdd(&a.st[cur_r].units_done, add);
             const RepParam& p = a.rep[cur_r];
             s_last = (old + add == p.nslot / APB) ? 1 : 0;
         }
         __syncthreads();
-        if (s_last && tid == 0) finalize_replica(a, cur_r);
+        if (s_last && tid == 0) finalize_replica(a, cur_r, kstat, en);
         __syncthreads();
         dke = dlj = dco = dbo = dan = dvi = 0.0;
+        trip = 0;
     };
 
     for (int u = u0; u < u1; u++) {
         const int slot0 = u * APB;
         const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
         if (r != cur_r) {
-            if (cur_r >= 0) flush();
+            if (cur_r >= 0 && run_r) flush();
             cur_r = r;
             nunits_r = 0;
+            const RepState s = a.st[r];
+            if (a.mode == MODE_FORCE) run_r = !stale(s);
+            else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
+            else run_r = s.phase == 1 && !stale(s);
+            if (!run_r) continue;
+            en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
+                                                        : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
+            kstat = (a.mode == MODE_FORCE) ? 0 : s.step - s.start;
+            const int cb = cur_buf(s);
+            pin = a.pos[cb];
+            pout = a.pos[cb ^ 1];
             const RepParam& p = a.rep[r];
             L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
             Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
             lam = (a.mode == MODE_LF) ? (float)a.th[r].S : 1.0f;
         }
+        if (!run_r) continue;
         nunits_r++;
         const int slot = slot0 + wave * APW + grp;
         const int id = a.ids[slot];
         const bool active = id >= 0;
-        const float4 xi = a.pos_in[slot];
+        const float4 xi = pin[slot];
         const int ti = active ? a.typ[slot] : 0;
         const int nn = active ? min(a.ncount[slot], a.cap) : 0;
         const float qi = xi.w * a.qfac * a.qq_scale;
         float fx = 0.f, fy = 0.f, fz = 0.f;
         double elj = 0.0, eco = 0.0, evi = 0.0;
         const uint32_t* row = a.nlist + (size_t)slot * a.cap;
-        const float4* ljrow = s_lj + ti * 8;
-        for (int k = sub; k < nn; k += G) {
-            const uint32_t e = row[k];
-            const int j = (int)(e & MD_JMASK);
-            const int tj = (int)(e >> MD_JBITS);
-            const float4 pj = a.pos_in[j];
-            const float4 lj = ljrow[tj];
-            const float dx = minimg(xi.x - pj.x, L0, Li0);
-            const float dy = minimg(xi.y - pj.y, L1, Li1);
-            const float dz = minimg(xi.z - pj.z, L2, Li2);
-            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            float ri = __builtin_amdgcn_rsqf(rsq);
-            float ff = 0.0f;
-            float ucoul = 0.0f;
-            if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
-                const float qq = qi * pj.w;
-                ff = qq * (ri - a.reacf * rsq);
-                if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
-            } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:178-183
-                const float qq = qi * pj.w;
-                const float r = rsq * ri;
-                const float t = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
-                const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
-                                    0.254829592f) * t;
-                const float ex = __builtin_amdgcn_exp2f(a.alfasq * rsq);
-                float c = fmaf(a.spiialfa, r, poly) * ex * ri;
-                if (COUL == 1) c -= a.wolf;
-                ff = c * qq;
-                if (EN) {
-                    float u = poly * ex * ri - a.ecut;
-                    if (COUL == 1) u += a.wolf * (__builtin_amdgcn_logf(r) - a.log2rc) * 0.69314718056f;
-                    ucoul = qq * u;
-                }
-            }
-            const float ri2 = ri * ri;
-            ff *= ri2;
-            const float ri8 = (ri2 * ri2) * (ri2 * ri2);
-            ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
-            const bool inside = !(rsq > a.rc2);                   // src/pair_forces.cu:195
-            ff = inside ? ff : 0.0f;
-            fx = fmaf(ff, dx, fx);
-            fy = fmaf(ff, dy, fy);
-            fz = fmaf(ff, dz, fz);
-            if (EN && inside) {
-                const float ri6 = ri2 * ri2 * ri2;
-                const float ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
-                elj += (double)ulj;
-                eco += (double)ucoul;
-                evi += (double)(ff * rsq);
-            }
-        }
+        if (ENMODE != 0 && en)
+            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+                                    fx, fy, fz, elj, eco, evi);
+        else
+            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+                                     fx, fy, fz, elj, eco, evi);
         // bonded terms (lanes of the atom's group share them)
         double ebo = 0.0, ean = 0.0, bvi = 0.0;
         if (a.bonded && active) {
@@ -238,7 +293,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
                 const uint32_t e = a.bond_list[(size_t)id * 4 + b];
                 const int js = a.slot_of[e & MD_JMASK];
                 const int bt = (int)(e >> MD_JBITS);
-                const float4 pj = a.pos_in[js];
+                const float4 pj = pin[js];
                 const float dx = minimg(xi.x - pj.x, L0, Li0);
                 const float dy = minimg(xi.y - pj.y, L1, Li1);
                 const float dz = minimg(xi.z - pj.z, L2, Li2);
@@ -248,7 +303,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
                 fx = fmaf(fa, dx, fx);
                 fy = fmaf(fa, dy, fy);
                 fz = fmaf(fa, dz, fz);
-                if (EN) {
+                if (ENMODE != 0 && en) {
                     const float dr = sqrtf(rsq) - r0;
                     ebo += 0.25 * (double)kk * (double)dr * (double)dr;   // each bond is seen from both atoms
                     bvi += 0.5 * (double)(fa * rsq);
@@ -258,8 +313,8 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
             for (int q = sub; q < na; q += G) {          // src/bonded_forces.cu:119-188
                 const int4 e = a.ang_list[(size_t)id * 4 + q];
                 const int role = e.x & 0xff, at = e.x >> 8;
-                const float4 pA = a.pos_in[a.slot_of[e.y]];
-                const float4 pk = a.pos_in[a.slot_of[e.z]];
+                const float4 pA = pin[a.slot_of[e.y]];
+                const float4 pk = pin[a.slot_of[e.z]];
                 float4 pi = xi, pc = pA;
                 if (role == 1) { pi = pA; pc = xi; }
                 float ax = minimg(pi.x - pc.x, L0, Li0), ay = minimg(pi.y - pc.y, L1, Li1), az = minimg(pi.z - pc.z, L2, Li2);
@@ -282,7 +337,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
                 fx = fmaf(fk, gx, fx);
                 fy = fmaf(fk, gy, fy);
                 fz = fmaf(fk, gz, fz);
-                if (EN) {
+                if (ENMODE != 0 && en) {
                     const double d = (double)th - (double)a.at0[at];
                     ean += (0.5 / 3.0) * (double)a.ak2[at] * d * d;   // each angle is seen from 3 atoms
                 }
@@ -294,7 +349,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
             fy += __shfl_xor(fy, o, 64);
             fz += __shfl_xor(fz, o, 64);
         }
-        if (EN) {
+        if (ENMODE != 0 && en) {
             dlj += elj;
             dco += eco;
             dvi += evi + 2.0 * bvi;
@@ -302,8 +357,7 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
             dan += ean;
         }
         if (active && sub == 0) {
-            const float4 f4 = make_float4(fx, fy, fz, 0.0f);
-            a.force[slot] = f4;
+            a.force[slot] = make_float4(fx, fy, fz, 0.0f);
             if (a.mode == MODE_LF) {
                 float4 v = a.vel[slot];
                 const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
@@ -315,11 +369,11 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
                 x = fmaf(-L0, truncf(x * 2.0f * Li0), x);   // src/nve_leapfrog.cu:50-55
                 y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
                 z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
-                a.pos_out[slot] = make_float4(x, y, z, xi.w);
+                pout[slot] = make_float4(x, y, z, xi.w);
                 dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
                 const float4 pb = a.pos_build[slot];
                 const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
-                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) a.flag_set[cur_r] = 1;   // src/neighborlist.cu:174
+                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
                 if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
             } else if (a.mode == MODE_VV) {
                 float4 v = a.vel[slot];
@@ -332,46 +386,69 @@ __global__ __launch_bounds__(256) void k_step(const StepArgs a) {
             }
         }
     }
-    if (cur_r >= 0) flush();
+    if (cur_r >= 0 && run_r) flush();
 }
 
 // Velocity-Verlet begin-of-step kernel: v = S v + F dt/2m; x += v dt; wrap;
 // skin check.  With apply_drift = 0 it only applies the end-of-step
-// thermostat scale (run end).
+// thermostat scale (run end).  The last wavefront of a replica (ticket over
+// its 64-slot tiles) folds the skin check into the list state and moves the
+// replica to the force phase.
 __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
+    if (a.apply_drift && a.gate->n_run == 0) return;   // nothing to kick: idle launch
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && a.flag_clear)
-        for (int r = threadIdx.x; r < a.nrep; r += blockDim.x) a.flag_clear[r] = 0;
-    if (s >= a.nslot) return;
-    if (a.ids[s] < 0) return;
-    const int r = a.tile_rep[s >> 6];
+    if (s >= a.nslot) return;   // whole wavefronts (nslot is a multiple of 64)
+    const int lane = threadIdx.x & 63;
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
+    const RepState st = a.st[r];
     const RepParam& p = a.rep[r];
-    float4 v = a.vel[s];
     if (!a.apply_drift) {
+        if (a.ids[s] < 0) return;
+        float4 v = a.vel[s];
         const float S = (float)a.th[r].S_end;
         v.x *= S; v.y *= S; v.z *= S;
         a.vel[s] = v;
         return;
     }
-    const float S = (float)a.th[r].S;
-    const float4 f = a.force[s];
-    const float h = 0.5f * a.dt * v.w;
-    v.x = fmaf(f.x, h, v.x * S);
-    v.y = fmaf(f.y, h, v.y * S);
-    v.z = fmaf(f.z, h, v.z * S);
-    a.vel[s] = v;
-    const float4 xi = a.pos_in[s];
-    float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
-    x = fmaf(-p.L[0], truncf(x * 2.0f * p.Linv[0]), x);
-    y = fmaf(-p.L[1], truncf(y * 2.0f * p.Linv[1]), y);
-    z = fmaf(-p.L[2], truncf(z * 2.0f * p.Linv[2]), z);
-    a.pos_out[s] = make_float4(x, y, z, xi.w);
-    const float4 pb = a.pos_build[s];
-    const float ddx = minimg(x - pb.x, p.L[0], p.Linv[0]);
-    const float ddy = minimg(y - pb.y, p.L[1], p.Linv[1]);
-    const float ddz = minimg(z - pb.z, p.L[2], p.Linv[2]);
-    if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) a.flag_set[r] = 1;
-    if (!(fabsf(x) <= p.L[0] && fabsf(y) <= p.L[1] && fabsf(z) <= p.L[2])) *a.err = 1;
+    if (!(st.phase == 0 && !stale(st) && st.step < st.target)) return;   // wave-uniform
+    const int cb = cur_buf(st);
+    bool trip = false;
+    if (a.ids[s] >= 0) {
+        float4 v = a.vel[s];
+        const float S = (float)a.th[r].S;
+        const float4 f = a.force[s];
+        const float h = 0.5f * a.dt * v.w;
+        v.x = fmaf(f.x, h, v.x * S);
+        v.y = fmaf(f.y, h, v.y * S);
+        v.z = fmaf(f.z, h, v.z * S);
+        a.vel[s] = v;
+        const float4 xi = a.pos[cb][s];
+        float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
+        x = fmaf(-p.L[0], truncf(x * 2.0f * p.Linv[0]), x);
+        y = fmaf(-p.L[1], truncf(y * 2.0f * p.Linv[1]), y);
+        z = fmaf(-p.L[2], truncf(z * 2.0f * p.Linv[2]), z);
+        a.pos[cb ^ 1][s] = make_float4(x, y, z, xi.w);
+        const float4 pb = a.pos_build[s];
+        const float ddx = minimg(x - pb.x, p.L[0], p.Linv[0]);
+        const float ddy = minimg(y - pb.y, p.L[1], p.Linv[1]);
+        const float ddz = minimg(z - pb.z, p.L[2], p.Linv[2]);
+        trip = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q;
+        if (!(fabsf(x) <= p.L[0] && fabsf(y) <= p.L[1] && fabsf(z) <= p.L[2])) *a.err = 1;
+    }
+    const unsigned long long tb = __ballot(trip);
+    if (lane == 0) {
+        const int tchk = tb ? atomicOr(&a.st[r].trip_new, 1) : 0;
+        const int old = atomicAdd(&a.st[r].tiles_done, 1 + (tchk < 0 ? 1 : 0));
+        if (old == p.nslot / 64 - 1) {
+            a.st[r].tiles_done = 0;
+            const int tn = __hip_atomic_exchange(&a.st[r].trip_new, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tn) a.st[r].trip[st.nb & 1] = 1;
+            a.st[r].phase = 1;
+            atomicSub(&a.gate->n_run, 1);
+            if (tn) atomicAdd(&a.gate->n_stale, 1);
+            else atomicAdd(&a.gate->n_force, 1);
+        }
+    }
 }
 
 // Thermostat bookkeeping at run boundaries (one workgroup).
@@ -394,33 +471,62 @@ __global__ void k_thermo_fix(ThState* th, const RepParam* rep, int nrep, int the
             t.xi = t.xi_end;
             t.vxi = t.vxi_end;
             t.S = 1.0;
+            t.S_end = 1.0;
         }
+    }
+}
+
+// Start of a run: target = step + nsteps for every replica; recount the gates.
+__global__ void k_set_target(RepState* st, Gate* g, int nrep, long long nsteps) {
+    __shared__ int cnt[3];
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int ns = 0, nr = 0, nf = 0;
+    for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+        RepState& s = st[r];
+        s.start = s.step;
+        s.target = s.step + nsteps;
+        const bool st_ = stale(s);
+        ns += st_;
+        nr += !st_ && s.phase == 0 && s.step < s.target;
+        nf += !st_ && s.phase == 1;
+    }
+    atomicAdd(&cnt[0], ns);
+    atomicAdd(&cnt[1], nr);
+    atomicAdd(&cnt[2], nf);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g->n_stale = cnt[0];
+        g->n_run = cnt[1];
+        g->n_force = cnt[2];
     }
 }
 
 // ---------------------------------------------------------------- launchers
 
-template <int G>
-static void launch_g(const StepArgs& a, int coul, bool en, int grid, hipStream_t st) {
-#define MD_STEP_CASE(C)                                                                   \
-    if (coul == C) {                                                                      \
-        if (en) hipLaunchKernelGGL((k_step<G, C, true>), dim3(grid), dim3(256), 0, st, a); \
-        else hipLaunchKernelGGL((k_step<G, C, false>), dim3(grid), dim3(256), 0, st, a);   \
-        return;                                                                           \
+template <int G, int EM>
+static void launch_ge(const StepArgs& a, int coul, int grid, hipStream_t st) {
+    switch (coul) {
+    case 0: hipLaunchKernelGGL((k_step<G, 0, EM>), dim3(grid), dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_step<G, 1, EM>), dim3(grid), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_step<G, 2, EM>), dim3(grid), dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_step<G, 3, EM>), dim3(grid), dim3(256), 0, st, a); break;
     }
-    MD_STEP_CASE(0)
-    MD_STEP_CASE(1)
-    MD_STEP_CASE(2)
-    MD_STEP_CASE(3)
-#undef MD_STEP_CASE
 }
 
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int energy, int grid, hipStream_t st) {
+template <int G>
+static void launch_g(const StepArgs& a, int coul, int grid, hipStream_t st) {
+    const bool em = (a.mode == MODE_FORCE) ? a.force_energy != 0 : a.energy_every > 0;
+    if (em) launch_ge<G, 1>(a, coul, grid, st);
+    else launch_ge<G, 0>(a, coul, grid, st);
+}
+
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st) {
     switch (lanes) {
-    case 4: launch_g<4>(a, coul, energy != 0, grid, st); break;
-    case 8: launch_g<8>(a, coul, energy != 0, grid, st); break;
-    case 16: launch_g<16>(a, coul, energy != 0, grid, st); break;
-    case 32: launch_g<32>(a, coul, energy != 0, grid, st); break;
+    case 4: launch_g<4>(a, coul, grid, st); break;
+    case 8: launch_g<8>(a, coul, grid, st); break;
+    case 16: launch_g<16>(a, coul, grid, st); break;
+    case 32: launch_g<32>(a, coul, grid, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -434,5 +540,10 @@ extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st) {
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st) {
     hipLaunchKernelGGL(k_thermo_fix, dim3(1), dim3(256), 0, st, th, rep, nrep, thermostat, dt, mode);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_target, dim3(1), dim3(256), 0, st, s, g, nrep, nsteps);
     return hipGetLastError();
 }

@@ -23,6 +23,16 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "call out of order", -4: 
           -5: "unstable trajectory"}
 
 
+class LaunchConfig(C.Structure):
+    """md_launch_config (md_api.h)."""
+    _fields_ = [("lanes", C.c_int), ("grid", C.c_int), ("cell_div", C.c_int), ("check_every", C.c_int),
+                ("build_lanes", C.c_int), ("submit", C.c_int)]
+
+    @classmethod
+    def names(cls):
+        return [f[0] for f in cls._fields_]
+
+
 class MDError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{ERRORS.get(code, code)}] {msg}")
@@ -66,7 +76,8 @@ def lib():
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
-            "md_get_info": ([P, P, P, P, P, P], C.c_int), "md_set_launch": ([P, C.c_int, C.c_int, C.c_int], C.c_int),
+            "md_get_info": ([P, P, P, P, P, P], C.c_int),
+            "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
             "md_autotune": ([P, C.c_int, P], C.c_int), "md_timing_enable": ([P, C.c_int], C.c_int),
             "md_timing_get": ([P, P, P, P, P], C.c_int), "md_copy_bench": ([P, C.c_longlong, C.c_int, P], C.c_int),
             "md_fm_set_frames": ([P, C.c_int, P, P], C.c_int),
@@ -83,7 +94,7 @@ def lib():
 def exported_symbols() -> List[str]:
     return ["md_last_error", "md_version", "md_create", "md_destroy", "md_set_forcefield", "md_set_integrator",
             "md_set_system", "md_set_topology", "md_upload_state", "md_download_state", "md_compute_forces",
-            "md_run", "md_get_neighbors", "md_get_thresholds", "md_get_info", "md_set_launch", "md_autotune",
+            "md_run", "md_get_neighbors", "md_get_thresholds", "md_get_info", "md_set_launch", "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_fm_set_frames", "md_fm_loss_grad"]
 
 
@@ -226,10 +237,22 @@ class Engine:
         return dict(rebuilds=reb.value, steps=st.value, cap=cap.value, lanes=ln.value, grid=gr.value)
 
     # -- launch control / measurement ------------------------------------
-    def set_launch(self, lanes: int = 16, grid: int = 0, cell_div: int = 2):
-        _check(lib().md_set_launch(self._h, lanes, grid, cell_div))
+    def set_launch(self, **kw):
+        """Change fields of the launch configuration (md_launch_config)."""
+        c = LaunchConfig()
+        _check(lib().md_get_launch(self._h, C.byref(c)))
+        for k, v in kw.items():
+            if k not in LaunchConfig.names():
+                raise TypeError(f"unknown launch field {k}")
+            setattr(c, k, int(v))
+        _check(lib().md_set_launch(self._h, C.byref(c)))
 
-    def autotune(self, nsteps: int = 20) -> float:
+    def get_launch(self):
+        c = LaunchConfig()
+        _check(lib().md_get_launch(self._h, C.byref(c)))
+        return {k: getattr(c, k) for k in LaunchConfig.names()}
+
+    def autotune(self, nsteps: int = 192) -> float:
         best = C.c_double()
         _check(lib().md_autotune(self._h, nsteps, C.byref(best)))
         return best.value

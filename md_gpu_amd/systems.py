@@ -292,11 +292,12 @@ def perturb_nacl_ff(ff: ForceField, seed: int, frac: float = 0.05):
 
 
 def eps_sig_from_lj(lj1: np.ndarray, lj2: np.ndarray):
-    """Invert lj1 = 48 eps sig^12, lj2 = 24 eps sig^6 (entries with lj2 = 0 give 0)."""
+    """Invert lj1 = 48 eps sig^12, lj2 = 24 eps sig^6, i.e. sig^6 = lj1 / (2 lj2),
+    eps = lj2 / (24 sig^6) (entries with lj2 = 0 give 0)."""
     lj1 = np.asarray(lj1, np.float64)
     lj2 = np.asarray(lj2, np.float64)
     with np.errstate(divide="ignore", invalid="ignore"):
-        s6 = np.where(lj2 > 0, 2.0 * lj1 / np.where(lj2 > 0, lj2, 1.0), 0.0)
+        s6 = np.where(lj2 > 0, lj1 / (2.0 * np.where(lj2 > 0, lj2, 1.0)), 0.0)
         sig = np.where(s6 > 0, s6 ** (1.0 / 6.0), 0.0)
         eps = np.where(s6 > 0, lj2 / (24.0 * np.where(s6 > 0, s6, 1.0)), 0.0)
     return eps, sig

@@ -232,3 +232,19 @@ def test_force_matching_gradient(E, orc):
     assert np.allclose(g1, o1, rtol=1e-4, atol=1e-4 * np.abs(o1).max())
     assert np.allclose(g2, o2, rtol=1e-4, atol=1e-4 * np.abs(o2).max())
     e.close()
+
+
+def test_logged_reference_forces_nacl_rf(E, orc):
+    """The engine reproduces the step-0 forces the reference logged for
+    examples/nacl (reaction field, 4096 ions), rows i % 128 == 0."""
+    from test_oracle_golden import load_fixture
+    s, _, idx, F_ref = load_fixture()
+    say("logged forces nacl_rf_step0", s.n)
+    e = E.Engine.from_system(s)
+    F, _, _ = e.compute_forces()
+    rel = np.linalg.norm(F[idx] - F_ref, axis=1) / np.linalg.norm(F_ref, axis=1)
+    Fo, _, _ = orc.forces(s)
+    say("  max rel vs log", rel.max(), "rel rms vs oracle", rel_rms(F, Fo))
+    assert rel.max() < 3e-5
+    assert rel_rms(F, Fo) < FORCE_TOL
+    e.close()

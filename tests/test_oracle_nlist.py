@@ -122,3 +122,11 @@ def test_generators_match_workload_definitions():
         assert np.mean(r.type == 1) == pytest.approx(0.3, abs=0.01)
         rl = r.ff.rc + r.ff.skin
         assert rl < r.box[0] / 2
+
+
+def test_eps_sig_inversion():
+    eps = np.array([[0.13, 0.2337], [0.2337, 0.42]])
+    sig = np.array([[2.35, 3.375], [3.375, 4.40]])
+    ff = S.ForceField.from_eps_sig(["Na", "Cl"], [23.0, 35.45], eps, sig)
+    e2, s2 = S.eps_sig_from_lj(ff.lj1, ff.lj2)
+    assert np.allclose(e2, eps, rtol=1e-12) and np.allclose(s2, sig, rtol=1e-12)
