@@ -80,12 +80,16 @@ def kernel_time(e, nsteps):
     return t["step_ms"] / max(t["step_launches"], 1), t["rebuild_ms"] / max(t["rebuilds"], 1), t
 
 
-def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=None, force_check=None):
+def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=None, force_check=None,
+                  launch=None):
     from md_gpu_amd import engine as E
     log(f"{name}: setting up")
     e = E.Engine.from_system(systems)
     tuned = None
-    if autotune:
+    if launch:
+        e.set_launch(**launch)
+        log(f"{name}: launch {e.get_launch()}")
+    elif autotune:
         tuned = e.autotune(192)
         log(f"{name}: autotuned {e.get_launch()} ({tuned:.4f} ms/step)")
     e.run(warmup, stats=False)
@@ -97,7 +101,8 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
     pos1, _, _ = e.download()
     flops1 = f_pair_fn(pos1)
     flops = 0.5 * (flops0 + flops1)
-    kstep_ms, kreb_ms, traw = kernel_time(e, min(steps, 2000))
+    ktime_steps = min(steps, 2000)
+    kstep_ms, kreb_ms, traw = kernel_time(e, ktime_steps)
     sps = steps / wall
     info = e.info()
     out = {
@@ -105,7 +110,8 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
         "ns_per_day": sps * dt * 0.0864, "steps": steps, "warmup": warmup,
         "rebuilds_per_step": (reb1 - reb0) / steps / max(len(systems) if isinstance(systems, list) else 1, 1),
         "launch": dict(e.get_launch(), list_cap=info["cap"]),
-        "step_kernel_ms": kstep_ms, "rebuild_chain_ms_avg": kreb_ms,
+        "step_kernel_ms": kstep_ms,
+        "rebuild_chain_ms_per_step": traw["rebuild_ms"] / max(ktime_steps, 1),
         "roofline": {"bound": "valu", "achieved": flops / (kstep_ms * 1e-3) / 1e12, "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": flops / (kstep_ms * 1e-3) / 1e12 / PEAK_FP32,
                      "traffic": None, "algorithmic_flop_per_launch": flops},
@@ -150,7 +156,10 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch", default="", help="fixed launch config for the headline workload, "
+                    "e.g. lanes=8,grid=2048,cell_div=2,check_every=16,build_lanes=8 (skips the autotuner)")
     args = ap.parse_args()
+    launch = dict((k, int(v)) for k, v in (kv.split("=") for kv in args.launch.split(",") if kv)) or None
     if args.gpus != 1:
         print(f"bench.py measures one MI355X; --gpus {args.gpus} is not supported by this build "
               "(the step path shards by spatial domains; see DESIGN.md)", file=sys.stderr)
@@ -178,7 +187,7 @@ def main():
                 "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
 
     res = step_workload("nacl32768", nacl, f_nacl, args.steps, args.warmup, tune, nacl.dt, cpu_nacl,
-                        (lambda e, s: force_error(e, s)) if want_cpu else None)
+                        (lambda e, s: force_error(e, s)) if want_cpu else None, launch)
     line = {
         "metric": METRIC, "value": res["value"], "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -188,7 +197,7 @@ def main():
                    "T_K": nacl.T0, "tau_fs": nacl.tau},
         "ns_per_day": res["ns_per_day"], "force_rel_rms": res.get("force_rel_rms"),
         "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline"),
-        "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_avg": res["rebuild_chain_ms_avg"],
+        "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_per_step": res["rebuild_chain_ms_per_step"],
         "rebuilds_per_step": res["rebuilds_per_step"], "launch": res["launch"],
     }
     traffic = os.path.join(ROOT, "profiles", "step_kernel_nacl32768_traffic.json")

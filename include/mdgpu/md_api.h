@@ -145,8 +145,9 @@ typedef struct md_launch_config {
     int grid;          /* persistent grid of the step kernel in workgroups (0 = automatic) */
     int cell_div;      /* cell edge >= r_list / cell_div, 1..3 */
     int check_every;   /* steps queued between two (device-gated) list-rebuild chains */
-    int build_lanes;   /* lanes per atom in the list build: 4, 8, 16 */
+    int build_lanes;   /* lanes per atom in the list build: 4, 8 */
     int submit;        /* 0: replayed HIP graph segments, 1: direct kernel launches */
+    int refresh;       /* 1: also rebuild a list after check_every steps, ahead of the skin trip */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);
@@ -159,6 +160,10 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms_per_step);
 int md_timing_enable(md_engine* e, int enable);
 int md_timing_get(md_engine* e, double* step_kernel_ms, long long* step_kernel_launches,
                   double* rebuild_ms, long long* rebuild_count);
+/* Force `reps` full list rebuilds of every replica at the current positions
+ * and report the mean time of one (k_force_trip + the five rebuild kernels),
+ * from HIP events on the engine stream.  No counterpart in the reference. */
+int md_rebuild_bench(md_engine* e, int reps, double* ms_per_rebuild);
 /* Plain device copy of nbytes (calibration of HBM counters). Returns ms. */
 int md_copy_bench(md_engine* e, long long nbytes, int reps, double* ms);
 

@@ -124,6 +124,44 @@ double as_poly(double t) {   // A&S 7.1.26 polynomial, src/simulation.h:39-43
     return ((((1.061405429 * t - 1.453152027) * t + 1.421413741) * t - 0.284496736) * t + 0.254829592) * t;
 }
 
+// A(r) = (2 alpha / sqrt(pi)) r exp(-alpha^2 r^2) + erfc_AS(alpha r) and dA/dr, in FP64,
+// with the reference's erfc (src/pair_forces.cu:174-183: E = poly(t) exp(-alpha^2 r^2),
+// t = 1 / (1 + p alpha r))
+void wolf_A(double r, double a, double* A, double* dA) {
+    const double p = 0.3275911, sp = 2.0 / std::sqrt(kPi) * a;
+    const double t = 1.0 / (1.0 + p * a * r);
+    const double poly = as_poly(t);
+    const double dpoly = ((((5 * 1.061405429 * t - 4 * 1.453152027) * t + 3 * 1.421413741) * t -
+                           2 * 0.284496736) * t + 0.254829592);   // d poly / dt
+    const double ex = std::exp(-a * a * r * r);
+    *A = (sp * r + poly) * ex;
+    *dA = (sp + dpoly * (-p * a * t * t)) * ex + (sp * r + poly) * ex * (-2.0 * a * a * r);
+}
+
+// Cubic Hermite table of h(r) = A(r) - r A(rc)/rc (Wolf) or A(r) (damped) on MD_CTAB
+// equal intervals of [0, rc].  The pair force factor is then qq h(r) / r^3, the
+// same law as src/pair_forces.cu:174-186 evaluated in FP64 at the nodes; the cubic
+// reproduces it to ~2e-7 relative in FP32.
+std::vector<float4> coulomb_table(double a, double rc, bool shifted) {
+    double Arc, dArc;
+    wolf_A(rc, a, &Arc, &dArc);
+    const double w = shifted ? Arc / rc : 0.0, h = rc / MD_CTAB;
+    std::vector<double> v(MD_CTAB + 1), d(MD_CTAB + 1);
+    for (int k = 0; k <= MD_CTAB; k++) {
+        double A, dA;
+        wolf_A(k * h, a, &A, &dA);
+        v[k] = A - w * k * h;
+        d[k] = (dA - w) * h;   // derivative per unit of the local coordinate
+    }
+    std::vector<float4> t(MD_CTAB);
+    for (int k = 0; k < MD_CTAB; k++) {
+        const double c2 = 3.0 * (v[k + 1] - v[k]) - 2.0 * d[k] - d[k + 1];
+        const double c3 = 2.0 * (v[k] - v[k + 1]) + d[k] + d[k + 1];
+        t[k] = make_float4((float)v[k], (float)d[k], (float)c2, (float)c3);
+    }
+    return t;
+}
+
 }  // namespace
 
 struct md_engine {
@@ -167,7 +205,7 @@ struct md_engine {
     DBuf<unsigned long long> reb_count;
     DBuf<StatRec> stats;
     int stat_cap = 0;
-    DBuf<float4> ljtab;
+    DBuf<float4> ljtab, ctab;
     // snapshot for overflow recovery
     DBuf<float4> snap_pos0, snap_pos1, snap_vel, snap_force;
     DBuf<int> snap_ids, snap_typ, snap_slot_of;
@@ -177,7 +215,7 @@ struct md_engine {
     int cap = 0;
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 16, submit = 0;
+    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0;
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -322,6 +360,8 @@ int estimate_cap(md_engine* e) {
 }
 
 int alloc_list(md_engine* e, int cap) {
+    // a multiple of 128 entries = 4G for every G: rows hold whole blocks (lpos) and stay 16-byte aligned
+    cap = (std::max(cap, 128) + 127) / 128 * 128;
     e->cap = cap;
     CHECK(e->nlist.alloc((size_t)e->S * cap));
     e->drop_graphs();
@@ -351,6 +391,7 @@ BuildArgs build_args(md_engine* e) {
     b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.build_lanes = e->build_lanes;
+    b.row_g = e->lanes;
     return b;
 }
 
@@ -361,6 +402,8 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.acc = e->acc.p;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
+    a.ctab = e->ctab.p;
+    a.ct_scale = (float)(MD_CTAB / (double)e->ff.rc);
     a.slot_of = e->slot_of.p; a.bond_list = e->bond_list.p; a.nbond = e->nbond.p; a.ang_list = e->ang_list.p; a.nang = e->nang.p;
     a.target = e->target.p;
     a.cap = e->cap; a.ntype = e->ff.ntype; a.nrep = e->R; a.nunits = e->nunits(); a.mode = mode;
@@ -369,6 +412,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.rc2 = e->rc2; a.qfac = e->ff.qfac; a.palfa = e->palfa; a.spiialfa = e->spiialfa; a.alfasq = e->alfasq;
     a.wolf = e->wolf; a.reacf = e->reacf; a.ecut = e->ecut; a.log2rc = e->log2rc; a.qq_scale = 1.0f;
     a.dt = e->in.dt; a.tau = e->in.tau; a.skin2q = e->skin2q;
+    a.max_age = e->refresh ? e->check_every : 0;
     for (int k = 0; k < 8; k++) {
         a.bk2[k] = e->ff.bond_k2[k]; a.br0[k] = e->ff.bond_r0[k];
         a.ak2[k] = e->ff.angle_k2[k]; a.at0[k] = e->ff.angle_t0[k];
@@ -514,7 +558,7 @@ int md_create(md_engine** out, int device) {
         delete e;
         return fail(MD_E_HIP, "hipStreamCreate failed");
     }
-    if (e->ljtab.alloc(64) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
+    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
         delete e;
         return MD_E_HIP;
     }
@@ -542,6 +586,10 @@ int md_set_forcefield(md_engine* e, const md_forcefield* f) {
     if (e->ff.qfac == 0.0f) e->ff.qfac = 1389.35427611722f;
     derive_ff(e);
     CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
+    if (e->ff.coulomb == MD_COUL_WOLF || e->ff.coulomb == MD_COUL_DAMPED) {
+        const std::vector<float4> t = coulomb_table(e->ff.alpha, e->ff.rc, e->ff.coulomb == MD_COUL_WOLF);
+        HIPCHECK(hipMemcpy(e->ctab.p, t.data(), MD_CTAB * sizeof(float4), hipMemcpyHostToDevice));
+    }
     e->has_ff = true;
     e->has_sys = false;   // cells depend on r_list: set the system again
     e->has_state = false;
@@ -1012,7 +1060,7 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
         counts[i] = nc[s];
         worst = std::max(worst, nc[s]);
         int c = std::min(std::min(nc[s], cap), e->cap);
-        for (int k = 0; k < c; k++) nbrs[(size_t)i * cap + k] = ids[L[(size_t)s * e->cap + k] & MD_JMASK];
+        for (int k = 0; k < c; k++) nbrs[(size_t)i * cap + k] = ids[L[(size_t)s * e->cap + lpos(k, e->lanes)] & MD_JMASK];
     }
     if (worst > cap) return fail(MD_E_CAPACITY, "rows need %d entries, cap is %d", worst, cap);
     if (worst > e->cap) return fail(MD_E_CAPACITY, "device list overflowed (%d > %d)", worst, e->cap);
@@ -1051,16 +1099,19 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (c->cell_div < 1 || c->cell_div > 3) return fail(MD_E_INVALID, "cell_div must be 1..3");
     if (c->check_every < 1 || c->check_every > kSegment)
         return fail(MD_E_INVALID, "check_every must be 1..%d", kSegment);
-    if (c->build_lanes != 4 && c->build_lanes != 8 && c->build_lanes != 16)
-        return fail(MD_E_INVALID, "build_lanes must be 4, 8 or 16");
+    if (c->build_lanes != 4 && c->build_lanes != 8)
+        return fail(MD_E_INVALID, "build_lanes must be 4 or 8");
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
+    if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
+    const bool relayout = c->lanes != e->lanes;   // the row layout (lpos) follows the step kernel's lanes
     e->lanes = c->lanes;
     e->grid = c->grid;
     e->check_every = c->check_every;
     e->build_lanes = c->build_lanes;
     e->submit = c->submit;
+    e->refresh = c->refresh;
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
@@ -1068,6 +1119,8 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
             CHECK(setup_cells(e));
             if (e->has_state) CHECK(full_rebuild(e));
         }
+    } else if (relayout && e->has_sys && e->has_state) {
+        CHECK(full_rebuild(e));
     }
     return MD_OK;
 }
@@ -1080,6 +1133,7 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->check_every = e->check_every;
     c->build_lanes = e->build_lanes;
     c->submit = e->submit;
+    c->refresh = e->refresh;
     return MD_OK;
 }
 
@@ -1126,7 +1180,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
     double best_build = 1e30;
     for (int d = 1; d <= 3; d++)
-        for (int bl : {8, 16}) {
+        for (int bl : {4, 8}) {
             md_launch_config c = best;
             c.cell_div = d;
             c.build_lanes = bl;
@@ -1159,26 +1213,33 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             CHECK(md_timing_enable(e, 0));
             if (per < best_step) { best_step = per; best.lanes = l; best.grid = c.grid; }
         }
-    // stage 3, whole runs: rebuild-check interval x submission mode
+    // stage 3, whole runs: rebuild-check interval x age refresh x submission mode
     double best_run = 1e30;
     const int kr = std::max(nsteps, 2 * kSegment);
     for (int sub : {0, 1})
-        for (int ce : {2, 4, 8, 16}) {
-            md_launch_config c = best;
-            c.check_every = ce;
-            c.submit = sub;
-            CHECK(md_set_launch(e, &c));
-            CHECK(restore());
-            CHECK(md_run(e, std::min(kr, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
-            HIPCHECK(hipEventRecord(t0, e->st));
-            CHECK(md_run(e, kr, 0, nullptr));
-            HIPCHECK(hipEventRecord(t1, e->st));
-            HIPCHECK(hipEventSynchronize(t1));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-            const double per = ms / kr;
-            if (per < best_run) { best_run = per; best.check_every = ce; best.submit = sub; }
-        }
+        for (int rf : {0, 1})
+            for (int ce : {4, 6, 8, 12, 16}) {
+                md_launch_config c = best;
+                c.check_every = ce;
+                c.submit = sub;
+                c.refresh = rf;
+                CHECK(md_set_launch(e, &c));
+                CHECK(restore());
+                CHECK(md_run(e, std::min(kr, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
+                HIPCHECK(hipEventRecord(t0, e->st));
+                CHECK(md_run(e, kr, 0, nullptr));
+                HIPCHECK(hipEventRecord(t1, e->st));
+                HIPCHECK(hipEventSynchronize(t1));
+                float ms = 0;
+                HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+                const double per = ms / kr;
+                if (per < best_run) {
+                    best_run = per;
+                    best.check_every = ce;
+                    best.submit = sub;
+                    best.refresh = rf;
+                }
+            }
     CHECK(md_set_launch(e, &best));
     CHECK(restore());
     e->timing = saved_timing;
@@ -1187,6 +1248,25 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     e->n_step = saved_n[0];
     e->n_reb = saved_n[1];
     if (best_ms) *best_ms = best_run;
+    return MD_OK;
+}
+
+int md_rebuild_bench(md_engine* e, int reps, double* ms) {
+    CHECK(check_ready(e, true));
+    if (reps < 1) return fail(MD_E_INVALID, "reps must be >= 1");
+    HIPCHECK(hipSetDevice(e->dev));
+    CHECK(full_rebuild(e));   // settles the capacity
+    hipEvent_t t0 = ev_get(e, 0), t1 = ev_get(e, 1);
+    HIPCHECK(hipEventRecord(t0, e->st));
+    for (int k = 0; k < reps; k++) {
+        HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
+        CHECK(enqueue_rebuild(e, nullptr));
+    }
+    HIPCHECK(hipEventRecord(t1, e->st));
+    HIPCHECK(hipEventSynchronize(t1));
+    float t = 0;
+    HIPCHECK(hipEventElapsedTime(&t, t0, t1));
+    if (ms) *ms = t / reps;
     return MD_OK;
 }
 

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* ga
         for (int t = 0; t < NT; t++) s1[t] = s2[t] = 0.0f;
         const uint32_t* row = a.nlist + (size_t)slot * a.cap;
         for (int k = sub; k < nn; k += G) {
-            const uint32_t e = row[k];
+            const uint32_t e = row[lpos(k, G)];   // rows are laid out for G lanes (the step kernel's)
             const int j = (int)(e & MD_JMASK);
             const int tj = (int)(e >> MD_JBITS);
             const float4 pj = pin[j];

@@ -8,7 +8,8 @@
 //   force[S]    float4 (fx, fy, fz, 0)
 //   ids[S]      int    original atom index (-1 = padding slot)
 //   typ[S]      int    atom type
-//   nlist[S*cap] uint32 (j_slot | type_j << 29), row-major per atom, ncount[S]
+//   nlist[S*cap] uint32 (j_slot | type_j << 29), one row per atom, ncount[S];
+//               entries stored in the transposed block order of lpos()
 // Replicas occupy 64-aligned slot ranges, so a wavefront never straddles two.
 //
 // Control: every replica carries a RepState on the device.  Kernels decide
@@ -24,6 +25,7 @@
 #define MD_JMASK 0x1FFFFFFFu
 #define MD_KB_REF 8.314e-3      // src/integrator.cu:155 (1000/8.314)
 #define MD_NSTAT 8              // accumulators per replica
+#define MD_CTAB 256             // intervals of the Coulomb force table on [0, rc]
 
 enum { ACC_KE2 = 0, ACC_LJ, ACC_COUL, ACC_BOND, ACC_ANGLE, ACC_VIR };
 enum { MODE_FORCE = 0, MODE_LF = 1, MODE_VV = 2 };
@@ -53,6 +55,7 @@ struct RepState {
     int units_done;    // ticket of the step kernel
     int tiles_done;    // ticket of the kick kernel
     int build_done;    // ticket of the list build
+    int age;           // steps completed since the last build
 };
 
 // Device-wide counters that let a launch with nothing to do exit on its first
@@ -63,6 +66,15 @@ struct Gate {
     int n_force;       // velocity Verlet: replicas whose force half is pending (list valid)
     int pad;
 };
+
+// Storage position of logical entry e in a neighbor row read by G lanes per atom.
+// Rows are stored in blocks of 4G entries; inside a block the layout is transposed
+// so that lane l's 16-byte quad holds logical entries l, l+G, l+2G, l+3G.  Each
+// position gather then reads consecutive entries (mostly consecutive slots: a row
+// lists the members of one cell together) across the atom's lanes.
+__host__ __device__ inline int lpos(int e, int G) {
+    return (e & ~(4 * G - 1)) + 4 * (e % G) + ((e / G) & 3);
+}
 
 // current position buffer of a replica
 __host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step + s.phase) & 1); }
@@ -107,6 +119,7 @@ struct StepArgs {
     int* err;             // [1] non-finite coordinates
     StatRec* stats;       // [stat_cap][R] or null
     const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)
+    const float4* ctab;   // [MD_CTAB] cubic coefficients of h(r) = A(r) - r A(rc)/rc (Wolf / damped)
     const int* slot_of;   // [N] original index -> slot
     const uint32_t* bond_list;  // [N*4] (partner | type << 29)
     const int* nbond;           // [N]
@@ -126,9 +139,11 @@ struct StepArgs {
     float rc2, qfac, palfa, spiialfa, alfasq, wolf, reacf;
     float ecut;           // Coulomb energy shift: Wolf E(rc)/rc, RF c_rf, else 0
     float log2rc;         // log2(rc) for the Wolf energy's ln(r/rc) term
+    float ct_scale;       // MD_CTAB / rc: table index per unit r
     float qq_scale;       // multiplies q_i q_j (force matching)
     float dt, tau;
     float skin2q;
+    int max_age;          // > 0: the list is also refreshed after this many steps
     float bk2[8], br0[8], ak2[8], at0[8];
 };
 
@@ -187,5 +202,6 @@ struct BuildArgs {
     int cap;
     float rl2;
     int has_excl;
-    int build_lanes;      // lanes per atom in k_nl_build (4, 8, 16)
+    int build_lanes;      // lanes per atom in k_nl_build (4, 8)
+    int row_g;            // lanes per atom of the step kernel (row layout, lpos)
 };

@@ -14,7 +14,7 @@
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
 //   k_nl_permute  deterministic order inside a cell (by original index);
 //                 gathers pos/vel/force/id/type into scratch in the new order
-//   k_nl_build    one workgroup per 64-slot tile, LPA (4, 8, 16) lanes per atom:
+//   k_nl_build    one workgroup per 64-slot tile, LPA (4, 8) lanes per atom:
 //                 the image cells within r_list of the tile's bounding box are
 //                 listed in LDS, their members streamed through LDS in
 //                 double-buffered chunks of 64*LPA, each tested with the
@@ -203,6 +203,8 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     __shared__ int s_tpref[kTab + 1];
     __shared__ int s_tshift[kTab];
     __shared__ int s_w[NW];
+    __shared__ int s_n[2];               // staged candidates per buffer
+    __shared__ unsigned short s_widx[NW][NT];   // per wave: staged candidates near the wave's box
     __shared__ float s_bb[NW][6];
     if (a.gate->n_stale == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -228,6 +230,8 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
         lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
         hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
     }
+    // the wave's own box (its APW atoms) prunes candidates once more before the pair tests
+    const float wlx = lx, wly = ly, wlz = lz, whx = hx, why = hy, whz = hz;
     if (lane == 0) {
         s_bb[wave][0] = lx; s_bb[wave][1] = ly; s_bb[wave][2] = lz;
         s_bb[wave][3] = hx; s_bb[wave][4] = hy; s_bb[wave][5] = hz;
@@ -289,68 +293,109 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
             }
             if (tid == 0) s_tpref[ntab] = T;
             __syncthreads();
-            // ---- stream the candidates through LDS
-            auto fetch = [&](int base, float4& rp, float4& rs) {
+            // ---- stream the candidates through LDS.  A candidate whose image lies
+            //      r_list or farther from the tile box cannot pass for any atom of the
+            //      tile: it is dropped before staging (stable compaction, so rows keep
+            //      candidate order).
+            auto fetch = [&](int base, float4& rp, float4& rs) -> bool {
                 const int q = base + tid;
-                if (q < T) {
-                    int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
-                    while (lo_ < hi_) {
-                        const int mid = (lo_ + hi_ + 1) >> 1;
-                        if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
-                    }
-                    const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
-                    const int sh = s_tshift[lo_];
-                    rp = a.pos_s[j];
-                    rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
-                    // -k L is exact for the |k| <= 1 images that can pass the predicate
-                    rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
-                                     -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
+                if (q >= T) return false;
+                int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
+                while (lo_ < hi_) {
+                    const int mid = (lo_ + hi_ + 1) >> 1;
+                    if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
                 }
+                const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
+                const int sh = s_tshift[lo_];
+                rp = a.pos_s[j];
+                // -k L is exact for the |k| <= 1 images that can pass the predicate
+                rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
+                                 -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
+                const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
+                const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
+                const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
+                const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
+                rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
+                return gx * gx + gy * gy + gz * gz < rlp2;
+            };
+            auto stage = [&](int buf, bool keep, const float4& rp, const float4& rs) {
+                int kept;
+                const int at = block_scan<NT>(keep ? 1 : 0, s_w, kept);
+                if (keep) {
+                    s_pos[buf][at] = rp;
+                    s_sh[buf][at] = rs;
+                }
+                if (tid == 0) s_n[buf] = kept;
+            };
+            auto test = [&](int buf, int k, int wn, uint32_t& ent) -> bool {
+                if (k >= wn) return false;
+                const int q = s_widx[wave][k];
+                const float4 pj = s_pos[buf][q];
+                const float4 sj = s_sh[buf][q];
+                // (xi - xj) - k L: the canonical minimum-image displacement for this image
+                const float dx = (xi.x - pj.x) + sj.x;
+                const float dy = (xi.y - pj.y) + sj.y;
+                const float dz = (xi.z - pj.z) + sj.z;
+                const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                ent = (uint32_t)__float_as_int(pj.w);
+                bool ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
+                if (EXCL) {
+                    const int jid = __float_as_int(sj.w);
+                    ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
+                }
+                return ok;
+            };
+            // append in candidate order: rank among this atom's LPA lanes
+            auto append = [&](bool ok, uint32_t ent) {
+                const unsigned long long m = __ballot(ok);
+                const uint32_t gb = (uint32_t)(m >> (lane & ~(LPA - 1))) & (uint32_t)((1ull << LPA) - 1);
+                const int before = __popc(gb & ((1u << sub) - 1u));
+                if (ok && cnt + before < a.cap) rowp[lpos(cnt + before, a.row_g)] = ent;
+                cnt += __popc(gb);
             };
             const int nch = (T + NT - 1) / NT;
             float4 rp = make_float4(0, 0, 0, 0), rs = make_float4(0, 0, 0, 0);
             if (nch > 0) {
-                fetch(0, rp, rs);
-                s_pos[0][tid] = rp;
-                s_sh[0][tid] = rs;
+                const bool keep = fetch(0, rp, rs);
+                stage(0, keep, rp, rs);
             }
             __syncthreads();
             for (int ch = 0; ch < nch; ch++) {
                 const int buf = ch & 1;
                 const bool more = ch + 1 < nch;
-                if (more) fetch((ch + 1) * NT, rp, rs);
-                const int n = min(NT, T - ch * NT);
-                for (int u = 0; u < n; u += LPA) {
-                    const int q = u + sub;
-                    bool ok = false;
-                    uint32_t ent = 0;
-                    if (q < n) {
-                        const float4 pj = s_pos[buf][q];
-                        const float4 sj = s_sh[buf][q];
-                        // (xi - xj) - k L: the canonical minimum-image displacement for this image
-                        const float dx = (xi.x - pj.x) + sj.x;
-                        const float dy = (xi.y - pj.y) + sj.y;
-                        const float dz = (xi.z - pj.z) + sj.z;
-                        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        ent = (uint32_t)__float_as_int(pj.w);
-                        ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
-                        if (EXCL) {
-                            const int jid = __float_as_int(sj.w);
-                            ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
-                        }
+                bool keep_next = false;
+                if (more) keep_next = fetch((ch + 1) * NT, rp, rs);
+                const int n = s_n[buf];
+                // this wave's candidates: staged ones within r_list of the wave's box (in order)
+                int wn = 0;
+                for (int c0 = 0; c0 < n; c0 += 64) {
+                    const int c = c0 + lane;
+                    bool near = false;
+                    if (c < n) {
+                        const float4 pj = s_pos[buf][c];
+                        const float4 sj = s_sh[buf][c];
+                        const float ix = pj.x - sj.x, iy = pj.y - sj.y, iz = pj.z - sj.z;
+                        const float gx = fmaxf(0.0f, fmaxf(wlx - ix, ix - whx));
+                        const float gy = fmaxf(0.0f, fmaxf(wly - iy, iy - why));
+                        const float gz = fmaxf(0.0f, fmaxf(wlz - iz, iz - whz));
+                        near = gx * gx + gy * gy + gz * gz < rlp2;
                     }
-                    // append in candidate order: rank among this atom's LPA lanes
-                    const unsigned long long m = __ballot(ok);
-                    const uint32_t gb = (uint32_t)(m >> (lane & ~(LPA - 1))) & (uint32_t)((1ull << LPA) - 1);
-                    const int before = __popc(gb & ((1u << sub) - 1u));
-                    if (ok && cnt + before < a.cap) rowp[cnt + before] = ent;
-                    cnt += __popc(gb);
+                    const unsigned long long m = __ballot(near);
+                    if (near) s_widx[wave][wn + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)c;
+                    wn += __popcll(m);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int u = 0; u < wn; u += 2 * LPA) {
+                    uint32_t e0 = 0, e1 = 0;
+                    const bool ok0 = test(buf, u + sub, wn, e0);
+                    const bool ok1 = test(buf, u + LPA + sub, wn, e1);
+                    append(ok0, e0);
+                    append(ok1, e1);
                 }
                 __syncthreads();
-                if (more) {
-                    s_pos[buf ^ 1][tid] = rp;
-                    s_sh[buf ^ 1][tid] = rs;
-                }
+                if (more) stage(buf ^ 1, keep_next, rp, rs);
                 __syncthreads();
             }
         }
@@ -376,6 +421,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
         const int old = atomicAdd(&a.st[r].build_done, 1);
         if (old == p.nslot / 64 - 1) {
             a.st[r].build_done = 0;
+            a.st[r].age = 0;
             a.st[r].nb = st.nb + 1;
             atomicSub(&a.gate->n_stale, 1);
             if (st.phase == 0 && st.step < st.target) atomicAdd(&a.gate->n_run, 1);
@@ -406,10 +452,8 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
     switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
     case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
     case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;
-    case 16:  hipLaunchKernelGGL((k_nl_build<false, 16>), ntiles, dim3(1024), 0, st, a); break;
     case 104: hipLaunchKernelGGL((k_nl_build<true, 4>), ntiles, dim3(256), 0, st, a); break;
     case 108: hipLaunchKernelGGL((k_nl_build<true, 8>), ntiles, dim3(512), 0, st, a); break;
-    case 116: hipLaunchKernelGGL((k_nl_build<true, 16>), ntiles, dim3(1024), 0, st, a); break;
     default:  return hipErrorInvalidValue;
     }
     return hipGetLastError();

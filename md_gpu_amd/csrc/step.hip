@@ -110,75 +110,120 @@ __device__ void finalize_replica(const StepArgs& a, int r, long long kstat, bool
     }
     for (int k = 0; k < MD_NSTAT; k++) __hip_atomic_exchange(acc + k, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_exchange(&st.units_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // list refresh by age (ahead of the skin trip), so queued steps do not stall
+    if (a.mode != MODE_FORCE) st.age = st.age + 1;
+    const bool aged = a.max_age > 0 && st.age >= a.max_age;
     if (a.mode == MODE_LF) {
         // skin check of the positions just written: the list is stale for them
         const int tn = __hip_atomic_exchange(&st.trip_new, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (tn) st.trip[st.nb & 1] = 1;
+        const bool tr = tn || aged;
+        if (tr) st.trip[st.nb & 1] = 1;
         st.step = st.step + 1;
-        if (tn) atomicAdd(&a.gate->n_stale, 1);
-        if (tn || st.step >= st.target) atomicSub(&a.gate->n_run, 1);
+        if (tr) atomicAdd(&a.gate->n_stale, 1);
+        if (tr || st.step >= st.target) atomicSub(&a.gate->n_run, 1);
     } else if (a.mode == MODE_VV) {
         st.phase = 0;
         st.step = st.step + 1;
         atomicSub(&a.gate->n_force, 1);
-        if (st.step < st.target) atomicAdd(&a.gate->n_run, 1);
+        if (aged) {
+            st.trip[st.nb & 1] = 1;
+            atomicAdd(&a.gate->n_stale, 1);
+        } else if (st.step < st.target) {
+            atomicAdd(&a.gate->n_run, 1);
+        }
     }
 }
 
-// Pair loop of one atom over its neighbor row (lanes sub, sub+G, ...).
+// One pair: force factor (F_i += ff * d) and, with EN, the pair energies.
+// Wolf / damped forces come from the cubic table of h(r) in LDS (s_ct); the
+// energies keep the closed form.
+template <int COUL, bool EN>
+__device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, float qi, float qj, float rsq,
+                                         const float4 lj, float& ulj, float& ucoul) {
+    const float ri = __builtin_amdgcn_rsqf(rsq);
+    float ff = 0.0f;
+    if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
+        const float qq = qi * qj;
+        ff = qq * (ri - a.reacf * rsq);
+        if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
+    } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:174-186
+        const float qq = qi * qj;
+        const float r = rsq * ri;
+        const float x = r * a.ct_scale;
+        const int k = min((int)x, MD_CTAB - 1);
+        const float t = x - (float)k;
+        const float4 c4 = s_ct[k];
+        const float hv = fmaf(fmaf(fmaf(c4.w, t, c4.z), t, c4.y), t, c4.x);   // A(r) - r A(rc)/rc
+        ff = (hv * ri) * qq;                                                   // (A(r)/r - A(rc)/rc) qq
+        if (EN) {
+            const float tt = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
+            const float poly = ((((1.061405429f * tt - 1.453152027f) * tt + 1.421413741f) * tt - 0.284496736f) * tt +
+                                0.254829592f) * tt;
+            const float ex = __builtin_amdgcn_exp2f(a.alfasq * rsq);
+            float u = poly * ex * ri - a.ecut;
+            if (COUL == 1) u += a.wolf * (__builtin_amdgcn_logf(r) - a.log2rc) * 0.69314718056f;
+            ucoul = qq * u;
+        }
+    }
+    const float ri2 = ri * ri;
+    ff *= ri2;
+    const float ri8 = (ri2 * ri2) * (ri2 * ri2);
+    ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
+    if (EN) {
+        const float ri6 = ri2 * ri2 * ri2;
+        ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
+    }
+    return ff;
+}
+
+// Pair loop of one atom over its neighbor row.  Rows are read as quads of
+// entries (16-byte loads; the capacity is a multiple of 4G, so rows and blocks
+// are 16-byte aligned).  Block b holds logical entries 4Gb .. 4Gb+4G-1 in the
+// transposed order of lpos(): lane `sub` loads quad sub of every block and gets
+// logical entries 4Gb + sub + G u (u = 0..3).  So the G lanes of an atom read
+// G*16 contiguous bytes of the row, every gather instruction covers G
+// consecutive entries, and each lane has four independent gathers in flight.
+// Entries past the row length are masked.
 template <int G, int COUL, bool EN>
 __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
-                                         const uint32_t* __restrict__ row, int nn, int sub, const float4 xi,
+                                         const float4* s_ct,
+                                         const uint32_t* __restrict__ row, int nn, int sub, int self, const float4 xi,
                                          float qi, float L0, float L1, float L2, float Li0, float Li1, float Li2,
                                          float& fx, float& fy, float& fz, double& elj, double& eco, double& evi) {
-    for (int k = sub; k < nn; k += G) {
-        const uint32_t e = row[k];
-        const int j = (int)(e & MD_JMASK);
-        const int tj = (int)(e >> MD_JBITS);
-        const float4 pj = pin[j];
-        const float4 lj = s_lj[tj];
-        const float dx = minimg(xi.x - pj.x, L0, Li0);
-        const float dy = minimg(xi.y - pj.y, L1, Li1);
-        const float dz = minimg(xi.z - pj.z, L2, Li2);
-        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float ri = __builtin_amdgcn_rsqf(rsq);
-        float ff = 0.0f;
-        float ucoul = 0.0f;
-        if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
-            const float qq = qi * pj.w;
-            ff = qq * (ri - a.reacf * rsq);
-            if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
-        } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:178-183
-            const float qq = qi * pj.w;
-            const float r = rsq * ri;
-            const float t = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
-            const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
-                                0.254829592f) * t;
-            const float ex = __builtin_amdgcn_exp2f(a.alfasq * rsq);
-            float c = fmaf(a.spiialfa, r, poly) * ex * ri;
-            if (COUL == 1) c -= a.wolf;
-            ff = c * qq;
-            if (EN) {
-                float u = poly * ex * ri - a.ecut;
-                if (COUL == 1) u += a.wolf * (__builtin_amdgcn_logf(r) - a.log2rc) * 0.69314718056f;
-                ucoul = qq * u;
-            }
+    const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(row);
+    const int nb = (nn + 4 * G - 1) / (4 * G);       // blocks
+    uint4 nxt = nb > 0 ? row4[sub] : make_uint4(0, 0, 0, 0);
+    for (int b = 0; b < nb; b++) {
+        const uint4 e4 = nxt;
+        if (b + 1 < nb) nxt = row4[(b + 1) * G + sub];   // next quad in flight while this one computes
+        const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
+        const int base = 4 * G * b + sub;                // logical entry of element u: base + G u
+        float4 pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = (base + G * u < nn) ? (int)(ev[u] & MD_JMASK) : self;
+            pj[u] = pin[j];
         }
-        const float ri2 = ri * ri;
-        ff *= ri2;
-        const float ri8 = (ri2 * ri2) * (ri2 * ri2);
-        ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
-        const bool inside = !(rsq > a.rc2);                   // src/pair_forces.cu:195
-        ff = inside ? ff : 0.0f;
-        fx = fmaf(ff, dx, fx);
-        fy = fmaf(ff, dy, fy);
-        fz = fmaf(ff, dz, fz);
-        if (EN && inside) {
-            const float ri6 = ri2 * ri2 * ri2;
-            const float ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
-            elj += (double)ulj;
-            eco += (double)ucoul;
-            evi += (double)(ff * rsq);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const float4 lj = s_lj[ev[u] >> MD_JBITS];
+            const float dx = minimg(xi.x - pj[u].x, L0, Li0);
+            const float dy = minimg(xi.y - pj[u].y, L1, Li1);
+            const float dz = minimg(xi.z - pj[u].z, L2, Li2);
+            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float ulj = 0.0f, ucoul = 0.0f;
+            float ff = pair_ff<COUL, EN>(a, s_ct, qi, pj[u].w, rsq, lj, ulj, ucoul);
+            // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); masked entries of the last block
+            const bool use = (base + G * u < nn) && !(rsq > a.rc2);
+            ff = use ? ff : 0.0f;
+            fx = fmaf(ff, dx, fx);
+            fy = fmaf(ff, dy, fy);
+            fz = fmaf(ff, dz, fz);
+            if (EN && use) {
+                elj += (double)ulj;
+                eco += (double)ucoul;
+                evi += (double)(ff * rsq);
+            }
         }
     }
 }
@@ -189,6 +234,8 @@ __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __rest
 template <int G, int COUL, int ENMODE>
 __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_lj[64];
+    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB : 1];
+    static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     __shared__ double s_red[4][MD_NSTAT];
     __shared__ int s_last;
     if (a.mode == MODE_LF && a.gate->n_run == 0) return;     // nothing can step: idle launch
@@ -198,6 +245,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
     if (tid < 64) s_lj[tid] = a.ljtab[tid];
+    if (COUL == 1 || COUL == 2) s_ct[tid] = a.ctab[tid];   // 256 threads, MD_CTAB entries
     __syncthreads();
 
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
@@ -280,10 +328,10 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         double elj = 0.0, eco = 0.0, evi = 0.0;
         const uint32_t* row = a.nlist + (size_t)slot * a.cap;
         if (ENMODE != 0 && en)
-            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, slot, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                     fx, fy, fz, elj, eco, evi);
         else
-            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, slot, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                      fx, fy, fz, elj, eco, evi);
         // bonded terms (lanes of the atom's group share them)
         double ebo = 0.0, ean = 0.0, bvi = 0.0;

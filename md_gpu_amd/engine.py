@@ -26,7 +26,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "call out of order", -4: 
 class LaunchConfig(C.Structure):
     """md_launch_config (md_api.h)."""
     _fields_ = [("lanes", C.c_int), ("grid", C.c_int), ("cell_div", C.c_int), ("check_every", C.c_int),
-                ("build_lanes", C.c_int), ("submit", C.c_int)]
+                ("build_lanes", C.c_int), ("submit", C.c_int), ("refresh", C.c_int)]
 
     @classmethod
     def names(cls):
@@ -80,6 +80,7 @@ def lib():
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
             "md_autotune": ([P, C.c_int, P], C.c_int), "md_timing_enable": ([P, C.c_int], C.c_int),
             "md_timing_get": ([P, P, P, P, P], C.c_int), "md_copy_bench": ([P, C.c_longlong, C.c_int, P], C.c_int),
+            "md_rebuild_bench": ([P, C.c_int, P], C.c_int),
             "md_fm_set_frames": ([P, C.c_int, P, P], C.c_int),
             "md_fm_loss_grad": ([P, P, P, C.c_float, P, P, P, P], C.c_int),
         }
@@ -95,7 +96,7 @@ def exported_symbols() -> List[str]:
     return ["md_last_error", "md_version", "md_create", "md_destroy", "md_set_forcefield", "md_set_integrator",
             "md_set_system", "md_set_topology", "md_upload_state", "md_download_state", "md_compute_forces",
             "md_run", "md_get_neighbors", "md_get_thresholds", "md_get_info", "md_set_launch", "md_get_launch", "md_autotune",
-            "md_timing_enable", "md_timing_get", "md_copy_bench", "md_fm_set_frames", "md_fm_loss_grad"]
+            "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad"]
 
 
 def _check(rc: int):
@@ -264,6 +265,12 @@ class Engine:
         a = C.c_double(); b = C.c_longlong(); c = C.c_double(); d = C.c_longlong()
         _check(lib().md_timing_get(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return dict(step_ms=a.value, step_launches=b.value, rebuild_ms=c.value, rebuilds=d.value)
+
+    def rebuild_bench(self, reps: int = 10) -> float:
+        """Mean ms of one forced full list rebuild (all replicas)."""
+        ms = C.c_double()
+        _check(lib().md_rebuild_bench(self._h, reps, C.byref(ms)))
+        return ms.value
 
     def copy_bench(self, nbytes: int, reps: int = 20) -> float:
         ms = C.c_double()

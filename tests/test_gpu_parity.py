@@ -248,3 +248,37 @@ def test_logged_reference_forces_nacl_rf(E, orc):
     assert rel.max() < 3e-5
     assert rel_rms(F, Fo) < FORCE_TOL
     e.close()
+
+
+LAUNCH_VARIANTS = [
+    dict(submit=1, refresh=0, check_every=4, build_lanes=4, lanes=4),
+    dict(submit=0, refresh=1, check_every=4, build_lanes=8, lanes=8),
+    dict(submit=1, refresh=1, check_every=6, build_lanes=4, lanes=32),
+    dict(submit=0, refresh=0, check_every=12, build_lanes=8, lanes=16, cell_div=3),
+]
+
+
+@pytest.mark.parametrize("variant", range(len(LAUNCH_VARIANTS)))
+@pytest.mark.parametrize("kind", ["nacl_lf_berendsen", "water_vv_nh"])
+def test_launch_variants_match_oracle(E, orc, kind, variant):
+    """Submission mode, age refresh and kernel shapes change only the schedule,
+    never the result: every variant follows the oracle trajectory."""
+    cfg = LAUNCH_VARIANTS[variant]
+    s = perturbed(S.nacl(nsites=16), 0.1) if kind == "nacl_lf_berendsen" else perturbed(S.water(nmol=512), 0.02)
+    say("launch variant", kind, cfg)
+    e = E.Engine.from_system(s)
+    e.set_launch(**cfg)
+    assert {k: e.get_launch()[k] for k in cfg} == cfg
+    pos0, vel0, _ = e.download()
+    so = s.copy()
+    so.pos, so.vel = pos0, vel0
+    nsteps = 30
+    st = e.run(nsteps, energy_every=10)
+    pos, vel, _ = e.download()
+    r = orc.md_run(so, nsteps)
+    dpos = pos.astype(np.float64) - r["pos"]
+    dpos -= s.box * np.rint(dpos / s.box)
+    say("  max |dx|", np.abs(dpos).max(), "rebuilds", e.info()["rebuilds"])
+    assert np.abs(dpos).max() < 2e-4
+    assert np.allclose(st["temp"][:, 0], r["temp"], rtol=1e-4)
+    e.close()

@@ -1,0 +1,95 @@
+"""GPU checks of state that only exists during runs: neighbor lists rebuilt
+inside replayed graph segments, run-to-run reproducibility, and energy
+conservation at the benchmark size."""
+import sys
+
+import numpy as np
+import pytest
+
+from md_gpu_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def E():
+    from md_gpu_amd import engine
+    return engine
+
+
+def pairs_within(orc, pos, box, r2):
+    counts, nb = orc.nlist(pos, box, float(np.float32(r2)))
+    return [set(nb[i, :counts[i]].tolist()) for i in range(len(pos))]
+
+
+@pytest.mark.parametrize("nsites,cfg", [(16, dict(check_every=8)), (32, dict(check_every=16, lanes=8, build_lanes=8)),
+                                        (32, dict(check_every=4, refresh=1, submit=1))])
+def test_lists_valid_after_in_run_rebuilds(E, orc, nsites, cfg):
+    """After a run with many device-side rebuilds, every pair within rc of the
+    current positions is in the list, and no row holds a pair beyond
+    rc + 2 skin (the list is never older than a skin/2 move per atom)."""
+    s = S.nacl(nsites=nsites)
+    say("lists after run", s.n, cfg)
+    e = E.Engine.from_system(s)
+    e.set_launch(**cfg)
+    reb0 = e.info()["rebuilds"]
+    e.run(120, energy_every=0, stats=False)
+    assert e.info()["rebuilds"] - reb0 >= 3
+    pos, _, _ = e.download()
+    counts, nbrs = e.neighbors()
+    rc2, _, _, _ = orc.thresholds(s.ff.rc, s.ff.skin, s.box)
+    need = pairs_within(orc, pos, s.box, rc2)
+    outer = pairs_within(orc, pos, s.box, (s.ff.rc + 2 * s.ff.skin) ** 2)
+    for i in range(s.n):
+        row = set(nbrs[i, :counts[i]].tolist())
+        assert need[i] <= row, f"atom {i}: missing {sorted(need[i] - row)[:5]}"
+        assert row <= outer[i], f"atom {i}: stale {sorted(row - outer[i])[:5]}"
+    e.close()
+
+
+def test_runs_reproducible(E):
+    """Two engines on the same input follow the same trajectory: the only
+    run-to-run difference allowed is the order of FP64 atomic sums."""
+    s = S.nacl()
+    say("reproducible nacl32768")
+    runs = []
+    for _ in range(2):
+        e = E.Engine.from_system(s)
+        e.set_launch(lanes=8, grid=2048, check_every=16, build_lanes=8)
+        st = e.run(150, energy_every=10)
+        pos, vel, _ = e.download()
+        runs.append((st["temp"][:, 0].copy(), pos, e.info()["rebuilds"]))
+        e.close()
+    (t0, p0, r0), (t1, p1, r1) = runs
+    say("  rebuilds", r0, r1, "max dT/T", np.abs(t0 - t1).max() / t0.mean(), "max dx", np.abs(p0 - p1).max())
+    assert r0 == r1
+    assert np.allclose(t0, t1, rtol=1e-9)
+    assert np.abs(p0 - p1).max() < 1e-4
+
+
+def test_nve_conservation_nacl32768(E):
+    """NaCl, 32,768 ions from the rocksalt lattice with 1200 K velocities, NVE
+    leapfrog dt 2 fs, 1000 steps: the total energy (leapfrog half-step kinetic
+    energy + potential) stays within 2 % of the kinetic energy.  (Half of the
+    initial kinetic energy goes into the lattice's potential energy, so the
+    temperature settles near 600 K.)"""
+    s = S.nacl()
+    s.thermostat = "none"
+    say("nve nacl32768")
+    e = E.Engine.from_system(s)
+    e.set_launch(lanes=8, grid=2048, check_every=16, build_lanes=8)
+    e.run(100, energy_every=0, stats=False)
+    st = e.run(1000, energy_every=50)
+    m = st["has_energy"][:, 0] == 1
+    H = st["conserved"][m, 0]
+    ke = st["ke"][m, 0]
+    span = H.max() - H.min()
+    say("  H span", span, "KE", ke.mean(), "T", st["temp"][-1, 0])
+    assert np.all(np.isfinite(H))
+    assert span < 0.02 * ke.mean()
+    assert 300.0 < st["temp"][:, 0].min() and st["temp"][:, 0].max() < 1300.0
+    e.close()
