@@ -81,7 +81,7 @@ def kernel_time(e, nsteps):
 
 
 def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=None, force_check=None,
-                  launch=None):
+                  launch=None, calibrate=False):
     from md_gpu_amd import engine as E
     log(f"{name}: setting up")
     e = E.Engine.from_system(systems)
@@ -117,6 +117,11 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
                      "traffic": None, "algorithmic_flop_per_launch": flops},
         "step_roofline_frac": flops / (1e-3 * 1e3 * wall / steps) / 1e12 / PEAK_FP32,
     }
+    if calibrate:
+        nbytes = 64 << 20
+        cms = e.copy_bench(nbytes, 5)
+        out["hbm_copy_GBps"] = 2 * nbytes / (cms * 1e-3) / 1e9   # read + write of a 64 MiB device copy
+        out["rebuild_ms"] = e.rebuild_bench(10)                    # one forced full list rebuild
     if force_check is not None:
         out["force_rel_rms"] = force_check(e, systems)
     if cpu_fn is not None:
@@ -187,7 +192,7 @@ def main():
                 "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
 
     res = step_workload("nacl32768", nacl, f_nacl, args.steps, args.warmup, tune, nacl.dt, cpu_nacl,
-                        (lambda e, s: force_error(e, s)) if want_cpu else None, launch)
+                        (lambda e, s: force_error(e, s)) if want_cpu else None, launch, calibrate=True)
     line = {
         "metric": METRIC, "value": res["value"], "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -199,6 +204,7 @@ def main():
         "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline"),
         "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_per_step": res["rebuild_chain_ms_per_step"],
         "rebuilds_per_step": res["rebuilds_per_step"], "launch": res["launch"],
+        "rebuild_ms": res.get("rebuild_ms"), "hbm_copy_GBps": res.get("hbm_copy_GBps"),
     }
     traffic = os.path.join(ROOT, "profiles", "step_kernel_nacl32768_traffic.json")
     if os.path.exists(traffic):
@@ -206,6 +212,7 @@ def main():
             tr = json.load(f)
         line["roofline"]["traffic"] = tr.get("bytes_per_launch")
         line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
+        line["roofline"]["valu_busy_counters"] = tr.get("valu_busy")
     work = {}
     if not args.quick:
         work = extra_workloads(args, want_cpu, tune)

@@ -215,7 +215,7 @@ struct md_engine {
     int cap = 0;
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0;
+    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 1;
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -392,6 +392,7 @@ BuildArgs build_args(md_engine* e) {
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.build_lanes = e->build_lanes;
     b.row_g = e->lanes;
+    b.build_kernel = e->build_kernel;
     return b;
 }
 
@@ -1103,15 +1104,18 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
         return fail(MD_E_INVALID, "build_lanes must be 4 or 8");
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
+    if (c->build_kernel != 0 && c->build_kernel != 1) return fail(MD_E_INVALID, "build_kernel must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
-    const bool relayout = c->lanes != e->lanes;   // the row layout (lpos) follows the step kernel's lanes
+    // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
+    const bool relayout = c->lanes != e->lanes || c->build_kernel != e->build_kernel || c->build_lanes != e->build_lanes;
     e->lanes = c->lanes;
     e->grid = c->grid;
     e->check_every = c->check_every;
     e->build_lanes = c->build_lanes;
     e->submit = c->submit;
     e->refresh = c->refresh;
+    e->build_kernel = c->build_kernel;
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
@@ -1134,6 +1138,7 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->build_lanes = e->build_lanes;
     c->submit = e->submit;
     c->refresh = e->refresh;
+    c->build_kernel = e->build_kernel;
     return MD_OK;
 }
 
@@ -1180,10 +1185,13 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
     double best_build = 1e30;
     for (int d = 1; d <= 3; d++)
-        for (int bl : {4, 8}) {
+        for (int bk : {0, 1})
+            for (int bl : {4, 8}) {
+            if (bk == 1 && bl != 8) continue;   // the per-wave kernel uses 8 lanes per atom
             md_launch_config c = best;
             c.cell_div = d;
             c.build_lanes = bl;
+            c.build_kernel = bk;
             CHECK(md_set_launch(e, &c));
             CHECK(restore());
             HIPCHECK(hipEventRecord(t0, e->st));
@@ -1195,7 +1203,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             HIPCHECK(hipEventSynchronize(t1));
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-            if (ms < best_build) { best_build = ms; best.cell_div = d; best.build_lanes = bl; }
+            if (ms < best_build) { best_build = ms; best.cell_div = d; best.build_lanes = bl; best.build_kernel = bk; }
         }
     // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events
     double best_step = 1e30;

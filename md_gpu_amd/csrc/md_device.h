@@ -204,4 +204,5 @@ struct BuildArgs {
     int has_excl;
     int build_lanes;      // lanes per atom in k_nl_build (4, 8)
     int row_g;            // lanes per atom of the step kernel (row layout, lpos)
+    int build_kernel;     // 0: k_nl_build (tile-staged), 1: k_nl_wbuild (per wavefront)
 };

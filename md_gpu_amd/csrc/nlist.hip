@@ -14,7 +14,8 @@
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
 //   k_nl_permute  deterministic order inside a cell (by original index);
 //                 gathers pos/vel/force/id/type into scratch in the new order
-//   k_nl_build    one workgroup per 64-slot tile, LPA (4, 8) lanes per atom:
+//   k_nl_build    (or k_nl_wbuild, below: the same per wavefront without block
+//                 synchronisation) one workgroup per 64-slot tile, LPA (4, 8) lanes per atom:
 //                 the image cells within r_list of the tile's bounding box are
 //                 listed in LDS, their members streamed through LDS in
 //                 double-buffered chunks of 64*LPA, each tested with the
@@ -430,6 +431,215 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     }
 }
 
+// Per-wave list build: one wavefront per APW = 64/LPA consecutive slots.  Each
+// wave lists the image cells within r_list of its own atoms' bounding box (a
+// wave-private table in LDS), streams their members in chunks of 64 (one
+// candidate per lane; the next chunk's loads are in flight while the current
+// one is tested), drops candidates whose image is r_list or farther from the
+// box, tests the rest with the canonical predicate and appends hits in
+// candidate order.  Hits are gathered per atom in LDS and written to the row
+// in bursts.  Waves never wait for each other until the tile's epilogue.
+template <bool EXCL>
+__global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
+    constexpr int LPA = 8, APW = 64 / LPA, NW = 8;   // 8 waves x 8 atoms = one 64-slot tile
+    constexpr int WT = 128;                         // table entries per wave and pass
+    constexpr int HB = 64;                          // buffered hits per atom
+    __shared__ int s_tstart[NW][WT];
+    __shared__ int s_tpref[NW][WT + 1];
+    __shared__ int s_tshift[NW][WT];
+    __shared__ float4 s_cpos[NW][64];               // staged candidates (x, y, z, bits of j | tj << 29)
+    __shared__ float4 s_csh[NW][64];                // (-kx L, -ky L, -kz L, bits of jid)
+    __shared__ uint32_t s_hits[NW][APW][HB];
+    if (a.gate->n_stale == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tile = blockIdx.x;
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
+    const RepState st = a.st[r];
+    if (!stale(st)) return;
+    const RepParam& p = a.rep[r];
+    const int cb = cur_buf(st);
+    const int grp = lane / LPA, sub = lane % LPA;
+    const int slot = tile * 64 + wave * APW + grp;
+    const int id = a.ids_s[slot];
+    const bool active = id >= 0;
+    const float4 xi = a.pos_s[slot];
+    int4 ex = make_int4(-1, -1, -1, -1);
+    if (EXCL && active) ex = a.excl[id];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    auto wave_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
+    // the wave's box over its active atoms
+    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
+    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
+    for (int o = 32; o >= LPA; o >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
+        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+    }
+    int cnt = 0;        // hits of this lane's atom (all lanes of the group agree)
+    int nbuf = 0;       // buffered, not yet written
+    uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
+    const int G = a.row_g;
+    // write the buffered hits of every atom of the wave to the rows (lanes of a group
+    // write consecutive logical entries)
+    auto flush = [&]() {
+        wave_sync();
+        const int base = cnt - nbuf;
+        for (int k = sub; k < nbuf; k += LPA) {
+            const int e = base + k;
+            if (e < a.cap) rowp[lpos(e, G)] = s_hits[wave][grp][k];
+        }
+        nbuf = 0;
+        wave_sync();
+    };
+    if (lx <= hx) {
+        const float rlp = sqrtf(a.rl2) * 1.0001f + 1e-4f;
+        const float rlp2 = rlp * rlp;
+        const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
+        const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
+        const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
+        const int nreg = ax.nr * ay.nr * az.nr;
+        const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
+        auto probe = [=](int c, int& cs, int& shift) -> int {
+            int kx, ky, kz, wx, wy, wz;
+            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
+            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
+            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
+            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
+            cs = a.cell_start[cs_off + rank];
+            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
+            return a.cell_start[cs_off + rank + 1] - cs;
+        };
+        int c0 = 0;
+        while (c0 < nreg) {
+            // ---- wave table: up to WT non-empty cells, in region order
+            int ntab = 0, T = 0;
+            while (c0 < nreg && ntab <= WT - 64) {
+                const int c = c0 + lane;
+                int cs = 0, sh = 0, n = 0;
+                if (c < nreg) n = probe(c, cs, sh);
+                const unsigned long long m = __ballot(n > 0);
+                int inc = n;   // inclusive scan of counts over the wave
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(inc, o, 64);
+                    if (lane >= o) inc += t;
+                }
+                if (n > 0) {
+                    const int k = ntab + __popcll(m & below);
+                    s_tstart[wave][k] = cs;
+                    s_tpref[wave][k] = T + inc - n;
+                    s_tshift[wave][k] = sh;
+                }
+                ntab += __popcll(m);
+                T += __shfl(inc, 63, 64);
+                c0 += 64;
+            }
+            if (lane == 0) s_tpref[wave][ntab] = T;
+            wave_sync();
+            // ---- stream the table's candidates, 64 per chunk
+            auto fetch = [&](int q, float4& rp, float4& rs) -> bool {
+                if (q >= T) return false;
+                int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
+                while (lo_ < hi_) {
+                    const int mid = (lo_ + hi_ + 1) >> 1;
+                    if (s_tpref[wave][mid] <= q) lo_ = mid; else hi_ = mid - 1;
+                }
+                const int j = s_tstart[wave][lo_] + (q - s_tpref[wave][lo_]);
+                const int sh = s_tshift[wave][lo_];
+                rp = a.pos_s[j];
+                // -k L is exact for the |k| <= 1 images that can pass the predicate
+                rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
+                                 -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
+                const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
+                const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
+                const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
+                const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
+                rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
+                return gx * gx + gy * gy + gz * gz < rlp2;
+            };
+            float4 rp = make_float4(0, 0, 0, 0), rs = rp;
+            bool keep = fetch(lane, rp, rs);
+            for (int q0 = 0; q0 < T; q0 += 64) {
+                // stage this chunk's survivors (stable compaction), then fetch the next chunk
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int k = __popcll(m & below);
+                    s_cpos[wave][k] = rp;
+                    s_csh[wave][k] = rs;
+                }
+                const int wn = __popcll(m);
+                keep = fetch(q0 + 64 + lane, rp, rs);
+                wave_sync();
+                for (int u = 0; u < wn; u += LPA) {
+                    const int q = u + sub;
+                    bool ok = false;
+                    uint32_t ent = 0;
+                    if (q < wn) {
+                        const float4 pj = s_cpos[wave][q];
+                        const float4 sj = s_csh[wave][q];
+                        // (xi - xj) - k L: the canonical minimum-image displacement for this image
+                        const float dx = (xi.x - pj.x) + sj.x;
+                        const float dy = (xi.y - pj.y) + sj.y;
+                        const float dz = (xi.z - pj.z) + sj.z;
+                        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        ent = (uint32_t)__float_as_int(pj.w);
+                        ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
+                        if (EXCL) {
+                            const int jid = __float_as_int(sj.w);
+                            ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
+                        }
+                    }
+                    // rank among this atom's lanes: hits stay in candidate order
+                    const unsigned long long hm = __ballot(ok);
+                    const uint32_t gb = (uint32_t)(hm >> (lane & ~(LPA - 1))) & 0xFFu;
+                    if (ok) s_hits[wave][grp][nbuf + __popc(gb & ((1u << sub) - 1u))] = ent;
+                    const int add = __popc(gb);
+                    cnt += add;
+                    nbuf += add;
+                    // some atom's buffer may not take another LPA hits: write all out
+                    const unsigned long long full = __ballot(nbuf > HB - LPA);
+                    if (full) flush();
+                }
+                wave_sync();
+            }
+        }
+    }
+    if (__ballot(nbuf > 0)) flush();   // wave-uniform
+    if (sub == 0) a.ncount[slot] = active ? cnt : 0;
+    int mx = sub == 0 ? cnt : 0;
+    for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) atomicMax(&a.maxcount[r], mx);
+    __syncthreads();
+    // copy the tile back from scratch into the primary arrays (new order)
+    if (tid < 64) {
+        const int s = tile * 64 + tid;
+        const float4 x = a.pos_s[s];
+        a.pos[cb][s] = x;
+        a.pos_build[s] = x;
+        a.vel[s] = a.vel_s[s];
+        a.force[s] = a.force_s[s];
+        a.ids[s] = a.ids_s[s];
+        a.typ[s] = a.typ_s[s];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // the last tile of the replica opens the new list epoch
+        const int old = atomicAdd(&a.st[r].build_done, 1);
+        if (old == p.nslot / 64 - 1) {
+            a.st[r].build_done = 0;
+            a.st[r].age = 0;
+            a.st[r].nb = st.nb + 1;
+            atomicSub(&a.gate->n_stale, 1);
+            if (st.phase == 0 && st.step < st.target) atomicAdd(&a.gate->n_run, 1);
+            if (st.phase == 1) atomicAdd(&a.gate->n_force, 1);
+        }
+    }
+}
+
 // Mark the lists of all replicas stale (host-requested full rebuild).
 __global__ void k_force_trip(RepState* st, Gate* g, int nrep) {
     for (int r = threadIdx.x; r < nrep; r += blockDim.x) st[r].trip[st[r].nb & 1] = 1;
@@ -449,6 +659,11 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
     const dim3 ntiles(a.nslot / 64);
+    if (a.build_kernel == 1) {
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_wbuild<true>), ntiles, dim3(512), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_wbuild<false>), ntiles, dim3(512), 0, st, a);
+        return hipGetLastError();
+    }
     switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
     case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
     case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;

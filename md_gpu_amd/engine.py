@@ -26,7 +26,8 @@ ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "call out of order", -4: 
 class LaunchConfig(C.Structure):
     """md_launch_config (md_api.h)."""
     _fields_ = [("lanes", C.c_int), ("grid", C.c_int), ("cell_div", C.c_int), ("check_every", C.c_int),
-                ("build_lanes", C.c_int), ("submit", C.c_int), ("refresh", C.c_int)]
+                ("build_lanes", C.c_int), ("submit", C.c_int), ("refresh", C.c_int),
+                ("build_kernel", C.c_int)]
 
     @classmethod
     def names(cls):

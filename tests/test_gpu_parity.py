@@ -57,11 +57,13 @@ def oracle_forces(orc, s):
     return orc.forces(s)
 
 
+@pytest.mark.parametrize("build_kernel", [0, 1])
 @pytest.mark.parametrize("name", list(CASES))
-def test_neighbor_sets_bit_exact(E, orc, name):
+def test_neighbor_sets_bit_exact(E, orc, name, build_kernel):
     s = CASES[name]
-    say("neighbors", name, s.n)
+    say("neighbors", name, s.n, "build kernel", build_kernel)
     e = E.Engine.from_system(s)
+    e.set_launch(build_kernel=build_kernel)
     pos, _, _ = e.download()
     rc2, rl2, sk = e.thresholds()
     orc_rc2, orc_rl2, orc_sk, _ = orc.thresholds(s.ff.rc, s.ff.skin, s.box)
@@ -251,10 +253,11 @@ def test_logged_reference_forces_nacl_rf(E, orc):
 
 
 LAUNCH_VARIANTS = [
-    dict(submit=1, refresh=0, check_every=4, build_lanes=4, lanes=4),
-    dict(submit=0, refresh=1, check_every=4, build_lanes=8, lanes=8),
-    dict(submit=1, refresh=1, check_every=6, build_lanes=4, lanes=32),
-    dict(submit=0, refresh=0, check_every=12, build_lanes=8, lanes=16, cell_div=3),
+    dict(submit=1, refresh=0, check_every=4, build_kernel=0, build_lanes=4, lanes=4),
+    dict(submit=0, refresh=1, check_every=4, build_kernel=0, build_lanes=8, lanes=8),
+    dict(submit=1, refresh=1, check_every=6, build_kernel=1, lanes=32),
+    dict(submit=0, refresh=0, check_every=12, build_kernel=1, lanes=16, cell_div=3),
+    dict(submit=0, refresh=0, check_every=8, build_kernel=1, lanes=8, cell_div=1),
 ]
 
 
