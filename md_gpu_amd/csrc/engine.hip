@@ -31,6 +31,7 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st);
 extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st);
 extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st);
+extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int grid, hipStream_t st);
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st);
@@ -192,6 +193,7 @@ struct md_engine {
     DBuf<int> cell_count, cell_start, cell_rank;
     DBuf<uint32_t> nlist;
     DBuf<int> ncount, maxcount;
+    DBuf<int> tilemax, tflag;      // [S/64] per-tile row maximum (list build), skin-check flag (kick)
     DBuf<int4> excl, ang_list;
     DBuf<uint32_t> bond_list;
     DBuf<int> nbond, nang;
@@ -200,7 +202,7 @@ struct md_engine {
     DBuf<RepState> rst;
     DBuf<Gate> gate;
     DBuf<ThState> th;
-    DBuf<double> acc, fm_acc;
+    DBuf<double> part, fm_acc;     // part: [S/8][MD_NSTAT] partial sums of the step kernel
     DBuf<int> err;
     DBuf<unsigned long long> reb_count;
     DBuf<StatRec> stats;
@@ -215,7 +217,7 @@ struct md_engine {
     int cap = 0;
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 16, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 1;
+    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 0;
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -387,6 +389,7 @@ BuildArgs build_args(md_engine* e) {
     b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p; b.tmpid = e->tmpid.p;
     b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
     b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p; b.maxcount = e->maxcount.p;
+    b.tilemax = e->tilemax.p;
     b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p; b.rebuild_count = e->reb_count.p;
     b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
@@ -400,7 +403,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     StepArgs a{};
     a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
     a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
-    a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.acc = e->acc.p;
+    a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
     a.ctab = e->ctab.p;
@@ -425,7 +428,7 @@ KickArgs kick_args(md_engine* e, int apply_drift) {
     KickArgs k{};
     k.pos[0] = e->pos[0].p; k.pos[1] = e->pos[1].p; k.vel = e->vel.p; k.force = e->force.p; k.ids = e->ids.p;
     k.pos_build = e->pos_build.p; k.tile_rep = e->tile_rep.p; k.rep = e->rep.p; k.st = e->rst.p; k.gate = e->gate.p; k.th = e->th.p;
-    k.err = e->err.p; k.nslot = e->S; k.nrep = e->R; k.dt = e->in.dt; k.skin2q = e->skin2q; k.apply_drift = apply_drift;
+    k.err = e->err.p; k.tflag = e->tflag.p; k.nslot = e->S; k.nrep = e->R; k.dt = e->in.dt; k.skin2q = e->skin2q; k.apply_drift = apply_drift;
     return k;
 }
 
@@ -463,6 +466,7 @@ int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts) {
     if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
     HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st));
     if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->steps.push_back(ev); }
+    HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
     return MD_OK;
 }
 
@@ -700,8 +704,10 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->snap_gate.alloc(1));
     CHECK(e->th.alloc(nrep));
     CHECK(e->snap_th.alloc(nrep));
-    CHECK(e->acc.alloc((size_t)nrep * MD_NSTAT));
+    CHECK(e->part.alloc((size_t)(S / 8) * MD_NSTAT));   // units hold at least 8 slots (lanes <= 32)
     CHECK(e->maxcount.alloc(nrep));
+    CHECK(e->tilemax.alloc(S / MD_TILE));
+    CHECK(e->tflag.alloc(S / MD_TILE));
     e->stat_cap = 0;
     CHECK(ensure_stats(e, 1));
     std::vector<int> neg(S, -1);
@@ -819,7 +825,6 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
     HIPCHECK(hipMemcpy(e->slot_of.p, slot_of.data(), e->N * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->th.p, th.data(), e->R * sizeof(ThState), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->rst.p, rs.data(), e->R * sizeof(RepState), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(e->acc.p, 0, e->acc.bytes()));
     HIPCHECK(hipMemset(e->err.p, 0, sizeof(int)));
     e->has_state = true;
     CHECK(full_rebuild(e));
@@ -1104,7 +1109,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
         return fail(MD_E_INVALID, "build_lanes must be 4 or 8");
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
-    if (c->build_kernel != 0 && c->build_kernel != 1) return fail(MD_E_INVALID, "build_kernel must be 0 or 1");
+    if (c->build_kernel < 0 || c->build_kernel > 2) return fail(MD_E_INVALID, "build_kernel must be 0, 1 or 2");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
@@ -1185,9 +1190,9 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
     double best_build = 1e30;
     for (int d = 1; d <= 3; d++)
-        for (int bk : {0, 1})
+        for (int bk : {0, 1, 2})
             for (int bl : {4, 8}) {
-            if (bk == 1 && bl != 8) continue;   // the per-wave kernel uses 8 lanes per atom
+            if (bk != 0 && bl != 8) continue;   // build_lanes only applies to the tile-staged kernel
             md_launch_config c = best;
             c.cell_div = d;
             c.build_lanes = bl;

@@ -51,11 +51,8 @@ struct RepState {
     int phase;         // velocity Verlet: 0 = between steps, 1 = kicked/drifted, forces pending
     int nb;            // completed neighbor-list builds
     int trip[2];       // trip[nb & 1] != 0: the list is stale for the current positions
-    int trip_new;      // raised (atomically) during a position update
-    int units_done;    // ticket of the step kernel
-    int tiles_done;    // ticket of the kick kernel
-    int build_done;    // ticket of the list build
     int age;           // steps completed since the last build
+    int pad;
 };
 
 // Device-wide counters that let a launch with nothing to do exit on its first
@@ -74,6 +71,11 @@ struct Gate {
 // lists the members of one cell together) across the atom's lanes.
 __host__ __device__ inline int lpos(int e, int G) {
     return (e & ~(4 * G - 1)) + 4 * (e % G) + ((e / G) & 3);
+}
+// the same for G = 1 << gs (shifts and masks only)
+__device__ __forceinline__ int lpos_sh(int e, int gs) {
+    const int G = 1 << gs;
+    return (e & ~(4 * G - 1)) | ((e & (G - 1)) << 2) | ((e >> gs) & 3);
 }
 
 // current position buffer of a replica
@@ -115,7 +117,7 @@ struct StepArgs {
     RepState* st;
     Gate* gate;
     ThState* th;
-    double* acc;          // [R][MD_NSTAT]
+    double* part;         // [nunits][MD_NSTAT] FP64 partial sums, one slot per run of units
     int* err;             // [1] non-finite coordinates
     StatRec* stats;       // [stat_cap][R] or null
     const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)
@@ -160,6 +162,7 @@ struct KickArgs {
     Gate* gate;
     const ThState* th;
     int* err;
+    int* tflag;           // [S/64] skin check tripped in the 64-slot tile
     int nslot;
     int nrep;
     float dt, skin2q;
@@ -190,7 +193,8 @@ struct BuildArgs {
     int* slot_of;         // [N]
     uint32_t* nlist;
     int* ncount;
-    int* maxcount;        // [R]
+    int* maxcount;        // [R] longest row of the replica's new list
+    int* tilemax;         // [S/64] longest row per 64-slot tile
     const int4* excl;     // [N] 1-2 partners (-1 empty)
     RepState* st;
     Gate* gate;

@@ -4,7 +4,7 @@
 // which never compiled; the list semantics are those of the O(N^2) build it
 // replaces, src/neighborlist.cu:231-297).
 //
-// A rebuild is five kernels.  Each runs only for replicas whose list is stale
+// A rebuild is six kernels.  Each runs only for replicas whose list is stale
 // (RepState.trip, raised by the position update when some atom moved by at
 // least skin/2 since the last build: the skin criterion of
 // src/neighborlist.cu:157-176), so the chain can sit in every graph segment:
@@ -14,15 +14,16 @@
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
 //   k_nl_permute  deterministic order inside a cell (by original index);
 //                 gathers pos/vel/force/id/type into scratch in the new order
-//   k_nl_build    (or k_nl_wbuild, below: the same per wavefront without block
-//                 synchronisation) one workgroup per 64-slot tile, LPA (4, 8) lanes per atom:
-//                 the image cells within r_list of the tile's bounding box are
-//                 listed in LDS, their members streamed through LDS in
-//                 double-buffered chunks of 64*LPA, each tested with the
-//                 canonical FP32 predicate; hits are appended in candidate
-//                 order (ballot + popcount inside the atom's LPA lanes), so
-//                 rows are deterministic.  The last workgroup of a replica bumps its
-//                 build epoch, which marks the list valid.
+//   build         one workgroup per 64-slot tile; the image cells within r_list
+//                 of a bounding box of atoms are listed in LDS and their members
+//                 streamed and tested with the canonical FP32 predicate.  Three
+//                 variants (build_kernel): k_nl_build (LPA lanes per atom, tile
+//                 box, double-buffered block staging), k_nl_wbuild (8 lanes per
+//                 atom, per-wavefront boxes, LDS hit buffers) and k_nl_jbuild
+//                 (a lane per candidate, 16 atoms per wavefront, one ballot per
+//                 atom and chunk).  Hits are appended in candidate order, so
+//                 rows are deterministic.  Each tile stores its longest row.
+//   k_nl_commit   per rebuilt replica: longest row -> maxcount, new list epoch.
 // Bytes per atom per rebuild (algorithmic): read pos/vel/force/id/type 56 B,
 // write them twice (scratch + copy back) 112 B, plus the list 4*K B.
 #include "md_device.h"
@@ -147,10 +148,7 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
         run += c;
         a.cell_count[p.cell_off + k] = 0;   // ready for the next rebuild
     }
-    if (threadIdx.x == 0) {
-        a.cell_start[p.cs_off + nc] = p.slot_off + p.natom;
-        atomicAdd(a.rebuild_count, 1ULL);
-    }
+    if (threadIdx.x == 0) a.cell_start[p.cs_off + nc] = p.slot_off + p.natom;
 }
 
 __global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
@@ -204,6 +202,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     __shared__ int s_tpref[kTab + 1];
     __shared__ int s_tshift[kTab];
     __shared__ int s_w[NW];
+    __shared__ int s_wmax[NW];
     __shared__ int s_n[2];               // staged candidates per buffer
     __shared__ unsigned short s_widx[NW][NT];   // per wave: staged candidates near the wave's box
     __shared__ float s_bb[NW][6];
@@ -244,6 +243,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     }
     int cnt = 0;
     uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
+    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
     if (lx <= hx) {
         const float rlp = sqrtf(a.rl2) * 1.0001f + 1e-4f;
         const float rlp2 = rlp * rlp;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
                 const unsigned long long m = __ballot(ok);
                 const uint32_t gb = (uint32_t)(m >> (lane & ~(LPA - 1))) & (uint32_t)((1ull << LPA) - 1);
                 const int before = __popc(gb & ((1u << sub) - 1u));
-                if (ok && cnt + before < a.cap) rowp[lpos(cnt + before, a.row_g)] = ent;
+                if (ok && cnt + before < a.cap) rowp[lpos_sh(cnt + before, gs)] = ent;
                 cnt += __popc(gb);
             };
             const int nch = (T + NT - 1) / NT;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     if (sub == 0) a.ncount[slot] = active ? cnt : 0;
     int m = sub == 0 ? cnt : 0;
     for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if (lane == 0) atomicMax(&a.maxcount[r], m);
+    if (lane == 0) s_wmax[wave] = m;
     // copy the tile back from scratch into the primary arrays (new order)
     if (tid < 64) {
         const int s = tile * 64 + tid;
@@ -417,17 +417,10 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
         a.typ[s] = a.typ_s[s];
     }
     __syncthreads();
-    if (tid == 0) {
-        // the last tile of the replica opens the new list epoch
-        const int old = atomicAdd(&a.st[r].build_done, 1);
-        if (old == p.nslot / 64 - 1) {
-            a.st[r].build_done = 0;
-            a.st[r].age = 0;
-            a.st[r].nb = st.nb + 1;
-            atomicSub(&a.gate->n_stale, 1);
-            if (st.phase == 0 && st.step < st.target) atomicAdd(&a.gate->n_run, 1);
-            if (st.phase == 1) atomicAdd(&a.gate->n_force, 1);
-        }
+    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
+        int m = s_wmax[0];
+        for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
+        a.tilemax[tile] = m;
     }
 }
 
@@ -450,6 +443,7 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
     __shared__ float4 s_cpos[NW][64];               // staged candidates (x, y, z, bits of j | tj << 29)
     __shared__ float4 s_csh[NW][64];                // (-kx L, -ky L, -kz L, bits of jid)
     __shared__ uint32_t s_hits[NW][APW][HB];
+    __shared__ int s_wmax[NW];
     if (a.gate->n_stale == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
@@ -482,7 +476,7 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
     int cnt = 0;        // hits of this lane's atom (all lanes of the group agree)
     int nbuf = 0;       // buffered, not yet written
     uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
-    const int G = a.row_g;
+    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
     // write the buffered hits of every atom of the wave to the rows (lanes of a group
     // write consecutive logical entries)
     auto flush = [&]() {
@@ -490,7 +484,7 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
         const int base = cnt - nbuf;
         for (int k = sub; k < nbuf; k += LPA) {
             const int e = base + k;
-            if (e < a.cap) rowp[lpos(e, G)] = s_hits[wave][grp][k];
+            if (e < a.cap) rowp[lpos_sh(e, gs)] = s_hits[wave][grp][k];
         }
         nbuf = 0;
         wave_sync();
@@ -612,7 +606,7 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
     if (sub == 0) a.ncount[slot] = active ? cnt : 0;
     int mx = sub == 0 ? cnt : 0;
     for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) atomicMax(&a.maxcount[r], mx);
+    if (lane == 0) s_wmax[wave] = mx;
     __syncthreads();
     // copy the tile back from scratch into the primary arrays (new order)
     if (tid < 64) {
@@ -626,17 +620,292 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
         a.typ[s] = a.typ_s[s];
     }
     __syncthreads();
-    if (tid == 0) {
-        // the last tile of the replica opens the new list epoch
-        const int old = atomicAdd(&a.st[r].build_done, 1);
-        if (old == p.nslot / 64 - 1) {
-            a.st[r].build_done = 0;
-            a.st[r].age = 0;
-            a.st[r].nb = st.nb + 1;
-            atomicSub(&a.gate->n_stale, 1);
-            if (st.phase == 0 && st.step < st.target) atomicAdd(&a.gate->n_run, 1);
-            if (st.phase == 1) atomicAdd(&a.gate->n_force, 1);
+    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
+        int m = s_wmax[0];
+        for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
+        a.tilemax[tile] = m;
+    }
+}
+
+// Candidate-parallel build: one workgroup (4 wavefronts) per 64-slot tile,
+// AW = 16 atoms per wavefront.  The wavefront lists the image cells within
+// r_list of its atoms' bounding box, streams their members, keeps those whose
+// image lies within r_list of the box (stable compaction into a 128-entry LDS
+// ring) and, per chunk of 64 kept candidates, lets each lane test one
+// candidate against the 16 atoms in turn.  The hits of one atom in a chunk are
+// one ballot: they go to count + (hits in lower lanes), so rows are filled in
+// candidate order with coalesced stores and no per-hit LDS traffic.
+template <bool EXCL>
+__global__ __launch_bounds__(256) void k_nl_jbuild(BuildArgs a) {
+    constexpr int NW = 4, AW = 16, WT = 256, RQ = 128;
+    __shared__ int s_tstart[NW][WT];
+    __shared__ int s_tpref[NW][WT + 1];
+    __shared__ int s_tshift[NW][WT];
+    __shared__ float4 s_qp[NW][RQ];     // ring of kept candidates (x, y, z, bits of j | tj << 29)
+    __shared__ float4 s_qs[NW][RQ];     // (-kx L, -ky L, -kz L, bits of jid)
+    __shared__ int4 s_ex[NW][EXCL ? AW : 1];
+    __shared__ int s_wmax[NW];
+    if (a.gate->n_stale == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tile = blockIdx.x;
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
+    const RepState st = a.st[r];
+    if (!stale(st)) return;
+    const RepParam& p = a.rep[r];
+    const int cb = cur_buf(st);
+    const int slot0 = tile * 64 + wave * AW;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    auto wave_sync = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    bool active = false;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < AW) {
+        const int slot = slot0 + lane;
+        const int id = a.ids_s[slot];
+        active = id >= 0;
+        xi = a.pos_s[slot];
+        if (EXCL) s_ex[wave][lane] = active ? a.excl[id] : make_int4(-1, -1, -1, -1);
+    }
+    // the wavefront's box over its active atoms
+    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
+    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
+    for (int o = 32; o >= 1; o >>= 1) {
+        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
+        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+    }
+    wave_sync();
+    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
+    const int cap = a.cap;
+    const float rl2 = a.rl2;
+    // the atoms as wavefront-uniform values (scalar registers); an empty slot sits
+    // far outside the box, so no candidate passes the distance test for it
+    float axr[AW], ayr[AW], azr[AW];
+    int cnt[AW];
+    {
+        const float fx = active ? xi.x : 1.0e30f, fy = active ? xi.y : 1.0e30f, fz = active ? xi.z : 1.0e30f;
+#pragma unroll
+        for (int i = 0; i < AW; i++) {
+            axr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fx), i));
+            ayr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fy), i));
+            azr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fz), i));
+            cnt[i] = 0;
         }
+    }
+
+    // test ring entries [head, head + n) against the wavefront's atoms
+    auto process = [&](int head, int n) {
+        const int q = (head + lane) & (RQ - 1);
+        const bool valid = lane < n;
+        const float4 pj = s_qp[wave][q];
+        const float4 sj = s_qs[wave][q];
+        const uint32_t ent = __float_as_uint(pj.w);
+        const int js = (int)(ent & MD_JMASK);
+        const int jid = __float_as_int(sj.w);
+#pragma unroll
+        for (int i = 0; i < AW; i++) {
+            const int si = slot0 + i;
+            // (xi - xj) - k L: the canonical minimum-image displacement for this image
+            const float dx = (axr[i] - pj.x) + sj.x;
+            const float dy = (ayr[i] - pj.y) + sj.y;
+            const float dz = (azr[i] - pj.z) + sj.z;
+            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            bool ok = valid & (rsq < rl2) & (js != si);
+            if (EXCL) {
+                const int4 ex = s_ex[wave][i];
+                ok = ok & (jid != ex.x) & (jid != ex.y) & (jid != ex.z) & (jid != ex.w);
+            }
+            const unsigned long long m = __ballot(ok);
+            if (m) {
+                const int e = cnt[i] + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (ok && e < cap) a.nlist[(size_t)si * cap + lpos_sh(e, gs)] = ent;
+                cnt[i] += __popcll(m);
+            }
+        }
+    };
+
+    if (lx <= hx) {
+        const float rlp = sqrtf(rl2) * 1.0001f + 1e-4f;
+        const float rlp2 = rlp * rlp;
+        const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
+        const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
+        const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
+        const int nreg = ax.nr * ay.nr * az.nr;
+        const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
+        // Cell probe without branches around the loads, so the loads of a group of
+        // probes are in flight together: a cell beyond r_list of the box reads
+        // entry 0 and reports no members.
+        auto probe = [=](int c, int& cs, int& shift) -> int {
+            int kx, ky, kz, wx, wy, wz;
+            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
+            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
+            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            const bool in = c < nreg && ddx * ddx + ddy * ddy + ddz * ddz < rlp2;
+            const int rank = a.cell_rank[in ? cell_off + wx + ncx * (wy + ncy * wz) : cell_off];
+            cs = a.cell_start[cs_off + rank];
+            const int ce = a.cell_start[cs_off + rank + 1];
+            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
+            return in ? ce - cs : 0;
+        };
+        // candidate q of the current table: binary search for its cell, then the
+        // loads (consumed one chunk later, so they overlap the chunk before)
+        struct Cand { float4 p; int typ, id, sh, j; bool valid; };
+        auto fetch = [&](int q, int ntab, int T) -> Cand {
+            Cand c;
+            c.valid = q < T;
+            int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
+            while (lo_ < hi_) {
+                const int mid = (lo_ + hi_ + 1) >> 1;
+                if (s_tpref[wave][mid] <= q) lo_ = mid; else hi_ = mid - 1;
+            }
+            c.j = c.valid ? s_tstart[wave][lo_] + (q - s_tpref[wave][lo_]) : slot0;
+            c.sh = s_tshift[wave][lo_];
+            c.p = a.pos_s[c.j];
+            c.typ = a.typ_s[c.j];
+            c.id = EXCL ? a.ids_s[c.j] : 0;
+            return c;
+        };
+        int head = 0, tail = 0;   // ring positions (wavefront-uniform, taken mod RQ)
+        int c0 = 0;
+        while (c0 < nreg) {
+            // ---- wavefront table: up to WT non-empty cells, in region order, probed
+            //      in groups of four 64-cell batches
+            int ntab = 0, T = 0;
+            while (c0 < nreg && ntab <= WT - 256) {
+                int cs[4], sh[4], n[4];
+#pragma unroll
+                for (int b = 0; b < 4; b++) n[b] = probe(c0 + 64 * b + lane, cs[b], sh[b]);
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const unsigned long long m = __ballot(n[b] > 0);
+                    int inc = n[b];   // inclusive scan of counts over the wavefront
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int t = __shfl_up(inc, o, 64);
+                        if (lane >= o) inc += t;
+                    }
+                    if (n[b] > 0) {
+                        const int k = ntab + __popcll(m & below);
+                        s_tstart[wave][k] = cs[b];
+                        s_tpref[wave][k] = T + inc - n[b];
+                        s_tshift[wave][k] = sh[b];
+                    }
+                    ntab += __popcll(m);
+                    T += __shfl(inc, 63, 64);
+                }
+                c0 += 256;
+            }
+            if (lane == 0) s_tpref[wave][ntab] = T;
+            wave_sync();
+            if (T == 0) continue;
+            // ---- stream the table's candidates, 64 at a time, through the ring
+            Cand nx = fetch(lane, ntab, T);
+            for (int q0 = 0; q0 < T; q0 += 64) {
+                const Cand cu = nx;
+                if (q0 + 64 < T) nx = fetch(q0 + 64 + lane, ntab, T);   // wavefront-uniform test
+                bool keep = false;
+                float4 rp = cu.p, rs = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (cu.valid) {
+                    // -k L is exact for the |k| <= 1 images that can pass the predicate
+                    rs = make_float4(-(float)((cu.sh & 15) - 8) * p.L[0], -(float)(((cu.sh >> 4) & 15) - 8) * p.L[1],
+                                     -(float)(((cu.sh >> 8) & 15) - 8) * p.L[2], __int_as_float(cu.id));
+                    const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
+                    const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
+                    const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
+                    const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
+                    rp.w = __int_as_float((int)((uint32_t)cu.j | ((uint32_t)cu.typ << MD_JBITS)));
+                    keep = gx * gx + gy * gy + gz * gz < rlp2;
+                }
+                const unsigned long long m = __ballot(keep);
+                if (keep) {
+                    const int k = (tail + __popcll(m & below)) & (RQ - 1);
+                    s_qp[wave][k] = rp;
+                    s_qs[wave][k] = rs;
+                }
+                tail += __popcll(m);
+                wave_sync();
+                if (tail - head >= 64) {   // at most 127 entries are pending
+                    process(head, 64);
+                    head += 64;
+                    wave_sync();
+                }
+            }
+        }
+        if (tail > head) process(head, tail - head);
+    }
+    int mine = 0, mx = 0;
+#pragma unroll
+    for (int i = 0; i < AW; i++) {
+        if (lane == i) mine = cnt[i];
+        mx = max(mx, cnt[i]);
+    }
+    if (lane < AW) a.ncount[slot0 + lane] = active ? mine : 0;
+    if (lane == 0) s_wmax[wave] = mx;
+    __syncthreads();
+    // copy the tile back from scratch into the primary arrays (new order)
+    if (tid < 64) {
+        const int s = tile * 64 + tid;
+        const float4 x = a.pos_s[s];
+        a.pos[cb][s] = x;
+        a.pos_build[s] = x;
+        a.vel[s] = a.vel_s[s];
+        a.force[s] = a.force_s[s];
+        a.ids[s] = a.ids_s[s];
+        a.typ[s] = a.typ_s[s];
+    }
+    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
+        int m = s_wmax[0];
+        for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
+        a.tilemax[tile] = m;
+    }
+}
+
+// Last kernel of the rebuild chain: for every rebuilt replica (one wavefront
+// each) the longest row over its tiles goes into maxcount (max over the run's
+// rebuilds; the host checks it against the capacity), and the new list epoch
+// opens.  The gate counters change once per workgroup.  No early exit: every
+// wavefront reaches the barrier.
+__global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a) {
+    __shared__ int s_d[4][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wave;
+    int done = 0, to_run = 0, to_force = 0;
+    if (r < a.nrep) {
+        const RepState st = a.st[r];         // independent loads, issued together
+        const int slot_off = a.rep[r].slot_off, nslot = a.rep[r].nslot;
+        const int mc0 = a.maxcount[r];
+        if (stale(st)) {
+            const int t0 = slot_off / 64, nt = nslot / 64;
+            int m = mc0;
+            for (int t = lane; t < nt; t += 64) m = max(m, a.tilemax[t0 + t]);
+            for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+            if (lane == 0) {
+                a.maxcount[r] = m;
+                a.st[r].age = 0;
+                a.st[r].nb = st.nb + 1;
+            }
+            done = 1;
+            to_run = st.phase == 0 && st.step < st.target;
+            to_force = st.phase == 1;
+        }
+    }
+    if (lane == 0) {
+        s_d[wave][0] = done;
+        s_d[wave][1] = to_run;
+        s_d[wave][2] = to_force;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0, nr = 0, nf = 0;
+        for (int w = 0; w < 4; w++) { n += s_d[w][0]; nr += s_d[w][1]; nf += s_d[w][2]; }
+        if (n) {
+            atomicSub(&a.gate->n_stale, n);
+            atomicAdd(a.rebuild_count, (unsigned long long)n);
+        }
+        if (nr) atomicAdd(&a.gate->n_run, nr);
+        if (nf) atomicAdd(&a.gate->n_force, nf);
     }
 }
 
@@ -659,18 +928,22 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
     const dim3 ntiles(a.nslot / 64);
-    if (a.build_kernel == 1) {
+    if (a.build_kernel == 2) {
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true>), ntiles, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_jbuild<false>), ntiles, dim3(256), 0, st, a);
+    } else if (a.build_kernel == 1) {
         if (a.has_excl) hipLaunchKernelGGL((k_nl_wbuild<true>), ntiles, dim3(512), 0, st, a);
         else hipLaunchKernelGGL((k_nl_wbuild<false>), ntiles, dim3(512), 0, st, a);
-        return hipGetLastError();
+    } else {
+        switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
+        case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
+        case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;
+        case 104: hipLaunchKernelGGL((k_nl_build<true, 4>), ntiles, dim3(256), 0, st, a); break;
+        case 108: hipLaunchKernelGGL((k_nl_build<true, 8>), ntiles, dim3(512), 0, st, a); break;
+        default:  return hipErrorInvalidValue;
+        }
     }
-    switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
-    case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
-    case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;
-    case 104: hipLaunchKernelGGL((k_nl_build<true, 4>), ntiles, dim3(256), 0, st, a); break;
-    case 108: hipLaunchKernelGGL((k_nl_build<true, 8>), ntiles, dim3(512), 0, st, a); break;
-    default:  return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k_nl_commit, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

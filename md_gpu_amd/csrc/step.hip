@@ -17,10 +17,12 @@
 // neighbor row), 64/G atoms per wavefront, 4 wavefronts per workgroup, and a
 // persistent grid walks contiguous ranges of 256/G-atom units.  Per-atom sums
 // are reduced across the G lanes with cross-lane shuffles; KE / energies /
-// virial are reduced per workgroup and added with FP64 atomics; the last
-// workgroup to finish a replica (atomic ticket) computes that replica's
-// temperature, thermostat update and statistics record, folds the skin-check
-// flag into the list state and advances the replica's step counter.
+// virial and the skin-check flag are reduced per workgroup into FP64 partial
+// slots (plain stores).  k_step_finalize, launched next, reduces a replica's
+// slots and computes its temperature, thermostat update and statistics record,
+// folds the skin-check flag into the list state and advances the step counter.
+// No device-scope atomics on shared addresses in the hot kernel: with a
+// thousand workgroups they serialise behind each other.
 // A replica is skipped when its list is stale or its run target is reached,
 // so a captured segment can be replayed with no host decision.
 //
@@ -32,10 +34,6 @@ namespace {
 
 __device__ __forceinline__ float minimg(float d, float L, float Linv) {
     return fmaf(-L, rintf(d * Linv), d);
-}
-
-__device__ __forceinline__ double atomic_load_f64(double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 
@@ -51,14 +49,13 @@ __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double 
     return s;
 }
 
-// Runs once per replica and kernel launch, by thread 0 of the last workgroup.
-__device__ void finalize_replica(const StepArgs& a, int r, long long kstat, bool en) {
-    const RepParam& p = a.rep[r];
-    RepState& st = a.st[r];
-    double* acc = a.acc + (size_t)r * MD_NSTAT;
-    double v[MD_NSTAT];
-    for (int k = 0; k < MD_NSTAT; k++) v[k] = atomic_load_f64(acc + k);
-    ThState& th = a.th[r];
+// Runs once per replica and step-kernel launch, by thread 0 of the replica's
+// workgroup in k_step_finalize; v = the replica's reduced partial sums, tn = the
+// skin check tripped during the position update.
+// p, st0 and th are register copies loaded at kernel start (their loads overlap
+// the partial-sum loads); the updated state is stored back at the end.
+__device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, const RepState& st0, ThState th,
+                                 long long kstat, bool en, const double* v, bool tn) {
     StatRec rec;
     rec.ke = rec.temp = rec.pe_lj = rec.pe_coul = rec.pe_bond = rec.pe_angle = rec.virial = rec.conserved = 0.0;
     rec.xi = th.xi;
@@ -108,27 +105,28 @@ __device__ void finalize_replica(const StepArgs& a, int r, long long kstat, bool
         rec.conserved = rec.ke + rec.pe_lj + rec.pe_coul + rec.pe_bond + rec.pe_angle + nh_terms;
         a.stats[(size_t)kstat * a.nrep + r] = rec;
     }
-    for (int k = 0; k < MD_NSTAT; k++) __hip_atomic_exchange(acc + k, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_exchange(&st.units_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.mode != MODE_FORCE) a.th[r] = th;
     // list refresh by age (ahead of the skin trip), so queued steps do not stall
-    if (a.mode != MODE_FORCE) st.age = st.age + 1;
-    const bool aged = a.max_age > 0 && st.age >= a.max_age;
+    if (a.mode == MODE_FORCE) return;
+    RepState& st = a.st[r];
+    const int age = st0.age + 1;
+    const long long step = st0.step + 1;
+    st.age = age;
+    st.step = step;
+    const bool aged = a.max_age > 0 && age >= a.max_age;
     if (a.mode == MODE_LF) {
         // skin check of the positions just written: the list is stale for them
-        const int tn = __hip_atomic_exchange(&st.trip_new, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const bool tr = tn || aged;
-        if (tr) st.trip[st.nb & 1] = 1;
-        st.step = st.step + 1;
+        if (tr) st.trip[st0.nb & 1] = 1;
         if (tr) atomicAdd(&a.gate->n_stale, 1);
-        if (tr || st.step >= st.target) atomicSub(&a.gate->n_run, 1);
-    } else if (a.mode == MODE_VV) {
+        if (tr || step >= st0.target) atomicSub(&a.gate->n_run, 1);
+    } else {
         st.phase = 0;
-        st.step = st.step + 1;
         atomicSub(&a.gate->n_force, 1);
         if (aged) {
-            st.trip[st.nb & 1] = 1;
+            st.trip[st0.nb & 1] = 1;
             atomicAdd(&a.gate->n_stale, 1);
-        } else if (st.step < st.target) {
+        } else if (step < st0.target) {
             atomicAdd(&a.gate->n_run, 1);
         }
     }
@@ -237,7 +235,6 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB : 1];
     static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     __shared__ double s_red[4][MD_NSTAT];
-    __shared__ int s_last;
     if (a.mode == MODE_LF && a.gate->n_run == 0) return;     // nothing can step: idle launch
     if (a.mode == MODE_VV && a.gate->n_force == 0) return;
     constexpr int APW = 64 / G;        // atoms per wavefront
@@ -250,17 +247,19 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
 
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
-    int cur_r = -1, nunits_r = 0;
+    int cur_r = -1, run_u0 = 0;
     bool run_r = false, en = false;
-    long long kstat = -1;
     const float4* pin = nullptr;
     float4* pout = nullptr;
     float L0 = 0, L1 = 0, L2 = 0, Li0 = 0, Li1 = 0, Li2 = 0, lam = 1.0f;
     double dke = 0.0, dlj = 0.0, dco = 0.0, dbo = 0.0, dan = 0.0, dvi = 0.0;
     int trip = 0;
 
+    // The workgroup's partial sums of a run of units of one replica go to the slot of
+    // the run's first unit (plain stores; k_step_finalize, the next kernel, reduces the
+    // replica's slots).  No atomics: same-address device atomics from every
+    // workgroup would serialise behind each other.
     auto flush = [&]() {
-        // workgroup reduction of the FP64 partials, one atomic per quantity
         double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, 0.0};
         for (int k = 0; k < MD_NSTAT - 2; k++)
             for (int o = 32; o >= 1; o >>= 1) vals[k] += __shfl_xor(vals[k], o, 64);
@@ -270,23 +269,8 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             s_red[wave][MD_NSTAT - 1] = tb ? 1.0 : 0.0;
         }
         __syncthreads();
-        if (tid == 0) {
-            double* acc = a.acc + (size_t)cur_r * MD_NSTAT;
-            double chk = 0.0;
-            for (int k = 0; k < MD_NSTAT - 2; k++) {
-                double t = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
-                if (t != 0.0) chk += atomicAdd(acc + k, t);   // the returned value orders the ticket below
-            }
-            const bool tr = (s_red[0][MD_NSTAT - 1] + s_red[1][MD_NSTAT - 1] + s_red[2][MD_NSTAT - 1] +
-                             s_red[3][MD_NSTAT - 1]) != 0.0;
-            int tchk = tr ? atomicOr(&a.st[cur_r].trip_new, 1) : 0;
-            int add = nunits_r + (chk != chk ? 1 : 0) + (tchk < 0 ? 1 : 0);
-            int old = atomicAdd(&a.st[cur_r].units_done, add);
-            const RepParam& p = a.rep[cur_r];
-            s_last = (old + add == p.nslot / APB) ? 1 : 0;
-        }
-        __syncthreads();
-        if (s_last && tid == 0) finalize_replica(a, cur_r, kstat, en);
+        if (tid < MD_NSTAT)
+            a.part[(size_t)run_u0 * MD_NSTAT + tid] = s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
         __syncthreads();
         dke = dlj = dco = dbo = dan = dvi = 0.0;
         trip = 0;
@@ -298,7 +282,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         if (r != cur_r) {
             if (cur_r >= 0 && run_r) flush();
             cur_r = r;
-            nunits_r = 0;
+            run_u0 = u;
             const RepState s = a.st[r];
             if (a.mode == MODE_FORCE) run_r = !stale(s);
             else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
@@ -306,7 +290,6 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             if (!run_r) continue;
             en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
                                                         : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
-            kstat = (a.mode == MODE_FORCE) ? 0 : s.step - s.start;
             const int cb = cur_buf(s);
             pin = a.pos[cb];
             pout = a.pos[cb ^ 1];
@@ -316,7 +299,6 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             lam = (a.mode == MODE_LF) ? (float)a.th[r].S : 1.0f;
         }
         if (!run_r) continue;
-        nunits_r++;
         const int slot = slot0 + wave * APW + grp;
         const int id = a.ids[slot];
         const bool active = id >= 0;
@@ -437,10 +419,61 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     if (cur_r >= 0 && run_r) flush();
 }
 
+// After k_step (same grid): one workgroup per replica sums the partial slots
+// the step kernel's workgroups wrote for the replica and finalizes it.  Slot u
+// was written in this launch iff u is the replica's first unit or the first
+// unit of a non-empty workgroup range, u0(b) = floor(b * nunits / grid); only
+// those are read, so no slot has to be cleared between launches.
+__global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb, int grid) {
+    __shared__ double s_red[4][MD_NSTAT];
+    // No gate test: when the step kernel left early no replica passes the test below.
+    // The loads of the state, layout and thermostat are independent and overlap.
+    const int r = blockIdx.x;
+    const RepState s = a.st[r];
+    const RepParam p = a.rep[r];
+    ThState th;
+    if (threadIdx.x == 0) th = a.th[r];
+    bool run;
+    if (a.mode == MODE_FORCE) run = !stale(s);
+    else if (a.mode == MODE_LF) run = !stale(s) && s.step < s.target;
+    else run = s.phase == 1 && !stale(s);
+    if (!run) return;   // block-uniform
+    const int ru0 = p.slot_off / apb, ru1 = (p.slot_off + p.nslot) / apb;
+    const long long n = a.nunits;
+    double v[MD_NSTAT];
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
+    for (int u = ru0 + (int)threadIdx.x; u < ru1; u += blockDim.x) {
+        const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
+        if (u == ru0 || (b < grid && (b * n) / grid == u)) {
+            const double2* q = reinterpret_cast<const double2*>(a.part + (size_t)u * MD_NSTAT);
+#pragma unroll
+            for (int k = 0; k < MD_NSTAT / 2; k++) {
+                const double2 t = q[k];
+                v[2 * k] += t.x;
+                v[2 * k + 1] += t.y;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < MD_NSTAT; k++)
+        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    if (lane == 0)
+        for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[MD_NSTAT];
+        for (int k = 0; k < MD_NSTAT; k++) t[k] = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+        const long long kstat = (a.mode == MODE_FORCE) ? 0 : s.step - s.start;
+        const bool en = (a.mode == MODE_FORCE) ? (a.force_energy != 0)
+                                               : (a.energy_every > 0 && (s.step % a.energy_every) == 0);
+        finalize_replica(a, r, p, s, th, kstat, en, t, t[MD_NSTAT - 1] != 0.0);
+    }
+}
+
 // Velocity-Verlet begin-of-step kernel: v = S v + F dt/2m; x += v dt; wrap;
 // skin check.  With apply_drift = 0 it only applies the end-of-step
-// thermostat scale (run end).  The last wavefront of a replica (ticket over
-// its 64-slot tiles) folds the skin check into the list state and moves the
+// thermostat scale (run end).  Each wavefront stores its 64-slot tile's skin
+// check flag; k_kick_finalize folds them into the list state and moves the
 // replica to the force phase.
 __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
     if (a.apply_drift && a.gate->n_run == 0) return;   // nothing to kick: idle launch
@@ -484,19 +517,7 @@ __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
         if (!(fabsf(x) <= p.L[0] && fabsf(y) <= p.L[1] && fabsf(z) <= p.L[2])) *a.err = 1;
     }
     const unsigned long long tb = __ballot(trip);
-    if (lane == 0) {
-        const int tchk = tb ? atomicOr(&a.st[r].trip_new, 1) : 0;
-        const int old = atomicAdd(&a.st[r].tiles_done, 1 + (tchk < 0 ? 1 : 0));
-        if (old == p.nslot / 64 - 1) {
-            a.st[r].tiles_done = 0;
-            const int tn = __hip_atomic_exchange(&a.st[r].trip_new, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (tn) a.st[r].trip[st.nb & 1] = 1;
-            a.st[r].phase = 1;
-            atomicSub(&a.gate->n_run, 1);
-            if (tn) atomicAdd(&a.gate->n_stale, 1);
-            else atomicAdd(&a.gate->n_force, 1);
-        }
-    }
+    if (lane == 0) a.tflag[s >> 6] = tb ? 1 : 0;
 }
 
 // Thermostat bookkeeping at run boundaries (one workgroup).
@@ -521,6 +542,45 @@ __global__ void k_thermo_fix(ThState* th, const RepParam* rep, int nrep, int the
             t.S = 1.0;
             t.S_end = 1.0;
         }
+    }
+}
+
+// After k_vv_kick: one wavefront per replica ORs the tile flags of the skin
+// check, marks the list stale when one tripped and moves the replica to the
+// force phase.  Gate counters change once per workgroup; no early exit before
+// the barrier.
+__global__ __launch_bounds__(256) void k_kick_finalize(const KickArgs a) {
+    __shared__ int s_d[4][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wave;
+    int moved = 0, tripped = 0;
+    if (r < a.nrep) {
+        const RepState st = a.st[r];
+        if (st.phase == 0 && !stale(st) && st.step < st.target) {   // the replica kicked
+            const RepParam& p = a.rep[r];
+            const int t0 = p.slot_off / 64, nt = p.nslot / 64;
+            int t = 0;
+            for (int k = lane; k < nt; k += 64) t |= a.tflag[t0 + k];
+            tripped = __ballot(t != 0) ? 1 : 0;
+            if (lane == 0) {
+                if (tripped) a.st[r].trip[st.nb & 1] = 1;
+                a.st[r].phase = 1;
+            }
+            moved = 1;
+        }
+    }
+    if (lane == 0) {
+        s_d[wave][0] = moved;
+        s_d[wave][1] = tripped;
+        s_d[wave][2] = moved - tripped;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int n = 0, nt = 0, nf = 0;
+        for (int w = 0; w < 4; w++) { n += s_d[w][0]; nt += s_d[w][1]; nf += s_d[w][2]; }
+        if (n) atomicSub(&a.gate->n_run, n);
+        if (nt) atomicAdd(&a.gate->n_stale, nt);
+        if (nf) atomicAdd(&a.gate->n_force, nf);
     }
 }
 
@@ -580,8 +640,15 @@ extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int
     return hipGetLastError();
 }
 
+extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int grid, hipStream_t st) {
+    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_step_finalize, dim3(a.nrep), dim3(256), 0, st, a, 256 / lanes, grid);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_vv_kick, dim3((a.nslot + 255) / 256), dim3(256), 0, st, a);
+    if (a.apply_drift) hipLaunchKernelGGL(k_kick_finalize, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
