@@ -149,7 +149,7 @@ typedef struct md_launch_config {
     int submit;        /* 0: replayed HIP graph segments, 1: direct kernel launches */
     int refresh;       /* 1: also rebuild a list after check_every steps, ahead of the skin trip */
     int build_kernel;  /* 0: tile-staged list build (build_lanes 4 or 8), 1: per-wavefront build (8 lanes),
-                          2: candidate-parallel build (a lane per candidate, 16 atoms per wavefront) */
+                          2, 3: candidate-parallel build (a lane per candidate, 16 or 8 atoms per wavefront) */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);

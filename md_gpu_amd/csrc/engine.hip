@@ -1109,7 +1109,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
         return fail(MD_E_INVALID, "build_lanes must be 4 or 8");
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
-    if (c->build_kernel < 0 || c->build_kernel > 2) return fail(MD_E_INVALID, "build_kernel must be 0, 1 or 2");
+    if (c->build_kernel < 0 || c->build_kernel > 3) return fail(MD_E_INVALID, "build_kernel must be 0..3");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
@@ -1190,7 +1190,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
     double best_build = 1e30;
     for (int d = 1; d <= 3; d++)
-        for (int bk : {0, 1, 2})
+        for (int bk : {0, 1, 2, 3})
             for (int bl : {4, 8}) {
             if (bk != 0 && bl != 8) continue;   // build_lanes only applies to the tile-staged kernel
             md_launch_config c = best;
@@ -1211,7 +1211,10 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             if (ms < best_build) { best_build = ms; best.cell_div = d; best.build_lanes = bl; best.build_kernel = bk; }
         }
     // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events
-    double best_step = 1e30;
+    // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events;
+    // the three fastest shapes go on to whole runs
+    struct Shape { double ms; int lanes, grid; };
+    std::vector<Shape> shapes;
     const int ks = std::max(8, std::min(nsteps, 64));
     for (int l : {4, 8, 16, 32})
         for (int w : {2, 4, 8}) {
@@ -1222,37 +1225,43 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             CHECK(restore());
             CHECK(md_timing_enable(e, 1));
             CHECK(md_run(e, ks, 0, nullptr));
-            const double per = e->t_step_ms / std::max(1LL, e->n_step);
+            shapes.push_back({e->t_step_ms / std::max(1LL, e->n_step), l, c.grid});
             CHECK(md_timing_enable(e, 0));
-            if (per < best_step) { best_step = per; best.lanes = l; best.grid = c.grid; }
         }
-    // stage 3, whole runs: rebuild-check interval x age refresh x submission mode
+    std::sort(shapes.begin(), shapes.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
+    shapes.resize(std::min<size_t>(shapes.size(), 3));
+    // stage 3, whole runs (best of two): shape x rebuild-check interval x age refresh x submission mode
     double best_run = 1e30;
     const int kr = std::max(nsteps, 2 * kSegment);
-    for (int sub : {0, 1})
-        for (int rf : {0, 1})
-            for (int ce : {4, 6, 8, 12, 16}) {
-                md_launch_config c = best;
-                c.check_every = ce;
-                c.submit = sub;
-                c.refresh = rf;
-                CHECK(md_set_launch(e, &c));
-                CHECK(restore());
-                CHECK(md_run(e, std::min(kr, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
-                HIPCHECK(hipEventRecord(t0, e->st));
-                CHECK(md_run(e, kr, 0, nullptr));
-                HIPCHECK(hipEventRecord(t1, e->st));
-                HIPCHECK(hipEventSynchronize(t1));
-                float ms = 0;
-                HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-                const double per = ms / kr;
-                if (per < best_run) {
-                    best_run = per;
-                    best.check_every = ce;
-                    best.submit = sub;
-                    best.refresh = rf;
+    md_launch_config base = best;
+    for (const Shape& sh : shapes)
+        for (int sub : {0, 1})
+            for (int rf : {0, 1})
+                for (int ce : {4, 6, 8, 12, 16}) {
+                    md_launch_config c = base;
+                    c.lanes = sh.lanes;
+                    c.grid = sh.grid;
+                    c.check_every = ce;
+                    c.submit = sub;
+                    c.refresh = rf;
+                    CHECK(md_set_launch(e, &c));
+                    double per = 1e30;
+                    for (int rep = 0; rep < 2; rep++) {
+                        CHECK(restore());
+                        CHECK(md_run(e, std::min(kr, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
+                        HIPCHECK(hipEventRecord(t0, e->st));
+                        CHECK(md_run(e, kr, 0, nullptr));
+                        HIPCHECK(hipEventRecord(t1, e->st));
+                        HIPCHECK(hipEventSynchronize(t1));
+                        float ms = 0;
+                        HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
+                        per = std::min(per, (double)ms / kr);
+                    }
+                    if (per < best_run) {
+                        best_run = per;
+                        best = c;
+                    }
                 }
-            }
     CHECK(md_set_launch(e, &best));
     CHECK(restore());
     e->timing = saved_timing;

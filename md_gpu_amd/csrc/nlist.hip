@@ -20,8 +20,8 @@
 //                 variants (build_kernel): k_nl_build (LPA lanes per atom, tile
 //                 box, double-buffered block staging), k_nl_wbuild (8 lanes per
 //                 atom, per-wavefront boxes, LDS hit buffers) and k_nl_jbuild
-//                 (a lane per candidate, 16 atoms per wavefront, one ballot per
-//                 atom and chunk).  Hits are appended in candidate order, so
+//                 (a lane per candidate, 16 or 8 atoms per wavefront, one ballot
+//                 per atom and chunk).  Hits are appended in candidate order, so
 //                 rows are deterministic.  Each tile stores its longest row.
 //   k_nl_commit   per rebuilt replica: longest row -> maxcount, new list epoch.
 // Bytes per atom per rebuild (algorithmic): read pos/vel/force/id/type 56 B,
@@ -627,17 +627,17 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
     }
 }
 
-// Candidate-parallel build: one workgroup (4 wavefronts) per 64-slot tile,
-// AW = 16 atoms per wavefront.  The wavefront lists the image cells within
+// Candidate-parallel build: one workgroup (64 / AW wavefronts) per 64-slot tile,
+// AW (8 or 16) atoms per wavefront.  The wavefront lists the image cells within
 // r_list of its atoms' bounding box, streams their members, keeps those whose
 // image lies within r_list of the box (stable compaction into a 128-entry LDS
 // ring) and, per chunk of 64 kept candidates, lets each lane test one
-// candidate against the 16 atoms in turn.  The hits of one atom in a chunk are
+// candidate against the AW atoms in turn.  The hits of one atom in a chunk are
 // one ballot: they go to count + (hits in lower lanes), so rows are filled in
 // candidate order with coalesced stores and no per-hit LDS traffic.
-template <bool EXCL>
-__global__ __launch_bounds__(256) void k_nl_jbuild(BuildArgs a) {
-    constexpr int NW = 4, AW = 16, WT = 256, RQ = 128;
+template <bool EXCL, int AW>
+__global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
+    constexpr int NW = 64 / AW, WT = 256, RQ = 128;
     __shared__ int s_tstart[NW][WT];
     __shared__ int s_tpref[NW][WT + 1];
     __shared__ int s_tshift[NW][WT];
@@ -929,8 +929,11 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
     const dim3 ntiles(a.nslot / 64);
     if (a.build_kernel == 2) {
-        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true>), ntiles, dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_nl_jbuild<false>), ntiles, dim3(256), 0, st, a);
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true, 16>), ntiles, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_jbuild<false, 16>), ntiles, dim3(256), 0, st, a);
+    } else if (a.build_kernel == 3) {
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true, 8>), ntiles, dim3(512), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_jbuild<false, 8>), ntiles, dim3(512), 0, st, a);
     } else if (a.build_kernel == 1) {
         if (a.has_excl) hipLaunchKernelGGL((k_nl_wbuild<true>), ntiles, dim3(512), 0, st, a);
         else hipLaunchKernelGGL((k_nl_wbuild<false>), ntiles, dim3(512), 0, st, a);

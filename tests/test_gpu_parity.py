@@ -57,7 +57,7 @@ def oracle_forces(orc, s):
     return orc.forces(s)
 
 
-@pytest.mark.parametrize("build_kernel", [0, 1, 2])
+@pytest.mark.parametrize("build_kernel", [0, 1, 2, 3])
 @pytest.mark.parametrize("name", list(CASES))
 def test_neighbor_sets_bit_exact(E, orc, name, build_kernel):
     s = CASES[name]
@@ -260,6 +260,7 @@ LAUNCH_VARIANTS = [
     dict(submit=0, refresh=0, check_every=8, build_kernel=1, lanes=8, cell_div=1),
     dict(submit=0, refresh=1, check_every=8, build_kernel=2, lanes=4, cell_div=2),
     dict(submit=1, refresh=0, check_every=16, build_kernel=2, lanes=16, cell_div=3),
+    dict(submit=0, refresh=1, check_every=6, build_kernel=3, lanes=8, cell_div=1),
 ]
 
 

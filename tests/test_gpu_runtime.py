@@ -30,6 +30,7 @@ def pairs_within(orc, pos, box, r2):
                                         (32, dict(check_every=16, lanes=8, build_kernel=0, build_lanes=8)),
                                         (32, dict(check_every=16, lanes=8, build_kernel=1)),
                                         (32, dict(check_every=8, lanes=8, build_kernel=2)),
+                                        (32, dict(check_every=8, lanes=16, build_kernel=3)),
                                         (32, dict(check_every=4, refresh=1, submit=1))])
 def test_lists_valid_after_in_run_rebuilds(E, orc, nsites, cfg):
     """After a run with many device-side rebuilds, every pair within rc of the
