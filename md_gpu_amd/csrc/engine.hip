@@ -217,7 +217,7 @@ struct md_engine {
     int cap = 0;
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 0;
+    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 1, build_kernel = 3;
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so

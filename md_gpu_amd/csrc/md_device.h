@@ -77,6 +77,10 @@ __device__ __forceinline__ int lpos_sh(int e, int gs) {
     const int G = 1 << gs;
     return (e & ~(4 * G - 1)) | ((e & (G - 1)) << 2) | ((e >> gs) & 3);
 }
+__device__ __forceinline__ uint32_t lpos_u(uint32_t e, int gs) {
+    const uint32_t G = 1u << gs;
+    return (e & ~(4u * G - 1u)) | ((e & (G - 1u)) << 2) | __builtin_amdgcn_ubfe(e, (uint32_t)gs, 2u);
+}
 
 // current position buffer of a replica
 __host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step + s.phase) & 1); }

@@ -638,9 +638,9 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
 template <bool EXCL, int AW>
 __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     constexpr int NW = 64 / AW, WT = 256, RQ = 128;
-    __shared__ int s_tstart[NW][WT];
-    __shared__ int s_tpref[NW][WT + 1];
-    __shared__ int s_tshift[NW][WT];
+    __shared__ float4 s_tcell[NW][WT];  // table row: (-kx L, -ky L, -kz L, bits of (cell start - pref))
+    __shared__ int s_tpref[NW][WT + 1]; // first candidate index of each row; [ntab] = total
+    __shared__ int s_mk[NW][64];        // row-boundary marks of a 64-candidate chunk (tagged)
     __shared__ float4 s_qp[NW][RQ];     // ring of kept candidates (x, y, z, bits of j | tj << 29)
     __shared__ float4 s_qs[NW][RQ];     // (-kx L, -ky L, -kz L, bits of jid)
     __shared__ int4 s_ex[NW][EXCL ? AW : 1];
@@ -655,6 +655,7 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     const int cb = cur_buf(st);
     const int slot0 = tile * 64 + wave * AW;
     const unsigned long long below = (1ull << lane) - 1ull;
+    s_mk[wave][lane] = 0;   // tags start at 1
     auto wave_sync = [&]() {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -719,9 +720,9 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
             }
             const unsigned long long m = __ballot(ok);
             if (m) {
-                const int e = cnt[i] + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                if (ok && e < cap) a.nlist[(size_t)si * cap + lpos_sh(e, gs)] = ent;
+                const uint32_t e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)cnt[i]));
+                if (ok && e < (uint32_t)cap) a.nlist[(size_t)si * cap + lpos_u(e, gs)] = ent;
                 cnt[i] += __popcll(m);
             }
         }
@@ -750,19 +751,35 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
             shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
             return in ? ce - cs : 0;
         };
-        // candidate q of the current table: binary search for its cell, then the
-        // loads (consumed one chunk later, so they overlap the chunk before)
-        struct Cand { float4 p; int typ, id, sh, j; bool valid; };
-        auto fetch = [&](int q, int ntab, int T) -> Cand {
-            Cand c;
-            c.valid = q < T;
-            int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
-            while (lo_ < hi_) {
-                const int mid = (lo_ + hi_ + 1) >> 1;
-                if (s_tpref[wave][mid] <= q) lo_ = mid; else hi_ = mid - 1;
+        // Row of candidate q0 + lane, given row k_lo holding q0: the rows starting
+        // inside the chunk mark their start position, and a lane's row is k_lo plus
+        // the marks at or below it.  Also advances k_lo to the row holding q0 + 64.
+        int ktag = 0;
+        auto locate = [&](int q0, int& k_lo, int ntab) -> int {
+            ++ktag;
+            const int kk = k_lo + 1 + lane;
+            if (kk < ntab) {
+                const int rel = s_tpref[wave][kk] - q0;   // >= 1: row k_lo holds q0
+                if (rel < 64) s_mk[wave][rel] = ktag;
             }
-            c.j = c.valid ? s_tstart[wave][lo_] + (q - s_tpref[wave][lo_]) : slot0;
-            c.sh = s_tshift[wave][lo_];
+            wave_sync();
+            const unsigned long long M = __ballot(s_mk[wave][lane] == ktag);
+            const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+            const int k = k_lo + __popcll(M & upto);
+            const int k63 = k_lo + __popcll(M);
+            k_lo = (k63 + 1 < ntab && s_tpref[wave][k63 + 1] <= q0 + 64) ? k63 + 1 : k63;
+            return k;
+        };
+        // candidate q0 + lane: its row, then the loads (consumed one chunk later, so
+        // they overlap the chunk before)
+        struct Cand { float4 p, s; int typ, id, j; bool valid; };
+        auto fetch = [&](int q0, int& k_lo, int ntab, int T) -> Cand {
+            Cand c;
+            const int q = q0 + lane;
+            c.valid = q < T;
+            const int k = locate(q0, k_lo, ntab);
+            c.s = s_tcell[wave][c.valid ? k : 0];
+            c.j = c.valid ? __float_as_int(c.s.w) + q : slot0;
             c.p = a.pos_s[c.j];
             c.typ = a.typ_s[c.j];
             c.id = EXCL ? a.ids_s[c.j] : 0;
@@ -788,9 +805,13 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
                     }
                     if (n[b] > 0) {
                         const int k = ntab + __popcll(m & below);
-                        s_tstart[wave][k] = cs[b];
-                        s_tpref[wave][k] = T + inc - n[b];
-                        s_tshift[wave][k] = sh[b];
+                        const int pref = T + inc - n[b];
+                        // -k L is exact for the image offsets of the region
+                        s_tcell[wave][k] = make_float4(-(float)((sh[b] & 15) - 8) * p.L[0],
+                                                       -(float)(((sh[b] >> 4) & 15) - 8) * p.L[1],
+                                                       -(float)(((sh[b] >> 8) & 15) - 8) * p.L[2],
+                                                       __int_as_float(cs[b] - pref));
+                        s_tpref[wave][k] = pref;
                     }
                     ntab += __popcll(m);
                     T += __shfl(inc, 63, 64);
@@ -801,16 +822,15 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
             wave_sync();
             if (T == 0) continue;
             // ---- stream the table's candidates, 64 at a time, through the ring
-            Cand nx = fetch(lane, ntab, T);
+            int k_lo = 0;
+            Cand nx = fetch(0, k_lo, ntab, T);
             for (int q0 = 0; q0 < T; q0 += 64) {
                 const Cand cu = nx;
-                if (q0 + 64 < T) nx = fetch(q0 + 64 + lane, ntab, T);   // wavefront-uniform test
+                if (q0 + 64 < T) nx = fetch(q0 + 64, k_lo, ntab, T);   // wavefront-uniform test
                 bool keep = false;
                 float4 rp = cu.p, rs = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (cu.valid) {
-                    // -k L is exact for the |k| <= 1 images that can pass the predicate
-                    rs = make_float4(-(float)((cu.sh & 15) - 8) * p.L[0], -(float)(((cu.sh >> 4) & 15) - 8) * p.L[1],
-                                     -(float)(((cu.sh >> 8) & 15) - 8) * p.L[2], __int_as_float(cu.id));
+                    rs = make_float4(cu.s.x, cu.s.y, cu.s.z, __int_as_float(cu.id));
                     const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
                     const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
                     const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
