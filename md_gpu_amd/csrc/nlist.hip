@@ -37,19 +37,23 @@ __device__ __forceinline__ int cell_coord(float x, float halfL, float cinv, int 
     return min(max(k, 0), nc - 1);
 }
 
-__device__ __forceinline__ int floordiv(int a, int b) {  // b > 0
-    int q = a / b;
-    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+// floor(a / b), b > 0, without an integer division: the estimate from the rounded
+// reciprocal of b is off by at most one for |a| < 2^24, and one correction step
+// makes it exact.  (Region walks never come near that range.)
+__device__ __forceinline__ int floor_div(int a, int b, float inv_b) {
+    int q = (int)floorf(((float)a + 0.5f) * inv_b);
+    const int r = a - q * b;
+    return q + (r >= b ? 1 : 0) - (r < 0 ? 1 : 0);
 }
 
 // One axis of the image-cell region around a tile's bounding box [bmin, bmax].
 struct Axis {
     int lo, nr, nc;
-    float cedge, halfL, bmin, bmax;
+    float cedge, halfL, bmin, bmax, inv_nc, inv_nr;
     // image index k, wrapped cell w of region coordinate c; returns the gap between
     // the cell slab and the box along this axis (0 when they overlap)
     __device__ float gap(int c, int& k, int& w) const {
-        k = floordiv(c, nc);
+        k = floor_div(c, nc, inv_nc);
         w = c - k * nc;
         const float cl = c * cedge - halfL, ch = cl + cedge;
         return fmaxf(0.0f, fmaxf(cl - bmax, bmin - ch));
@@ -64,6 +68,8 @@ __device__ __forceinline__ Axis make_axis(float bmin, float bmax, float rlp, flo
     x.lo = l;
     x.nr = max(h - l + 1, 0);
     x.nc = nc;
+    x.inv_nc = 1.0f / (float)nc;
+    x.inv_nr = 1.0f / (float)max(x.nr, 1);
     x.cedge = cedge;
     x.halfL = halfL;
     x.bmin = bmin;
@@ -258,9 +264,10 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
         // region cell c -> (cell start, member count, packed image shift); count 0 when pruned or empty
         auto probe = [=](int c, int& cs, int& shift) -> int {
             int kx, ky, kz, wx, wy, wz;
-            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
-            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
-            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
+            const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
+            const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
+            const float ddz = az.gap(az.lo + cz, kz, wz);
             if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
             const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
             cs = a.cell_start[cs_off + rank];
@@ -499,9 +506,10 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
         const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
         auto probe = [=](int c, int& cs, int& shift) -> int {
             int kx, ky, kz, wx, wy, wz;
-            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
-            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
-            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
+            const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
+            const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
+            const float ddz = az.gap(az.lo + cz, kz, wz);
             if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
             const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
             cs = a.cell_start[cs_off + rank];
@@ -741,9 +749,10 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
         // entry 0 and reports no members.
         auto probe = [=](int c, int& cs, int& shift) -> int {
             int kx, ky, kz, wx, wy, wz;
-            const float ddx = ax.gap(ax.lo + c % ax.nr, kx, wx);
-            const float ddy = ay.gap(ay.lo + (c / ax.nr) % ay.nr, ky, wy);
-            const float ddz = az.gap(az.lo + c / (ax.nr * ay.nr), kz, wz);
+            const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
+            const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
+            const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
+            const float ddz = az.gap(az.lo + cz, kz, wz);
             const bool in = c < nreg && ddx * ddx + ddy * ddy + ddz * ddz < rlp2;
             const int rank = a.cell_rank[in ? cell_off + wx + ncx * (wy + ncy * wz) : cell_off];
             cs = a.cell_start[cs_off + rank];

@@ -199,8 +199,9 @@ __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __rest
         float4 pj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int j = (base + G * u < nn) ? (int)(ev[u] & MD_JMASK) : self;
-            pj[u] = pin[j];
+            // 32-bit byte offset from the position array (scalar base + vector offset)
+            const uint32_t off = ((base + G * u < nn) ? (ev[u] & MD_JMASK) : (uint32_t)self) << 4;
+            pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + off);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -427,24 +428,34 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
 __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb, int grid) {
     __shared__ double s_red[4][MD_NSTAT];
     // No gate test: when the step kernel left early no replica passes the test below.
-    // The loads of the state, layout and thermostat are independent and overlap.
+    // Every load is issued before any is consumed: with one replica its units are
+    // all units, and with at least one workgroup per unit every unit starts a run,
+    // so the slot loads need neither the layout nor a division.
     const int r = blockIdx.x;
     const RepState s = a.st[r];
-    const RepParam p = a.rep[r];
+    RepParam p;
     ThState th;
-    if (threadIdx.x == 0) th = a.th[r];
-    bool run;
-    if (a.mode == MODE_FORCE) run = !stale(s);
-    else if (a.mode == MODE_LF) run = !stale(s) && s.step < s.target;
-    else run = s.phase == 1 && !stale(s);
-    if (!run) return;   // block-uniform
-    const int ru0 = p.slot_off / apb, ru1 = (p.slot_off + p.nslot) / apb;
+    if (threadIdx.x == 0) {
+        p = a.rep[r];
+        th = a.th[r];
+    }
+    int ru0 = 0, ru1 = a.nunits;
+    if (a.nrep > 1) {
+        const int so = a.rep[r].slot_off, ns = a.rep[r].nslot;
+        ru0 = so / apb;
+        ru1 = (so + ns) / apb;
+    }
+    const bool every = grid >= a.nunits;
     const long long n = a.nunits;
     double v[MD_NSTAT];
     for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
     for (int u = ru0 + (int)threadIdx.x; u < ru1; u += blockDim.x) {
-        const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
-        if (u == ru0 || (b < grid && (b * n) / grid == u)) {
+        bool start = every || u == ru0;
+        if (!start) {
+            const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
+            start = b < grid && (b * n) / grid == u;
+        }
+        if (start) {
             const double2* q = reinterpret_cast<const double2*>(a.part + (size_t)u * MD_NSTAT);
 #pragma unroll
             for (int k = 0; k < MD_NSTAT / 2; k++) {
@@ -454,6 +465,11 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
             }
         }
     }
+    bool run;
+    if (a.mode == MODE_FORCE) run = !stale(s);
+    else if (a.mode == MODE_LF) run = !stale(s) && s.step < s.target;
+    else run = s.phase == 1 && !stale(s);
+    if (!run) return;   // block-uniform
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int k = 0; k < MD_NSTAT; k++)
         for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
