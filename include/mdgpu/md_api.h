@@ -150,6 +150,8 @@ typedef struct md_launch_config {
     int refresh;       /* 1: also rebuild a list after check_every steps, ahead of the skin trip */
     int build_kernel;  /* 0: tile-staged list build (build_lanes 4 or 8), 1: per-wavefront build (8 lanes),
                           2, 3: candidate-parallel build (a lane per candidate, 16 or 8 atoms per wavefront) */
+    int defer;         /* leapfrog: 1 = each step kernel finalizes the previous step of its block
+                          (temperature, thermostat, statistics) instead of a finalize kernel per step */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);

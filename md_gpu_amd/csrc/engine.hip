@@ -202,7 +202,8 @@ struct md_engine {
     DBuf<RepState> rst;
     DBuf<Gate> gate;
     DBuf<ThState> th;
-    DBuf<double> part, fm_acc;     // part: [S/8][MD_NSTAT] partial sums of the step kernel
+    DBuf<double> part, fm_acc;     // part: [2][S/8][MD_NSTAT] partial sums of the step kernel
+    DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
     DBuf<int> err;
     DBuf<unsigned long long> reb_count;
     DBuf<StatRec> stats;
@@ -218,6 +219,7 @@ struct md_engine {
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
     int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 1, build_kernel = 3;
+    int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -404,6 +406,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
     a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
+    a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
     a.ctab = e->ctab.p;
@@ -461,11 +464,26 @@ int enqueue_rebuild(md_engine* e, TimedSpan* ts) {
     return MD_OK;
 }
 
-int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts) {
+int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool finalize = true) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
     HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st));
     if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->steps.push_back(ev); }
+    if (finalize) HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
+    return MD_OK;
+}
+
+// A leapfrog block of n steps with deferred finalize: each step kernel finalizes
+// the previous one of the block on its way in; one finalize kernel closes the
+// block (the rebuild chain and the host see only closed blocks).
+int enqueue_lf_block(md_engine* e, int energy_every, int n) {
+    StepArgs a = step_args(e, MODE_LF, energy_every);
+    a.lagmode = 1;
+    for (int k = 0; k < n; k++) {
+        a.lag = k > 0 ? 1 : 0;
+        a.par = k & 1;
+        CHECK(enqueue_stepkernel(e, a, nullptr, false));
+    }
     HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
     return MD_OK;
 }
@@ -640,7 +658,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
         N += natom[r];
         S += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
     }
-    if (S >= (1LL << MD_JBITS)) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << MD_JBITS);
+    if (S >= (1LL << (MD_JBITS - 4))) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << (MD_JBITS - 4));
     for (long long i = 0; i < N; i++) {
         if (type[i] < 0 || type[i] >= e->ff.ntype) return fail(MD_E_INVALID, "atom %lld: type %d outside 0..%d", i, type[i], e->ff.ntype - 1);
         if (!(mass[i] > 0)) return fail(MD_E_INVALID, "atom %lld: mass must be > 0", i);
@@ -704,7 +722,8 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->snap_gate.alloc(1));
     CHECK(e->th.alloc(nrep));
     CHECK(e->snap_th.alloc(nrep));
-    CHECK(e->part.alloc((size_t)(S / 8) * MD_NSTAT));   // units hold at least 8 slots (lanes <= 32)
+    CHECK(e->part.alloc(2 * (size_t)(S / 8) * MD_NSTAT));   // units hold at least 8 slots (lanes <= 32)
+    CHECK(e->lst.alloc(2 * (size_t)nrep));
     CHECK(e->maxcount.alloc(nrep));
     CHECK(e->tilemax.alloc(S / MD_TILE));
     CHECK(e->tflag.alloc(S / MD_TILE));
@@ -902,7 +921,8 @@ int enqueue_segment(md_engine* e, int energy_every) {
     const int blocks = std::max(1, kSegment / ce);
     for (int b = 0; b < blocks; b++) {
         CHECK(enqueue_rebuild(e, nullptr));
-        for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
+        if (e->defer && e->in.integrator == MD_LEAPFROG) CHECK(enqueue_lf_block(e, energy_every, ce));
+        else for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
     }
     return MD_OK;
 }
@@ -1110,6 +1130,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
     if (c->build_kernel < 0 || c->build_kernel > 3) return fail(MD_E_INVALID, "build_kernel must be 0..3");
+    if (c->defer != 0 && c->defer != 1) return fail(MD_E_INVALID, "defer must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
@@ -1121,6 +1142,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     e->submit = c->submit;
     e->refresh = c->refresh;
     e->build_kernel = c->build_kernel;
+    e->defer = c->defer;
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
@@ -1144,6 +1166,7 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->submit = e->submit;
     c->refresh = e->refresh;
     c->build_kernel = e->build_kernel;
+    c->defer = e->defer;
     return MD_OK;
 }
 
@@ -1231,19 +1254,24 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     std::sort(shapes.begin(), shapes.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
     shapes.resize(std::min<size_t>(shapes.size(), 3));
     // stage 3, whole runs (best of two): shape x rebuild-check interval x age refresh x submission mode
+    // x deferred finalize
     double best_run = 1e30;
     const int kr = std::max(nsteps, 2 * kSegment);
     md_launch_config base = best;
+    const bool lf = e->in.integrator == MD_LEAPFROG;
     for (const Shape& sh : shapes)
         for (int sub : {0, 1})
             for (int rf : {0, 1})
+                for (int df : {1, 0})
                 for (int ce : {4, 6, 8, 12, 16}) {
+                    if (df == 0 && !lf) continue;   // deferral applies to leapfrog only
                     md_launch_config c = base;
                     c.lanes = sh.lanes;
                     c.grid = sh.grid;
                     c.check_every = ce;
                     c.submit = sub;
                     c.refresh = rf;
+                    c.defer = df;
                     CHECK(md_set_launch(e, &c));
                     double per = 1e30;
                     for (int rep = 0; rep < 2; rep++) {

@@ -52,7 +52,8 @@ struct RepState {
     int nb;            // completed neighbor-list builds
     int trip[2];       // trip[nb & 1] != 0: the list is stale for the current positions
     int age;           // steps completed since the last build
-    int pad;
+    int ran;           // deferred-finalize copies: the step kernel ran this step for the replica
+    double lam;        // deferred-finalize copies: Berendsen factor for the replica's next step
 };
 
 // Device-wide counters that let a launch with nothing to do exit on its first
@@ -121,7 +122,12 @@ struct StepArgs {
     RepState* st;
     Gate* gate;
     ThState* th;
-    double* part;         // [nunits][MD_NSTAT] FP64 partial sums, one slot per run of units
+    double* part;         // [2][nunits][MD_NSTAT] FP64 partial sums, one slot per run of units
+                          // (buffer par; the deferred finalize reads buffer par ^ 1)
+    RepState* lst;        // [2][R] state handed from one step launch to the next (deferred finalize)
+    int lagmode;          // leapfrog block with deferred finalize: the step kernel writes lst[par]
+    int lag;              // the previous launch was a step kernel of the same block: finalize it first
+    int par;              // buffer parity of this launch
     int* err;             // [1] non-finite coordinates
     StatRec* stats;       // [stat_cap][R] or null
     const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)

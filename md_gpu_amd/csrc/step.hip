@@ -49,6 +49,56 @@ __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double 
     return s;
 }
 
+// Leapfrog end of step for one replica: temperature, the Berendsen factor for
+// the next step (src/integrator.cu:146-147), list age, skin/age trip and step
+// count.  The finalize kernel and the deferred finalize inside the step kernel
+// both call it, so the two schedules give bit-identical trajectories.
+__device__ __forceinline__ void lf_done(const StepArgs& a, const RepParam& p, RepState& s, double ke2, bool tn,
+                                        double& T, double& lam) {
+    const double dt = a.dt, tau = a.tau;
+    T = ke2 / ((double)p.nf * MD_KB_REF);
+    lam = 1.0;
+    if (a.thermostat == 1) lam = sqrt(1.0 + dt / tau * ((double)p.T0 / T - 1.0));
+    s.age = s.age + 1;
+    s.step = s.step + 1;
+    // list refresh by age (ahead of the skin trip), so queued steps do not stall;
+    // the skin check of the positions just written makes the list stale for them
+    const bool aged = a.max_age > 0 && s.age >= a.max_age;
+    if (tn || aged) {
+        if (s.nb & 1) s.trip[1] = 1;
+        else s.trip[0] = 1;
+    }
+}
+
+// Statistics record of a finished step (index kstat of the run).
+__device__ __forceinline__ void put_record(const StepArgs& a, int r, long long kstat, bool en, const double* v,
+                                           double ke, double temp, double xi, double vxi, double nh_terms) {
+    if (!(a.stats && kstat >= 0 && kstat < a.stat_cap)) return;
+    StatRec rec;
+    rec.ke = ke;
+    rec.temp = temp;
+    rec.xi = xi;
+    rec.vxi = vxi;
+    rec.has_energy = en ? 1 : 0;
+    rec.pad = 0;
+    rec.pe_lj = 0.5 * v[ACC_LJ];
+    rec.pe_coul = 0.5 * v[ACC_COUL];
+    rec.pe_bond = v[ACC_BOND];
+    rec.pe_angle = v[ACC_ANGLE];
+    rec.virial = 0.5 * v[ACC_VIR];
+    rec.conserved = rec.ke + rec.pe_lj + rec.pe_coul + rec.pe_bond + rec.pe_angle + nh_terms;
+    a.stats[(size_t)kstat * a.nrep + r] = rec;
+}
+
+// Gate counters after a replica went from state o to state n within one leapfrog
+// launch or block (target unchanged).
+__device__ __forceinline__ void lf_gates(const StepArgs& a, const RepState& o, const RepState& n) {
+    const int ds = (int)stale(n) - (int)stale(o);
+    const int dr = (int)(!stale(n) && n.step < n.target) - (int)(!stale(o) && o.step < o.target);
+    if (ds) atomicAdd(&a.gate->n_stale, ds);
+    if (dr) atomicAdd(&a.gate->n_run, dr);
+}
+
 // Runs once per replica and step-kernel launch, by thread 0 of the replica's
 // workgroup in k_step_finalize; v = the replica's reduced partial sums, tn = the
 // skin check tripped during the position update.
@@ -56,22 +106,23 @@ __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double 
 // the partial-sum loads); the updated state is stored back at the end.
 __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, const RepState& st0, ThState th,
                                  long long kstat, bool en, const double* v, bool tn) {
-    StatRec rec;
-    rec.ke = rec.temp = rec.pe_lj = rec.pe_coul = rec.pe_bond = rec.pe_angle = rec.virial = rec.conserved = 0.0;
-    rec.xi = th.xi;
-    rec.vxi = th.vxi;
-    rec.has_energy = en ? 1 : 0;
-    rec.pad = 0;
     const double dt = a.dt, tau = a.tau;
-    double nh_terms = 0.0;
     if (a.mode == MODE_LF) {
-        double T = v[ACC_KE2] / ((double)p.nf * MD_KB_REF);
-        double lam = 1.0;
-        if (a.thermostat == 1) lam = sqrt(1.0 + dt / tau * ((double)p.T0 / T - 1.0));   // src/integrator.cu:146-147
-        th.S = lam;
-        rec.ke = 0.5 * v[ACC_KE2];
-        rec.temp = T;
-    } else if (a.mode == MODE_VV) {
+        RepState s = st0;
+        double T, lam;
+        lf_done(a, p, s, v[ACC_KE2], tn, T, lam);
+        put_record(a, r, kstat, en, v, 0.5 * v[ACC_KE2], T, th.xi, th.vxi, 0.0);
+        a.th[r].S = lam;
+        RepState& st = a.st[r];
+        st.age = s.age;
+        st.step = s.step;
+        st.trip[0] = s.trip[0];
+        st.trip[1] = s.trip[1];
+        lf_gates(a, st0, s);
+        return;
+    }
+    double nh_terms = 0.0, ke = 0.0, temp = 0.0, rxi = th.xi, rvxi = th.vxi;
+    if (a.mode == MODE_VV) {
         double K = 0.5 * v[ACC_KE2];
         double xi = th.xi, vxi = th.vxi, s1 = 1.0;
         if (a.thermostat == 2) {
@@ -85,10 +136,10 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
         th.xi_end = xi;
         th.vxi_end = vxi;
         th.ke2_cur = 2.0 * K;
-        rec.ke = K;
-        rec.temp = 2.0 * K / ((double)p.nf * MD_KB_REF);
-        rec.xi = xi;
-        rec.vxi = vxi;
+        ke = K;
+        temp = 2.0 * K / ((double)p.nf * MD_KB_REF);
+        rxi = xi;
+        rvxi = vxi;
         if (a.thermostat == 2) nh_terms = 0.5 * p.Q * vxi * vxi + (double)p.nf * p.kT0 * xi;
         double s2 = 1.0;
         if (a.thermostat == 2) s2 = nh_half(xi, vxi, K, p.nf, p.kT0, p.Q, dt);
@@ -96,40 +147,74 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
         th.vxi = vxi;
         th.S = s1 * s2;
     }
-    if (a.stats && kstat >= 0 && kstat < a.stat_cap) {
-        rec.pe_lj = 0.5 * v[ACC_LJ];
-        rec.pe_coul = 0.5 * v[ACC_COUL];
-        rec.pe_bond = v[ACC_BOND];
-        rec.pe_angle = v[ACC_ANGLE];
-        rec.virial = 0.5 * v[ACC_VIR];
-        rec.conserved = rec.ke + rec.pe_lj + rec.pe_coul + rec.pe_bond + rec.pe_angle + nh_terms;
-        a.stats[(size_t)kstat * a.nrep + r] = rec;
-    }
-    if (a.mode != MODE_FORCE) a.th[r] = th;
-    // list refresh by age (ahead of the skin trip), so queued steps do not stall
+    put_record(a, r, kstat, en, v, ke, temp, rxi, rvxi, nh_terms);
     if (a.mode == MODE_FORCE) return;
+    a.th[r] = th;
     RepState& st = a.st[r];
     const int age = st0.age + 1;
     const long long step = st0.step + 1;
     st.age = age;
     st.step = step;
     const bool aged = a.max_age > 0 && age >= a.max_age;
-    if (a.mode == MODE_LF) {
-        // skin check of the positions just written: the list is stale for them
-        const bool tr = tn || aged;
-        if (tr) st.trip[st0.nb & 1] = 1;
-        if (tr) atomicAdd(&a.gate->n_stale, 1);
-        if (tr || step >= st0.target) atomicSub(&a.gate->n_run, 1);
-    } else {
-        st.phase = 0;
-        atomicSub(&a.gate->n_force, 1);
-        if (aged) {
-            st.trip[st0.nb & 1] = 1;
-            atomicAdd(&a.gate->n_stale, 1);
-        } else if (step < st0.target) {
-            atomicAdd(&a.gate->n_run, 1);
+    st.phase = 0;
+    atomicSub(&a.gate->n_force, 1);
+    if (aged) {
+        st.trip[st0.nb & 1] = 1;
+        atomicAdd(&a.gate->n_stale, 1);
+    } else if (step < st0.target) {
+        atomicAdd(&a.gate->n_run, 1);
+    }
+}
+
+// Sums of a replica's partial slots of one step-kernel launch (grid workgroups;
+// slot u holds the run of the workgroup that started it: the replica's first
+// unit and the first unit of every non-empty workgroup range,
+// u0(b) = floor(b * nunits / grid)).  All 256 threads of the workgroup take part
+// and every thread gets the totals; the order of the additions is fixed, so
+// every caller gets the same bits.  ALL = false loads only KE and the trip flag.
+template <bool ALL>
+__device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, int grid, int nunits,
+                                          double (*s_red)[MD_NSTAT], double* v) {
+    const bool every = grid >= nunits;
+    const long long n = nunits;
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
+    for (int u = ru0 + (int)threadIdx.x; u < ru1; u += blockDim.x) {
+        bool start = every || u == ru0;
+        if (!start) {
+            const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
+            start = b < grid && (b * n) / grid == u;
+        }
+        if (start) {
+            const double* q = part + (size_t)u * MD_NSTAT;
+            if (ALL) {
+                const double2* q2 = reinterpret_cast<const double2*>(q);
+#pragma unroll
+                for (int k = 0; k < MD_NSTAT / 2; k++) {
+                    const double2 t = q2[k];
+                    v[2 * k] += t.x;
+                    v[2 * k + 1] += t.y;
+                }
+            } else {
+                v[ACC_KE2] += q[ACC_KE2];
+                v[MD_NSTAT - 1] += q[MD_NSTAT - 1];
+            }
         }
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) {
+        if (!ALL && k != ACC_KE2 && k != MD_NSTAT - 1) continue;
+        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    }
+    __syncthreads();   // s_red is free
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
 }
 
 // One pair: force factor (F_i += ff * d) and, with EN, the pair energies.
@@ -205,7 +290,9 @@ __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __rest
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const float4 lj = s_lj[ev[u] >> MD_JBITS];
+            // type_j * 16 bytes: slots stay below 2^(MD_JBITS - 4), so the bits of j above
+            // the shift are zero (enforced when the system is set)
+            const float4 lj = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_lj) + (ev[u] >> (MD_JBITS - 4)));
             const float dx = minimg(xi.x - pj[u].x, L0, Li0);
             const float dy = minimg(xi.y - pj[u].y, L1, Li1);
             const float dz = minimg(xi.z - pj[u].z, L2, Li2);
@@ -244,10 +331,22 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const int sub = lane & (G - 1), grp = lane / G;
     if (tid < 64) s_lj[tid] = a.ljtab[tid];
     if (COUL == 1 || COUL == 2) s_ct[tid] = a.ctab[tid];   // 256 threads, MD_CTAB entries
-    __syncthreads();
-
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
+    // One replica: its state and, with a deferred finalize, the previous launch's
+    // partial sums are loaded together with the LDS tables (one memory round trip).
+    const bool one = a.nrep == 1 && u0 < u1;
+    RepState s_one;
+    double v_one[MD_NSTAT];
+    if (one) {
+        s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
+        if (a.lag) {
+            const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
+            if (u0 == 0 && a.stats) sum_slots<true>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+            else sum_slots<false>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+        }
+    }
+    __syncthreads();
     int cur_r = -1, run_u0 = 0;
     bool run_r = false, en = false;
     const float4* pin = nullptr;
@@ -271,7 +370,8 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         }
         __syncthreads();
         if (tid < MD_NSTAT)
-            a.part[(size_t)run_u0 * MD_NSTAT + tid] = s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
+            a.part[((size_t)a.par * a.nunits + run_u0) * MD_NSTAT + tid] =
+                s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
         __syncthreads();
         dke = dlj = dco = dbo = dan = dvi = 0.0;
         trip = 0;
@@ -284,10 +384,49 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             if (cur_r >= 0 && run_r) flush();
             cur_r = r;
             run_u0 = u;
-            const RepState s = a.st[r];
+            int ru0 = 0;
+            if (a.nrep > 1) ru0 = a.rep[r].slot_off / APB;
+            const bool leader = u == ru0;   // the workgroup holding the replica's first unit
+            RepState s;
+            double lamd = 1.0;
+            if (a.lag) {
+                // Deferred finalize of the previous launch (same block, same grid): every
+                // workgroup sums the replica's KE and trip flag from the previous partial
+                // slots and advances the hand-over state; the leader also writes the
+                // statistics record.  Same arithmetic as k_step_finalize.
+                s = one ? s_one : a.lst[(size_t)(a.par ^ 1) * a.nrep + r];
+                lamd = s.lam;
+                if (s.ran) {
+                    const bool en_prev = a.energy_every > 0 && (s.step % a.energy_every) == 0;
+                    double v[MD_NSTAT];
+                    if (one) {
+#pragma unroll
+                        for (int k = 0; k < MD_NSTAT; k++) v[k] = v_one[k];
+                    } else {
+                        const int ru1 = (a.rep[r].slot_off + a.rep[r].nslot) / APB;
+                        const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
+                        if (leader && en_prev && a.stats) sum_slots<true>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
+                        else sum_slots<false>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
+                    }
+                    const long long kst = s.step - s.start;
+                    double T;
+                    lf_done(a, a.rep[r], s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
+                    if (leader && tid == 0 && a.stats)
+                        put_record(a, r, kst, en_prev, v, 0.5 * v[ACC_KE2], T, a.th[r].xi, a.th[r].vxi, 0.0);
+                }
+            } else {
+                s = one ? s_one : a.st[r];
+                if (a.mode == MODE_LF) lamd = a.th[r].S;
+            }
             if (a.mode == MODE_FORCE) run_r = !stale(s);
             else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
             else run_r = s.phase == 1 && !stale(s);
+            if (a.lagmode && leader && tid == 0) {   // hand-over to the next launch
+                RepState o = s;
+                o.ran = run_r ? 1 : 0;
+                o.lam = lamd;
+                a.lst[(size_t)a.par * a.nrep + r] = o;
+            }
             if (!run_r) continue;
             en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
                                                         : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
@@ -297,7 +436,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             const RepParam& p = a.rep[r];
             L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
             Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
-            lam = (a.mode == MODE_LF) ? (float)a.th[r].S : 1.0f;
+            lam = (float)lamd;
         }
         if (!run_r) continue;
         const int slot = slot0 + wave * APW + grp;
@@ -421,23 +560,26 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
 }
 
 // After k_step (same grid): one workgroup per replica sums the partial slots
-// the step kernel's workgroups wrote for the replica and finalizes it.  Slot u
-// was written in this launch iff u is the replica's first unit or the first
-// unit of a non-empty workgroup range, u0(b) = floor(b * nunits / grid); only
-// those are read, so no slot has to be cleared between launches.
+// the step kernel's workgroups wrote for the replica and finalizes it (only the
+// slots written in this launch are read, so none has to be cleared).  After a
+// leapfrog block with deferred finalize (a.lagmode) the replica's state comes
+// from the last launch's hand-over copy lst[par]: this kernel finalizes that
+// launch's step, writes the state back and settles the gate counters for the
+// whole block.
 __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb, int grid) {
     __shared__ double s_red[4][MD_NSTAT];
-    // No gate test: when the step kernel left early no replica passes the test below.
-    // Every load is issued before any is consumed: with one replica its units are
-    // all units, and with at least one workgroup per unit every unit starts a run,
-    // so the slot loads need neither the layout nor a division.
     const int r = blockIdx.x;
-    const RepState s = a.st[r];
+    // Every load is issued before any is consumed: with one replica its units are
+    // all units, so the slot loads need no layout load first.
+    const double* part = a.part + (size_t)a.par * a.nunits * MD_NSTAT;
+    const RepState s = a.lagmode ? a.lst[(size_t)a.par * a.nrep + r] : a.st[r];
     RepParam p;
     ThState th;
+    RepState s_old;
     if (threadIdx.x == 0) {
         p = a.rep[r];
         th = a.th[r];
+        if (a.lagmode) s_old = a.st[r];
     }
     int ru0 = 0, ru1 = a.nunits;
     if (a.nrep > 1) {
@@ -445,44 +587,41 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
         ru0 = so / apb;
         ru1 = (so + ns) / apb;
     }
-    const bool every = grid >= a.nunits;
-    const long long n = a.nunits;
     double v[MD_NSTAT];
-    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
-    for (int u = ru0 + (int)threadIdx.x; u < ru1; u += blockDim.x) {
-        bool start = every || u == ru0;
-        if (!start) {
-            const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
-            start = b < grid && (b * n) / grid == u;
-        }
-        if (start) {
-            const double2* q = reinterpret_cast<const double2*>(a.part + (size_t)u * MD_NSTAT);
-#pragma unroll
-            for (int k = 0; k < MD_NSTAT / 2; k++) {
-                const double2 t = q[k];
-                v[2 * k] += t.x;
-                v[2 * k + 1] += t.y;
+    sum_slots<true>(part, ru0, ru1, grid, a.nunits, s_red, v);   // used only if the replica ran
+    if (a.lagmode) {
+        // No launch of the block passed the gate when none could step, and lst[par]
+        // was not written: nothing to do.  (A replica that could step keeps
+        // n_run >= 1 until its own workgroup here; one that could not is unchanged.)
+        if (threadIdx.x == 0 && a.gate->n_run != 0) {
+            RepState n = s;
+            double lam = s.lam;
+            if (s.ran) {
+                double T;
+                lf_done(a, p, n, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lam);
+                const bool en = a.energy_every > 0 && (s.step % a.energy_every) == 0;
+                put_record(a, r, s.step - s.start, en, v, 0.5 * v[ACC_KE2], T, th.xi, th.vxi, 0.0);
             }
+            a.th[r].S = lam;
+            RepState& st = a.st[r];
+            st.age = n.age;
+            st.step = n.step;
+            st.trip[0] = n.trip[0];
+            st.trip[1] = n.trip[1];
+            lf_gates(a, s_old, n);
         }
+        return;
     }
+    // No gate test: when the step kernel left early no replica passes the test below.
     bool run;
     if (a.mode == MODE_FORCE) run = !stale(s);
     else if (a.mode == MODE_LF) run = !stale(s) && s.step < s.target;
     else run = s.phase == 1 && !stale(s);
-    if (!run) return;   // block-uniform
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < MD_NSTAT; k++)
-        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
-    if (lane == 0)
-        for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[MD_NSTAT];
-        for (int k = 0; k < MD_NSTAT; k++) t[k] = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+    if (run && threadIdx.x == 0) {
         const long long kstat = (a.mode == MODE_FORCE) ? 0 : s.step - s.start;
         const bool en = (a.mode == MODE_FORCE) ? (a.force_energy != 0)
                                                : (a.energy_every > 0 && (s.step % a.energy_every) == 0);
-        finalize_replica(a, r, p, s, th, kstat, en, t, t[MD_NSTAT - 1] != 0.0);
+        finalize_replica(a, r, p, s, th, kstat, en, v, v[MD_NSTAT - 1] != 0.0);
     }
 }
 

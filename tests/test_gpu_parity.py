@@ -258,9 +258,10 @@ LAUNCH_VARIANTS = [
     dict(submit=1, refresh=1, check_every=6, build_kernel=1, lanes=32),
     dict(submit=0, refresh=0, check_every=12, build_kernel=1, lanes=16, cell_div=3),
     dict(submit=0, refresh=0, check_every=8, build_kernel=1, lanes=8, cell_div=1),
-    dict(submit=0, refresh=1, check_every=8, build_kernel=2, lanes=4, cell_div=2),
+    dict(submit=0, refresh=1, check_every=8, build_kernel=2, lanes=4, cell_div=2, defer=1),
     dict(submit=1, refresh=0, check_every=16, build_kernel=2, lanes=16, cell_div=3),
-    dict(submit=0, refresh=1, check_every=6, build_kernel=3, lanes=8, cell_div=1),
+    dict(submit=0, refresh=1, check_every=6, build_kernel=3, lanes=8, cell_div=1, defer=1),
+    dict(submit=1, refresh=0, check_every=12, build_kernel=3, lanes=16, defer=1),
 ]
 
 

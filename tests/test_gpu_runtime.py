@@ -56,8 +56,9 @@ def test_lists_valid_after_in_run_rebuilds(E, orc, nsites, cfg):
 
 
 def test_runs_reproducible(E):
-    """Two engines on the same input follow the same trajectory: the only
-    run-to-run difference allowed is the order of FP64 atomic sums."""
+    """Two engines on the same input follow the same trajectory bit for bit:
+    every reduction has a fixed order (no floating-point atomics) and the cell
+    sort orders each cell by original index."""
     s = S.nacl()
     say("reproducible nacl32768")
     runs = []
@@ -71,8 +72,37 @@ def test_runs_reproducible(E):
     (t0, p0, r0), (t1, p1, r1) = runs
     say("  rebuilds", r0, r1, "max dT/T", np.abs(t0 - t1).max() / t0.mean(), "max dx", np.abs(p0 - p1).max())
     assert r0 == r1
-    assert np.allclose(t0, t1, rtol=1e-9)
-    assert np.abs(p0 - p1).max() < 1e-4
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("cfg", [dict(check_every=8, refresh=1), dict(check_every=6, refresh=0, submit=1),
+                                 dict(check_every=16, refresh=0, lanes=4)])
+def test_deferred_finalize_bit_identical(E, cfg):
+    """Leapfrog blocks whose step kernels finalize the previous step (defer=1)
+    follow the same trajectory bit for bit as a finalize kernel after every step:
+    positions, velocities, forces and every statistics record, through list
+    rebuilds, stalls on a stale list and energy steps."""
+    s = S.nacl(nsites=16)
+    say("deferred finalize", s.n, cfg)
+    out = []
+    for defer in (0, 1):
+        e = E.Engine.from_system(s)
+        e.set_launch(defer=defer, **cfg)
+        st = e.run(203, energy_every=7)
+        st2 = e.run(50, energy_every=0)
+        pos, vel, frc = e.download()
+        out.append((st, st2, pos, vel, frc, e.info()["rebuilds"]))
+        e.close()
+    a, b = out
+    say("  rebuilds", a[5], b[5])
+    assert a[5] == b[5] and a[5] > 10
+    for k in range(5):
+        if k < 2:
+            for name in a[k].dtype.names:
+                assert np.array_equal(a[k][name], b[k][name]), name
+        else:
+            assert np.array_equal(a[k], b[k])
 
 
 def test_nve_conservation_nacl32768(E):
