@@ -35,8 +35,8 @@ extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int 
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st);
-extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* gacc, int lanes, int coul, int nt, int grid,
-                                           hipStream_t st);
+extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
+                                           int grid, hipStream_t st);
 
 namespace {
 
@@ -203,6 +203,7 @@ struct md_engine {
     DBuf<Gate> gate;
     DBuf<ThState> th;
     DBuf<double> part, fm_acc;     // part: [2][S/8][MD_NSTAT] partial sums of the step kernel
+    DBuf<double> fm_part;          // [grid][2 + 2 nt^2] workgroup slots of the force-matching adjoint
     DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
     DBuf<int> err;
     DBuf<unsigned long long> reb_count;
@@ -1390,9 +1391,10 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     a.qq_scale = qscale * qscale;
     const int coul = e->coul_kernel;
     HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st));
-    HIPCHECK(hipMemsetAsync(e->fm_acc.p, 0, e->fm_acc.bytes(), e->st));
     a.qq_scale = qscale;
-    HIPCHECK(md_launch_fm_adjoint(a, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+    const size_t nslot = (size_t)e->grid_blocks() * (2 + 2 * ntk * ntk);
+    if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
+    HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
     std::vector<double> g(2 + 2 * ntk * ntk);
     HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
     CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));

@@ -9,7 +9,8 @@
 //   d ff / d lj1[ti,tj] = r^-14,  d ff / d lj2[ti,tj] = -r^-8,
 //   d ff / d qscale    = 2 qscale qfac q_i q_j C(r) / r^2   (C: Wolf / RF / damped factor)
 // Per-pair terms are FP32; each atom's partial sums are promoted to FP64
-// before they are accumulated across atoms, workgroups and frames.
+// before they are accumulated across atoms, workgroups and frames.  The
+// Coulomb factor is the force pass's cubic table (Wolf / damped).
 // Algorithmic work per ordered pair: forward 65 flop + adjoint ~45 flop.
 #include "md_device.h"
 
@@ -19,13 +20,23 @@ __device__ __forceinline__ float minimg(float d, float L, float Linv) {
 }
 }  // namespace
 
+// Row walk as in the step kernel: lane `sub` reads 16-byte quads of its atom's
+// row (logical entries sub + G m in order), the next quad in flight; four
+// gathers per quad.  The Coulomb factor C(r)/r^2 comes from the same cubic
+// table of the force pass (Wolf / damped), so the gradient is that of the
+// force the GPU computes.  Each workgroup stores its FP64 sums in its own slot
+// of part (no atomics); k_fm_reduce adds the slots.
 template <int G, int COUL, int NT>
-__global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* gacc) {
+__global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* part) {
     constexpr int APW = 64 / G, APB = 4 * APW;
     constexpr int NACC = 2 + 2 * NT * NT;
     __shared__ double s_red[4][NACC];
+    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB : 1];
+    static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
+    if (COUL == 1 || COUL == 2) s_ct[tid] = a.ctab[tid];
+    __syncthreads();
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
     double acc[NACC];
@@ -49,40 +60,52 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* ga
         float s1[NT], s2[NT], sq = 0.0f;
 #pragma unroll
         for (int t = 0; t < NT; t++) s1[t] = s2[t] = 0.0f;
-        const uint32_t* row = a.nlist + (size_t)slot * a.cap;
-        for (int k = sub; k < nn; k += G) {
-            const uint32_t e = row[lpos(k, G)];   // rows are laid out for G lanes (the step kernel's)
-            const int j = (int)(e & MD_JMASK);
-            const int tj = (int)(e >> MD_JBITS);
-            const float4 pj = pin[j];
-            const float dx = minimg(xi.x - pj.x, L0, Li0);
-            const float dy = minimg(xi.y - pj.y, L1, Li1);
-            const float dz = minimg(xi.z - pj.z, L2, Li2);
-            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const bool inside = !(rsq > a.rc2);
-            const float g = inside ? fmaf(Rz, dz, fmaf(Ry, dy, Rx * dx)) : 0.0f;
-            const float ri = __builtin_amdgcn_rsqf(rsq);
-            const float ri2 = ri * ri;
-            const float ri8 = (ri2 * ri2) * (ri2 * ri2);
-            const float t14 = g * ri8 * ri2 * ri2 * ri2;
-            const float t8 = g * ri8;
+        const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(a.nlist + (size_t)slot * a.cap);
+        const int nb = (nn + 4 * G - 1) / (4 * G);
+        uint4 nxt = nb > 0 ? row4[sub] : make_uint4(0, 0, 0, 0);
+        for (int b = 0; b < nb; b++) {
+            const uint4 e4 = nxt;
+            if (b + 1 < nb) nxt = row4[(b + 1) * G + sub];
+            const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
+            const int base = 4 * G * b + sub;
+            float4 pj[4];
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                s1[t] += (tj == t) ? t14 : 0.0f;
-                s2[t] -= (tj == t) ? t8 : 0.0f;
+            for (int q = 0; q < 4; q++) {
+                const uint32_t off = ((base + G * q < nn) ? (ev[q] & MD_JMASK) : (uint32_t)slot) << 4;
+                pj[q] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + off);
             }
-            float c = 0.0f;
-            if (COUL == 0) {
-                c = ri - a.reacf * rsq;
-            } else if (COUL == 1 || COUL == 2) {
-                const float rr = rsq * ri;
-                const float t = __builtin_amdgcn_rcpf(fmaf(a.palfa, rr, 1.0f));
-                const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
-                                    0.254829592f) * t;
-                c = fmaf(a.spiialfa, rr, poly) * __builtin_amdgcn_exp2f(a.alfasq * rsq) * ri;
-                if (COUL == 1) c -= a.wolf;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const bool valid = base + G * q < nn;
+                const int tj = (int)(ev[q] >> MD_JBITS);
+                const float dx = minimg(xi.x - pj[q].x, L0, Li0);
+                const float dy = minimg(xi.y - pj[q].y, L1, Li1);
+                const float dz = minimg(xi.z - pj[q].z, L2, Li2);
+                // a masked entry is the atom itself: give it a distance beyond the cutoff
+                const float rsq = valid ? fmaf(dz, dz, fmaf(dy, dy, dx * dx)) : 2.0f * a.rc2;
+                const bool inside = !(rsq > a.rc2);
+                const float g = inside ? fmaf(Rz, dz, fmaf(Ry, dy, Rx * dx)) : 0.0f;
+                const float ri = __builtin_amdgcn_rsqf(rsq);
+                const float ri2 = ri * ri;
+                const float ri8 = (ri2 * ri2) * (ri2 * ri2);
+                const float t8 = g * ri8;
+                const float t14 = t8 * ri2 * ri2 * ri2;
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    s1[t] += (tj == t) ? t14 : 0.0f;
+                    s2[t] -= (tj == t) ? t8 : 0.0f;
+                }
+                if (COUL == 0) {
+                    sq = fmaf(g * pj[q].w, (ri - a.reacf * rsq) * ri2, sq);
+                } else if (COUL == 1 || COUL == 2) {
+                    const float x = rsq * ri * a.ct_scale;
+                    const int k = min((int)x, MD_CTAB - 1);
+                    const float tt = x - (float)k;
+                    const float4 c4 = s_ct[k];
+                    const float hv = fmaf(fmaf(fmaf(c4.w, tt, c4.z), tt, c4.y), tt, c4.x);
+                    sq = fmaf(g * pj[q].w, (hv * ri) * ri2, sq);   // C(r) / r^2, as in the force pass
+                }
             }
-            if (COUL != 3) sq = fmaf(g * pj.w, c * ri2, sq);
         }
         // fold the atom's partials into FP64 (all lanes of the group hold parts)
         double* a1 = acc + 2;
@@ -106,14 +129,27 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* ga
     if (lane == 0)
         for (int k = 0; k < NACC; k++) s_red[wave][k] = acc[k];
     __syncthreads();
-    if (tid < NACC) {
-        double t = s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
-        if (t != 0.0) atomicAdd(gacc + tid, t);
+    if (tid < NACC) part[(size_t)blockIdx.x * NACC + tid] = s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
+}
+
+// One workgroup: out[k] = sum over the adjoint kernel's workgroup slots.
+__global__ __launch_bounds__(256) void k_fm_reduce(const double* part, int nslot, int nacc, double* out) {
+    __shared__ double s_red[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < nacc; k++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nslot; b += blockDim.x) v += part[(size_t)b * nacc + k];
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) s_red[wave] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) out[k] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+        __syncthreads();
     }
 }
 
 template <int G, int NT>
 static void launch_fm_g(const StepArgs& a, double* gacc, int coul, int grid, hipStream_t st) {
+    // gacc: [grid][2 + 2 NT^2] workgroup slots
     switch (coul) {
     case 0: hipLaunchKernelGGL((k_fm_adjoint<G, 0, NT>), dim3(grid), dim3(256), 0, st, a, gacc); break;
     case 1: hipLaunchKernelGGL((k_fm_adjoint<G, 1, NT>), dim3(grid), dim3(256), 0, st, a, gacc); break;
@@ -135,10 +171,14 @@ static hipError_t launch_fm_nt(const StepArgs& a, double* gacc, int lanes, int c
 }
 
 // ntype must be 1, 2 or 4 (the host rounds up; unused type rows stay zero)
-extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* gacc, int lanes, int coul, int nt, int grid,
-                                           hipStream_t st) {
-    if (nt == 1) return launch_fm_nt<1>(a, gacc, lanes, coul, grid, st);
-    if (nt == 2) return launch_fm_nt<2>(a, gacc, lanes, coul, grid, st);
-    if (nt == 4) return launch_fm_nt<4>(a, gacc, lanes, coul, grid, st);
-    return hipErrorInvalidValue;
+// part: [grid][2 + 2 nt^2] scratch; out: [2 + 2 nt^2] sums (loss, qscale, lj1, lj2 terms)
+extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
+                                           int grid, hipStream_t st) {
+    hipError_t e = hipErrorInvalidValue;
+    if (nt == 1) e = launch_fm_nt<1>(a, part, lanes, coul, grid, st);
+    if (nt == 2) e = launch_fm_nt<2>(a, part, lanes, coul, grid, st);
+    if (nt == 4) e = launch_fm_nt<4>(a, part, lanes, coul, grid, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fm_reduce, dim3(1), dim3(256), 0, st, (const double*)part, grid, 2 + 2 * nt * nt, out);
+    return hipGetLastError();
 }
