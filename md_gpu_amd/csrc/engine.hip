@@ -204,6 +204,8 @@ struct md_engine {
     DBuf<ThState> th;
     DBuf<double> part, fm_acc;     // part: [2][S/8][MD_NSTAT] partial sums of the step kernel
     DBuf<double> fm_part;          // [grid][2 + 2 nt^2] workgroup slots of the force-matching adjoint
+    DBuf<float4> fm_ljtab;         // LJ table of the trial parameters (force matching)
+    std::vector<float4> fm_ljtab_h;    // its host staging copy (alive until the next evaluation)
     DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
     DBuf<int> err;
     DBuf<unsigned long long> reb_count;
@@ -290,9 +292,10 @@ void derive_ff(md_engine* e) {
     }
 }
 
-int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
+// LJ table of the step kernel: (lj1, lj2, energy shift at rc, 0) per type pair
+void make_ljtab(const md_engine* e, const float* lj1, const float* lj2, float4* t) {
     const int nt = e->ff.ntype;
-    std::vector<float4> t(64, make_float4(0, 0, 0, 0));
+    for (int k = 0; k < 64; k++) t[k] = make_float4(0, 0, 0, 0);
     double rc2 = (double)e->ff.rc * e->ff.rc;
     for (int i = 0; i < nt; i++)
         for (int j = 0; j < nt; j++) {
@@ -301,6 +304,11 @@ int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
             double shift = l1 / 12.0 * ri6 * ri6 - l2 / 6.0 * ri6;
             t[i * 8 + j] = make_float4((float)l1, (float)l2, (float)shift, 0.f);
         }
+}
+
+int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
+    std::vector<float4> t(64);
+    make_ljtab(e, lj1, lj2, t.data());
     HIPCHECK(hipMemcpyAsync(e->ljtab.p, t.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
@@ -1385,8 +1393,15 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     if (nt > 4) return fail(MD_E_CAPACITY, "force matching supports at most 4 atom types");
     const int ntk = nt == 1 ? 1 : (nt == 2 ? 2 : 4);
     HIPCHECK(hipSetDevice(e->dev));
-    CHECK(upload_ljtab(e, lj1, lj2));
+    // trial LJ table in its own buffer: the engine's table is left alone, and the
+    // upload is ordered on the stream (one synchronisation per evaluation, at the end)
+    if (!e->fm_ljtab.p) CHECK(e->fm_ljtab.alloc(64));
+    e->fm_ljtab_h.resize(64);
+    HIPCHECK(hipStreamSynchronize(e->st));   // the staging copy of the previous evaluation is free
+    make_ljtab(e, lj1, lj2, e->fm_ljtab_h.data());
+    HIPCHECK(hipMemcpyAsync(e->fm_ljtab.p, e->fm_ljtab_h.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
     StepArgs a = step_args(e, MODE_FORCE, 0);
+    a.ljtab = e->fm_ljtab.p;
     a.stats = nullptr;
     a.qq_scale = qscale * qscale;
     const int coul = e->coul_kernel;
@@ -1397,7 +1412,7 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
     std::vector<double> g(2 + 2 * ntk * ntk);
     HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
-    CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
+    HIPCHECK(hipStreamSynchronize(e->st));
     const double M = 3.0 * (double)e->N;
     if (loss) *loss = g[0] / M;
     if (g_qscale) *g_qscale = 2.0 / M * g[1];
