@@ -226,7 +226,7 @@ def extra_workloads(args, want_cpu, tune):
     # argon 864, NVE velocity Verlet, 1000 steps
     ar = S.argon()
     out["argon864_nve_vv"] = step_workload(
-        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 1000, 100, False, ar.dt,
+        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 1000, 100, tune, ar.dt,
         (lambda e, s: cpu_small(e, s, 5.0)) if want_cpu else None)
     # water 196,608 atoms, NVT velocity Verlet + Nose-Hoover
     wa = S.water()

@@ -221,7 +221,7 @@ struct md_engine {
     int cap = 0;
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 1, build_kernel = 3;
+    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 3;
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
@@ -1265,7 +1265,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     // stage 3, whole runs (best of two): shape x rebuild-check interval x age refresh x submission mode
     // x deferred finalize
     double best_run = 1e30;
-    const int kr = std::max(nsteps, 2 * kSegment);
+    const int kr = std::max(nsteps, 4 * kSegment);
     md_launch_config base = best;
     const bool lf = e->in.integrator == MD_LEAPFROG;
     for (const Shape& sh : shapes)

@@ -1,0 +1,47 @@
+"""Short fixed-launch run of the molten-NaCl workload (32,768 ions) for
+hardware-counter collection.
+
+    python -m md_gpu_amd.profrun [--launch k=v,...] [--steps N] [--timed]
+
+Counter collection serialises every dispatch, so this driver keeps the run
+short, needs no autotuning and reports (rather than raises) a failed run.
+With --timed every step is launched directly with HIP events around it (the
+same timing path bench.py uses for kernel durations).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--launch", default="lanes=8,grid=1024,cell_div=2,check_every=8,build_lanes=8,build_kernel=3,"
+                                        "refresh=1,defer=1,submit=1")
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--timed", action="store_true")
+    args = ap.parse_args()
+    from md_gpu_amd import engine as E, systems as S
+    cfg = dict((k, int(v)) for k, v in (kv.split("=") for kv in args.launch.split(",") if kv))
+    e = E.Engine.from_system(S.nacl())
+    e.set_launch(**cfg)
+    try:
+        if args.warmup:
+            e.run(args.warmup, stats=False)
+        if args.timed:
+            e.timing(True)
+        e.run(args.steps, stats=False)
+        out = dict(ok=True, launch=e.get_launch(), steps=args.steps)
+        if args.timed:
+            out["timing"] = e.timing_get()
+    except E.MDError as x:
+        out = dict(ok=False, error=str(x))
+    print(json.dumps(out), flush=True)
+    e.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
