@@ -152,6 +152,8 @@ typedef struct md_launch_config {
                           2, 3: candidate-parallel build (a lane per candidate, 16 or 8 atoms per wavefront) */
     int defer;         /* leapfrog: 1 = each step kernel finalizes the previous step of its block
                           (temperature, thermostat, statistics) instead of a finalize kernel per step */
+    int vv_fuse;       /* NVE velocity Verlet: 1 = one step kernel per step ends the step and begins the
+                          next (leapfrog form, so defer applies); 0 = kick, force and finalize kernels */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);

@@ -223,6 +223,7 @@ struct md_engine {
     long long step = 0;          // steps completed (all replicas advance together between calls)
     int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 3;
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
+    int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -391,6 +392,13 @@ int ensure_stats(md_engine* e, int nrec) {
     return MD_OK;
 }
 
+// NVE velocity Verlet runs in leapfrog form (one step kernel per step)
+bool vv_fused(const md_engine* e) {
+    return e->in.integrator == MD_VVERLET && e->in.thermostat == MD_TH_NONE && e->vv_fuse;
+}
+// the step kernel runs in leapfrog form: leapfrog, or fused velocity Verlet
+bool lf_form(const md_engine* e) { return e->in.integrator == MD_LEAPFROG || vv_fused(e); }
+
 BuildArgs build_args(md_engine* e) {
     BuildArgs b{};
     b.pos[0] = e->pos[0].p; b.pos[1] = e->pos[1].p; b.vel = e->vel.p; b.force = e->force.p;
@@ -407,6 +415,7 @@ BuildArgs build_args(md_engine* e) {
     b.build_lanes = e->build_lanes;
     b.row_g = e->lanes;
     b.build_kernel = e->build_kernel;
+    b.lfgate = lf_form(e) ? 1 : 0;
     return b;
 }
 
@@ -416,6 +425,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
     a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0;
+    a.vvfused = vv_fused(e) ? 1 : 0;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
     a.ctab = e->ctab.p;
@@ -441,6 +451,7 @@ KickArgs kick_args(md_engine* e, int apply_drift) {
     k.pos[0] = e->pos[0].p; k.pos[1] = e->pos[1].p; k.vel = e->vel.p; k.force = e->force.p; k.ids = e->ids.p;
     k.pos_build = e->pos_build.p; k.tile_rep = e->tile_rep.p; k.rep = e->rep.p; k.st = e->rst.p; k.gate = e->gate.p; k.th = e->th.p;
     k.err = e->err.p; k.tflag = e->tflag.p; k.nslot = e->S; k.nrep = e->R; k.dt = e->in.dt; k.skin2q = e->skin2q; k.apply_drift = apply_drift;
+    k.lfgate = lf_form(e) ? 1 : 0;
     return k;
 }
 
@@ -501,7 +512,7 @@ int enqueue_lf_block(md_engine* e, int energy_every, int n) {
 // chain_mid a gated rebuild chain sits between the velocity-Verlet kick and
 // the force kernel, so the step always completes (timed launches).
 int enqueue_step(md_engine* e, int energy_every, TimedSpan* ts, bool chain_mid = false) {
-    if (e->in.integrator == MD_LEAPFROG) {
+    if (lf_form(e)) {
         CHECK(enqueue_stepkernel(e, step_args(e, MODE_LF, energy_every), ts));
     } else {
         HIPCHECK(md_launch_kick(kick_args(e, 1), e->st));
@@ -639,6 +650,16 @@ int md_set_integrator(md_engine* e, const md_integrator* in) {
     if (in->thermostat != MD_TH_NONE && !(in->tau > 0)) return fail(MD_E_INVALID, "tau must be > 0");
     if (e->has_state && e->in.integrator != in->integrator)
         return fail(MD_E_STATE, "switching the integrator needs a new upload (velocity convention differs)");
+    // A new velocity-Verlet thermostat starts from the current velocities (leapfrog-form NVE
+    // runs do not keep the thermostat's kinetic-energy record): a state round trip resets it.
+    const bool rethermo = e->has_state && e->has_integ && in->integrator == MD_VVERLET &&
+                          in->thermostat != e->in.thermostat;
+    std::vector<float> P, V;
+    if (rethermo) {
+        P.resize(3 * (size_t)e->N);
+        V.resize(3 * (size_t)e->N);
+        CHECK(md_download_state(e, P.data(), V.data(), nullptr));
+    }
     e->in = *in;
     e->has_integ = true;
     e->drop_graphs();
@@ -649,6 +670,7 @@ int md_set_integrator(md_engine* e, const md_integrator* in) {
         }
         HIPCHECK(hipMemcpy(e->rep.p, e->rep_h.data(), e->R * sizeof(RepParam), hipMemcpyHostToDevice));
     }
+    if (rethermo) CHECK(md_upload_state(e, P.data(), V.data()));
     return MD_OK;
 }
 
@@ -930,7 +952,7 @@ int enqueue_segment(md_engine* e, int energy_every) {
     const int blocks = std::max(1, kSegment / ce);
     for (int b = 0; b < blocks; b++) {
         CHECK(enqueue_rebuild(e, nullptr));
-        if (e->defer && e->in.integrator == MD_LEAPFROG) CHECK(enqueue_lf_block(e, energy_every, ce));
+        if (e->defer && lf_form(e)) CHECK(enqueue_lf_block(e, energy_every, ce));
         else for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
     }
     return MD_OK;
@@ -988,7 +1010,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     if (nsteps == 0) return MD_OK;
     if (energy_every < 0) return fail(MD_E_INVALID, "energy_every < 0");
     HIPCHECK(hipSetDevice(e->dev));
-    const bool vv = e->in.integrator == MD_VVERLET;
+    const bool vv = e->in.integrator == MD_VVERLET, fused = vv_fused(e);
     if (stats) CHECK(ensure_stats(e, nsteps));
     const long long target = e->step + nsteps;
     int retries = 0;
@@ -997,6 +1019,12 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
         HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
         HIPCHECK(md_launch_set_target(e->rst.p, e->gate.p, e->R, nsteps, e->st));
         if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
+        if (fused) {
+            // leapfrog-form velocity Verlet: a valid list, then the first half-kick and drift of
+            // the run's first step; every step kernel then ends one step and begins the next
+            CHECK(enqueue_rebuild(e, nullptr));
+            HIPCHECK(md_launch_kick(kick_args(e, 1), e->st));
+        }
         long long launched = 0;
         std::vector<RepState> rs;
         if (e->timing) {
@@ -1056,7 +1084,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
         }
         break;
     }
-    if (vv) {
+    if (vv && !fused) {   // (leapfrog form: the last step kernel left full-step velocities)
         HIPCHECK(md_launch_kick(kick_args(e, 0), e->st));
         HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 1, e->st));
     }
@@ -1140,6 +1168,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
     if (c->build_kernel < 0 || c->build_kernel > 3) return fail(MD_E_INVALID, "build_kernel must be 0..3");
     if (c->defer != 0 && c->defer != 1) return fail(MD_E_INVALID, "defer must be 0 or 1");
+    if (c->vv_fuse != 0 && c->vv_fuse != 1) return fail(MD_E_INVALID, "vv_fuse must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
@@ -1152,6 +1181,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     e->refresh = c->refresh;
     e->build_kernel = c->build_kernel;
     e->defer = c->defer;
+    e->vv_fuse = c->vv_fuse;
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
@@ -1176,6 +1206,7 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->refresh = e->refresh;
     c->build_kernel = e->build_kernel;
     c->defer = e->defer;
+    c->vv_fuse = e->vv_fuse;
     return MD_OK;
 }
 
@@ -1267,13 +1298,13 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     double best_run = 1e30;
     const int kr = std::max(nsteps, 4 * kSegment);
     md_launch_config base = best;
-    const bool lf = e->in.integrator == MD_LEAPFROG;
+    const bool lf = lf_form(e);
     for (const Shape& sh : shapes)
         for (int sub : {0, 1})
             for (int rf : {0, 1})
                 for (int df : {1, 0})
                 for (int ce : {4, 6, 8, 12, 16}) {
-                    if (df == 0 && !lf) continue;   // deferral applies to leapfrog only
+                    if (df == 0 && !lf) continue;   // deferral applies to leapfrog-form steps only
                     md_launch_config c = base;
                     c.lanes = sh.lanes;
                     c.grid = sh.grid;

@@ -125,6 +125,9 @@ struct StepArgs {
     double* part;         // [2][nunits][MD_NSTAT] FP64 partial sums, one slot per run of units
                           // (buffer par; the deferred finalize reads buffer par ^ 1)
     RepState* lst;        // [2][R] state handed from one step launch to the next (deferred finalize)
+    int vvfused;          // NVE velocity Verlet in leapfrog form (mode MODE_LF): the step kernel ends the
+                          // step (second half-kick) and, unless the run ends, begins the next one (first
+                          // half-kick and drift); between launches the replica is kicked (phase 1)
     int lagmode;          // leapfrog block with deferred finalize: the step kernel writes lst[par]
     int lag;              // the previous launch was a step kernel of the same block: finalize it first
     int par;              // buffer parity of this launch
@@ -177,6 +180,7 @@ struct KickArgs {
     int nrep;
     float dt, skin2q;
     int apply_drift;      // 0: run end (v *= S_end only)
+    int lfgate;           // the step kernel is in leapfrog form: a kicked replica stays counted in n_run
 };
 
 struct BuildArgs {
@@ -218,5 +222,7 @@ struct BuildArgs {
     int has_excl;
     int build_lanes;      // lanes per atom in k_nl_build (4, 8)
     int row_g;            // lanes per atom of the step kernel (row layout, lpos)
-    int build_kernel;     // 0: k_nl_build (tile-staged), 1: k_nl_wbuild (per wavefront)
+    int build_kernel;     // 0: k_nl_build (tile-staged), 1: k_nl_wbuild (per wavefront), 2/3: k_nl_jbuild
+    int lfgate;           // the step kernel is in leapfrog form: n_run counts every valid replica short of
+                          // its target, whatever its phase
 };

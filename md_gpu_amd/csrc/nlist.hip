@@ -916,8 +916,8 @@ __global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a) {
                 a.st[r].nb = st.nb + 1;
             }
             done = 1;
-            to_run = st.phase == 0 && st.step < st.target;
-            to_force = st.phase == 1;
+            to_run = (a.lfgate || st.phase == 0) && st.step < st.target;
+            to_force = !a.lfgate && st.phase == 1;
         }
     }
     if (lane == 0) {

@@ -68,6 +68,8 @@ __device__ __forceinline__ void lf_done(const StepArgs& a, const RepParam& p, Re
         if (s.nb & 1) s.trip[1] = 1;
         else s.trip[0] = 1;
     }
+    // velocity Verlet in leapfrog form: the run's last step began no next step
+    if (a.vvfused && s.step >= s.target) s.phase = 0;
 }
 
 // Statistics record of a finished step (index kstat of the run).
@@ -116,6 +118,7 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
         RepState& st = a.st[r];
         st.age = s.age;
         st.step = s.step;
+        st.phase = s.phase;
         st.trip[0] = s.trip[0];
         st.trip[1] = s.trip[1];
         lf_gates(a, st0, s);
@@ -352,6 +355,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const float4* pin = nullptr;
     float4* pout = nullptr;
     float L0 = 0, L1 = 0, L2 = 0, Li0 = 0, Li1 = 0, Li2 = 0, lam = 1.0f;
+    bool drift = true;   // leapfrog form: the positions advance (velocity Verlet: not after the run's last step)
     double dke = 0.0, dlj = 0.0, dco = 0.0, dbo = 0.0, dan = 0.0, dvi = 0.0;
     int trip = 0;
 
@@ -437,6 +441,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
             Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
             lam = (float)lamd;
+            drift = !a.vvfused || s.step + 1 < s.target;
         }
         if (!run_r) continue;
         const int slot = slot0 + wave * APW + grp;
@@ -530,21 +535,39 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             a.force[slot] = make_float4(fx, fy, fz, 0.0f);
             if (a.mode == MODE_LF) {
                 float4 v = a.vel[slot];
-                const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
-                v.x = fmaf(fx, massif, v.x) * lam;
-                v.y = fmaf(fy, massif, v.y) * lam;
-                v.z = fmaf(fz, massif, v.z) * lam;
+                if (a.vvfused) {
+                    // velocity Verlet, NVE: v(t) = v(t - dt/2) + F dt/2m ends the step (its kinetic
+                    // energy is the step's); v(t + dt/2) = v(t) + F dt/2m begins the next.  The same
+                    // operations as the kick kernel and the force-half epilogue below.
+                    const float h = 0.5f * a.dt * v.w;
+                    v.x = fmaf(fx, h, v.x);
+                    v.y = fmaf(fy, h, v.y);
+                    v.z = fmaf(fz, h, v.z);
+                    dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+                    if (drift) {
+                        v.x = fmaf(fx, h, v.x * lam);
+                        v.y = fmaf(fy, h, v.y * lam);
+                        v.z = fmaf(fz, h, v.z * lam);
+                    }
+                } else {
+                    const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
+                    v.x = fmaf(fx, massif, v.x) * lam;
+                    v.y = fmaf(fy, massif, v.y) * lam;
+                    v.z = fmaf(fz, massif, v.z) * lam;
+                    dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+                }
                 a.vel[slot] = v;
-                float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
-                x = fmaf(-L0, truncf(x * 2.0f * Li0), x);   // src/nve_leapfrog.cu:50-55
-                y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
-                z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
-                pout[slot] = make_float4(x, y, z, xi.w);
-                dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
-                const float4 pb = a.pos_build[slot];
-                const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
-                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
-                if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
+                if (drift) {
+                    float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
+                    x = fmaf(-L0, truncf(x * 2.0f * Li0), x);   // src/nve_leapfrog.cu:50-55
+                    y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
+                    z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
+                    pout[slot] = make_float4(x, y, z, xi.w);
+                    const float4 pb = a.pos_build[slot];
+                    const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
+                    if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
+                    if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
+                }
             } else if (a.mode == MODE_VV) {
                 float4 v = a.vel[slot];
                 const float h = 0.5f * a.dt * v.w;
@@ -606,6 +629,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
             RepState& st = a.st[r];
             st.age = n.age;
             st.step = n.step;
+            st.phase = n.phase;
             st.trip[0] = n.trip[0];
             st.trip[1] = n.trip[1];
             lf_gates(a, s_old, n);
@@ -733,6 +757,7 @@ __global__ __launch_bounds__(256) void k_kick_finalize(const KickArgs a) {
     if (threadIdx.x == 0) {
         int n = 0, nt = 0, nf = 0;
         for (int w = 0; w < 4; w++) { n += s_d[w][0]; nt += s_d[w][1]; nf += s_d[w][2]; }
+        if (a.lfgate) n = nt, nf = 0;   // leapfrog-form steps follow: only a tripped replica stops
         if (n) atomicSub(&a.gate->n_run, n);
         if (nt) atomicAdd(&a.gate->n_stale, nt);
         if (nf) atomicAdd(&a.gate->n_force, nf);

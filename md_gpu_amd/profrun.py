@@ -1,7 +1,10 @@
-"""Short fixed-launch run of the molten-NaCl workload (32,768 ions) for
-hardware-counter collection.
+"""Short fixed-launch run of one benchmark workload for hardware-counter
+collection and kernel traces: molten NaCl (32,768 ions, the default), argon
+(864 atoms, NVE velocity Verlet), water (196,608 atoms, NVT velocity Verlet)
+or the 512 batched Ar-Xe replicas.
 
-    python -m md_gpu_amd.profrun [--launch k=v,...] [--steps N] [--timed]
+    python -m md_gpu_amd.profrun [--system nacl|argon|water|arxe]
+                                 [--launch k=v,...] [--steps N] [--timed]
 
 Counter collection serialises every dispatch, so this driver keeps the run
 short, needs no autotuning and reports (rather than raises) a failed run.
@@ -19,13 +22,15 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--launch", default="lanes=8,grid=1024,cell_div=2,check_every=8,build_lanes=8,build_kernel=3,"
                                         "refresh=1,defer=1,submit=1")
+    ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe"])
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--timed", action="store_true")
     args = ap.parse_args()
     from md_gpu_amd import engine as E, systems as S
     cfg = dict((k, int(v)) for k, v in (kv.split("=") for kv in args.launch.split(",") if kv))
-    e = E.Engine.from_system(S.nacl())
+    build = {"nacl": S.nacl, "argon": S.argon, "water": S.water, "arxe": S.arxe_replicas}[args.system]
+    e = E.Engine.from_system(build())
     e.set_launch(**cfg)
     try:
         if args.warmup:
@@ -33,7 +38,7 @@ def main() -> int:
         if args.timed:
             e.timing(True)
         e.run(args.steps, stats=False)
-        out = dict(ok=True, launch=e.get_launch(), steps=args.steps)
+        out = dict(ok=True, system=args.system, launch=e.get_launch(), steps=args.steps)
         if args.timed:
             out["timing"] = e.timing_get()
     except E.MDError as x:

@@ -105,6 +105,68 @@ def test_deferred_finalize_bit_identical(E, cfg):
             assert np.array_equal(a[k], b[k])
 
 
+def _nve_vv(s):
+    s = s.copy()
+    s.integrator, s.thermostat = "vverlet", "none"
+    return s
+
+
+@pytest.mark.parametrize("name,cfg", [("nacl", dict(check_every=8)),
+                                      ("nacl", dict(check_every=6, defer=0, submit=1)),
+                                      ("nacl", dict(check_every=4, lanes=16)),
+                                      ("argon", dict(check_every=16, lanes=32))])
+def test_fused_vv_bit_identical(E, name, cfg):
+    """NVE velocity Verlet in leapfrog form (one step kernel per step that ends
+    the step and begins the next, vv_fuse=1) follows the same trajectory bit for
+    bit as the kick / force / finalize kernels (vv_fuse=0): positions, full-step
+    velocities, forces and every statistics record, through list rebuilds,
+    stalls on a stale list, energy steps and two runs."""
+    s = _nve_vv(S.nacl(nsites=16) if name == "nacl" else S.argon(t_star=2.5))
+    say("fused vv", name, s.n, cfg)
+    out = []
+    for fuse in (0, 1):
+        e = E.Engine.from_system(s)
+        e.set_launch(vv_fuse=fuse, refresh=0, **cfg)
+        st = e.run(203, energy_every=7)
+        st2 = e.run(50, energy_every=0)
+        pos, vel, frc = e.download()
+        out.append((st, st2, pos, vel, frc, e.info()["rebuilds"]))
+        e.close()
+    a, b = out
+    say("  rebuilds", a[5], b[5])
+    assert a[5] == b[5] and a[5] >= 5
+    for k in range(5):
+        if k < 2:
+            for nm in a[k].dtype.names:
+                assert np.array_equal(a[k][nm], b[k][nm]), nm
+        else:
+            assert np.array_equal(a[k], b[k])
+
+
+def test_vv_thermostat_switch_starts_from_velocities(E):
+    """Switching an NVE velocity-Verlet engine (leapfrog form) to Nose-Hoover
+    restarts the thermostat from the current velocities: the run matches bit for
+    bit an engine uploaded fresh with the same positions and velocities."""
+    s = _nve_vv(S.argon())
+    say("thermostat switch argon")
+    e = E.Engine.from_system(s)
+    e.set_launch(check_every=8)
+    e.run(100, stats=False)
+    pos, vel, _ = e.download()
+    e.set_integrator("vverlet", "nosehoover", s.dt, 200.0)
+    st = e.run(150)
+    e.close()
+    t = s.copy()
+    t.pos, t.vel, t.thermostat, t.tau = pos, vel, "nosehoover", 200.0
+    f = E.Engine.from_system(t)
+    f.set_launch(check_every=8)
+    st2 = f.run(150)
+    f.close()
+    say("  T", st["temp"][-1, 0], st2["temp"][-1, 0])
+    for nm in ("ke", "temp", "conserved", "xi", "vxi"):
+        assert np.array_equal(st[nm], st2[nm]), nm
+
+
 def test_nve_conservation_nacl32768(E):
     """NaCl, 32,768 ions from the rocksalt lattice with 1200 K velocities, NVE
     leapfrog dt 2 fs, 1000 steps: the total energy (leapfrog half-step kinetic
