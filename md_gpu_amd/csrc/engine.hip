@@ -154,12 +154,13 @@ std::vector<float4> coulomb_table(double a, double rc, bool shifted) {
         v[k] = A - w * k * h;
         d[k] = (dA - w) * h;   // derivative per unit of the local coordinate
     }
-    std::vector<float4> t(MD_CTAB);
+    std::vector<float4> t(MD_CTAB + 1);
     for (int k = 0; k < MD_CTAB; k++) {
         const double c2 = 3.0 * (v[k + 1] - v[k]) - 2.0 * d[k] - d[k + 1];
         const double c3 = 2.0 * (v[k] - v[k + 1]) + d[k] + d[k + 1];
         t[k] = make_float4((float)v[k], (float)d[k], (float)c2, (float)c3);
     }
+    t[MD_CTAB] = make_float4((float)v[MD_CTAB], 0.0f, 0.0f, 0.0f);   // h(rc), for r = rc exactly
     return t;
 }
 
@@ -601,7 +602,7 @@ int md_create(md_engine** out, int device) {
         delete e;
         return fail(MD_E_HIP, "hipStreamCreate failed");
     }
-    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
+    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
         delete e;
         return MD_E_HIP;
     }
@@ -631,7 +632,7 @@ int md_set_forcefield(md_engine* e, const md_forcefield* f) {
     CHECK(upload_ljtab(e, e->ff.lj1, e->ff.lj2));
     if (e->ff.coulomb == MD_COUL_WOLF || e->ff.coulomb == MD_COUL_DAMPED) {
         const std::vector<float4> t = coulomb_table(e->ff.alpha, e->ff.rc, e->ff.coulomb == MD_COUL_WOLF);
-        HIPCHECK(hipMemcpy(e->ctab.p, t.data(), MD_CTAB * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(e->ctab.p, t.data(), (MD_CTAB + 1) * sizeof(float4), hipMemcpyHostToDevice));
     }
     e->has_ff = true;
     e->has_sys = false;   // cells depend on r_list: set the system again

@@ -31,11 +31,14 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* pa
     constexpr int APW = 64 / G, APB = 4 * APW;
     constexpr int NACC = 2 + 2 * NT * NT;
     __shared__ double s_red[4][NACC];
-    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB : 1];
+    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
     static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
-    if (COUL == 1 || COUL == 2) s_ct[tid] = a.ctab[tid];
+    if (COUL == 1 || COUL == 2) {
+        s_ct[tid] = a.ctab[tid];
+        if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
+    }
     __syncthreads();
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
@@ -98,12 +101,8 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* pa
                 if (COUL == 0) {
                     sq = fmaf(g * pj[q].w, (ri - a.reacf * rsq) * ri2, sq);
                 } else if (COUL == 1 || COUL == 2) {
-                    const float x = rsq * ri * a.ct_scale;
-                    const int k = min((int)x, MD_CTAB - 1);
-                    const float tt = x - (float)k;
-                    const float4 c4 = s_ct[k];
-                    const float hv = fmaf(fmaf(fmaf(c4.w, tt, c4.z), tt, c4.y), tt, c4.x);
-                    sq = fmaf(g * pj[q].w, (hv * ri) * ri2, sq);   // C(r) / r^2, as in the force pass
+                    const float hv = ctab_eval(s_ct, (rsq * ri) * a.ct_scale);
+                    sq = fmaf(g * pj[q].w, hv * (ri2 * ri), sq);   // C(r) / r^2, as in the force pass
                 }
             }
         }

@@ -83,6 +83,16 @@ __device__ __forceinline__ uint32_t lpos_u(uint32_t e, int gs) {
     return (e & ~(4u * G - 1u)) | ((e & (G - 1u)) << 2) | __builtin_amdgcn_ubfe(e, (uint32_t)gs, 2u);
 }
 
+// h at table coordinate x = r * MD_CTAB / rc >= 0 from the cubic Coulomb table in
+// LDS (MD_CTAB intervals and an end entry h(rc); beyond rc the value is finite
+// and unused).  x - (int)x is exact for x >= 0, so fract gives the offset.
+__device__ __forceinline__ float ctab_eval(const float4* s_ct, float x) {
+    const int k = min((int)x, MD_CTAB);
+    const float t = __builtin_amdgcn_fractf(x);
+    const float4 c4 = s_ct[k];
+    return fmaf(fmaf(fmaf(c4.w, t, c4.z), t, c4.y), t, c4.x);
+}
+
 // current position buffer of a replica
 __host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step + s.phase) & 1); }
 
@@ -134,7 +144,7 @@ struct StepArgs {
     int* err;             // [1] non-finite coordinates
     StatRec* stats;       // [stat_cap][R] or null
     const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)
-    const float4* ctab;   // [MD_CTAB] cubic coefficients of h(r) = A(r) - r A(rc)/rc (Wolf / damped)
+    const float4* ctab;   // [MD_CTAB + 1] cubic coefficients of h(r) = A(r) - r A(rc)/rc (Wolf / damped)
     const int* slot_of;   // [N] original index -> slot
     const uint32_t* bond_list;  // [N*4] (partner | type << 29)
     const int* nbond;           // [N]

@@ -227,20 +227,19 @@ template <int COUL, bool EN>
 __device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, float qi, float qj, float rsq,
                                          const float4 lj, float& ulj, float& ucoul) {
     const float ri = __builtin_amdgcn_rsqf(rsq);
-    float ff = 0.0f;
+    const float ri2 = ri * ri, ri3 = ri2 * ri;
+    const float ri6 = ri3 * ri3, ri8 = ri6 * ri2;
+    const float flj = fmaf(lj.x, ri6, -lj.y) * ri8;   // lj1 r^-14 - lj2 r^-8, src/pair_forces.cu:190-193
+    float ff = flj;
     if (COUL == 0) {           // reaction field, src/pair_forces.cu:88-91
         const float qq = qi * qj;
-        ff = qq * (ri - a.reacf * rsq);
+        ff = fmaf(qq * (ri - a.reacf * rsq), ri2, flj);
         if (EN) ucoul = qq * (ri + 0.5f * a.reacf * rsq - a.ecut);
     } else if (COUL == 1 || COUL == 2) {   // Wolf / damped, src/pair_forces.cu:174-186
         const float qq = qi * qj;
         const float r = rsq * ri;
-        const float x = r * a.ct_scale;
-        const int k = min((int)x, MD_CTAB - 1);
-        const float t = x - (float)k;
-        const float4 c4 = s_ct[k];
-        const float hv = fmaf(fmaf(fmaf(c4.w, t, c4.z), t, c4.y), t, c4.x);   // A(r) - r A(rc)/rc
-        ff = (hv * ri) * qq;                                                   // (A(r)/r - A(rc)/rc) qq
+        const float hv = ctab_eval(s_ct, r * a.ct_scale);   // A(r) - r A(rc)/rc
+        ff = fmaf(hv * qq, ri3, flj);                      // (A(r)/r - A(rc)/rc) qq / r^2
         if (EN) {
             const float tt = __builtin_amdgcn_rcpf(fmaf(a.palfa, r, 1.0f));
             const float poly = ((((1.061405429f * tt - 1.453152027f) * tt + 1.421413741f) * tt - 0.284496736f) * tt +
@@ -251,14 +250,7 @@ __device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, 
             ucoul = qq * u;
         }
     }
-    const float ri2 = ri * ri;
-    ff *= ri2;
-    const float ri8 = (ri2 * ri2) * (ri2 * ri2);
-    ff = fmaf(fmaf(lj.x, ri8 * rsq, -lj.y), ri8, ff);   // src/pair_forces.cu:190-193
-    if (EN) {
-        const float ri6 = ri2 * ri2 * ri2;
-        ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
-    }
+    if (EN) ulj = fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z;
     return ff;
 }
 
@@ -269,7 +261,47 @@ __device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, 
 // logical entries 4Gb + sub + G u (u = 0..3).  So the G lanes of an atom read
 // G*16 contiguous bytes of the row, every gather instruction covers G
 // consecutive entries, and each lane has four independent gathers in flight.
-// Entries past the row length are masked.
+// Full blocks run without per-entry tests; the last, partial block masks the
+// entries past the row length (they read the atom itself and are dropped).
+template <int G, int COUL, bool EN, bool TAIL>
+__device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
+                                          const float4* s_ct, const uint4 e4, int base, int nn, int self,
+                                          const float4 xi, float qi, float L0, float L1, float L2, float Li0, float Li1,
+                                          float Li2, float& fx, float& fy, float& fz, double& elj, double& eco,
+                                          double& evi) {
+    const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
+    float4 pj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        // 32-bit byte offset from the position array (scalar base + vector offset)
+        const uint32_t off = ((!TAIL || base + G * u < nn) ? (ev[u] & MD_JMASK) : (uint32_t)self) << 4;
+        pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + off);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        // type_j * 16 bytes: slots stay below 2^(MD_JBITS - 4), so the bits of j above
+        // the shift are zero (enforced when the system is set)
+        const float4 lj = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_lj) + (ev[u] >> (MD_JBITS - 4)));
+        const float dx = minimg(xi.x - pj[u].x, L0, Li0);
+        const float dy = minimg(xi.y - pj[u].y, L1, Li1);
+        const float dz = minimg(xi.z - pj[u].z, L2, Li2);
+        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        float ulj = 0.0f, ucoul = 0.0f;
+        float ff = pair_ff<COUL, EN>(a, s_ct, qi, pj[u].w, rsq, lj, ulj, ucoul);
+        // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); masked entries of the last block
+        const bool use = (!TAIL || base + G * u < nn) && !(rsq > a.rc2);
+        ff = use ? ff : 0.0f;
+        fx = fmaf(ff, dx, fx);
+        fy = fmaf(ff, dy, fy);
+        fz = fmaf(ff, dz, fz);
+        if (EN && use) {
+            elj += (double)ulj;
+            eco += (double)ucoul;
+            evi += (double)(ff * rsq);
+        }
+    }
+}
+
 template <int G, int COUL, bool EN>
 __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
                                          const float4* s_ct,
@@ -277,44 +309,26 @@ __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __rest
                                          float qi, float L0, float L1, float L2, float Li0, float Li1, float Li2,
                                          float& fx, float& fy, float& fz, double& elj, double& eco, double& evi) {
     const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(row);
-    const int nb = (nn + 4 * G - 1) / (4 * G);       // blocks
+    const uint32_t un = (uint32_t)nn;
+    const int nb = (int)((un + 4 * G - 1) / (4 * G));   // blocks
+    const int nfull = (int)(un / (4 * G));              // blocks without a masked entry
     uint4 nxt = nb > 0 ? row4[sub] : make_uint4(0, 0, 0, 0);
-    for (int b = 0; b < nb; b++) {
+    auto full = [&](int b) {
         const uint4 e4 = nxt;
-        if (b + 1 < nb) nxt = row4[(b + 1) * G + sub];   // next quad in flight while this one computes
-        const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
-        const int base = 4 * G * b + sub;                // logical entry of element u: base + G u
-        float4 pj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            // 32-bit byte offset from the position array (scalar base + vector offset)
-            const uint32_t off = ((base + G * u < nn) ? (ev[u] & MD_JMASK) : (uint32_t)self) << 4;
-            pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + off);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            // type_j * 16 bytes: slots stay below 2^(MD_JBITS - 4), so the bits of j above
-            // the shift are zero (enforced when the system is set)
-            const float4 lj = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_lj) + (ev[u] >> (MD_JBITS - 4)));
-            const float dx = minimg(xi.x - pj[u].x, L0, Li0);
-            const float dy = minimg(xi.y - pj[u].y, L1, Li1);
-            const float dz = minimg(xi.z - pj[u].z, L2, Li2);
-            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            float ulj = 0.0f, ucoul = 0.0f;
-            float ff = pair_ff<COUL, EN>(a, s_ct, qi, pj[u].w, rsq, lj, ulj, ucoul);
-            // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); masked entries of the last block
-            const bool use = (base + G * u < nn) && !(rsq > a.rc2);
-            ff = use ? ff : 0.0f;
-            fx = fmaf(ff, dx, fx);
-            fy = fmaf(ff, dy, fy);
-            fz = fmaf(ff, dz, fz);
-            if (EN && use) {
-                elj += (double)ulj;
-                eco += (double)ucoul;
-                evi += (double)(ff * rsq);
-            }
-        }
+        nxt = row4[min(b + 1, nb - 1) * G + sub];   // next quad in flight while this one computes
+        pair_quad<G, COUL, EN, false>(a, pin, s_lj, s_ct, e4, 4 * G * b + sub, nn, self, xi, qi, L0, L1, L2, Li0, Li1,
+                                      Li2, fx, fy, fz, elj, eco, evi);
+    };
+    int b = 0;
+    if (EN) {   // energy steps: more live registers, no unrolling
+        for (; b < nfull; b++) full(b);
+    } else {    // two blocks per iteration: the prefetched quad needs no register copy
+#pragma unroll 2
+        for (; b < nfull; b++) full(b);
     }
+    if (b < nb)
+        pair_quad<G, COUL, EN, true>(a, pin, s_lj, s_ct, nxt, 4 * G * b + sub, nn, self, xi, qi, L0, L1, L2, Li0, Li1,
+                                     Li2, fx, fy, fz, elj, eco, evi);
 }
 
 }  // namespace
@@ -323,7 +337,7 @@ __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __rest
 template <int G, int COUL, int ENMODE>
 __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_lj[64];
-    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB : 1];
+    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
     static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     __shared__ double s_red[4][MD_NSTAT];
     if (a.mode == MODE_LF && a.gate->n_run == 0) return;     // nothing can step: idle launch
@@ -333,7 +347,10 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
     if (tid < 64) s_lj[tid] = a.ljtab[tid];
-    if (COUL == 1 || COUL == 2) s_ct[tid] = a.ctab[tid];   // 256 threads, MD_CTAB entries
+    if (COUL == 1 || COUL == 2) {   // 256 threads, MD_CTAB + 1 entries
+        s_ct[tid] = a.ctab[tid];
+        if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
+    }
     const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
     const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
     // One replica: its state and, with a deferred finalize, the previous launch's
