@@ -261,22 +261,24 @@ __device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, 
 // logical entries 4Gb + sub + G u (u = 0..3).  So the G lanes of an atom read
 // G*16 contiguous bytes of the row, every gather instruction covers G
 // consecutive entries, and each lane has four independent gathers in flight.
-// Full blocks run without per-entry tests; the last, partial block masks the
-// entries past the row length (they read the atom itself and are dropped).
-template <int G, int COUL, bool EN, bool TAIL>
-__device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
-                                          const float4* s_ct, const uint4 e4, int base, int nn, int self,
-                                          const float4 xi, float qi, float L0, float L1, float L2, float Li0, float Li1,
-                                          float Li2, float& fx, float& fy, float& fz, double& elj, double& eco,
-                                          double& evi) {
+// The gathers of block b + 1 are issued before block b computes (the row quads
+// two blocks ahead), so their latency hides behind a block of arithmetic.
+// Every list entry holds a slot index (the list is zeroed when allocated and
+// builds write only slot indices), so the entries past the row length in the
+// last block gather some atom in bounds; their pairs are dropped.
+__device__ __forceinline__ void gather4(const float4* __restrict__ pin, const uint4 e4, float4* pj) {
     const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
-    float4 pj[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        // 32-bit byte offset from the position array (scalar base + vector offset)
-        const uint32_t off = ((!TAIL || base + G * u < nn) ? (ev[u] & MD_JMASK) : (uint32_t)self) << 4;
-        pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + off);
-    }
+    for (int u = 0; u < 4; u++)   // 32-bit byte offset from the position array (scalar base + vector offset)
+        pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + ((ev[u] & MD_JMASK) << 4));
+}
+
+template <int G, int COUL, bool EN, bool TAIL>
+__device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* s_lj, const float4* s_ct, const uint4 e4,
+                                          const float4* pj, int base, int nn, const float4 xi, float qi, float L0,
+                                          float L1, float L2, float Li0, float Li1, float Li2, float& fx, float& fy,
+                                          float& fz, double& elj, double& eco, double& evi) {
+    const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
 #pragma unroll
     for (int u = 0; u < 4; u++) {
         // type_j * 16 bytes: slots stay below 2^(MD_JBITS - 4), so the bits of j above
@@ -288,7 +290,7 @@ __device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* __res
         const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         float ulj = 0.0f, ucoul = 0.0f;
         float ff = pair_ff<COUL, EN>(a, s_ct, qi, pj[u].w, rsq, lj, ulj, ucoul);
-        // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); masked entries of the last block
+        // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); entries past the row length
         const bool use = (!TAIL || base + G * u < nn) && !(rsq > a.rc2);
         ff = use ? ff : 0.0f;
         fx = fmaf(ff, dx, fx);
@@ -305,30 +307,36 @@ __device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* __res
 template <int G, int COUL, bool EN>
 __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
                                          const float4* s_ct,
-                                         const uint32_t* __restrict__ row, int nn, int sub, int self, const float4 xi,
+                                         const uint32_t* __restrict__ row, int nn, int sub, const float4 xi,
                                          float qi, float L0, float L1, float L2, float Li0, float Li1, float Li2,
                                          float& fx, float& fy, float& fz, double& elj, double& eco, double& evi) {
     const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(row);
-    const uint32_t un = (uint32_t)nn;
-    const int nb = (int)((un + 4 * G - 1) / (4 * G));   // blocks
-    const int nfull = (int)(un / (4 * G));              // blocks without a masked entry
-    uint4 nxt = nb > 0 ? row4[sub] : make_uint4(0, 0, 0, 0);
-    auto full = [&](int b) {
-        const uint4 e4 = nxt;
-        nxt = row4[min(b + 1, nb - 1) * G + sub];   // next quad in flight while this one computes
-        pair_quad<G, COUL, EN, false>(a, pin, s_lj, s_ct, e4, 4 * G * b + sub, nn, self, xi, qi, L0, L1, L2, Li0, Li1,
-                                      Li2, fx, fy, fz, elj, eco, evi);
+    const int nb = (int)(((uint32_t)nn + 4 * G - 1) / (4 * G));   // blocks; all but the last are full
+    if (nb == 0) return;
+    uint4 e0 = row4[sub];
+    float4 p0[4];
+    gather4(pin, e0, p0);
+    uint4 e1 = row4[min(1, nb - 1) * G + sub];
+    auto block = [&](int b) {
+        float4 p1[4];
+        gather4(pin, e1, p1);                               // block b + 1
+        const uint4 e2 = row4[min(b + 2, nb - 1) * G + sub];   // row quad of block b + 2
+        pair_quad<G, COUL, EN, false>(a, s_lj, s_ct, e0, p0, 0, nn, xi, qi, L0, L1, L2, Li0, Li1, Li2, fx, fy, fz,
+                                      elj, eco, evi);
+        e0 = e1;
+        e1 = e2;
+#pragma unroll
+        for (int u = 0; u < 4; u++) p0[u] = p1[u];
     };
     int b = 0;
     if (EN) {   // energy steps: more live registers, no unrolling
-        for (; b < nfull; b++) full(b);
-    } else {    // two blocks per iteration: the prefetched quad needs no register copy
+        for (; b < nb - 1; b++) block(b);
+    } else {    // two blocks per iteration: no register copies of the pipelined values
 #pragma unroll 2
-        for (; b < nfull; b++) full(b);
+        for (; b < nb - 1; b++) block(b);
     }
-    if (b < nb)
-        pair_quad<G, COUL, EN, true>(a, pin, s_lj, s_ct, nxt, 4 * G * b + sub, nn, self, xi, qi, L0, L1, L2, Li0, Li1,
-                                     Li2, fx, fy, fz, elj, eco, evi);
+    pair_quad<G, COUL, EN, true>(a, s_lj, s_ct, e0, p0, 4 * G * (nb - 1) + sub, nn, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+                                 fx, fy, fz, elj, eco, evi);
 }
 
 }  // namespace
@@ -472,10 +480,10 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         double elj = 0.0, eco = 0.0, evi = 0.0;
         const uint32_t* row = a.nlist + (size_t)slot * a.cap;
         if (ENMODE != 0 && en)
-            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, slot, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                     fx, fy, fz, elj, eco, evi);
         else
-            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, slot, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                      fx, fy, fz, elj, eco, evi);
         // bonded terms (lanes of the atom's group share them)
         double ebo = 0.0, ean = 0.0, bvi = 0.0;
