@@ -133,6 +133,7 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
     const int per = (nc + 1023) / 1024;
     const int b = min(nc, (int)threadIdx.x * per), e = min(nc, b + per);
     int sum = 0;
+#pragma unroll 8
     for (int k = b; k < e; k++) sum += a.cell_count[p.cell_off + k];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = sum;
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
     }
     __syncthreads();
     int run = p.slot_off + wsum[w] + inc - sum;
+#pragma unroll 8
     for (int k = b; k < e; k++) {
         int c = a.cell_count[p.cell_off + k];
         a.cell_start[p.cs_off + k] = run;
@@ -185,6 +187,7 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     int gc = a.key[o];
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
+#pragma unroll 4
     for (int u = cs; u < ce; u++) cnt += (a.tmpid[u] < myid);
     int f = cs + cnt;
     a.pos_s[f] = a.pos[cur_buf(st)][o];
@@ -892,38 +895,49 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
 }
 
 // Last kernel of the rebuild chain: for every rebuilt replica (one wavefront
-// each) the longest row over its tiles goes into maxcount (max over the run's
-// rebuilds; the host checks it against the capacity), and the new list epoch
-// opens.  The gate counters change once per workgroup.  No early exit: every
-// wavefront reaches the barrier.
-__global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a) {
+// each, or with wide = 1 a whole workgroup for the few-replica case) the longest
+// row over its tiles goes into maxcount (max over the run's rebuilds; the host
+// checks it against the capacity), and the new list epoch opens.  The gate
+// counters change once per workgroup.  No early exit: every thread reaches the
+// barriers.
+__global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a, int wide) {
     __shared__ int s_d[4][3];
+    __shared__ int s_m[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + wave;
-    int done = 0, to_run = 0, to_force = 0;
+    const int r = wide ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
+    int done = 0, to_run = 0, to_force = 0, m = 0, nb = 0;
     if (r < a.nrep) {
         const RepState st = a.st[r];         // independent loads, issued together
         const int slot_off = a.rep[r].slot_off, nslot = a.rep[r].nslot;
-        const int mc0 = a.maxcount[r];
+        m = a.maxcount[r];
         if (stale(st)) {
             const int t0 = slot_off / 64, nt = nslot / 64;
-            int m = mc0;
-            for (int t = lane; t < nt; t += 64) m = max(m, a.tilemax[t0 + t]);
+            const int k0 = wide ? (int)threadIdx.x : lane, kstep = wide ? 256 : 64;
+#pragma unroll 4
+            for (int t = k0; t < nt; t += kstep) m = max(m, a.tilemax[t0 + t]);
             for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-            if (lane == 0) {
-                a.maxcount[r] = m;
-                a.st[r].age = 0;
-                a.st[r].nb = st.nb + 1;
-            }
             done = 1;
             to_run = (a.lfgate || st.phase == 0) && st.step < st.target;
             to_force = !a.lfgate && st.phase == 1;
+            nb = st.nb;
         }
     }
+    if (wide) {
+        if (lane == 0) s_m[wave] = m;
+        __syncthreads();
+        m = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+    }
+    const bool owner = wide ? threadIdx.x == 0 : lane == 0;
+    if (owner && done) {
+        a.maxcount[r] = m;
+        a.st[r].age = 0;
+        a.st[r].nb = nb + 1;
+    }
     if (lane == 0) {
-        s_d[wave][0] = done;
-        s_d[wave][1] = to_run;
-        s_d[wave][2] = to_force;
+        const bool count = !wide || wave == 0;
+        s_d[wave][0] = count ? done : 0;
+        s_d[wave][1] = count ? to_run : 0;
+        s_d[wave][2] = count ? to_force : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -975,7 +989,8 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
         default:  return hipErrorInvalidValue;
         }
     }
-    hipLaunchKernelGGL(k_nl_commit, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
+    const int wide = a.nrep <= 32 ? 1 : 0;   // few replicas: a workgroup each
+    hipLaunchKernelGGL(k_nl_commit, dim3(wide ? a.nrep : (a.nrep + 3) / 4), dim3(256), 0, st, a, wide);
     return hipGetLastError();
 }
 

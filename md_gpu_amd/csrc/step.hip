@@ -171,37 +171,40 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
 
 // Sums of a replica's partial slots of one step-kernel launch (grid workgroups;
 // slot u holds the run of the workgroup that started it: the replica's first
-// unit and the first unit of every non-empty workgroup range,
-// u0(b) = floor(b * nunits / grid)).  All 256 threads of the workgroup take part
-// and every thread gets the totals; the order of the additions is fixed, so
-// every caller gets the same bits.  ALL = false loads only KE and the trip flag.
+// unit ru0 and the first unit u0(b) = floor(b * nunits / grid) of every
+// workgroup b whose range starts inside the replica, so the slots are ru0 and
+// u0(b) for b in [ceil((ru0 + 1) grid / nunits), ceil(ru1 grid / nunits));
+// with grid >= nunits every unit starts a range).  All 256 threads of the
+// workgroup take part (slot k of the list goes to thread k mod 256) and every
+// thread gets the totals; the order of the additions is fixed, so every caller
+// gets the same bits.  ALL = false loads only KE and the trip flag.
 template <bool ALL>
 __device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, int grid, int nunits,
                                           double (*s_red)[MD_NSTAT], double* v) {
     const bool every = grid >= nunits;
-    const long long n = nunits;
+    const long long n = nunits, g = grid;
+    int cnt = ru1 - ru0, b_lo = 0;
+    if (!every) {
+        b_lo = (int)(((long long)(ru0 + 1) * g + n - 1) / n);
+        const int b_hi = (int)(((long long)ru1 * g + n - 1) / n);
+        cnt = 1 + max(0, b_hi - b_lo);
+    }
 #pragma unroll
     for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
-    for (int u = ru0 + (int)threadIdx.x; u < ru1; u += blockDim.x) {
-        bool start = every || u == ru0;
-        if (!start) {
-            const long long b = ((long long)u * grid + n - 1) / n;   // first workgroup with u0(b) >= u
-            start = b < grid && (b * n) / grid == u;
-        }
-        if (start) {
-            const double* q = part + (size_t)u * MD_NSTAT;
-            if (ALL) {
-                const double2* q2 = reinterpret_cast<const double2*>(q);
+    for (int k = (int)threadIdx.x; k < cnt; k += blockDim.x) {
+        const int u = (every || k == 0) ? ru0 + k : (int)(((long long)(b_lo + k - 1) * n) / g);
+        const double* q = part + (size_t)u * MD_NSTAT;
+        if (ALL) {
+            const double2* q2 = reinterpret_cast<const double2*>(q);
 #pragma unroll
-                for (int k = 0; k < MD_NSTAT / 2; k++) {
-                    const double2 t = q2[k];
-                    v[2 * k] += t.x;
-                    v[2 * k + 1] += t.y;
-                }
-            } else {
-                v[ACC_KE2] += q[ACC_KE2];
-                v[MD_NSTAT - 1] += q[MD_NSTAT - 1];
+            for (int j = 0; j < MD_NSTAT / 2; j++) {
+                const double2 t = q2[j];
+                v[2 * j] += t.x;
+                v[2 * j + 1] += t.y;
             }
+        } else {
+            v[ACC_KE2] += q[ACC_KE2];
+            v[MD_NSTAT - 1] += q[MD_NSTAT - 1];
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -749,34 +752,40 @@ __global__ void k_thermo_fix(ThState* th, const RepParam* rep, int nrep, int the
     }
 }
 
-// After k_vv_kick: one wavefront per replica ORs the tile flags of the skin
-// check, marks the list stale when one tripped and moves the replica to the
-// force phase.  Gate counters change once per workgroup; no early exit before
-// the barrier.
-__global__ __launch_bounds__(256) void k_kick_finalize(const KickArgs a) {
+// After k_vv_kick: per replica (one wavefront each, or with wide = 1 a whole
+// workgroup for the few-replica case) OR the tile flags of the skin check, mark
+// the list stale when one tripped and move the replica to the force phase.  Gate
+// counters change once per workgroup; no early exit before the barriers.
+__global__ __launch_bounds__(256) void k_kick_finalize(const KickArgs a, int wide) {
     __shared__ int s_d[4][3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = blockIdx.x * 4 + wave;
-    int moved = 0, tripped = 0;
+    const int r = wide ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
+    int moved = 0, tripped = 0, nbpar = 0;
     if (r < a.nrep) {
         const RepState st = a.st[r];
         if (st.phase == 0 && !stale(st) && st.step < st.target) {   // the replica kicked
             const RepParam& p = a.rep[r];
             const int t0 = p.slot_off / 64, nt = p.nslot / 64;
+            const int k0 = wide ? (int)threadIdx.x : lane, kstep = wide ? 256 : 64;
             int t = 0;
-            for (int k = lane; k < nt; k += 64) t |= a.tflag[t0 + k];
+#pragma unroll 4
+            for (int k = k0; k < nt; k += kstep) t |= a.tflag[t0 + k];
             tripped = __ballot(t != 0) ? 1 : 0;
-            if (lane == 0) {
-                if (tripped) a.st[r].trip[st.nb & 1] = 1;
-                a.st[r].phase = 1;
-            }
             moved = 1;
+            nbpar = st.nb & 1;
         }
     }
+    if (wide) tripped = __syncthreads_or(tripped);
+    const bool owner = wide ? threadIdx.x == 0 : lane == 0;
+    if (owner && moved) {
+        if (tripped) a.st[r].trip[nbpar] = 1;
+        a.st[r].phase = 1;
+    }
     if (lane == 0) {
-        s_d[wave][0] = moved;
-        s_d[wave][1] = tripped;
-        s_d[wave][2] = moved - tripped;
+        const bool count = !wide || wave == 0;
+        s_d[wave][0] = count ? moved : 0;
+        s_d[wave][1] = count ? tripped : 0;
+        s_d[wave][2] = count ? moved - tripped : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -853,7 +862,10 @@ extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int 
 
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_vv_kick, dim3((a.nslot + 255) / 256), dim3(256), 0, st, a);
-    if (a.apply_drift) hipLaunchKernelGGL(k_kick_finalize, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
+    if (a.apply_drift) {
+        const int wide = a.nrep <= 32 ? 1 : 0;   // few replicas: a workgroup each
+        hipLaunchKernelGGL(k_kick_finalize, dim3(wide ? a.nrep : (a.nrep + 3) / 4), dim3(256), 0, st, a, wide);
+    }
     return hipGetLastError();
 }
 
