@@ -40,8 +40,9 @@ __global__ __launch_bounds__(256) void k_fm_adjoint(const StepArgs a, double* pa
         if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
     }
     __syncthreads();
-    const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
-    const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);   // logical workgroup: contiguous per XCD
+    const int u0 = (int)(((long long)lb * a.nunits) / gridDim.x);
+    const int u1 = (int)(((long long)(lb + 1) * a.nunits) / gridDim.x);
     double acc[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; k++) acc[k] = 0.0;

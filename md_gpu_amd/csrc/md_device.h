@@ -93,6 +93,16 @@ __device__ __forceinline__ float ctab_eval(const float4* s_ct, float x) {
     return fmaf(fmaf(fmaf(c4.w, t, c4.z), t, c4.y), t, c4.x);
 }
 
+// Workgroups are dispatched round-robin over MI355X's 8 XCDs (workgroup b on
+// XCD b mod 8), each with its own L2.  xcd_block maps the launch index b to a
+// logical index so that every XCD takes one contiguous range of the n logical
+// workgroups (a bijection on [0, n)): neighboring atoms, and the positions their
+// rows gather, stay in one L2.
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    const int x = b & 7, k = b >> 3, q = n >> 3, rem = n & 7;
+    return x * q + min(x, rem) + k;
+}
+
 // current position buffer of a replica
 __host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step + s.phase) & 1); }
 

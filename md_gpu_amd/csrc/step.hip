@@ -362,8 +362,9 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         s_ct[tid] = a.ctab[tid];
         if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
     }
-    const int u0 = (int)(((long long)blockIdx.x * a.nunits) / gridDim.x);
-    const int u1 = (int)(((long long)(blockIdx.x + 1) * a.nunits) / gridDim.x);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);   // logical workgroup: contiguous per XCD
+    const int u0 = (int)(((long long)lb * a.nunits) / gridDim.x);
+    const int u1 = (int)(((long long)(lb + 1) * a.nunits) / gridDim.x);
     // One replica: its state and, with a deferred finalize, the previous launch's
     // partial sums are loaded together with the LDS tables (one memory round trip).
     const bool one = a.nrep == 1 && u0 < u1;
