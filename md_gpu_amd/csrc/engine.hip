@@ -27,7 +27,7 @@
 #include "md_device.h"
 #include "mdgpu/md_api.h"
 
-extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st);
+extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStream_t st);
 extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st);
 extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st);
 extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st);
@@ -193,7 +193,7 @@ struct md_engine {
     DBuf<int> ids, typ, ids_s, typ_s, slot_of, key, rank, tmp, tmpid;
     DBuf<int> cell_count, cell_start, cell_rank;
     DBuf<uint32_t> nlist;
-    DBuf<int> ncount, maxcount;
+    DBuf<int> ncount;
     DBuf<int> tilemax, tflag;      // [S/64] per-tile row maximum (list build), skin-check flag (kick)
     DBuf<int4> excl, ang_list;
     DBuf<uint32_t> bond_list;
@@ -209,7 +209,7 @@ struct md_engine {
     std::vector<float4> fm_ljtab_h;    // its host staging copy (alive until the next evaluation)
     DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
     DBuf<int> err;
-    DBuf<unsigned long long> reb_count;
+    long long reb_base = 0;        // list builds of states uploaded before the current one
     DBuf<StatRec> stats;
     int stat_cap = 0;
     DBuf<float4> ljtab, ctab;
@@ -408,9 +408,9 @@ BuildArgs build_args(md_engine* e) {
     b.pos_build = e->pos_build.p;
     b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p; b.tmpid = e->tmpid.p;
     b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
-    b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p; b.maxcount = e->maxcount.p;
+    b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p;
     b.tilemax = e->tilemax.p;
-    b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p; b.rebuild_count = e->reb_count.p;
+    b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p;
     b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.build_lanes = e->build_lanes;
@@ -425,7 +425,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
     a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
-    a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0;
+    a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0; a.open = 0;
     a.vvfused = vv_fused(e) ? 1 : 0;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
@@ -476,11 +476,13 @@ struct TimedSpan {
     }
 };
 
-int enqueue_rebuild(md_engine* e, TimedSpan* ts) {
+// commit = false: the caller queues a deferred-finalize block whose first step
+// kernel opens the rebuilt lists
+int enqueue_rebuild(md_engine* e, TimedSpan* ts, bool commit = true) {
     BuildArgs b = build_args(e);
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
-    HIPCHECK(md_launch_rebuild(b, e->st));
+    HIPCHECK(md_launch_rebuild(b, commit ? 1 : 0, e->st));
     if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->rebs.push_back(ev); }
     return MD_OK;
 }
@@ -500,6 +502,7 @@ int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool fina
 int enqueue_lf_block(md_engine* e, int energy_every, int n) {
     StepArgs a = step_args(e, MODE_LF, energy_every);
     a.lagmode = 1;
+    a.open = 1;
     for (int k = 0; k < n; k++) {
         a.lag = k > 0 ? 1 : 0;
         a.par = k & 1;
@@ -538,17 +541,30 @@ int snapshot(md_engine* e, bool restore) {
     return MD_OK;
 }
 
+// The builds' longest rows per tile accumulate in tilemax until cleared.
+int clear_rowmax(md_engine* e) {
+    HIPCHECK(hipMemsetAsync(e->tilemax.p, 0, e->tilemax.bytes(), e->st));
+    return MD_OK;
+}
+// Queue the copy of the row maxima (read after the next stream synchronisation).
+int fetch_rowmax(md_engine* e, std::vector<int>& tm) {
+    tm.resize(e->tilemax.n);
+    HIPCHECK(hipMemcpyAsync(tm.data(), e->tilemax.p, e->tilemax.bytes(), hipMemcpyDeviceToHost, e->st));
+    return MD_OK;
+}
+int max_of(const std::vector<int>& v) { return v.empty() ? 0 : *std::max_element(v.begin(), v.end()); }
+
 // Rebuild all replicas now at the current positions (host synchronous),
 // growing the list capacity until every row fits.
 int full_rebuild(md_engine* e) {
     for (int attempt = 0; attempt < 6; attempt++) {
         HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
-        HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+        CHECK(clear_rowmax(e));
         CHECK(enqueue_rebuild(e, nullptr));
-        std::vector<int> mc(e->R);
-        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+        std::vector<int> tm;
+        CHECK(fetch_rowmax(e, tm));
         HIPCHECK(hipStreamSynchronize(e->st));
-        int m = *std::max_element(mc.begin(), mc.end());
+        const int m = max_of(tm);
         if (m <= e->cap) return MD_OK;
         CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
     }
@@ -557,12 +573,12 @@ int full_rebuild(md_engine* e) {
 
 // Queue a rebuild chain (it runs only for replicas whose list is stale).
 int refresh_lists(md_engine* e) {
-    HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+    CHECK(clear_rowmax(e));
     CHECK(enqueue_rebuild(e, nullptr));
-    std::vector<int> mc(e->R);
-    HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+    std::vector<int> tm;
+    CHECK(fetch_rowmax(e, tm));
     HIPCHECK(hipStreamSynchronize(e->st));
-    if (*std::max_element(mc.begin(), mc.end()) > e->cap) CHECK(full_rebuild(e));
+    if (max_of(tm) > e->cap) CHECK(full_rebuild(e));
     return MD_OK;
 }
 
@@ -602,7 +618,7 @@ int md_create(md_engine** out, int device) {
         delete e;
         return fail(MD_E_HIP, "hipStreamCreate failed");
     }
-    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) || e->reb_count.alloc(1)) {
+    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64)) {
         delete e;
         return MD_E_HIP;
     }
@@ -756,7 +772,6 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->snap_th.alloc(nrep));
     CHECK(e->part.alloc(2 * (size_t)(S / 8) * MD_NSTAT));   // units hold at least 8 slots (lanes <= 32)
     CHECK(e->lst.alloc(2 * (size_t)nrep));
-    CHECK(e->maxcount.alloc(nrep));
     CHECK(e->tilemax.alloc(S / MD_TILE));
     CHECK(e->tflag.alloc(S / MD_TILE));
     e->stat_cap = 0;
@@ -832,6 +847,11 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
     if (!pos) return fail(MD_E_INVALID, "null positions");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
+    if (e->has_state) {   // the builds of the state being replaced stay counted
+        std::vector<RepState> old;
+        CHECK(read_states(e, old));
+        for (const RepState& x : old) e->reb_base += x.nb;
+    }
     const int S = e->S;
     std::vector<float4> P(S, make_float4(0, 0, 0, 0)), V(S, make_float4(0, 0, 0, 0));
     std::vector<int> ids(S, -1), typ(S, 0), slot_of(e->N);
@@ -952,9 +972,13 @@ int enqueue_segment(md_engine* e, int energy_every) {
     const int ce = std::max(1, e->check_every);
     const int blocks = std::max(1, kSegment / ce);
     for (int b = 0; b < blocks; b++) {
-        CHECK(enqueue_rebuild(e, nullptr));
-        if (e->defer && lf_form(e)) CHECK(enqueue_lf_block(e, energy_every, ce));
-        else for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
+        if (e->defer && lf_form(e)) {   // the block's first step kernel opens the rebuilt lists
+            CHECK(enqueue_rebuild(e, nullptr, false));
+            CHECK(enqueue_lf_block(e, energy_every, ce));
+        } else {
+            CHECK(enqueue_rebuild(e, nullptr));
+            for (int k = 0; k < ce; k++) CHECK(enqueue_step(e, energy_every, nullptr));
+        }
     }
     return MD_OK;
 }
@@ -1017,7 +1041,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     int retries = 0;
     for (;;) {   // one attempt; repeated only after a list-capacity overflow
         CHECK(snapshot(e, false));
-        HIPCHECK(hipMemsetAsync(e->maxcount.p, 0, e->R * sizeof(int), e->st));
+        CHECK(clear_rowmax(e));
         HIPCHECK(md_launch_set_target(e->rst.p, e->gate.p, e->R, nsteps, e->st));
         if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
         if (fused) {
@@ -1064,12 +1088,12 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
             }
         }
         int errf = 0;
-        std::vector<int> mc(e->R);
-        HIPCHECK(hipMemcpyAsync(mc.data(), e->maxcount.p, e->R * sizeof(int), hipMemcpyDeviceToHost, e->st));
+        std::vector<int> tm;
+        CHECK(fetch_rowmax(e, tm));
         HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
         HIPCHECK(hipStreamSynchronize(e->st));
         if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
-        int m = *std::max_element(mc.begin(), mc.end());
+        const int m = max_of(tm);
         if (m > e->cap) {   // a list row overflowed: roll the run back with a larger capacity
             if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
             CHECK(snapshot(e, true));
@@ -1141,12 +1165,15 @@ int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q) {
 
 int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid) {
     if (!e) return fail(MD_E_INVALID, "null engine");
-    if (rebuilds) {
-        unsigned long long c = 0;
-        HIPCHECK(hipSetDevice(e->dev));
-        HIPCHECK(hipStreamSynchronize(e->st));
-        HIPCHECK(hipMemcpy(&c, e->reb_count.p, sizeof c, hipMemcpyDeviceToHost));
-        *rebuilds = (long long)c;
+    if (rebuilds) {   // every list build opens an epoch: the replicas' epoch counters add up the builds
+        long long c = e->reb_base;
+        if (e->has_state) {
+            std::vector<RepState> rs;
+            HIPCHECK(hipSetDevice(e->dev));
+            CHECK(read_states(e, rs));
+            for (const RepState& x : rs) c += x.nb;
+        }
+        *rebuilds = c;
     }
     if (steps) *steps = e->step;
     if (cap) *cap = e->cap;

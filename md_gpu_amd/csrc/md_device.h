@@ -149,6 +149,8 @@ struct StepArgs {
                           // step (second half-kick) and, unless the run ends, begins the next one (first
                           // half-kick and drift); between launches the replica is kicked (phase 1)
     int lagmode;          // leapfrog block with deferred finalize: the step kernel writes lst[par]
+    int open;             // the block's rebuild chain had no commit: the first step kernel (lag = 0)
+                          // opens the new list epoch of every stale (just rebuilt) replica
     int lag;              // the previous launch was a step kernel of the same block: finalize it first
     int par;              // buffer parity of this launch
     int* err;             // [1] non-finite coordinates
@@ -227,12 +229,10 @@ struct BuildArgs {
     int* slot_of;         // [N]
     uint32_t* nlist;
     int* ncount;
-    int* maxcount;        // [R] longest row of the replica's new list
-    int* tilemax;         // [S/64] longest row per 64-slot tile
+    int* tilemax;         // [S/64] longest row per 64-slot tile over the builds since the host cleared it
     const int4* excl;     // [N] 1-2 partners (-1 empty)
     RepState* st;
     Gate* gate;
-    unsigned long long* rebuild_count;  // replicas rebuilt (counter)
     const int* tile_rep;
     const RepParam* rep;
     int nslot;

@@ -23,7 +23,7 @@
 //                 (a lane per candidate, 16 or 8 atoms per wavefront, one ballot
 //                 per atom and chunk).  Hits are appended in candidate order, so
 //                 rows are deterministic.  Each tile stores its longest row.
-//   k_nl_commit   per rebuilt replica: longest row -> maxcount, new list epoch.
+//   k_nl_commit   per rebuilt replica: new list epoch (or the next step kernel opens it).
 // Bytes per atom per rebuild (algorithmic): read pos/vel/force/id/type 56 B,
 // write them twice (scratch + copy back) 112 B, plus the list 4*K B.
 #include "md_device.h"
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
     if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
         int m = s_wmax[0];
         for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = m;
+        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
     }
 }
 
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
     if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
         int m = s_wmax[0];
         for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = m;
+        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
     }
 }
 
@@ -890,63 +890,42 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
         int m = s_wmax[0];
         for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = m;
+        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
     }
 }
 
-// Last kernel of the rebuild chain: for every rebuilt replica (one wavefront
-// each, or with wide = 1 a whole workgroup for the few-replica case) the longest
-// row over its tiles goes into maxcount (max over the run's rebuilds; the host
-// checks it against the capacity), and the new list epoch opens.  The gate
-// counters change once per workgroup.  No early exit: every thread reaches the
-// barriers.
-__global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a, int wide) {
+// Last kernel of a rebuild chain that is not followed by a deferred-finalize
+// block: every rebuilt replica (one wavefront each) opens its new list epoch
+// (a deferred-finalize block's first step kernel opens it instead).  The gate
+// counters change once per workgroup.  No early exit: every wavefront reaches
+// the barrier.
+__global__ __launch_bounds__(256) void k_nl_commit(BuildArgs a) {
     __shared__ int s_d[4][3];
-    __shared__ int s_m[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = wide ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
-    int done = 0, to_run = 0, to_force = 0, m = 0, nb = 0;
+    const int r = blockIdx.x * 4 + wave;
+    int done = 0, to_run = 0, to_force = 0;
     if (r < a.nrep) {
-        const RepState st = a.st[r];         // independent loads, issued together
-        const int slot_off = a.rep[r].slot_off, nslot = a.rep[r].nslot;
-        m = a.maxcount[r];
+        const RepState st = a.st[r];
         if (stale(st)) {
-            const int t0 = slot_off / 64, nt = nslot / 64;
-            const int k0 = wide ? (int)threadIdx.x : lane, kstep = wide ? 256 : 64;
-#pragma unroll 4
-            for (int t = k0; t < nt; t += kstep) m = max(m, a.tilemax[t0 + t]);
-            for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+            if (lane == 0) {
+                a.st[r].age = 0;
+                a.st[r].nb = st.nb + 1;
+            }
             done = 1;
             to_run = (a.lfgate || st.phase == 0) && st.step < st.target;
             to_force = !a.lfgate && st.phase == 1;
-            nb = st.nb;
         }
     }
-    if (wide) {
-        if (lane == 0) s_m[wave] = m;
-        __syncthreads();
-        m = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
-    }
-    const bool owner = wide ? threadIdx.x == 0 : lane == 0;
-    if (owner && done) {
-        a.maxcount[r] = m;
-        a.st[r].age = 0;
-        a.st[r].nb = nb + 1;
-    }
     if (lane == 0) {
-        const bool count = !wide || wave == 0;
-        s_d[wave][0] = count ? done : 0;
-        s_d[wave][1] = count ? to_run : 0;
-        s_d[wave][2] = count ? to_force : 0;
+        s_d[wave][0] = done;
+        s_d[wave][1] = to_run;
+        s_d[wave][2] = to_force;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int n = 0, nr = 0, nf = 0;
         for (int w = 0; w < 4; w++) { n += s_d[w][0]; nr += s_d[w][1]; nf += s_d[w][2]; }
-        if (n) {
-            atomicSub(&a.gate->n_stale, n);
-            atomicAdd(a.rebuild_count, (unsigned long long)n);
-        }
+        if (n) atomicSub(&a.gate->n_stale, n);
         if (nr) atomicAdd(&a.gate->n_run, nr);
         if (nf) atomicAdd(&a.gate->n_force, nf);
     }
@@ -964,7 +943,8 @@ __global__ void k_force_trip(RepState* st, Gate* g, int nrep) {
 
 // ---------------------------------------------------------------- launchers
 
-extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
+// commit = 0: a deferred-finalize block follows and its first step kernel opens the new lists
+extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStream_t st) {
     const int nb = (a.nslot + 255) / 256;
     hipLaunchKernelGGL(k_nl_bin, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_scan, dim3(a.nrep), dim3(1024), 0, st, a);
@@ -989,8 +969,7 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, hipStream_t st) {
         default:  return hipErrorInvalidValue;
         }
     }
-    const int wide = a.nrep <= 32 ? 1 : 0;   // few replicas: a workgroup each
-    hipLaunchKernelGGL(k_nl_commit, dim3(wide ? a.nrep : (a.nrep + 3) / 4), dim3(256), 0, st, a, wide);
+    if (commit) hipLaunchKernelGGL(k_nl_commit, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

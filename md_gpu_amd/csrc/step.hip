@@ -351,7 +351,8 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
     static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     __shared__ double s_red[4][MD_NSTAT];
-    if (a.mode == MODE_LF && a.gate->n_run == 0) return;     // nothing can step: idle launch
+    // nothing can step (and no list was just rebuilt for this block to open): idle launch
+    if (a.mode == MODE_LF && a.gate->n_run == 0 && !(a.open && a.gate->n_stale != 0)) return;
     if (a.mode == MODE_VV && a.gate->n_force == 0) return;
     constexpr int APW = 64 / G;        // atoms per wavefront
     constexpr int APB = 4 * APW;       // atoms per unit (workgroup pass)
@@ -449,6 +450,10 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                 }
             } else {
                 s = one ? s_one : a.st[r];
+                if (a.open && stale(s)) {   // rebuilt by the block's chain: the new list epoch opens
+                    s.nb += 1;
+                    s.age = 0;
+                }
                 if (a.mode == MODE_LF) lamd = a.th[r].S;
             }
             if (a.mode == MODE_FORCE) run_r = !stale(s);
@@ -645,7 +650,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
         // No launch of the block passed the gate when none could step, and lst[par]
         // was not written: nothing to do.  (A replica that could step keeps
         // n_run >= 1 until its own workgroup here; one that could not is unchanged.)
-        if (threadIdx.x == 0 && a.gate->n_run != 0) {
+        if (threadIdx.x == 0 && (a.gate->n_run != 0 || (a.open && a.gate->n_stale != 0))) {
             RepState n = s;
             double lam = s.lam;
             if (s.ran) {
@@ -659,6 +664,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
             st.age = n.age;
             st.step = n.step;
             st.phase = n.phase;
+            st.nb = n.nb;
             st.trip[0] = n.trip[0];
             st.trip[1] = n.trip[1];
             lf_gates(a, s_old, n);
