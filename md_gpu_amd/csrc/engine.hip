@@ -30,7 +30,8 @@
 extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStream_t st);
 extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st);
 extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st);
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st);
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st,
+                                     hipEvent_t ev0, hipEvent_t ev1);
 extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int grid, hipStream_t st);
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
@@ -489,9 +490,9 @@ int enqueue_rebuild(md_engine* e, TimedSpan* ts, bool commit = true) {
 
 int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool finalize = true) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
-    if (ts) { ev = ts->take(); HIPCHECK(hipEventRecord(ev.first, e->st)); }
-    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st));
-    if (ts) { HIPCHECK(hipEventRecord(ev.second, e->st)); ts->steps.push_back(ev); }
+    if (ts) ev = ts->take();   // the launch records the kernel's own start and end
+    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st, ev.first, ev.second));
+    if (ts) ts->steps.push_back(ev);
     if (finalize) HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
     return MD_OK;
 }
@@ -1464,7 +1465,7 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     a.stats = nullptr;
     a.qq_scale = qscale * qscale;
     const int coul = e->coul_kernel;
-    HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st));
+    HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st, nullptr, nullptr));
     a.qq_scale = qscale;
     const size_t nslot = (size_t)e->grid_blocks() * (2 + 2 * ntk * ntk);
     if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));

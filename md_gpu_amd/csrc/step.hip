@@ -28,6 +28,8 @@
 //
 // Algorithmic work per ordered pair (FMA = 2, add/mul/rsqrt/rcp/exp2/rint = 1):
 // LJ + Wolf 65 flop, LJ + RF 45, LJ 36, Wolf only 60 (src/pair_forces.cu:151-200).
+#include <hip/hip_ext.h>
+
 #include "md_device.h"
 
 namespace {
@@ -833,29 +835,38 @@ __global__ void k_set_target(RepState* st, Gate* g, int nrep, long long nsteps) 
 
 // ---------------------------------------------------------------- launchers
 
+// With ev0/ev1 the launch records the kernel's own start and end in the two events
+// (hipExtLaunchKernelGGL), so their elapsed time is the kernel's execution time.
+template <typename K>
+static void launch_k(K kern, const StepArgs& a, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, a);
+}
+
 template <int G, int EM>
-static void launch_ge(const StepArgs& a, int coul, int grid, hipStream_t st) {
+static void launch_ge(const StepArgs& a, int coul, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     switch (coul) {
-    case 0: hipLaunchKernelGGL((k_step<G, 0, EM>), dim3(grid), dim3(256), 0, st, a); break;
-    case 1: hipLaunchKernelGGL((k_step<G, 1, EM>), dim3(grid), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_step<G, 2, EM>), dim3(grid), dim3(256), 0, st, a); break;
-    default: hipLaunchKernelGGL((k_step<G, 3, EM>), dim3(grid), dim3(256), 0, st, a); break;
+    case 0: launch_k(k_step<G, 0, EM>, a, grid, st, ev0, ev1); break;
+    case 1: launch_k(k_step<G, 1, EM>, a, grid, st, ev0, ev1); break;
+    case 2: launch_k(k_step<G, 2, EM>, a, grid, st, ev0, ev1); break;
+    default: launch_k(k_step<G, 3, EM>, a, grid, st, ev0, ev1); break;
     }
 }
 
 template <int G>
-static void launch_g(const StepArgs& a, int coul, int grid, hipStream_t st) {
+static void launch_g(const StepArgs& a, int coul, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const bool em = (a.mode == MODE_FORCE) ? a.force_energy != 0 : a.energy_every > 0;
-    if (em) launch_ge<G, 1>(a, coul, grid, st);
-    else launch_ge<G, 0>(a, coul, grid, st);
+    if (em) launch_ge<G, 1>(a, coul, grid, st, ev0, ev1);
+    else launch_ge<G, 0>(a, coul, grid, st, ev0, ev1);
 }
 
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st) {
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st,
+                                     hipEvent_t ev0, hipEvent_t ev1) {
     switch (lanes) {
-    case 4: launch_g<4>(a, coul, grid, st); break;
-    case 8: launch_g<8>(a, coul, grid, st); break;
-    case 16: launch_g<16>(a, coul, grid, st); break;
-    case 32: launch_g<32>(a, coul, grid, st); break;
+    case 4: launch_g<4>(a, coul, grid, st, ev0, ev1); break;
+    case 8: launch_g<8>(a, coul, grid, st, ev0, ev1); break;
+    case 16: launch_g<16>(a, coul, grid, st, ev0, ev1); break;
+    case 32: launch_g<32>(a, coul, grid, st, ev0, ev1); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
