@@ -154,6 +154,8 @@ typedef struct md_launch_config {
                           (temperature, thermostat, statistics) instead of a finalize kernel per step */
     int vv_fuse;       /* NVE velocity Verlet: 1 = one step kernel per step ends the step and begins the
                           next (leapfrog form, so defer applies); 0 = kick, force and finalize kernels */
+    int fm_fused;      /* force matching without bonded terms: 1 = one pass computes the force and the
+                          gradient sums; 0 = a force pass, then the adjoint pass */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);

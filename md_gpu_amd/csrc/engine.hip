@@ -38,6 +38,8 @@ extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int
                                            int mode, hipStream_t st);
 extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
                                            int grid, hipStream_t st);
+extern "C" hipError_t md_launch_fm_fused(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
+                                         int grid, hipStream_t st);
 
 namespace {
 
@@ -226,6 +228,7 @@ struct md_engine {
     int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 3;
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
+    int fm_fused = 1;              // force matching without bonded terms: one pass for loss and gradient
     double queue_ratio = 1.15;   // launched / completed steps, learned
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
@@ -1198,6 +1201,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (c->build_kernel < 0 || c->build_kernel > 3) return fail(MD_E_INVALID, "build_kernel must be 0..3");
     if (c->defer != 0 && c->defer != 1) return fail(MD_E_INVALID, "defer must be 0 or 1");
     if (c->vv_fuse != 0 && c->vv_fuse != 1) return fail(MD_E_INVALID, "vv_fuse must be 0 or 1");
+    if (c->fm_fused != 0 && c->fm_fused != 1) return fail(MD_E_INVALID, "fm_fused must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
@@ -1211,6 +1215,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     e->build_kernel = c->build_kernel;
     e->defer = c->defer;
     e->vv_fuse = c->vv_fuse;
+    e->fm_fused = c->fm_fused;
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
@@ -1236,6 +1241,7 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->build_kernel = e->build_kernel;
     c->defer = e->defer;
     c->vv_fuse = e->vv_fuse;
+    c->fm_fused = e->fm_fused;
     return MD_OK;
 }
 
@@ -1463,13 +1469,18 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     StepArgs a = step_args(e, MODE_FORCE, 0);
     a.ljtab = e->fm_ljtab.p;
     a.stats = nullptr;
-    a.qq_scale = qscale * qscale;
     const int coul = e->coul_kernel;
-    HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st, nullptr, nullptr));
-    a.qq_scale = qscale;
     const size_t nslot = (size_t)e->grid_blocks() * (2 + 2 * ntk * ntk);
     if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
-    HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+    if (e->fm_fused && !e->bonded) {   // one pass: force and gradient sums together
+        a.qq_scale = qscale;
+        HIPCHECK(md_launch_fm_fused(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+    } else {                            // force pass (bonded terms included), then the adjoint walk
+        a.qq_scale = qscale * qscale;
+        HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st, nullptr, nullptr));
+        a.qq_scale = qscale;
+        HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+    }
     std::vector<double> g(2 + 2 * ntk * ntk);
     HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));

@@ -27,7 +27,8 @@ class LaunchConfig(C.Structure):
     """md_launch_config (md_api.h)."""
     _fields_ = [("lanes", C.c_int), ("grid", C.c_int), ("cell_div", C.c_int), ("check_every", C.c_int),
                 ("build_lanes", C.c_int), ("submit", C.c_int), ("refresh", C.c_int),
-                ("build_kernel", C.c_int), ("defer", C.c_int), ("vv_fuse", C.c_int)]
+                ("build_kernel", C.c_int), ("defer", C.c_int), ("vv_fuse", C.c_int),
+                ("fm_fused", C.c_int)]
 
     @classmethod
     def names(cls):

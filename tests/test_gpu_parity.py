@@ -208,7 +208,11 @@ def test_batched_replicas_run(E, orc):
     e.close()
 
 
-def test_force_matching_gradient(E, orc):
+@pytest.mark.parametrize("fused", [1, 0])
+def test_force_matching_gradient(E, orc, fused):
+    """Loss and gradient of the force-matching loss against the FP64 oracle,
+    for the single pass (force and gradient sums together) and for the force
+    pass followed by the adjoint walk."""
     base = S.nacl(nsites=16)
     rng = np.random.default_rng(3)
     nf = 4
@@ -223,8 +227,9 @@ def test_force_matching_gradient(E, orc):
         Ft, _, _ = orc.forces(dataclasses.replace(s, ff=tgt_ff))
         targets.append(Ft)
     targets = np.stack(targets)
-    say("fm", nf, base.n)
+    say("fm", nf, base.n, "fused", fused)
     e = E.Engine.from_system(base)
+    e.set_launch(fm_fused=fused)
     e.fm_set_frames(frames, targets)
     loss, g1, g2, gq = e.fm_loss_grad(base.ff.lj1, base.ff.lj2, 1.0)
     lo, o1, o2, oq = orc.fm_loss_grad(frames, base.type, base.charge, base.box, base.ff, 1.0, targets)
@@ -234,6 +239,35 @@ def test_force_matching_gradient(E, orc):
     assert np.allclose(g1, o1, rtol=1e-4, atol=1e-4 * np.abs(o1).max())
     assert np.allclose(g2, o2, rtol=1e-4, atol=1e-4 * np.abs(o2).max())
     e.close()
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16])
+def test_force_matching_single_pass_matches_two_pass(E, lanes):
+    """The single-pass loss and gradient equal the force pass + adjoint walk to
+    FP32 summation-order differences, at the base and at perturbed parameters."""
+    base = S.nacl(nsites=16)
+    rng = np.random.default_rng(5)
+    nf = 3
+    frames = np.stack([S.wrap_centered(base.pos + rng.normal(0, 0.25, base.pos.shape), base.box)
+                       for _ in range(nf)]).astype(np.float32)
+    targets = rng.normal(0, 50.0, (nf, base.n, 3)).astype(np.float32)
+    tgt_ff, tgt_q = S.perturb_nacl_ff(base.ff, seed=11)
+    say("fm single vs two pass", lanes)
+    out = {}
+    for fused in (1, 0):
+        e = E.Engine.from_system(base)
+        e.set_launch(fm_fused=fused, lanes=lanes)
+        e.fm_set_frames(frames, targets)
+        out[fused] = [e.fm_loss_grad(base.ff.lj1, base.ff.lj2, 1.0), e.fm_loss_grad(tgt_ff.lj1, tgt_ff.lj2, tgt_q)]
+        e.close()
+    for a, b in zip(out[1], out[0]):
+        la, g1a, g2a, gqa = a
+        lb, g1b, g2b, gqb = b
+        say("  loss", la, lb, "gq", gqa, gqb)
+        assert abs(la - lb) <= 1e-6 * lb
+        assert abs(gqa - gqb) <= 1e-5 * abs(gqb)
+        assert np.allclose(g1a, g1b, rtol=1e-5, atol=1e-6 * np.abs(g1b).max())
+        assert np.allclose(g2a, g2b, rtol=1e-5, atol=1e-6 * np.abs(g2b).max())
 
 
 def test_logged_reference_forces_nacl_rf(E, orc):
