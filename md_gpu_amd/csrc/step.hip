@@ -312,13 +312,14 @@ __device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* s_lj,
 template <int G, int COUL, bool EN>
 __device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
                                          const float4* s_ct,
-                                         const uint32_t* __restrict__ row, int nn, int sub, const float4 xi,
-                                         float qi, float L0, float L1, float L2, float Li0, float Li1, float Li2,
-                                         float& fx, float& fy, float& fz, double& elj, double& eco, double& evi) {
+                                         const uint32_t* __restrict__ row, const uint4 q0, int nn, int sub,
+                                         const float4 xi, float qi, float L0, float L1, float L2, float Li0,
+                                         float Li1, float Li2, float& fx, float& fy, float& fz, double& elj,
+                                         double& eco, double& evi) {
     const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(row);
     const int nb = (int)(((uint32_t)nn + 4 * G - 1) / (4 * G));   // blocks; all but the last are full
     if (nb == 0) return;
-    uint4 e0 = row4[sub];
+    uint4 e0 = q0;   // the row's first quad, loaded by the caller
     float4 p0[4];
     gather4(pin, e0, p0);
     uint4 e1 = row4[min(1, nb - 1) * G + sub];
@@ -353,33 +354,46 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
     static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
     __shared__ double s_red[4][MD_NSTAT];
-    // nothing can step (and no list was just rebuilt for this block to open): idle launch
-    if (a.mode == MODE_LF && a.gate->n_run == 0 && !(a.open && a.gate->n_stale != 0)) return;
-    if (a.mode == MODE_VV && a.gate->n_force == 0) return;
     constexpr int APW = 64 / G;        // atoms per wavefront
     constexpr int APB = 4 * APW;       // atoms per unit (workgroup pass)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane & (G - 1), grp = lane / G;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);   // logical workgroup: contiguous per XCD
+    const int u0 = (int)(((long long)lb * a.nunits) / gridDim.x);
+    const int u1 = (int)(((long long)(lb + 1) * a.nunits) / gridDim.x);
+    // One memory round trip before any work: the gate counters, the LDS tables,
+    // the first unit's per-atom data (none of it depends on the replica state; both
+    // position buffers are read) and, with one replica, its state.
+    const Gate gate = *a.gate;
     if (tid < 64) s_lj[tid] = a.ljtab[tid];
     if (COUL == 1 || COUL == 2) {   // 256 threads, MD_CTAB + 1 entries
         s_ct[tid] = a.ctab[tid];
         if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
     }
-    const int lb = xcd_block(blockIdx.x, gridDim.x);   // logical workgroup: contiguous per XCD
-    const int u0 = (int)(((long long)lb * a.nunits) / gridDim.x);
-    const int u1 = (int)(((long long)(lb + 1) * a.nunits) / gridDim.x);
-    // One replica: its state and, with a deferred finalize, the previous launch's
-    // partial sums are loaded together with the LDS tables (one memory round trip).
+    const int fslot = u0 * APB + wave * APW + grp;
+    int f_id = -1, f_typ = 0, f_nc = 0;
+    float4 f_x0 = make_float4(0.f, 0.f, 0.f, 0.f), f_x1 = f_x0;
+    uint4 f_q = make_uint4(0, 0, 0, 0);
+    if (ENMODE == 0 && u0 < u1) {   // (the energy variants have no registers to spare)
+        f_id = a.ids[fslot];
+        f_typ = a.typ[fslot];
+        f_nc = a.ncount[fslot];
+        f_x0 = a.pos[0][fslot];
+        f_x1 = a.pos[1][fslot];
+        f_q = reinterpret_cast<const uint4*>(a.nlist + (size_t)fslot * a.cap)[sub];
+    }
     const bool one = a.nrep == 1 && u0 < u1;
     RepState s_one;
+    if (one) s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
+    // nothing can step (and no list was just rebuilt for this block to open): idle launch
+    if (a.mode == MODE_LF && gate.n_run == 0 && !(a.open && gate.n_stale != 0)) return;
+    if (a.mode == MODE_VV && gate.n_force == 0) return;
+    // With one replica and a deferred finalize, the previous launch's partial sums.
     double v_one[MD_NSTAT];
-    if (one) {
-        s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
-        if (a.lag) {
-            const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
-            if (u0 == 0 && a.stats) sum_slots<true>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
-            else sum_slots<false>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
-        }
+    if (one && a.lag) {
+        const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
+        if (u0 == 0 && a.stats) sum_slots<true>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+        else sum_slots<false>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
     }
     __syncthreads();
     int cur_r = -1, run_u0 = 0;
@@ -481,20 +495,30 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         }
         if (!run_r) continue;
         const int slot = slot0 + wave * APW + grp;
-        const int id = a.ids[slot];
+        const uint32_t* row = a.nlist + (size_t)slot * a.cap;
+        int id, tyi, nci;
+        float4 xi;
+        uint4 q0;
+        if (ENMODE == 0 && u == u0) {   // loaded in the prologue
+            id = f_id; tyi = f_typ; nci = f_nc;
+            xi = pin == a.pos[0] ? f_x0 : f_x1;
+            q0 = f_q;
+        } else {
+            id = a.ids[slot]; tyi = a.typ[slot]; nci = a.ncount[slot];
+            xi = pin[slot];
+            q0 = reinterpret_cast<const uint4*>(row)[sub];
+        }
         const bool active = id >= 0;
-        const float4 xi = pin[slot];
-        const int ti = active ? a.typ[slot] : 0;
-        const int nn = active ? min(a.ncount[slot], a.cap) : 0;
+        const int ti = active ? tyi : 0;
+        const int nn = active ? min(nci, a.cap) : 0;
         const float qi = xi.w * a.qfac * a.qq_scale;
         float fx = 0.f, fy = 0.f, fz = 0.f;
         double elj = 0.0, eco = 0.0, evi = 0.0;
-        const uint32_t* row = a.nlist + (size_t)slot * a.cap;
         if (ENMODE != 0 && en)
-            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, q0, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                     fx, fy, fz, elj, eco, evi);
         else
-            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
+            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, q0, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
                                      fx, fy, fz, elj, eco, evi);
         // bonded terms (lanes of the atom's group share them)
         double ebo = 0.0, ean = 0.0, bvi = 0.0;
