@@ -26,6 +26,8 @@
 #define MD_KB_REF 8.314e-3      // src/integrator.cu:155 (1000/8.314)
 #define MD_NSTAT 8              // accumulators per replica
 #define MD_CTAB 256             // intervals of the Coulomb force table on [0, rc]
+#define MD_SUBSHIFT 25          // list build: ranks and original indices stay below 2^25 (the slot limit);
+                                // the eighth of the cell an atom sits in is packed above them
 
 enum { ACC_KE2 = 0, ACC_LJ, ACC_COUL, ACC_BOND, ACC_ANGLE, ACC_VIR };
 enum { MODE_FORCE = 0, MODE_LF = 1, MODE_VV = 2 };

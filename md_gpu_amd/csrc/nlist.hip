@@ -119,8 +119,13 @@ __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
     int cz = cell_coord(x.z, p.halfL[2], p.cinv[2], p.nc[2]);
     int lin = cx + p.nc[0] * (cy + p.nc[1] * cz);
     int gc = p.cell_off + a.cell_rank[p.cell_off + lin];
+    // the atom's eighth of the cell, numbered along a path through adjacent eighths
+    const int sx = (x.x + p.halfL[0]) * p.cinv[0] - (float)cx >= 0.5f;
+    const int sy = (x.y + p.halfL[1]) * p.cinv[1] - (float)cy >= 0.5f;
+    const int sz = (x.z + p.halfL[2]) * p.cinv[2] - (float)cz >= 0.5f;
+    const int sub = (0x54672310 >> (4 * (sx | (sy << 1) | (sz << 2)))) & 7;
     a.key[s] = gc;
-    a.rank[s] = atomicAdd(&a.cell_count[gc], 1);
+    a.rank[s] = atomicAdd(&a.cell_count[gc], 1) | (sub << MD_SUBSHIFT);
 }
 
 __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
@@ -168,9 +173,10 @@ __global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
     int id = a.ids[s];
     if (id < 0) return;
     int gc = a.key[s];
-    int t = a.cell_start[gc + r] + a.rank[s];
+    const int rk = a.rank[s];
+    int t = a.cell_start[gc + r] + (rk & ((1 << MD_SUBSHIFT) - 1));
     a.tmp[t] = s;
-    a.tmpid[t] = id;
+    a.tmpid[t] = id | ((rk >> MD_SUBSHIFT) << MD_SUBSHIFT);   // order key: eighth of the cell, then index
 }
 
 __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
@@ -183,12 +189,13 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     const RepParam& p = a.rep[r];
     if (t >= p.slot_off + p.natom) return;
     int o = a.tmp[t];
-    int myid = a.tmpid[t];
+    const int mykey = a.tmpid[t];
+    const int myid = mykey & ((1 << MD_SUBSHIFT) - 1);
     int gc = a.key[o];
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
 #pragma unroll 4
-    for (int u = cs; u < ce; u++) cnt += (a.tmpid[u] < myid);
+    for (int u = cs; u < ce; u++) cnt += (a.tmpid[u] < mykey);
     int f = cs + cnt;
     a.pos_s[f] = a.pos[cur_buf(st)][o];
     a.vel_s[f] = a.vel[o];
