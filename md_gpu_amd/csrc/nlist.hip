@@ -12,7 +12,8 @@
 //                 arrival rank in the cell by atomic count
 //   k_nl_scan     exclusive scan of cell counts -> cell_start (one block per replica)
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
-//   k_nl_permute  deterministic order inside a cell (by original index);
+//   k_nl_permute  deterministic order inside a cell (by eighth of the cell, then
+//                 original index);
 //                 gathers pos/vel/force/id/type into scratch in the new order
 //   build         one workgroup per 64-slot tile; the image cells within r_list
 //                 of a bounding box of atoms are listed in LDS and their members
@@ -102,18 +103,23 @@ __device__ int block_scan(int v, int* s_w, int& total) {
 
 }  // namespace
 
+// The short rebuild kernels issue every load that does not depend on another
+// (the gate counter included) in one round trip: an idle launch costs about the
+// same, a working one saves round trips.
 __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
-    if (a.gate->n_stale == 0) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
-    int r = a.tile_rep[s >> 6];
+    const int n_stale = a.gate->n_stale;
+    const int r = a.tile_rep[s >> 6];
+    const int id = a.ids[s];
+    const float4 x0 = a.pos[0][s], x1 = a.pos[1][s];
+    if (n_stale == 0) return;
     const RepState st = a.st[r];
     if (!stale(st)) return;
     const RepParam& p = a.rep[r];
     if (s == p.slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
-    int id = a.ids[s];
     if (id < 0) return;
-    float4 x = a.pos[cur_buf(st)][s];
+    float4 x = cur_buf(st) ? x1 : x0;
     int cx = cell_coord(x.x, p.halfL[0], p.cinv[0], p.nc[0]);
     int cy = cell_coord(x.y, p.halfL[1], p.cinv[1], p.nc[1]);
     int cz = cell_coord(x.z, p.halfL[2], p.cinv[2], p.nc[2]);
@@ -165,43 +171,48 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
-    if (a.gate->n_stale == 0) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
-    int r = a.tile_rep[s >> 6];
-    if (!stale(a.st[r])) return;
-    int id = a.ids[s];
-    if (id < 0) return;
-    int gc = a.key[s];
+    const int n_stale = a.gate->n_stale;
+    const int r = a.tile_rep[s >> 6];
+    const int id = a.ids[s];
+    const int gc = a.key[s];
     const int rk = a.rank[s];
+    if (n_stale == 0) return;
+    if (!stale(a.st[r])) return;
+    if (id < 0) return;
     int t = a.cell_start[gc + r] + (rk & ((1 << MD_SUBSHIFT) - 1));
     a.tmp[t] = s;
     a.tmpid[t] = id | ((rk >> MD_SUBSHIFT) << MD_SUBSHIFT);   // order key: eighth of the cell, then index
 }
 
 __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
-    if (a.gate->n_stale == 0) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nslot) return;
-    int r = a.tile_rep[t >> 6];
-    const RepState st = a.st[r];
-    if (!stale(st)) return;
-    const RepParam& p = a.rep[r];
-    if (t >= p.slot_off + p.natom) return;
-    int o = a.tmp[t];
+    const int n_stale = a.gate->n_stale;
+    const int r = a.tile_rep[t >> 6];
+    const int o = a.tmp[t];
     const int mykey = a.tmpid[t];
+    if (n_stale == 0) return;
+    const RepState st = a.st[r];
+    const RepParam& p = a.rep[r];
+    if (!stale(st)) return;
+    if (t >= p.slot_off + p.natom) return;
+    // tmp holds a slot of this replica for every t below slot_off + natom
+    const int gc = a.key[o];
+    const float4 x = a.pos[cur_buf(st)][o], v = a.vel[o], f4 = a.force[o];
+    const int ty = a.typ[o];
     const int myid = mykey & ((1 << MD_SUBSHIFT) - 1);
-    int gc = a.key[o];
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
 #pragma unroll 4
     for (int u = cs; u < ce; u++) cnt += (a.tmpid[u] < mykey);
     int f = cs + cnt;
-    a.pos_s[f] = a.pos[cur_buf(st)][o];
-    a.vel_s[f] = a.vel[o];
-    a.force_s[f] = a.force[o];
+    a.pos_s[f] = x;
+    a.vel_s[f] = v;
+    a.force_s[f] = f4;
     a.ids_s[f] = myid;
-    a.typ_s[f] = a.typ[o];
+    a.typ_s[f] = ty;
     a.slot_of[myid] = f;
 }
 
