@@ -137,6 +137,13 @@ int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q);
  * list capacity per atom, current lanes-per-atom and grid of the step kernel. */
 int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid);
 
+/* Neighbor-list capacity (entries per atom), the reference's nmaxlist
+ * (.inp, src/neighborlist.cu:61).  The lists are rebuilt at the next call
+ * that needs them; a row that does not fit does not corrupt anything (the
+ * reference has no guard): the run that met it is rolled back to its start
+ * and replayed with a larger capacity. */
+int md_set_list_capacity(md_engine* e, int cap);
+
 /* Launch configuration (no counterpart in the reference, whose launch shape
  * is fixed by src/gpu_properties.cu:31-67).  Setting it rebuilds the lists
  * when the cell grid changes. */

@@ -1096,15 +1096,17 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
         CHECK(fetch_rowmax(e, tm));
         HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
         HIPCHECK(hipStreamSynchronize(e->st));
-        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
         const int m = max_of(tm);
         if (m > e->cap) {   // a list row overflowed: roll the run back with a larger capacity
+            // (checked first: an attempt with truncated rows may also have gone unstable)
             if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
             CHECK(snapshot(e, true));
+            HIPCHECK(hipMemsetAsync(e->err.p, 0, sizeof(int), e->st));
             CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
             CHECK(full_rebuild(e));
             continue;
         }
+        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
         for (auto& s : rs)
             if (s.step != target) return fail(MD_E_STATE, "replica stopped at step %lld of %lld", s.step, target);
         if (!e->timing) {   // learn how many launches a step costs (stalls until the next rebuild chain)
@@ -1183,6 +1185,17 @@ int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, i
     if (cap) *cap = e->cap;
     if (lanes) *lanes = e->lanes;
     if (grid) *grid = e->has_sys ? e->grid_blocks() : e->grid;
+    return MD_OK;
+}
+
+int md_set_list_capacity(md_engine* e, int cap) {
+    CHECK(check_ready(e, true));
+    if (cap < 1) return fail(MD_E_INVALID, "capacity must be >= 1");
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    CHECK(alloc_list(e, cap));
+    HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));   // every list is rebuilt when next needed
+    HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
 }
 

@@ -78,7 +78,7 @@ def lib():
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
-            "md_get_info": ([P, P, P, P, P, P], C.c_int),
+            "md_get_info": ([P, P, P, P, P, P], C.c_int), "md_set_list_capacity": ([P, C.c_int], C.c_int),
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
             "md_autotune": ([P, C.c_int, P], C.c_int), "md_timing_enable": ([P, C.c_int], C.c_int),
             "md_timing_get": ([P, P, P, P, P], C.c_int), "md_copy_bench": ([P, C.c_longlong, C.c_int, P], C.c_int),
@@ -98,7 +98,8 @@ def exported_symbols() -> List[str]:
     return ["md_last_error", "md_version", "md_create", "md_destroy", "md_set_forcefield", "md_set_integrator",
             "md_set_system", "md_set_topology", "md_upload_state", "md_download_state", "md_compute_forces",
             "md_run", "md_get_neighbors", "md_get_thresholds", "md_get_info", "md_set_launch", "md_get_launch", "md_autotune",
-            "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad"]
+            "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
+            "md_set_list_capacity"]
 
 
 def _check(rc: int):
@@ -240,6 +241,10 @@ class Engine:
         return dict(rebuilds=reb.value, steps=st.value, cap=cap.value, lanes=ln.value, grid=gr.value)
 
     # -- launch control / measurement ------------------------------------
+    def set_list_capacity(self, cap: int):
+        """Neighbor-list entries per atom (grown by the engine when a row overflows)."""
+        _check(lib().md_set_list_capacity(self._h, int(cap)))
+
     def set_launch(self, **kw):
         """Change fields of the launch configuration (md_launch_config)."""
         c = LaunchConfig()

@@ -189,3 +189,64 @@ def test_nve_conservation_nacl32768(E):
     assert span < 0.02 * ke.mean()
     assert 300.0 < st["temp"][:, 0].min() and st["temp"][:, 0].max() < 1300.0
     e.close()
+
+
+def test_invalid_calls_raise_and_leave_the_engine_usable(E):
+    """Calls out of order, invalid launch fields, non-finite positions and a
+    system beyond the slot limit return errors (MDError) instead of aborting,
+    and the engine keeps working afterwards."""
+    import ctypes as C
+    s = S.argon(ncell=4)
+    say("error paths")
+    e = E.Engine()
+    with pytest.raises(E.MDError):
+        e.run(10)                                   # no force field, no system
+    e.set_forcefield(s.ff)
+    e.set_integrator(s.integrator, s.thermostat, s.dt, s.tau)
+    n = 1 << 25                                     # slot limit: 2^25 slots
+    natom = np.array([n], np.int32)
+    box = np.full(3, 1000.0, np.float32)
+    T0 = np.array([100.0], np.float32)
+    typ = np.zeros(n, np.int32)
+    q = np.zeros(n, np.float32)
+    m = np.full(n, 40.0, np.float32)
+    rc = E.lib().md_set_system(e._h, 1, E._ptr(natom), E._ptr(box), E._ptr(T0), E._ptr(typ), E._ptr(q), E._ptr(m))
+    assert rc == -4, rc                              # capacity
+    del typ, q, m
+    e.set_systems(s)
+    for bad in (dict(lanes=3), dict(cell_div=4), dict(check_every=0), dict(build_kernel=7), dict(defer=2)):
+        with pytest.raises(E.MDError):
+            e.set_launch(**bad)
+    pos = s.pos.copy()
+    pos[5, 1] = np.nan
+    with pytest.raises(E.MDError):
+        e.upload(pos)
+    e.upload(s.pos, s.vel)
+    st = e.run(50)
+    assert np.all(np.isfinite(st["temp"])) and st["temp"][-1, 0] > 0
+    e.close()
+
+
+def test_list_overflow_rolls_back_and_replays(E):
+    """A list capacity below the rows' length (the reference's nmaxlist too
+    small, which it does not guard) makes the run roll back to its start, grow
+    the capacity and replay: the result equals a run that never overflowed."""
+    s = S.nacl(nsites=16)
+    say("list overflow replay", s.n)
+    out = []
+    for small in (False, True):
+        e = E.Engine.from_system(s)
+        e.set_launch(check_every=8)
+        if small:
+            e.set_list_capacity(128)          # NaCl rows hold ~410 entries
+            assert e.info()["cap"] == 128
+        st = e.run(60, energy_every=20)
+        pos, vel, frc = e.download()
+        out.append((st, pos, vel, frc, e.info()["cap"]))
+        e.close()
+    say("  cap", out[0][4], out[1][4])
+    assert out[1][4] > 128
+    for nm in out[0][0].dtype.names:
+        assert np.array_equal(out[0][0][nm], out[1][0][nm]), nm
+    for k in (1, 2, 3):
+        assert np.array_equal(out[0][k], out[1][k])
