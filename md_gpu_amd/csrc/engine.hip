@@ -229,7 +229,8 @@ struct md_engine {
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
     int fm_fused = 1;              // force matching without bonded terms: one pass for loss and gradient
-    double queue_ratio = 1.15;   // launched / completed steps, learned
+    static constexpr double kQueueRatio0 = 1.15;
+    double queue_ratio = kQueueRatio0;   // launched / completed steps, learned per launch configuration
     int fm_nframes = 0;
     // segment graphs by key; two executable copies each, launched alternately, so
     // the next replay can be submitted while the previous one still runs
@@ -1110,8 +1111,10 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
         for (auto& s : rs)
             if (s.step != target) return fail(MD_E_STATE, "replica stopped at step %lld of %lld", s.step, target);
         if (!e->timing) {   // learn how many launches a step costs (stalls until the next rebuild chain)
-            double ratio = (double)launched / nsteps;
-            e->queue_ratio = 0.7 * e->queue_ratio + 0.3 * std::max(1.0, ratio * 0.98);
+            // Over-queuing costs an idle launch per surplus step; under-queuing one more host
+            // round trip.  Follow a lower ratio at once, a higher one gradually.
+            const double ratio = std::max(1.0, (double)launched / nsteps * 0.98);
+            e->queue_ratio = ratio < e->queue_ratio ? ratio : 0.7 * e->queue_ratio + 0.3 * ratio;
         }
         break;
     }
@@ -1229,6 +1232,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     e->defer = c->defer;
     e->vv_fuse = c->vv_fuse;
     e->fm_fused = c->fm_fused;
+    e->queue_ratio = md_engine::kQueueRatio0;   // what a step costs in launches depends on the configuration
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
         e->cell_div = c->cell_div;
