@@ -206,7 +206,7 @@ struct md_engine {
     DBuf<RepState> rst;
     DBuf<Gate> gate;
     DBuf<ThState> th;
-    DBuf<double> part, fm_acc;     // part: [2][S/8][MD_NSTAT] partial sums of the step kernel
+    DBuf<double> part, fm_acc;     // part: [2][MD_NSTAT][nunits <= S/8] partial sums of the step kernel
     DBuf<double> fm_part;          // [grid][2 + 2 nt^2] workgroup slots of the force-matching adjoint
     DBuf<float4> fm_ljtab;         // LJ table of the trial parameters (force matching)
     std::vector<float4> fm_ljtab_h;    // its host staging copy (alive until the next evaluation)

@@ -144,7 +144,7 @@ struct StepArgs {
     RepState* st;
     Gate* gate;
     ThState* th;
-    double* part;         // [2][nunits][MD_NSTAT] FP64 partial sums, one slot per run of units
+    double* part;         // [2][MD_NSTAT][nunits] FP64 partial sums, one slot per run of units
                           // (buffer par; the deferred finalize reads buffer par ^ 1)
     RepState* lst;        // [2][R] state handed from one step launch to the next (deferred finalize)
     int vvfused;          // NVE velocity Verlet in leapfrog form (mode MODE_LF): the step kernel ends the

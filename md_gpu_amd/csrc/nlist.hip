@@ -710,6 +710,7 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
     const int cap = a.cap;
     const float rl2 = a.rl2;
+    uint32_t* const wrow = a.nlist + (size_t)slot0 * cap;   // row of the wavefront's first atom
     // the atoms as wavefront-uniform values (scalar registers); an empty slot sits
     // far outside the box, so no candidate passes the distance test for it
     float axr[AW], ayr[AW], azr[AW];
@@ -734,24 +735,24 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
         const uint32_t ent = __float_as_uint(pj.w);
         const int js = (int)(ent & MD_JMASK);
         const int jid = __float_as_int(sj.w);
+        const int jrel = js - slot0;   // the candidate's offset from the wavefront's first atom
 #pragma unroll
         for (int i = 0; i < AW; i++) {
-            const int si = slot0 + i;
             // (xi - xj) - k L: the canonical minimum-image displacement for this image
             const float dx = (axr[i] - pj.x) + sj.x;
             const float dy = (ayr[i] - pj.y) + sj.y;
             const float dz = (azr[i] - pj.z) + sj.z;
             const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool ok = valid & (rsq < rl2) & (js != si);
+            bool ok = valid & (rsq < rl2) & (jrel != i);
             if (EXCL) {
                 const int4 ex = s_ex[wave][i];
                 ok = ok & (jid != ex.x) & (jid != ex.y) & (jid != ex.z) & (jid != ex.w);
             }
-            const unsigned long long m = __ballot(ok);
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
             if (m) {
                 const uint32_t e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)cnt[i]));
-                if (ok && e < (uint32_t)cap) a.nlist[(size_t)si * cap + lpos_u(e, gs)] = ent;
+                if (ok && e < (uint32_t)cap) wrow[(uint32_t)(i * cap) + lpos_u(e, gs)] = ent;
                 cnt[i] += __popcll(m);
             }
         }

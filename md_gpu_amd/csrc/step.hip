@@ -171,44 +171,35 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
     }
 }
 
-// Sums of a replica's partial slots of one step-kernel launch (grid workgroups;
+// The partial slots of a replica in one step-kernel launch (grid workgroups;
 // slot u holds the run of the workgroup that started it: the replica's first
 // unit ru0 and the first unit u0(b) = floor(b * nunits / grid) of every
 // workgroup b whose range starts inside the replica, so the slots are ru0 and
 // u0(b) for b in [ceil((ru0 + 1) grid / nunits), ceil(ru1 grid / nunits));
-// with grid >= nunits every unit starts a range).  All 256 threads of the
-// workgroup take part (slot k of the list goes to thread k mod 256) and every
-// thread gets the totals; the order of the additions is fixed, so every caller
-// gets the same bits.  ALL = false loads only KE and the trip flag.
-template <bool ALL>
-__device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, int grid, int nunits,
-                                          double (*s_red)[MD_NSTAT], double* v) {
-    const bool every = grid >= nunits;
-    const long long n = nunits, g = grid;
-    int cnt = ru1 - ru0, b_lo = 0;
-    if (!every) {
-        b_lo = (int)(((long long)(ru0 + 1) * g + n - 1) / n);
-        const int b_hi = (int)(((long long)ru1 * g + n - 1) / n);
-        cnt = 1 + max(0, b_hi - b_lo);
-    }
-#pragma unroll
-    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
-    for (int k = (int)threadIdx.x; k < cnt; k += blockDim.x) {
-        const int u = (every || k == 0) ? ru0 + k : (int)(((long long)(b_lo + k - 1) * n) / g);
-        const double* q = part + (size_t)u * MD_NSTAT;
-        if (ALL) {
-            const double2* q2 = reinterpret_cast<const double2*>(q);
-#pragma unroll
-            for (int j = 0; j < MD_NSTAT / 2; j++) {
-                const double2 t = q2[j];
-                v[2 * j] += t.x;
-                v[2 * j + 1] += t.y;
-            }
-        } else {
-            v[ACC_KE2] += q[ACC_KE2];
-            v[MD_NSTAT - 1] += q[MD_NSTAT - 1];
+// with grid >= nunits every unit starts a range).
+struct SlotList {
+    int ru0, cnt, b_lo;
+    bool every;
+    long long n, g;
+    __device__ SlotList(int ru0_, int ru1, int grid, int nunits)
+        : ru0(ru0_), cnt(ru1 - ru0_), b_lo(0), every(grid >= nunits), n(nunits), g(grid) {
+        if (!every) {
+            b_lo = (int)(((long long)(ru0 + 1) * g + n - 1) / n);
+            const int b_hi = (int)(((long long)ru1 * g + n - 1) / n);
+            cnt = 1 + max(0, b_hi - b_lo);
         }
     }
+    __device__ int operator()(int k) const {   // the k-th slot, 0 <= k < cnt
+        return (every || k == 0) ? ru0 + k : (int)(((long long)(b_lo + k - 1) * n) / g);
+    }
+};
+
+// Totals over the 256 threads of the workgroup of per-thread sums v (the slots
+// k = tid, tid + 256, ... of a SlotList, added in that order); every thread gets
+// the totals.  The order of the additions is fixed, so every caller gets the same
+// bits.  ALL = false: only KE and the trip flag.
+template <bool ALL>
+__device__ __forceinline__ void reduce_slots(double (*s_red)[MD_NSTAT], double* v) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < MD_NSTAT; k++) {
@@ -223,6 +214,25 @@ __device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < MD_NSTAT; k++) v[k] = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+}
+
+// Sums of a replica's partial slots in one buffer of part ([MD_NSTAT][nunits]:
+// one plane per quantity, so that ALL = false, which reads only KE and the trip
+// flag, touches 16 bytes a slot; every workgroup of a deferred finalize reads
+// them all).
+template <bool ALL>
+__device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, int grid, int nunits,
+                                          double (*s_red)[MD_NSTAT], double* v) {
+    const SlotList sl(ru0, ru1, grid, nunits);
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
+    for (int k = (int)threadIdx.x; k < sl.cnt; k += blockDim.x) {
+        const int u = sl(k);
+#pragma unroll
+        for (int j = 0; j < MD_NSTAT; j++)
+            if (ALL || j == ACC_KE2 || j == MD_NSTAT - 1) v[j] += part[(size_t)j * nunits + u];
+    }
+    reduce_slots<ALL>(s_red, v);
 }
 
 // One pair: force factor (F_i += ff * d) and, with EN, the pair energies.
@@ -385,15 +395,53 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const bool one = a.nrep == 1 && u0 < u1;
     RepState s_one;
     if (one) s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
+    // With one replica and a deferred finalize, the previous launch's KE and trip
+    // flags: each thread's first four slots are loaded in the same round trip (the
+    // workgroup that writes the statistics record sums the full slots instead; the
+    // energy variants sum after the idle test).
+    const bool one_lag = one && a.lag;
+    const bool one_all = one_lag && u0 == 0 && a.stats;
+    const bool one_pre = ENMODE == 0 && one_lag && !one_all;
+    const double* prev_one = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.nunits;
+    const SlotList sl_one(0, a.nunits, gridDim.x, a.nunits);
+    double kpre[4], tpre[4];
+    if (one_pre) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = tid + 256 * j;
+            const int u = k < sl_one.cnt ? sl_one(k) : 0;
+            kpre[j] = k < sl_one.cnt ? prev_one[(size_t)ACC_KE2 * a.nunits + u] : 0.0;
+            tpre[j] = k < sl_one.cnt ? prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u] : 0.0;
+        }
+    }
     // nothing can step (and no list was just rebuilt for this block to open): idle launch
     if (a.mode == MODE_LF && gate.n_run == 0 && !(a.open && gate.n_stale != 0)) return;
     if (a.mode == MODE_VV && gate.n_force == 0) return;
-    // With one replica and a deferred finalize, the previous launch's partial sums.
+    // With one replica and a deferred finalize, the previous launch's partial sums
+    // (the additions in the order of sum_slots).
     double v_one[MD_NSTAT];
-    if (one && a.lag) {
-        const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
-        if (u0 == 0 && a.stats) sum_slots<true>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
-        else sum_slots<false>(prev, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+    if (one_lag) {
+        if (one_all) {
+            sum_slots<true>(prev_one, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+        } else if (!one_pre) {
+            sum_slots<false>(prev_one, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
+        } else {
+#pragma unroll
+            for (int k = 0; k < MD_NSTAT; k++) v_one[k] = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (tid + 256 * j < sl_one.cnt) {
+                    v_one[ACC_KE2] += kpre[j];
+                    v_one[MD_NSTAT - 1] += tpre[j];
+                }
+            }
+            for (int k = tid + 1024; k < sl_one.cnt; k += 256) {
+                const int u = sl_one(k);
+                v_one[ACC_KE2] += prev_one[(size_t)ACC_KE2 * a.nunits + u];
+                v_one[MD_NSTAT - 1] += prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u];
+            }
+            reduce_slots<false>(s_red, v_one);
+        }
     }
     __syncthreads();
     int cur_r = -1, run_u0 = 0;
@@ -420,7 +468,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
         }
         __syncthreads();
         if (tid < MD_NSTAT)
-            a.part[((size_t)a.par * a.nunits + run_u0) * MD_NSTAT + tid] =
+            a.part[((size_t)a.par * MD_NSTAT + tid) * a.nunits + run_u0] =
                 s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
         __syncthreads();
         dke = dlj = dco = dbo = dan = dvi = 0.0;
@@ -454,7 +502,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                         for (int k = 0; k < MD_NSTAT; k++) v[k] = v_one[k];
                     } else {
                         const int ru1 = (a.rep[r].slot_off + a.rep[r].nslot) / APB;
-                        const double* prev = a.part + (size_t)(a.par ^ 1) * a.nunits * MD_NSTAT;
+                        const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.nunits;
                         if (leader && en_prev && a.stats) sum_slots<true>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
                         else sum_slots<false>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
                     }
@@ -654,7 +702,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
     const int r = blockIdx.x;
     // Every load is issued before any is consumed: with one replica its units are
     // all units, so the slot loads need no layout load first.
-    const double* part = a.part + (size_t)a.par * a.nunits * MD_NSTAT;
+    const double* part = a.part + (size_t)a.par * MD_NSTAT * a.nunits;
     const RepState s = a.lagmode ? a.lst[(size_t)a.par * a.nrep + r] : a.st[r];
     RepParam p;
     ThState th;
