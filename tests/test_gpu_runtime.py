@@ -105,6 +105,31 @@ def test_deferred_finalize_bit_identical(E, cfg):
             assert np.array_equal(a[k], b[k])
 
 
+@pytest.mark.parametrize("grid", [0, 16])
+def test_deferred_finalize_bit_identical_batched(E, grid):
+    """The same for six Ar-Xe replicas of uneven size: every workgroup sums the
+    slots of its replicas, and with a grid of 16 workgroups (fewer than the
+    units) workgroups span several replicas and the slots of a replica are
+    those of the workgroups that start inside it."""
+    reps = S.arxe_replicas(nrep=6, seed=6, nmax=1024)
+    say("deferred finalize batched", [x.n for x in reps], "grid", grid)
+    out = []
+    for defer in (0, 1):
+        e = E.Engine.from_system(reps)
+        e.set_launch(defer=defer, grid=grid, check_every=8, refresh=1)
+        st = e.run(120, energy_every=5)
+        pos, vel, frc = e.download()
+        out.append((st, pos, vel, frc, e.info()["rebuilds"]))
+        e.close()
+    a, b = out
+    say("  rebuilds", a[4], b[4])
+    assert a[4] == b[4] and a[4] > 6
+    for name in a[0].dtype.names:
+        assert np.array_equal(a[0][name], b[0][name]), name
+    for k in range(1, 4):
+        assert np.array_equal(a[k], b[k])
+
+
 def _nve_vv(s):
     s = s.copy()
     s.integrator, s.thermostat = "vverlet", "none"
