@@ -374,11 +374,15 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     // One memory round trip before any work: the gate counters, the LDS tables,
     // the first unit's per-atom data (none of it depends on the replica state; both
     // position buffers are read) and, with one replica, its state.
+    // The table loads go to registers first and to LDS after every other load of
+    // the prologue is issued: an LDS store waits for its data, and loads placed
+    // after it would start a round trip of their own.
     const Gate gate = *a.gate;
-    if (tid < 64) s_lj[tid] = a.ljtab[tid];
+    const float4 lj_v = a.ljtab[tid & 63];
+    float4 ct_v = make_float4(0.f, 0.f, 0.f, 0.f), ct_e = ct_v;
     if (COUL == 1 || COUL == 2) {   // 256 threads, MD_CTAB + 1 entries
-        s_ct[tid] = a.ctab[tid];
-        if (tid == 0) s_ct[MD_CTAB] = a.ctab[MD_CTAB];
+        ct_v = a.ctab[tid];
+        ct_e = a.ctab[MD_CTAB];
     }
     const int fslot = u0 * APB + wave * APW + grp;
     int f_id = -1, f_typ = 0, f_nc = 0;
@@ -407,12 +411,16 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     double kpre[4], tpre[4];
     if (one_pre) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = tid + 256 * j;
-            const int u = k < sl_one.cnt ? sl_one(k) : 0;
-            kpre[j] = k < sl_one.cnt ? prev_one[(size_t)ACC_KE2 * a.nunits + u] : 0.0;
-            tpre[j] = k < sl_one.cnt ? prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u] : 0.0;
+        for (int j = 0; j < 4; j++) {   // unconditional loads (a clamped slot): no wait here
+            const int u = sl_one(min(tid + 256 * j, sl_one.cnt - 1));
+            kpre[j] = prev_one[(size_t)ACC_KE2 * a.nunits + u];
+            tpre[j] = prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u];
         }
+    }
+    if (tid < 64) s_lj[tid] = lj_v;
+    if (COUL == 1 || COUL == 2) {
+        s_ct[tid] = ct_v;
+        if (tid == 0) s_ct[MD_CTAB] = ct_e;
     }
     // nothing can step (and no list was just rebuilt for this block to open): idle launch
     if (a.mode == MODE_LF && gate.n_run == 0 && !(a.open && gate.n_stale != 0)) return;
