@@ -95,6 +95,11 @@ __device__ __forceinline__ float ctab_eval(const float4* s_ct, float x) {
     return fmaf(fmaf(fmaf(c4.w, t, c4.z), t, c4.y), t, c4.x);
 }
 
+// Compiler-only ordering point (no instruction is emitted): the loads above it
+// are not moved below it, e.g. into the branch after an early exit that is the
+// only user of their values, so that they are in flight together.
+__device__ __forceinline__ void loads_issued() { __asm__ __volatile__("" ::: "memory"); }
+
 // Workgroups are dispatched round-robin over MI355X's 8 XCDs (workgroup b on
 // XCD b mod 8), each with its own L2.  xcd_block maps the launch index b to a
 // logical index so that every XCD takes one contiguous range of the n logical

@@ -104,31 +104,38 @@ __device__ int block_scan(int v, int* s_w, int& total) {
 }  // namespace
 
 // The short rebuild kernels issue every load that does not depend on another
-// (the gate counter included) in one round trip: an idle launch costs about the
-// same, a working one saves round trips.
+// (the gate counter included) in one round trip, and the loads that depend on
+// the replica index in the next: an idle launch costs about the same, a working
+// one saves round trips.  A wavefront's 64 slots are one tile, so the replica
+// index is wavefront-uniform and its state and parameters are scalar loads.
 __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
-    const int r = a.tile_rep[s >> 6];
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
     const int id = a.ids[s];
     const float4 x0 = a.pos[0][s], x1 = a.pos[1][s];
+    loads_issued();
     if (n_stale == 0) return;
     const RepState st = a.st[r];
-    if (!stale(st)) return;
     const RepParam& p = a.rep[r];
-    if (s == p.slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
+    const float h0 = p.halfL[0], h1 = p.halfL[1], h2 = p.halfL[2];
+    const float c0 = p.cinv[0], c1 = p.cinv[1], c2 = p.cinv[2];
+    const int n0 = p.nc[0], n1 = p.nc[1], n2 = p.nc[2], cell_off = p.cell_off, slot_off = p.slot_off;
+    loads_issued();
+    if (!stale(st)) return;
+    if (s == slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
     if (id < 0) return;
     float4 x = cur_buf(st) ? x1 : x0;
-    int cx = cell_coord(x.x, p.halfL[0], p.cinv[0], p.nc[0]);
-    int cy = cell_coord(x.y, p.halfL[1], p.cinv[1], p.nc[1]);
-    int cz = cell_coord(x.z, p.halfL[2], p.cinv[2], p.nc[2]);
-    int lin = cx + p.nc[0] * (cy + p.nc[1] * cz);
-    int gc = p.cell_off + a.cell_rank[p.cell_off + lin];
+    int cx = cell_coord(x.x, h0, c0, n0);
+    int cy = cell_coord(x.y, h1, c1, n1);
+    int cz = cell_coord(x.z, h2, c2, n2);
+    int lin = cx + n0 * (cy + n1 * cz);
+    int gc = cell_off + a.cell_rank[cell_off + lin];
     // the atom's eighth of the cell, numbered along a path through adjacent eighths
-    const int sx = (x.x + p.halfL[0]) * p.cinv[0] - (float)cx >= 0.5f;
-    const int sy = (x.y + p.halfL[1]) * p.cinv[1] - (float)cy >= 0.5f;
-    const int sz = (x.z + p.halfL[2]) * p.cinv[2] - (float)cz >= 0.5f;
+    const int sx = (x.x + h0) * c0 - (float)cx >= 0.5f;
+    const int sy = (x.y + h1) * c1 - (float)cy >= 0.5f;
+    const int sz = (x.z + h2) * c2 - (float)cz >= 0.5f;
     const int sub = (0x54672310 >> (4 * (sx | (sy << 1) | (sz << 2)))) & 7;
     a.key[s] = gc;
     a.rank[s] = atomicAdd(&a.cell_count[gc], 1) | (sub << MD_SUBSHIFT);
@@ -136,16 +143,20 @@ __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
 
 __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
     __shared__ int wsum[16];
-    if (a.gate->n_stale == 0) return;
     const int r = blockIdx.x;
-    if (!stale(a.st[r])) return;
+    const int n_stale = a.gate->n_stale;
+    const RepState st = a.st[r];
     const RepParam& p = a.rep[r];
     const int nc = p.nc[0] * p.nc[1] * p.nc[2];
+    const int cell_off = p.cell_off, cs_off = p.cs_off, slot_off = p.slot_off, natom = p.natom;
+    loads_issued();
+    if (n_stale == 0) return;
+    if (!stale(st)) return;
     const int per = (nc + 1023) / 1024;
     const int b = min(nc, (int)threadIdx.x * per), e = min(nc, b + per);
     int sum = 0;
 #pragma unroll 8
-    for (int k = b; k < e; k++) sum += a.cell_count[p.cell_off + k];
+    for (int k = b; k < e; k++) sum += a.cell_count[cell_off + k];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = sum;
     for (int o = 1; o < 64; o <<= 1) {
@@ -159,27 +170,30 @@ __global__ __launch_bounds__(1024) void k_nl_scan(BuildArgs a) {
         for (int k = 0; k < 16; k++) { int t = wsum[k]; wsum[k] = run; run += t; }
     }
     __syncthreads();
-    int run = p.slot_off + wsum[w] + inc - sum;
+    int run = slot_off + wsum[w] + inc - sum;
 #pragma unroll 8
     for (int k = b; k < e; k++) {
-        int c = a.cell_count[p.cell_off + k];
-        a.cell_start[p.cs_off + k] = run;
+        int c = a.cell_count[cell_off + k];
+        a.cell_start[cs_off + k] = run;
         run += c;
-        a.cell_count[p.cell_off + k] = 0;   // ready for the next rebuild
+        a.cell_count[cell_off + k] = 0;   // ready for the next rebuild
     }
-    if (threadIdx.x == 0) a.cell_start[p.cs_off + nc] = p.slot_off + p.natom;
+    if (threadIdx.x == 0) a.cell_start[cs_off + nc] = slot_off + natom;
 }
 
 __global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
-    const int r = a.tile_rep[s >> 6];
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
     const int id = a.ids[s];
     const int gc = a.key[s];
     const int rk = a.rank[s];
+    loads_issued();
     if (n_stale == 0) return;
-    if (!stale(a.st[r])) return;
+    const RepState st = a.st[r];   // whole (no load that waits for nb)
+    loads_issued();
+    if (!stale(st)) return;
     if (id < 0) return;
     int t = a.cell_start[gc + r] + (rk & ((1 << MD_SUBSHIFT) - 1));
     a.tmp[t] = s;
@@ -190,18 +204,26 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
-    const int r = a.tile_rep[t >> 6];
-    const int o = a.tmp[t];
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[t >> 6]);
+    const int o_raw = a.tmp[t];
     const int mykey = a.tmpid[t];
+    loads_issued();
     if (n_stale == 0) return;
+    // Second round trip: the replica state and parameters and the atom at slot o.
+    // tmp holds a slot of this replica for every t below slot_off + natom; above
+    // that, and for a replica that is not rebuilt, o is only clamped into range
+    // and the loaded values are not used.
     const RepState st = a.st[r];
     const RepParam& p = a.rep[r];
-    if (!stale(st)) return;
-    if (t >= p.slot_off + p.natom) return;
-    // tmp holds a slot of this replica for every t below slot_off + natom
+    const int lim = p.slot_off + p.natom;
+    const int o = min(max(o_raw, 0), a.nslot - 1);
     const int gc = a.key[o];
-    const float4 x = a.pos[cur_buf(st)][o], v = a.vel[o], f4 = a.force[o];
+    const float4 x0 = a.pos[0][o], x1 = a.pos[1][o], v = a.vel[o], f4 = a.force[o];
     const int ty = a.typ[o];
+    loads_issued();
+    if (!stale(st)) return;
+    if (t >= lim) return;
+    const float4 x = cur_buf(st) ? x1 : x0;
     const int myid = mykey & ((1 << MD_SUBSHIFT) - 1);
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
@@ -674,15 +696,21 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     __shared__ float4 s_qs[NW][RQ];     // (-kx L, -ky L, -kz L, bits of jid)
     __shared__ int4 s_ex[NW][EXCL ? AW : 1];
     __shared__ int s_wmax[NW];
-    if (a.gate->n_stale == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
+    const int slot0 = tile * 64 + wave * AW;
+    // the gate, the replica index and the wavefront's atoms in one round trip
+    const int n_stale = a.gate->n_stale;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
+    const int id_pre = a.ids_s[slot0 + (lane & (AW - 1))];
+    const float4 x_pre = a.pos_s[slot0 + (lane & (AW - 1))];
+    loads_issued();
+    if (n_stale == 0) return;
     const RepState st = a.st[r];
+    loads_issued();
     if (!stale(st)) return;
     const RepParam& p = a.rep[r];
     const int cb = cur_buf(st);
-    const int slot0 = tile * 64 + wave * AW;
     const unsigned long long below = (1ull << lane) - 1ull;
     s_mk[wave][lane] = 0;   // tags start at 1
     auto wave_sync = [&]() {
@@ -693,10 +721,9 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     bool active = false;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < AW) {
-        const int slot = slot0 + lane;
-        const int id = a.ids_s[slot];
+        const int id = id_pre;
         active = id >= 0;
-        xi = a.pos_s[slot];
+        xi = x_pre;
         if (EXCL) s_ex[wave][lane] = active ? a.excl[id] : make_int4(-1, -1, -1, -1);
     }
     // the wavefront's box over its active atoms
