@@ -651,6 +651,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             a.force[slot] = make_float4(fx, fy, fz, 0.0f);
             if (a.mode == MODE_LF) {
                 float4 v = a.vel[slot];
+                const float4 pb = a.pos_build[slot];   // for the skin check: same round trip as v
                 if (a.vvfused) {
                     // velocity Verlet, NVE: v(t) = v(t - dt/2) + F dt/2m ends the step (its kinetic
                     // energy is the step's); v(t + dt/2) = v(t) + F dt/2m begins the next.  The same
@@ -679,7 +680,6 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                     y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
                     z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
                     pout[slot] = make_float4(x, y, z, xi.w);
-                    const float4 pb = a.pos_build[slot];
                     const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
                     if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
                     if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
