@@ -55,12 +55,12 @@ __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double 
 // the next step (src/integrator.cu:146-147), list age, skin/age trip and step
 // count.  The finalize kernel and the deferred finalize inside the step kernel
 // both call it, so the two schedules give bit-identical trajectories.
-__device__ __forceinline__ void lf_done(const StepArgs& a, const RepParam& p, RepState& s, double ke2, bool tn,
+__device__ __forceinline__ void lf_done(const StepArgs& a, float nf, float T0, RepState& s, double ke2, bool tn,
                                         double& T, double& lam) {
     const double dt = a.dt, tau = a.tau;
-    T = ke2 / ((double)p.nf * MD_KB_REF);
+    T = ke2 / ((double)nf * MD_KB_REF);
     lam = 1.0;
-    if (a.thermostat == 1) lam = sqrt(1.0 + dt / tau * ((double)p.T0 / T - 1.0));
+    if (a.thermostat == 1) lam = sqrt(1.0 + dt / tau * ((double)T0 / T - 1.0));
     s.age = s.age + 1;
     s.step = s.step + 1;
     // list refresh by age (ahead of the skin trip), so queued steps do not stall;
@@ -114,7 +114,7 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
     if (a.mode == MODE_LF) {
         RepState s = st0;
         double T, lam;
-        lf_done(a, p, s, v[ACC_KE2], tn, T, lam);
+        lf_done(a, p.nf, p.T0, s, v[ACC_KE2], tn, T, lam);
         put_record(a, r, kstat, en, v, 0.5 * v[ACC_KE2], T, th.xi, th.vxi, 0.0);
         a.th[r].S = lam;
         RepState& st = a.st[r];
@@ -399,6 +399,19 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     const bool one = a.nrep == 1 && u0 < u1;
     RepState s_one;
     if (one) s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
+    // With one replica its box and thermostat constants (wavefront-uniform) are
+    // loaded here as well; the unit loop would load them per replica change, each
+    // after the one before.
+    float g_L0 = 0.f, g_L1 = 0.f, g_L2 = 0.f, g_Li0 = 0.f, g_Li1 = 0.f, g_Li2 = 0.f, g_nf = 1.f, g_T0 = 0.f;
+    double g_S = 1.0;
+    if (one) {
+        const RepParam& p0 = a.rep[0];
+        g_L0 = p0.L[0]; g_L1 = p0.L[1]; g_L2 = p0.L[2];
+        g_Li0 = p0.Linv[0]; g_Li1 = p0.Linv[1]; g_Li2 = p0.Linv[2];
+        g_nf = p0.nf;
+        g_T0 = p0.T0;
+        g_S = a.th[0].S;
+    }
     // With one replica and a deferred finalize, the previous launch's KE and trip
     // flags: each thread's first four slots are loaded in the same round trip (the
     // workgroup that writes the statistics record sums the full slots instead; the
@@ -485,7 +498,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
 
     for (int u = u0; u < u1; u++) {
         const int slot0 = u * APB;
-        const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
+        const int r = one ? 0 : __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
         if (r != cur_r) {
             if (cur_r >= 0 && run_r) flush();
             cur_r = r;
@@ -516,7 +529,8 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                     }
                     const long long kst = s.step - s.start;
                     double T;
-                    lf_done(a, a.rep[r], s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
+                    lf_done(a, one ? g_nf : a.rep[r].nf, one ? g_T0 : a.rep[r].T0, s, v[ACC_KE2],
+                            v[MD_NSTAT - 1] != 0.0, T, lamd);
                     if (leader && tid == 0 && a.stats)
                         put_record(a, r, kst, en_prev, v, 0.5 * v[ACC_KE2], T, a.th[r].xi, a.th[r].vxi, 0.0);
                 }
@@ -526,7 +540,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                     s.nb += 1;
                     s.age = 0;
                 }
-                if (a.mode == MODE_LF) lamd = a.th[r].S;
+                if (a.mode == MODE_LF) lamd = one ? g_S : a.th[r].S;
             }
             if (a.mode == MODE_FORCE) run_r = !stale(s);
             else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
@@ -543,9 +557,14 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             const int cb = cur_buf(s);
             pin = a.pos[cb];
             pout = a.pos[cb ^ 1];
-            const RepParam& p = a.rep[r];
-            L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
-            Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
+            if (one) {
+                L0 = g_L0; L1 = g_L1; L2 = g_L2;
+                Li0 = g_Li0; Li1 = g_Li1; Li2 = g_Li2;
+            } else {
+                const RepParam& p = a.rep[r];
+                L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
+                Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
+            }
             lam = (float)lamd;
             drift = !a.vvfused || s.step + 1 < s.target;
         }
@@ -737,7 +756,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
             double lam = s.lam;
             if (s.ran) {
                 double T;
-                lf_done(a, p, n, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lam);
+                lf_done(a, p.nf, p.T0, n, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lam);
                 const bool en = a.energy_every > 0 && (s.step % a.energy_every) == 0;
                 put_record(a, r, s.step - s.start, en, v, 0.5 * v[ACC_KE2], T, th.xi, th.vxi, 0.0);
             }
