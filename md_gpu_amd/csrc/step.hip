@@ -194,6 +194,36 @@ struct SlotList {
     }
 };
 
+// x from another lane of the row by a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum of x over the 64 lanes of the wavefront (every lane active), the same bits
+// in every lane: pairs, quads (quad permutations), halves of 16-lane rows and
+// rows (mirrors; addition is commutative, so both partners get the same sum),
+// then (row 0 + row 1) + (row 2 + row 3) from lanes 0, 16, 32 and 48.  DPP moves
+// instead of LDS permutes: no LDS round trip per level.
+__device__ __forceinline__ double wave_sum_d(double x) {
+    x += dpp_d<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+    x += dpp_d<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+    x += dpp_d<0x141>(x);   // row_half_mirror
+    x += dpp_d<0x140>(x);   // row_mirror
+    const long long b = __double_as_longlong(x);
+    double r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 16 * q);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 16 * q);
+        r[q] = __longlong_as_double(((long long)hi << 32) | (long long)lo);
+    }
+    return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
 // Totals over the 256 threads of the workgroup of per-thread sums v (the slots
 // k = tid, tid + 256, ... of a SlotList, added in that order); every thread gets
 // the totals.  The order of the additions is fixed, so every caller gets the same
@@ -204,7 +234,7 @@ __device__ __forceinline__ void reduce_slots(double (*s_red)[MD_NSTAT], double* 
 #pragma unroll
     for (int k = 0; k < MD_NSTAT; k++) {
         if (!ALL && k != ACC_KE2 && k != MD_NSTAT - 1) continue;
-        for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+        v[k] = wave_sum_d(v[k]);
     }
     __syncthreads();   // s_red is free
     if (lane == 0) {
@@ -480,8 +510,7 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
     // workgroup would serialise behind each other.
     auto flush = [&]() {
         double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, 0.0};
-        for (int k = 0; k < MD_NSTAT - 2; k++)
-            for (int o = 32; o >= 1; o >>= 1) vals[k] += __shfl_xor(vals[k], o, 64);
+        for (int k = 0; k < MD_NSTAT - 2; k++) vals[k] = wave_sum_d(vals[k]);
         const unsigned long long tb = __ballot(trip != 0);
         if (lane == 0) {
             for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = vals[k];
