@@ -16,7 +16,7 @@
 // Work decomposition: G lanes cooperate on one atom (each strides over its
 // neighbor row), 64/G atoms per wavefront, 4 wavefronts per workgroup, and a
 // persistent grid walks contiguous ranges of 256/G-atom units.  Per-atom sums
-// are reduced across the G lanes with cross-lane shuffles; KE / energies /
+// are reduced across the G lanes with DPP row moves; KE / energies /
 // virial and the skin-check flag are reduced per workgroup into FP64 partial
 // slots (plain stores).  k_step_finalize, launched next, reduces a replica's
 // slots and computes its temperature, thermostat update and statistics record,
@@ -201,6 +201,25 @@ __device__ __forceinline__ double dpp_d(double x) {
     const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+// Sum of x over the G lanes of each aligned lane group (G = 4, 8, 16 or 32), the
+// same bits in every lane of the group: pairs and quads by quad permutations,
+// then mirrors within 8 and 16 lanes, then the two 16-lane halves of 32.
+template <int G>
+__device__ __forceinline__ float group_sum(float x) {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per atom");
+    x += dpp_f<0xB1>(x);                          // quad_perm [1, 0, 3, 2]
+    x += dpp_f<0x4E>(x);                          // quad_perm [2, 3, 0, 1]
+    if constexpr (G >= 8) x += dpp_f<0x141>(x);   // row_half_mirror
+    if constexpr (G >= 16) x += dpp_f<0x140>(x);  // row_mirror
+    if constexpr (G >= 32) x += __shfl_xor(x, 16, 64);
+    return x;
 }
 
 // Sum of x over the 64 lanes of the wavefront (every lane active), the same bits
@@ -682,12 +701,9 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
                 }
             }
         }
-#pragma unroll
-        for (int o = G / 2; o >= 1; o >>= 1) {
-            fx += __shfl_xor(fx, o, 64);
-            fy += __shfl_xor(fy, o, 64);
-            fz += __shfl_xor(fz, o, 64);
-        }
+        fx = group_sum<G>(fx);
+        fy = group_sum<G>(fy);
+        fz = group_sum<G>(fz);
         if (ENMODE != 0 && en) {
             dlj += elj;
             dco += eco;
