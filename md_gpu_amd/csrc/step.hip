@@ -284,6 +284,37 @@ __device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, 
     reduce_slots<ALL>(s_red, v);
 }
 
+// sum_slots<true> for the finalize kernel: each thread's slots are read four at
+// a time, all loads in flight before the first addition (the additions keep the
+// order of sum_slots, so the bits are the same).
+__device__ __forceinline__ void sum_slots_all4(const double* part, int ru0, int ru1, int grid, int nunits,
+                                               double (*s_red)[MD_NSTAT], double* v) {
+    const SlotList sl(ru0, ru1, grid, nunits);
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
+    int k = (int)threadIdx.x;
+    for (; k + 3 * (int)blockDim.x < sl.cnt; k += 4 * (int)blockDim.x) {
+        double x[4][MD_NSTAT];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int u = sl(k + q * (int)blockDim.x);
+#pragma unroll
+            for (int j = 0; j < MD_NSTAT; j++) x[q][j] = part[(size_t)j * nunits + u];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+#pragma unroll
+            for (int j = 0; j < MD_NSTAT; j++) v[j] += x[q][j];
+        }
+    }
+    for (; k < sl.cnt; k += blockDim.x) {
+        const int u = sl(k);
+#pragma unroll
+        for (int j = 0; j < MD_NSTAT; j++) v[j] += part[(size_t)j * nunits + u];
+    }
+    reduce_slots<true>(s_red, v);
+}
+
 // One pair: force factor (F_i += ff * d) and, with EN, the pair energies.
 // Wolf / damped forces come from the cubic table of h(r) in LDS (s_ct); the
 // energies keep the closed form.
@@ -791,7 +822,7 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
         ru1 = (so + ns) / apb;
     }
     double v[MD_NSTAT];
-    sum_slots<true>(part, ru0, ru1, grid, a.nunits, s_red, v);   // used only if the replica ran
+    sum_slots_all4(part, ru0, ru1, grid, a.nunits, s_red, v);   // used only if the replica ran
     if (a.lagmode) {
         // No launch of the block passed the gate when none could step, and lst[par]
         // was not written: nothing to do.  (A replica that could step keeps
