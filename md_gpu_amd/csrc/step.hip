@@ -582,17 +582,33 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             if (cur_r >= 0 && run_r) flush();
             cur_r = r;
             run_u0 = u;
-            int ru0 = 0;
-            if (a.nrep > 1) ru0 = a.rep[r].slot_off / APB;
+            // the replica's parameters and state, wavefront-uniform, in one round trip
+            // (with one replica they were loaded in the prologue)
+            float c_nf = g_nf, c_T0 = g_T0, c_L0 = g_L0, c_L1 = g_L1, c_L2 = g_L2;
+            float c_Li0 = g_Li0, c_Li1 = g_Li1, c_Li2 = g_Li2;
+            double c_S = g_S;
+            int c_so = 0, c_ns = 0;
+            RepState s = s_one;
+            if (!one) {
+                const RepParam& pr = a.rep[r];
+                c_so = pr.slot_off;
+                c_ns = pr.nslot;
+                c_nf = pr.nf;
+                c_T0 = pr.T0;
+                c_L0 = pr.L[0]; c_L1 = pr.L[1]; c_L2 = pr.L[2];
+                c_Li0 = pr.Linv[0]; c_Li1 = pr.Linv[1]; c_Li2 = pr.Linv[2];
+                c_S = a.th[r].S;
+                s = a.lag ? a.lst[(size_t)(a.par ^ 1) * a.nrep + r] : a.st[r];
+                loads_issued();
+            }
+            const int ru0 = c_so / APB;
             const bool leader = u == ru0;   // the workgroup holding the replica's first unit
-            RepState s;
             double lamd = 1.0;
             if (a.lag) {
                 // Deferred finalize of the previous launch (same block, same grid): every
                 // workgroup sums the replica's KE and trip flag from the previous partial
                 // slots and advances the hand-over state; the leader also writes the
                 // statistics record.  Same arithmetic as k_step_finalize.
-                s = one ? s_one : a.lst[(size_t)(a.par ^ 1) * a.nrep + r];
                 lamd = s.lam;
                 if (s.ran) {
                     const bool en_prev = a.energy_every > 0 && (s.step % a.energy_every) == 0;
@@ -601,25 +617,23 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
 #pragma unroll
                         for (int k = 0; k < MD_NSTAT; k++) v[k] = v_one[k];
                     } else {
-                        const int ru1 = (a.rep[r].slot_off + a.rep[r].nslot) / APB;
+                        const int ru1 = (c_so + c_ns) / APB;
                         const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.nunits;
                         if (leader && en_prev && a.stats) sum_slots<true>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
                         else sum_slots<false>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
                     }
                     const long long kst = s.step - s.start;
                     double T;
-                    lf_done(a, one ? g_nf : a.rep[r].nf, one ? g_T0 : a.rep[r].T0, s, v[ACC_KE2],
-                            v[MD_NSTAT - 1] != 0.0, T, lamd);
+                    lf_done(a, c_nf, c_T0, s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
                     if (leader && tid == 0 && a.stats)
                         put_record(a, r, kst, en_prev, v, 0.5 * v[ACC_KE2], T, a.th[r].xi, a.th[r].vxi, 0.0);
                 }
             } else {
-                s = one ? s_one : a.st[r];
                 if (a.open && stale(s)) {   // rebuilt by the block's chain: the new list epoch opens
                     s.nb += 1;
                     s.age = 0;
                 }
-                if (a.mode == MODE_LF) lamd = one ? g_S : a.th[r].S;
+                if (a.mode == MODE_LF) lamd = c_S;
             }
             if (a.mode == MODE_FORCE) run_r = !stale(s);
             else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
@@ -636,14 +650,8 @@ __global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
             const int cb = cur_buf(s);
             pin = a.pos[cb];
             pout = a.pos[cb ^ 1];
-            if (one) {
-                L0 = g_L0; L1 = g_L1; L2 = g_L2;
-                Li0 = g_Li0; Li1 = g_Li1; Li2 = g_Li2;
-            } else {
-                const RepParam& p = a.rep[r];
-                L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
-                Li0 = p.Linv[0]; Li1 = p.Linv[1]; Li2 = p.Linv[2];
-            }
+            L0 = c_L0; L1 = c_L1; L2 = c_L2;
+            Li0 = c_Li0; Li1 = c_Li1; Li2 = c_Li2;
             lam = (float)lamd;
             drift = !a.vvfused || s.step + 1 < s.target;
         }
