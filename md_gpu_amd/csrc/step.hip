@@ -874,18 +874,28 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
 // thermostat scale (run end).  Each wavefront stores its 64-slot tile's skin
 // check flag; k_kick_finalize folds them into the list state and moves the
 // replica to the force phase.
+// Two round trips: the gate, the replica index and every per-atom value (both
+// position buffers) first, the replica's state and constants second.
 __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
-    if (a.apply_drift && a.gate->n_run == 0) return;   // nothing to kick: idle launch
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;   // whole wavefronts (nslot is a multiple of 64)
     const int lane = threadIdx.x & 63;
+    const int n_run = a.gate->n_run;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
+    const int id = a.ids[s];
+    float4 v = a.vel[s];
+    const float4 f = a.force[s];
+    const float4 x0 = a.pos[0][s], x1 = a.pos[1][s];
+    const float4 pb = a.pos_build[s];
+    loads_issued();
+    if (a.apply_drift && n_run == 0) return;   // nothing to kick: idle launch
     const RepState st = a.st[r];
     const RepParam& p = a.rep[r];
+    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
+    const float S = (float)(a.apply_drift ? a.th[r].S : a.th[r].S_end);
+    loads_issued();
     if (!a.apply_drift) {
-        if (a.ids[s] < 0) return;
-        float4 v = a.vel[s];
-        const float S = (float)a.th[r].S_end;
+        if (id < 0) return;
         v.x *= S; v.y *= S; v.z *= S;
         a.vel[s] = v;
         return;
@@ -893,27 +903,23 @@ __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
     if (!(st.phase == 0 && !stale(st) && st.step < st.target)) return;   // wave-uniform
     const int cb = cur_buf(st);
     bool trip = false;
-    if (a.ids[s] >= 0) {
-        float4 v = a.vel[s];
-        const float S = (float)a.th[r].S;
-        const float4 f = a.force[s];
+    if (id >= 0) {
         const float h = 0.5f * a.dt * v.w;
         v.x = fmaf(f.x, h, v.x * S);
         v.y = fmaf(f.y, h, v.y * S);
         v.z = fmaf(f.z, h, v.z * S);
         a.vel[s] = v;
-        const float4 xi = a.pos[cb][s];
+        const float4 xi = cb ? x1 : x0;
         float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
-        x = fmaf(-p.L[0], truncf(x * 2.0f * p.Linv[0]), x);
-        y = fmaf(-p.L[1], truncf(y * 2.0f * p.Linv[1]), y);
-        z = fmaf(-p.L[2], truncf(z * 2.0f * p.Linv[2]), z);
+        x = fmaf(-L0, truncf(x * 2.0f * Li0), x);
+        y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
+        z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
         a.pos[cb ^ 1][s] = make_float4(x, y, z, xi.w);
-        const float4 pb = a.pos_build[s];
-        const float ddx = minimg(x - pb.x, p.L[0], p.Linv[0]);
-        const float ddy = minimg(y - pb.y, p.L[1], p.Linv[1]);
-        const float ddz = minimg(z - pb.z, p.L[2], p.Linv[2]);
+        const float ddx = minimg(x - pb.x, L0, Li0);
+        const float ddy = minimg(y - pb.y, L1, Li1);
+        const float ddz = minimg(z - pb.z, L2, Li2);
         trip = fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q;
-        if (!(fabsf(x) <= p.L[0] && fabsf(y) <= p.L[1] && fabsf(z) <= p.L[2])) *a.err = 1;
+        if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
     }
     const unsigned long long tb = __ballot(trip);
     if (lane == 0) a.tflag[s >> 6] = tb ? 1 : 0;
