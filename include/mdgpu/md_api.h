@@ -129,6 +129,19 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats);
  * (counts still complete). */
 int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs);
 
+/* Consistency of the device's sorted atom order and neighbor lists against the
+ * positions of the last list build (test support; no counterpart in the
+ * reference, whose list has no guard, src/neighborlist.cu:284).  counts[MD_NCHECK]
+ * receives the number of violations of each invariant, all zero when sound:
+ *   [0] an atom slot s with slot_of[ids[s]] != s (the order is not a bijection)
+ *   [1] a row entry outside its tile's neighbor set
+ *   [2] a listed pair at distance >= r_list at the build positions, or i with itself
+ *   [3] a tile-set entry that is not an atom of the tile's replica
+ *   [4] a row or tile set beyond its capacity
+ *   [5] row padding that is not the empty entry */
+#define MD_NCHECK 6
+int md_check_lists(md_engine* e, int* counts);
+
 /* Canonical FP32 thresholds the kernels use (for the parity checker):
  * rc2 = rc*rc, rl2 = (float)((skin + sqrt(rc2))^2), skin2q = skin*skin/4. */
 int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q);
@@ -138,7 +151,7 @@ int md_get_thresholds(md_engine* e, float* rc2, float* rl2, float* skin2q);
 int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, int* lanes, int* grid);
 
 /* Neighbor-list capacity (entries per atom), the reference's nmaxlist
- * (.inp, src/neighborlist.cu:61).  The lists are rebuilt at the next call
+ * (.inp, src/neighborlist.cu:61); at most 65535.  The lists are rebuilt at the next call
  * that needs them; a row that does not fit does not corrupt anything (the
  * reference has no guard): the run that met it is rolled back to its start
  * and replayed with a larger capacity. */
@@ -148,15 +161,15 @@ int md_set_list_capacity(md_engine* e, int cap);
  * is fixed by src/gpu_properties.cu:31-67).  Setting it rebuilds the lists
  * when the cell grid changes. */
 typedef struct md_launch_config {
-    int lanes;         /* lanes cooperating on one atom in the step kernel: 4, 8, 16, 32 */
-    int grid;          /* persistent grid of the step kernel in workgroups (0 = automatic) */
+    int lanes;         /* lanes cooperating on one atom in the step kernel: 4, 8, 16 (a workgroup of
+                          64 x lanes threads per 64-atom tile) */
+    int grid;          /* unused (0): the step kernel runs one workgroup per 64-atom tile */
     int cell_div;      /* cell edge >= r_list / cell_div, 1..3 */
     int check_every;   /* steps queued between two (device-gated) list-rebuild chains */
-    int build_lanes;   /* lanes per atom in the list build: 4, 8 */
+    int build_lanes;   /* unused (4 or 8 accepted) */
     int submit;        /* 0: replayed HIP graph segments, 1: direct kernel launches */
     int refresh;       /* 1: also rebuild a list after check_every steps, ahead of the skin trip */
-    int build_kernel;  /* 0: tile-staged list build (build_lanes 4 or 8), 1: per-wavefront build (8 lanes),
-                          2, 3: candidate-parallel build (a lane per candidate, 16 or 8 atoms per wavefront) */
+    int build_kernel;  /* 0: the tile neighbor-set build (the only list build) */
     int defer;         /* leapfrog: 1 = each step kernel finalizes the previous step of its block
                           (temperature, thermostat, statistics) instead of a finalize kernel per step */
     int vv_fuse;       /* NVE velocity Verlet: 1 = one step kernel per step ends the step and begins the

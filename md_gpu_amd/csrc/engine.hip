@@ -30,16 +30,21 @@
 extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStream_t st);
 extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st);
 extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st);
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st,
-                                     hipEvent_t ev0, hipEvent_t ev1);
-extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int grid, hipStream_t st);
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, hipStream_t st, hipEvent_t ev0,
+                                     hipEvent_t ev1);
+extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, hipStream_t st);
 extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st);
 extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                           int grid, hipStream_t st);
+                                           hipStream_t st);
 extern "C" hipError_t md_launch_fm_fused(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                         int grid, hipStream_t st);
+                                         hipStream_t st);
+extern "C" hipError_t md_launch_copies(int npair, const void* const* src, void* const* dst, const long long* bytes,
+                                       hipStream_t st);
+extern "C" hipError_t md_launch_check_lists(const BuildArgs& a, int row_g, int* bad, hipStream_t st);
+extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
+                                       RunReport* out, hipStream_t stream);
 
 namespace {
 
@@ -68,9 +73,14 @@ int fail(int code, const char* fmt, ...) {
         if (rc_ != MD_OK) return rc_; \
     } while (0)
 
-const int kSegment = 96;         // steps queued per graph segment (blocks x check_every)
+const int kSegment = 96;         // steps queued per full graph segment (blocks x check_every)
 const double kPi = 3.14159265358979323846;
 
+// Device buffer of n elements; the allocation is a multiple of 16 bytes (so a
+// buffer can be copied in 16-byte units) and zero when alloc returns.  The
+// zero fill runs on the null stream and is waited for here: the engine's own
+// stream is non-blocking and does not order itself after null-stream work, so
+// a fill still in flight could land after the engine's first kernels.
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -78,13 +88,15 @@ struct DBuf {
     int alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        const size_t nb = (count * sizeof(T) + 15) / 16 * 16;
+        hipError_t e = hipMalloc((void**)&p, nb);
         if (e != hipSuccess) {
             p = nullptr;
-            return fail(MD_E_HIP, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+            return fail(MD_E_HIP, "hipMalloc(%zu bytes): %s", nb, hipGetErrorString(e));
         }
         n = count;
-        e = hipMemset(p, 0, count * sizeof(T));
+        e = hipMemset(p, 0, nb);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) return fail(MD_E_HIP, "hipMemset: %s", hipGetErrorString(e));
         return MD_OK;
     }
@@ -93,7 +105,7 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
-    size_t bytes() const { return n * sizeof(T); }
+    size_t bytes() const { return (n * sizeof(T) + 15) / 16 * 16; }
 };
 
 // Skilling's transpose form of the Hilbert index ("Programming the Hilbert
@@ -195,9 +207,13 @@ struct md_engine {
     DBuf<float4> pos[2], vel, force, pos_build, pos_s, vel_s, force_s, target;
     DBuf<int> ids, typ, ids_s, typ_s, slot_of, key, rank, tmp, tmpid;
     DBuf<int> cell_count, cell_start, cell_rank;
-    DBuf<uint32_t> nlist;
-    DBuf<int> ncount;
-    DBuf<int> tilemax, tflag;      // [S/64] per-tile row maximum (list build), skin-check flag (kick)
+    DBuf<uint16_t> nl16;           // [S][cap] rows of local indices into the tile sets
+    DBuf<uint32_t> jset;           // [S/64][jcap] tile neighbor sets
+    DBuf<int> ncount, jcount;
+    DBuf<int> tmax;                // [2][S/64] builds' longest row / largest set per tile (running maxima)
+    DBuf<int> tflag;               // [S/64] skin-check flag per tile (kick)
+    DBuf<RunReport> report;        // run progress for the host
+    DBuf<int> badcnt;              // list invariant counters (md_check_lists)
     DBuf<int4> excl, ang_list;
     DBuf<uint32_t> bond_list;
     DBuf<int> nbond, nang;
@@ -206,8 +222,8 @@ struct md_engine {
     DBuf<RepState> rst;
     DBuf<Gate> gate;
     DBuf<ThState> th;
-    DBuf<double> part, fm_acc;     // part: [2][MD_NSTAT][nunits <= S/8] partial sums of the step kernel
-    DBuf<double> fm_part;          // [grid][2 + 2 nt^2] workgroup slots of the force-matching adjoint
+    DBuf<double> part, fm_acc;     // part: [2][MD_NSTAT][S/64] partial sums of the step kernel, one per tile
+    DBuf<double> fm_part;          // [S/64][2 + 2 nt^2] per-tile slots of the force-matching kernels
     DBuf<float4> fm_ljtab;         // LJ table of the trial parameters (force matching)
     std::vector<float4> fm_ljtab_h;    // its host staging copy (alive until the next evaluation)
     DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
@@ -222,18 +238,21 @@ struct md_engine {
     DBuf<ThState> snap_th;
     DBuf<RepState> snap_rst;
     DBuf<Gate> snap_gate;
-    int cap = 0;
+    int cap = 0;                   // row capacity, entries per atom (a multiple of 128)
+    int jcap = 0;                  // tile set capacity, entries per tile
+    int jcap_max = 0;              // largest set the step kernel's LDS holds on this device
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 3;
+    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 0;
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
     int fm_fused = 1;              // force matching without bonded terms: one pass for loss and gradient
     static constexpr double kQueueRatio0 = 1.15;
     double queue_ratio = kQueueRatio0;   // launched / completed steps, learned per launch configuration
     int fm_nframes = 0;
-    // segment graphs by key; two executable copies each, launched alternately, so
-    // the next replay can be submitted while the previous one still runs
+    // segment graphs by key (check interval, energy interval, blocks); two
+    // executable copies each, launched alternately, so the next replay can be
+    // submitted while the previous one still runs
     struct SegGraph { hipGraphExec_t ex[2]; int next; };
     std::map<std::string, SegGraph> graphs;
 
@@ -252,12 +271,7 @@ struct md_engine {
             for (auto x : kv.second.ex) (void)hipGraphExecDestroy(x);
         graphs.clear();
     }
-    int apb() const { return 256 / lanes; }
-    int nunits() const { return S / apb(); }
-    int grid_blocks() const {
-        int g = grid > 0 ? grid : ncu * 4;
-        return std::max(1, std::min(g, nunits()));
-    }
+    int ntiles() const { return S / MD_TILE; }
 };
 
 namespace {
@@ -366,26 +380,55 @@ int setup_cells(md_engine* e) {
     return MD_OK;
 }
 
-int estimate_cap(md_engine* e) {
-    double rl = std::sqrt((double)e->rl2);
+double max_density(const md_engine* e) {
     double dens = 0;
     for (int r = 0; r < e->R; r++) {
         double V = (double)e->box[3 * r] * e->box[3 * r + 1] * e->box[3 * r + 2];
         dens = std::max(dens, e->natom[r] / V);
     }
-    double k = dens * 4.0 / 3.0 * kPi * rl * rl * rl;
+    return dens;
+}
+
+// row capacity: the atoms within r_list at the densest replica, with margin
+int estimate_cap(md_engine* e) {
+    double rl = std::sqrt((double)e->rl2);
+    double k = max_density(e) * 4.0 / 3.0 * kPi * rl * rl * rl;
     int c = (int)(k * 1.4 + 64);
     c = std::min(c, e->N);
     return std::max(32, (c + 31) / 32 * 32);
 }
 
-int alloc_list(md_engine* e, int cap) {
-    // a multiple of 128 entries = 4G for every G: rows hold whole blocks (lpos) and stay 16-byte aligned
+// tile set capacity: the atoms within r_list of a 64-atom tile (a cube of the
+// tile's volume grown by r_list: Steiner's formula for the parallel body)
+int estimate_jcap(md_engine* e) {
+    double rl = std::sqrt((double)e->rl2), dens = max_density(e);
+    double a = std::cbrt(64.0 / std::max(dens, 1e-12));
+    double V = a * a * a + 6 * a * a * rl + 3 * kPi * a * rl * rl + 4.0 / 3.0 * kPi * rl * rl * rl;
+    int c = (int)(V * dens * 1.3 + 128);
+    return (c + 63) / 64 * 64;
+}
+
+int alloc_list(md_engine* e, int cap, int jcap) {
+    // a multiple of 128 entries = 8G for every G: rows hold whole blocks (lpos16) and stay 256-byte aligned
     cap = (std::max(cap, 128) + 127) / 128 * 128;
+    jcap = (std::max(jcap, 64) + 63) / 64 * 64;
+    if (jcap > e->jcap_max)
+        return fail(MD_E_CAPACITY, "a 64-atom tile has %d neighbors; the step kernel's LDS holds %d on this device",
+                    jcap, e->jcap_max);
+    if (cap > 65535) return fail(MD_E_CAPACITY, "rows of %d entries exceed the 16-bit set index", cap);
     e->cap = cap;
-    CHECK(e->nlist.alloc((size_t)e->S * cap));
+    e->jcap = jcap;
+    CHECK(e->nl16.alloc((size_t)e->S * cap));
+    CHECK(e->jset.alloc((size_t)e->ntiles() * jcap));
     e->drop_graphs();
     return MD_OK;
+}
+
+// grow the capacities after a build reported rows of `row` and sets of `set` entries
+int grow_list(md_engine* e, int row, int set) {
+    const int nc = row > e->cap ? (int)(row * 1.25) / 32 * 32 + 64 : e->cap;
+    const int nj = set > e->jcap ? std::min((int)(set * 1.2) / 64 * 64 + 128, std::max(set, e->jcap_max)) : e->jcap;
+    return alloc_list(e, nc, nj);
 }
 
 int ensure_stats(md_engine* e, int nrec) {
@@ -413,14 +456,13 @@ BuildArgs build_args(md_engine* e) {
     b.pos_build = e->pos_build.p;
     b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p; b.tmpid = e->tmpid.p;
     b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
-    b.slot_of = e->slot_of.p; b.nlist = e->nlist.p; b.ncount = e->ncount.p;
-    b.tilemax = e->tilemax.p;
+    b.slot_of = e->slot_of.p; b.nl16 = e->nl16.p; b.ncount = e->ncount.p;
+    b.jset = e->jset.p; b.jcount = e->jcount.p;
+    b.tilemax = e->tmax.p; b.jmax = e->tmax.p + e->ntiles();
     b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p;
     b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
-    b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
-    b.build_lanes = e->build_lanes;
+    b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.jcap = e->jcap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.row_g = e->lanes;
-    b.build_kernel = e->build_kernel;
     b.lfgate = lf_form(e) ? 1 : 0;
     return b;
 }
@@ -428,7 +470,8 @@ BuildArgs build_args(md_engine* e) {
 StepArgs step_args(md_engine* e, int mode, int energy_every) {
     StepArgs a{};
     a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
-    a.ids = e->ids.p; a.typ = e->typ.p; a.nlist = e->nlist.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
+    a.ids = e->ids.p; a.typ = e->typ.p; a.nl16 = e->nl16.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
+    a.jset = e->jset.p; a.jcount = e->jcount.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
     a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0; a.open = 0;
     a.vvfused = vv_fused(e) ? 1 : 0;
@@ -438,7 +481,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.ct_scale = (float)(MD_CTAB / (double)e->ff.rc);
     a.slot_of = e->slot_of.p; a.bond_list = e->bond_list.p; a.nbond = e->nbond.p; a.ang_list = e->ang_list.p; a.nang = e->nang.p;
     a.target = e->target.p;
-    a.cap = e->cap; a.ntype = e->ff.ntype; a.nrep = e->R; a.nunits = e->nunits(); a.mode = mode;
+    a.cap = e->cap; a.jcap = e->jcap; a.ntype = e->ff.ntype; a.nrep = e->R; a.ntiles = e->ntiles(); a.mode = mode;
     a.thermostat = e->in.thermostat; a.bonded = e->bonded ? 1 : 0;
     a.energy_every = energy_every; a.force_energy = 0;
     a.rc2 = e->rc2; a.qfac = e->ff.qfac; a.palfa = e->palfa; a.spiialfa = e->spiialfa; a.alfasq = e->alfasq;
@@ -495,9 +538,9 @@ int enqueue_rebuild(md_engine* e, TimedSpan* ts, bool commit = true) {
 int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool finalize = true) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) ev = ts->take();   // the launch records the kernel's own start and end
-    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->grid_blocks(), e->st, ev.first, ev.second));
+    HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->st, ev.first, ev.second));
     if (ts) ts->steps.push_back(ev);
-    if (finalize) HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
+    if (finalize) HIPCHECK(md_launch_step_finalize(a, e->lanes, e->st));
     return MD_OK;
 }
 
@@ -513,7 +556,7 @@ int enqueue_lf_block(md_engine* e, int energy_every, int n) {
         a.par = k & 1;
         CHECK(enqueue_stepkernel(e, a, nullptr, false));
     }
-    HIPCHECK(md_launch_step_finalize(a, e->lanes, e->grid_blocks(), e->st));
+    HIPCHECK(md_launch_step_finalize(a, e->lanes, e->st));
     return MD_OK;
 }
 
@@ -531,47 +574,55 @@ int enqueue_step(md_engine* e, int energy_every, TimedSpan* ts, bool chain_mid =
     return MD_OK;
 }
 
+// Copy of the state a run may be rolled back to (restore = false), or back
+// (restore = true): one copy kernel for all arrays.
 int snapshot(md_engine* e, bool restore) {
     struct Pair { void* a; void* b; size_t n; };
-    Pair list[] = {
+    const Pair list[] = {
         {e->snap_pos0.p, e->pos[0].p, e->pos[0].bytes()}, {e->snap_pos1.p, e->pos[1].p, e->pos[1].bytes()},
         {e->snap_vel.p, e->vel.p, e->vel.bytes()}, {e->snap_force.p, e->force.p, e->force.bytes()},
         {e->snap_ids.p, e->ids.p, e->ids.bytes()}, {e->snap_typ.p, e->typ.p, e->typ.bytes()},
         {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()}, {e->snap_th.p, e->th.p, e->th.bytes()},
         {e->snap_rst.p, e->rst.p, e->rst.bytes()}, {e->snap_gate.p, e->gate.p, e->gate.bytes()}};
-    for (auto& x : list) {
-        if (restore) HIPCHECK(hipMemcpyAsync(x.b, x.a, x.n, hipMemcpyDeviceToDevice, e->st));
-        else HIPCHECK(hipMemcpyAsync(x.a, x.b, x.n, hipMemcpyDeviceToDevice, e->st));
+    const void* src[10];
+    void* dst[10];
+    long long nb[10];
+    for (int k = 0; k < 10; k++) {
+        src[k] = restore ? list[k].a : list[k].b;
+        dst[k] = restore ? list[k].b : list[k].a;
+        nb[k] = (long long)list[k].n;
     }
+    HIPCHECK(md_launch_copies(10, src, dst, nb, e->st));
     return MD_OK;
 }
 
-// The builds' longest rows per tile accumulate in tilemax until cleared.
+// The builds' longest rows and largest sets per tile accumulate in tmax until cleared.
 int clear_rowmax(md_engine* e) {
-    HIPCHECK(hipMemsetAsync(e->tilemax.p, 0, e->tilemax.bytes(), e->st));
+    HIPCHECK(hipMemsetAsync(e->tmax.p, 0, e->tmax.bytes(), e->st));
     return MD_OK;
 }
-// Queue the copy of the row maxima (read after the next stream synchronisation).
-int fetch_rowmax(md_engine* e, std::vector<int>& tm) {
-    tm.resize(e->tilemax.n);
-    HIPCHECK(hipMemcpyAsync(tm.data(), e->tilemax.p, e->tilemax.bytes(), hipMemcpyDeviceToHost, e->st));
+
+// Queue the run report and its copy; the caller synchronises the stream.
+int queue_report(md_engine* e, RunReport* host) {
+    HIPCHECK(md_launch_report(e->rst.p, e->R, e->tmax.p, e->ntiles(), e->err.p, e->report.p, e->st));
+    HIPCHECK(hipMemcpyAsync(host, e->report.p, sizeof(RunReport), hipMemcpyDeviceToHost, e->st));
     return MD_OK;
 }
-int max_of(const std::vector<int>& v) { return v.empty() ? 0 : *std::max_element(v.begin(), v.end()); }
+
+bool overflowed(const md_engine* e, const RunReport& r) { return r.rowmax > e->cap || r.setmax > e->jcap; }
 
 // Rebuild all replicas now at the current positions (host synchronous),
-// growing the list capacity until every row fits.
+// growing the list capacities until every row and tile set fits.
 int full_rebuild(md_engine* e) {
     for (int attempt = 0; attempt < 6; attempt++) {
         HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
         CHECK(clear_rowmax(e));
         CHECK(enqueue_rebuild(e, nullptr));
-        std::vector<int> tm;
-        CHECK(fetch_rowmax(e, tm));
+        RunReport rr;
+        CHECK(queue_report(e, &rr));
         HIPCHECK(hipStreamSynchronize(e->st));
-        const int m = max_of(tm);
-        if (m <= e->cap) return MD_OK;
-        CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
+        if (!overflowed(e, rr)) return MD_OK;
+        CHECK(grow_list(e, rr.rowmax, rr.setmax));
     }
     return fail(MD_E_CAPACITY, "neighbor list capacity could not be sized");
 }
@@ -580,10 +631,10 @@ int full_rebuild(md_engine* e) {
 int refresh_lists(md_engine* e) {
     CHECK(clear_rowmax(e));
     CHECK(enqueue_rebuild(e, nullptr));
-    std::vector<int> tm;
-    CHECK(fetch_rowmax(e, tm));
+    RunReport rr;
+    CHECK(queue_report(e, &rr));
     HIPCHECK(hipStreamSynchronize(e->st));
-    if (max_of(tm) > e->cap) CHECK(full_rebuild(e));
+    if (overflowed(e, rr)) CHECK(full_rebuild(e));
     return MD_OK;
 }
 
@@ -615,15 +666,21 @@ int md_create(md_engine** out, int device) {
     md_engine* e = new md_engine();
     e->dev = device;
     hipDeviceProp_t prop;
+    size_t lds = 65536;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         e->ncu = prop.multiProcessorCount;
         e->arch = prop.gcnArchName;
+        lds = prop.sharedMemPerBlock;
     }
+    // the step and force-matching kernels hold the staged set and the dummy
+    // (16 bytes each) next to at most 8 KB of static LDS
+    e->jcap_max = (int)std::min<size_t>(65535, (lds - 8192) / 16 - 1) / 64 * 64;
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return fail(MD_E_HIP, "hipStreamCreate failed");
     }
-    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64)) {
+    if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) ||
+        e->report.alloc(1) || e->badcnt.alloc(8)) {
         delete e;
         return MD_E_HIP;
     }
@@ -711,7 +768,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
         N += natom[r];
         S += (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
     }
-    if (S >= (1LL << (MD_JBITS - 4))) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << (MD_JBITS - 4));
+    if (S > (1LL << MD_SLOTBITS)) return fail(MD_E_CAPACITY, "too many atoms (%lld slots; limit %d)", S, 1 << MD_SLOTBITS);
     for (long long i = 0; i < N; i++) {
         if (type[i] < 0 || type[i] >= e->ff.ntype) return fail(MD_E_INVALID, "atom %lld: type %d outside 0..%d", i, type[i], e->ff.ntype - 1);
         if (!(mass[i] > 0)) return fail(MD_E_INVALID, "atom %lld: mass must be > 0", i);
@@ -765,6 +822,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     for (auto* b : {&e->ids, &e->typ, &e->ids_s, &e->typ_s, &e->key, &e->rank, &e->tmp, &e->tmpid, &e->ncount,
                     &e->snap_ids, &e->snap_typ})
         CHECK(b->alloc(Sz));
+    CHECK(e->jcount.alloc(S / MD_TILE));
     CHECK(e->slot_of.alloc(N));
     CHECK(e->snap_slot_of.alloc(N));
     CHECK(e->rep.alloc(nrep));
@@ -775,9 +833,9 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->snap_gate.alloc(1));
     CHECK(e->th.alloc(nrep));
     CHECK(e->snap_th.alloc(nrep));
-    CHECK(e->part.alloc(2 * (size_t)(S / 8) * MD_NSTAT));   // units hold at least 8 slots (lanes <= 32)
+    CHECK(e->part.alloc(2 * (size_t)(S / MD_TILE) * MD_NSTAT));
     CHECK(e->lst.alloc(2 * (size_t)nrep));
-    CHECK(e->tilemax.alloc(S / MD_TILE));
+    CHECK(e->tmax.alloc(2 * (size_t)(S / MD_TILE)));
     CHECK(e->tflag.alloc(S / MD_TILE));
     e->stat_cap = 0;
     CHECK(ensure_stats(e, 1));
@@ -786,7 +844,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     HIPCHECK(hipMemcpy(e->ids_s.p, neg.data(), S * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->tile_rep.p, e->tile_rep_h.data(), (S / MD_TILE) * sizeof(int), hipMemcpyHostToDevice));
     CHECK(setup_cells(e));
-    CHECK(alloc_list(e, estimate_cap(e)));
+    CHECK(alloc_list(e, estimate_cap(e), std::min(estimate_jcap(e), e->jcap_max)));
     e->nb = e->na = 0;
     e->bonded = e->has_excl = false;
     CHECK(e->excl.alloc(N));
@@ -936,7 +994,14 @@ int md_download_state(md_engine* e, float* pos, float* vel, float* force) {
         for (int s = 0; s < S; s++) {
             int i = ids[s];
             if (i < 0) continue;
-            const float4& v = (k == 0 && cur_buf(rs[e->tile_rep_h[s / MD_TILE]]) == 1) ? buf2[s] : buf[s];
+            const int r = e->tile_rep_h[s / MD_TILE];
+            float4 v = (k == 0 && cur_buf(rs[r]) == 1) ? buf2[s] : buf[s];
+            if (k == 0) {   // positions run unwrapped between list builds: wrap as the build does
+                const RepParam& p = e->rep_h[r];
+                v.x = wrap_coord(v.x, p.L[0], p.Linv[0]);
+                v.y = wrap_coord(v.y, p.L[1], p.Linv[1]);
+                v.z = wrap_coord(v.z, p.L[2], p.Linv[2]);
+            }
             outs[k][3 * (size_t)i] = v.x;
             outs[k][3 * (size_t)i + 1] = v.y;
             outs[k][3 * (size_t)i + 2] = v.z;
@@ -973,9 +1038,8 @@ int md_compute_forces(md_engine* e, float* force, double* energy, double* virial
 namespace {
 
 // One segment: `blocks` x ([gated rebuild chain] + check_every steps).
-int enqueue_segment(md_engine* e, int energy_every) {
+int enqueue_segment(md_engine* e, int energy_every, int blocks) {
     const int ce = std::max(1, e->check_every);
-    const int blocks = std::max(1, kSegment / ce);
     for (int b = 0; b < blocks; b++) {
         if (e->defer && lf_form(e)) {   // the block's first step kernel opens the rebuilt lists
             CHECK(enqueue_rebuild(e, nullptr, false));
@@ -988,21 +1052,21 @@ int enqueue_segment(md_engine* e, int energy_every) {
     return MD_OK;
 }
 
-// Queue `nseg` segments: replays of a captured graph, or direct launches.
-int launch_segments(md_engine* e, int energy_every, int nseg) {
-    if (e->submit == 1) {
-        for (int s = 0; s < nseg; s++) CHECK(enqueue_segment(e, energy_every));
-        return MD_OK;
-    }
+// blocks of a full segment
+int segment_blocks(const md_engine* e) { return std::max(1, kSegment / std::max(1, e->check_every)); }
+
+// Queue one segment of `blocks` blocks: a replay of its captured graph, or direct launches.
+int launch_one(md_engine* e, int energy_every, int blocks) {
+    if (e->submit == 1) return enqueue_segment(e, energy_every, blocks);
     const int ce = std::max(1, e->check_every);
     char keyb[96];
-    snprintf(keyb, sizeof keyb, "%d:%d", ce, energy_every);
+    snprintf(keyb, sizeof keyb, "%d:%d:%d", ce, energy_every, blocks);
     std::string key(keyb);
     auto it = e->graphs.find(key);
     if (it == e->graphs.end()) {
         hipGraph_t g;
         HIPCHECK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_segment(e, energy_every);
+        const int rc = enqueue_segment(e, energy_every, blocks);
         hipError_t cerr = hipStreamEndCapture(e->st, &g);
         if (rc != MD_OK) return rc;
         if (cerr != hipSuccess) return fail(MD_E_HIP, "graph capture: %s", hipGetErrorString(cerr));
@@ -1017,17 +1081,22 @@ int launch_segments(md_engine* e, int energy_every, int nseg) {
         }
         it = e->graphs.emplace(key, sg).first;
     }
-    for (int s = 0; s < nseg; s++) {
-        md_engine::SegGraph& sg = it->second;
-        HIPCHECK(hipGraphLaunch(sg.ex[sg.next], e->st));
-        sg.next ^= 1;
-    }
+    md_engine::SegGraph& sg = it->second;
+    HIPCHECK(hipGraphLaunch(sg.ex[sg.next], e->st));
+    sg.next ^= 1;
     return MD_OK;
 }
 
-int segment_steps(md_engine* e) {
-    const int ce = std::max(1, e->check_every);
-    return std::max(1, kSegment / ce) * ce;
+// Queue `nblocks` blocks: full segments, then the rest as segments of
+// power-of-two block counts (a short run does not replay a whole segment of
+// idle launches, and at most log2 of a segment's blocks graphs per interval
+// are captured).
+int launch_blocks(md_engine* e, int energy_every, long long nblocks) {
+    const int full = segment_blocks(e);
+    for (; nblocks >= full; nblocks -= full) CHECK(launch_one(e, energy_every, full));
+    for (int b = 1 << 30; b >= 1; b >>= 1)
+        if (nblocks & b) CHECK(launch_one(e, energy_every, b));
+    return MD_OK;
 }
 
 }  // namespace
@@ -1047,6 +1116,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     for (;;) {   // one attempt; repeated only after a list-capacity overflow
         CHECK(snapshot(e, false));
         CHECK(clear_rowmax(e));
+        HIPCHECK(hipMemsetAsync(e->err.p, 0, sizeof(int), e->st));
         HIPCHECK(md_launch_set_target(e->rst.p, e->gate.p, e->R, nsteps, e->st));
         if (vv) HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 0, e->st));
         if (fused) {
@@ -1056,7 +1126,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
             HIPCHECK(md_launch_kick(kick_args(e, 1), e->st));
         }
         long long launched = 0;
-        std::vector<RepState> rs;
+        RunReport rr{};
         if (e->timing) {
             // direct launches, one rebuild chain per step, HIP events around each kernel
             TimedSpan span{e, {}, {}, 2};
@@ -1065,7 +1135,8 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
                 CHECK(enqueue_step(e, energy_every, &span, true));
             }
             launched = nsteps;
-            CHECK(read_states(e, rs));
+            CHECK(queue_report(e, &rr));
+            HIPCHECK(hipStreamSynchronize(e->st));
             for (auto& pr : span.steps) {
                 float ms = 0;
                 HIPCHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
@@ -1079,37 +1150,36 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
                 e->n_reb++;
             }
         } else {
-            const int per = segment_steps(e);
+            const int ce = std::max(1, e->check_every);
             long long want = (long long)std::ceil(nsteps * e->queue_ratio);
             for (int round = 0; round < 64; round++) {
-                int nseg = (int)((want + per - 1) / per);
-                CHECK(launch_segments(e, energy_every, std::max(nseg, 1)));
-                launched += (long long)std::max(nseg, 1) * per;
-                CHECK(read_states(e, rs));
-                long long lag = 0;
-                for (auto& s : rs) lag = std::max(lag, target - s.step);
-                if (lag <= 0) break;
+                const long long nbl = std::max(1LL, (want + ce - 1) / ce);
+                CHECK(launch_blocks(e, energy_every, nbl));
+                launched += nbl * ce;
+                CHECK(queue_report(e, &rr));
+                HIPCHECK(hipStreamSynchronize(e->st));
+                const long long lag = target - rr.min_step;
+                if (lag <= 0 || overflowed(e, rr) || rr.err) break;
                 want = (long long)std::ceil(lag * std::max(e->queue_ratio, 1.0) * 1.1) + 1;
             }
         }
-        int errf = 0;
-        std::vector<int> tm;
-        CHECK(fetch_rowmax(e, tm));
-        HIPCHECK(hipMemcpyAsync(&errf, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
-        HIPCHECK(hipStreamSynchronize(e->st));
-        const int m = max_of(tm);
-        if (m > e->cap) {   // a list row overflowed: roll the run back with a larger capacity
+        if (overflowed(e, rr)) {   // a row or tile set overflowed: roll the run back with larger capacities
             // (checked first: an attempt with truncated rows may also have gone unstable)
-            if (++retries > 4) return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (%d > %d)", m, e->cap);
+            if (++retries > 4)
+                return fail(MD_E_CAPACITY, "neighbor list capacity keeps overflowing (rows %d > %d or sets %d > %d)",
+                            rr.rowmax, e->cap, rr.setmax, e->jcap);
             CHECK(snapshot(e, true));
-            HIPCHECK(hipMemsetAsync(e->err.p, 0, sizeof(int), e->st));
-            CHECK(alloc_list(e, (int)(m * 1.25) / 32 * 32 + 64));
+            CHECK(grow_list(e, rr.rowmax, rr.setmax));
             CHECK(full_rebuild(e));
             continue;
         }
-        if (errf) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
-        for (auto& s : rs)
-            if (s.step != target) return fail(MD_E_STATE, "replica stopped at step %lld of %lld", s.step, target);
+        if (rr.err) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
+        if (rr.min_step != target) {
+            std::vector<RepState> rs;
+            CHECK(read_states(e, rs));
+            for (auto& s : rs)
+                if (s.step != target) return fail(MD_E_STATE, "replica stopped at step %lld of %lld", s.step, target);
+        }
         if (!e->timing) {   // learn how many launches a step costs (stalls until the next rebuild chain)
             // Over-queuing costs an idle launch per surplus step; under-queuing one more host
             // round trip.  Follow a lower ratio at once, a higher one gradually.
@@ -1144,20 +1214,28 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
     if (!counts || cap < 0 || (cap > 0 && !nbrs)) return fail(MD_E_INVALID, "bad output arrays");
     HIPCHECK(hipSetDevice(e->dev));
     CHECK(refresh_lists(e));
-    const int S = e->S;
-    std::vector<int> ids(S), nc(S);
-    std::vector<uint32_t> L((size_t)S * e->cap);
+    const int S = e->S, nt = e->ntiles();
+    std::vector<int> ids(S), nc(S), jc(nt);
+    std::vector<uint16_t> L((size_t)S * e->cap);
+    std::vector<uint32_t> J((size_t)nt * e->jcap);
     HIPCHECK(hipMemcpy(ids.data(), e->ids.p, S * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHECK(hipMemcpy(nc.data(), e->ncount.p, S * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHECK(hipMemcpy(L.data(), e->nlist.p, L.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(jc.data(), e->jcount.p, nt * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(L.data(), e->nl16.p, L.size() * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(J.data(), e->jset.p, J.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     int worst = 0;
     for (int s = 0; s < S; s++) {
         int i = ids[s];
         if (i < 0) continue;
         counts[i] = nc[s];
         worst = std::max(worst, nc[s]);
+        const int t = s / MD_TILE;
         int c = std::min(std::min(nc[s], cap), e->cap);
-        for (int k = 0; k < c; k++) nbrs[(size_t)i * cap + k] = ids[L[(size_t)s * e->cap + lpos(k, e->lanes)] & MD_JMASK];
+        for (int k = 0; k < c; k++) {
+            const int li = L[(size_t)s * e->cap + lpos16(k, e->lanes)];
+            if (li < 1 || li > jc[t]) return fail(MD_E_STATE, "row of slot %d holds set index %d outside 1..%d", s, li, jc[t]);
+            nbrs[(size_t)i * cap + k] = ids[J[(size_t)t * e->jcap + li - 1] & MD_SLOTMASK];
+        }
     }
     if (worst > cap) return fail(MD_E_CAPACITY, "rows need %d entries, cap is %d", worst, cap);
     if (worst > e->cap) return fail(MD_E_CAPACITY, "device list overflowed (%d > %d)", worst, e->cap);
@@ -1187,7 +1265,7 @@ int md_get_info(md_engine* e, long long* rebuilds, long long* steps, int* cap, i
     if (steps) *steps = e->step;
     if (cap) *cap = e->cap;
     if (lanes) *lanes = e->lanes;
-    if (grid) *grid = e->has_sys ? e->grid_blocks() : e->grid;
+    if (grid) *grid = e->has_sys ? e->ntiles() : 0;   // one step-kernel workgroup per 64-slot tile
     return MD_OK;
 }
 
@@ -1196,7 +1274,7 @@ int md_set_list_capacity(md_engine* e, int cap) {
     if (cap < 1) return fail(MD_E_INVALID, "capacity must be >= 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
-    CHECK(alloc_list(e, cap));
+    CHECK(alloc_list(e, cap, e->jcap));
     HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));   // every list is rebuilt when next needed
     HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
@@ -1204,8 +1282,8 @@ int md_set_list_capacity(md_engine* e, int cap) {
 
 int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (!e || !c) return fail(MD_E_INVALID, "null argument");
-    if (c->lanes != 4 && c->lanes != 8 && c->lanes != 16 && c->lanes != 32)
-        return fail(MD_E_INVALID, "lanes must be 4, 8, 16 or 32");
+    if (c->lanes != 4 && c->lanes != 8 && c->lanes != 16)
+        return fail(MD_E_INVALID, "lanes must be 4, 8 or 16");
     if (c->grid < 0) return fail(MD_E_INVALID, "grid must be >= 0");
     if (c->cell_div < 1 || c->cell_div > 3) return fail(MD_E_INVALID, "cell_div must be 1..3");
     if (c->check_every < 1 || c->check_every > kSegment)
@@ -1214,14 +1292,14 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
         return fail(MD_E_INVALID, "build_lanes must be 4 or 8");
     if (c->submit != 0 && c->submit != 1) return fail(MD_E_INVALID, "submit must be 0 (graphs) or 1 (direct)");
     if (c->refresh != 0 && c->refresh != 1) return fail(MD_E_INVALID, "refresh must be 0 or 1");
-    if (c->build_kernel < 0 || c->build_kernel > 3) return fail(MD_E_INVALID, "build_kernel must be 0..3");
+    if (c->build_kernel != 0) return fail(MD_E_INVALID, "build_kernel must be 0 (the tile neighbor-set build)");
     if (c->defer != 0 && c->defer != 1) return fail(MD_E_INVALID, "defer must be 0 or 1");
     if (c->vv_fuse != 0 && c->vv_fuse != 1) return fail(MD_E_INVALID, "vv_fuse must be 0 or 1");
     if (c->fm_fused != 0 && c->fm_fused != 1) return fail(MD_E_INVALID, "fm_fused must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
-    // the row layout (lpos) follows the step kernel's lanes; a new build kernel rebuilds too
-    const bool relayout = c->lanes != e->lanes || c->build_kernel != e->build_kernel || c->build_lanes != e->build_lanes;
+    // the row layout (lpos16) follows the step kernel's lanes
+    const bool relayout = c->lanes != e->lanes;
     e->lanes = c->lanes;
     e->grid = c->grid;
     e->check_every = c->check_every;
@@ -1302,17 +1380,18 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     };
     md_launch_config best;
     CHECK(md_get_launch(e, &best));
-    // stage 1, list build: cell divisor x lanes per atom, three forced rebuilds each
-    double best_build = 1e30;
+    // stage 1, cell grid x lanes per atom: the cell edge sets the build's candidate
+    // count, the lanes the step kernel's shape; build time (three forced rebuilds)
+    // plus step-kernel time (HIP events) of each pair
+    struct Shape { double ms; int cell_div, lanes; };
+    std::vector<Shape> shapes;
+    const int ks = std::max(8, std::min(nsteps, 64));
     for (int d = 1; d <= 3; d++)
-        for (int bk : {0, 1, 2, 3})
-            for (int bl : {4, 8}) {
-            if (bk != 0 && bl != 8) continue;   // build_lanes only applies to the tile-staged kernel
+        for (int l : {4, 8, 16}) {
             md_launch_config c = best;
             c.cell_div = d;
-            c.build_lanes = bl;
-            c.build_kernel = bk;
-            CHECK(md_set_launch(e, &c));
+            c.lanes = l;
+            if (md_set_launch(e, &c) != MD_OK) continue;   // e.g. a tile set beyond the LDS
             CHECK(restore());
             HIPCHECK(hipEventRecord(t0, e->st));
             for (int k = 0; k < 3; k++) {
@@ -1321,32 +1400,21 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             }
             HIPCHECK(hipEventRecord(t1, e->st));
             HIPCHECK(hipEventSynchronize(t1));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-            if (ms < best_build) { best_build = ms; best.cell_div = d; best.build_lanes = bl; best.build_kernel = bk; }
-        }
-    // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events
-    // stage 2, step kernel: lanes per atom x workgroups per CU, kernel time from events;
-    // the three fastest shapes go on to whole runs
-    struct Shape { double ms; int lanes, grid; };
-    std::vector<Shape> shapes;
-    const int ks = std::max(8, std::min(nsteps, 64));
-    for (int l : {4, 8, 16, 32})
-        for (int w : {2, 4, 8}) {
-            md_launch_config c = best;
-            c.lanes = l;
-            c.grid = e->ncu * w;
-            CHECK(md_set_launch(e, &c));
+            float bms = 0;
+            HIPCHECK(hipEventElapsedTime(&bms, t0, t1));
             CHECK(restore());
             CHECK(md_timing_enable(e, 1));
             CHECK(md_run(e, ks, 0, nullptr));
-            shapes.push_back({e->t_step_ms / std::max(1LL, e->n_step), l, c.grid});
+            const double sms = e->t_step_ms / std::max(1LL, e->n_step);
             CHECK(md_timing_enable(e, 0));
+            // a build every ~8 steps
+            shapes.push_back({sms + bms / 3.0 / 8.0, d, l});
         }
+    if (shapes.empty()) return fail(MD_E_CAPACITY, "no launch shape fits this system");
     std::sort(shapes.begin(), shapes.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
     shapes.resize(std::min<size_t>(shapes.size(), 3));
-    // stage 3, whole runs (best of two): shape x rebuild-check interval x age refresh x submission mode
-    // x deferred finalize
+    // stage 2, whole runs (best of two): shape x rebuild-check interval x age refresh x
+    // submission mode x deferred finalize
     double best_run = 1e30;
     const int kr = std::max(nsteps, 4 * kSegment);
     md_launch_config base = best;
@@ -1377,8 +1445,8 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
                 for (int ce : {4, 6, 8, 9, 12, 16, 24, 32}) {
                     if (df == 0 && !lf) continue;   // deferral applies to leapfrog-form steps only
                     md_launch_config c = base;
+                    c.cell_div = sh.cell_div;
                     c.lanes = sh.lanes;
-                    c.grid = sh.grid;
                     c.check_every = ce;
                     c.submit = sub;
                     c.refresh = rf;
@@ -1387,7 +1455,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
                     CHECK(time_run(c, kr, 2, per));
                     trials.push_back({per, c});
                 }
-    // stage 4: the four fastest again, longer and three times each (the minimum
+    // stage 3: the four fastest again, longer and three times each (the minimum
     // over many short noisy trials favours a lucky one)
     std::sort(trials.begin(), trials.end(), [](const Trial& x, const Trial& y) { return x.ms < y.ms; });
     trials.resize(std::min<size_t>(trials.size(), 4));
@@ -1450,6 +1518,20 @@ int md_copy_bench(md_engine* e, long long nbytes, int reps, double* ms) {
     return MD_OK;
 }
 
+// ---------------------------------------------------------------- list invariants
+int md_check_lists(md_engine* e, int* counts) {
+    CHECK(check_ready(e, true));
+    if (!counts) return fail(MD_E_INVALID, "null counts");
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipMemsetAsync(e->badcnt.p, 0, e->badcnt.bytes(), e->st));
+    HIPCHECK(md_launch_check_lists(build_args(e), e->lanes, e->badcnt.p, e->st));
+    int bad[8];
+    HIPCHECK(hipMemcpyAsync(bad, e->badcnt.p, sizeof bad, hipMemcpyDeviceToHost, e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    for (int k = 0; k < MD_NCHECK; k++) counts[k] = bad[k];
+    return MD_OK;
+}
+
 // ---------------------------------------------------------------- force matching
 int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* target_force) {
     CHECK(check_ready(e, false));
@@ -1504,16 +1586,16 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     a.ljtab = e->fm_ljtab.p;
     a.stats = nullptr;
     const int coul = e->coul_kernel;
-    const size_t nslot = (size_t)e->grid_blocks() * (2 + 2 * ntk * ntk);
+    const size_t nslot = (size_t)e->ntiles() * (2 + 2 * ntk * ntk);
     if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
     if (e->fm_fused && !e->bonded) {   // one pass: force and gradient sums together
         a.qq_scale = qscale;
-        HIPCHECK(md_launch_fm_fused(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+        HIPCHECK(md_launch_fm_fused(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->st));
     } else {                            // force pass (bonded terms included), then the adjoint walk
         a.qq_scale = qscale * qscale;
-        HIPCHECK(md_launch_step(a, e->lanes, coul, e->grid_blocks(), e->st, nullptr, nullptr));
+        HIPCHECK(md_launch_step(a, e->lanes, coul, e->st, nullptr, nullptr));
         a.qq_scale = qscale;
-        HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->grid_blocks(), e->st));
+        HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->st));
     }
     std::vector<double> g(2 + 2 * ntk * ntk);
     HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));

@@ -3,13 +3,22 @@
 //
 // Layout in HBM (all arrays sorted along the space-filling curve; "slot" =
 // position in that order):
-//   pos[2][S]   float4 (x, y, z, q)   double-buffered: a step reads one, writes the other
+//   pos[2][S]   float4 (x, y, z, q)   double-buffered: a step reads one, writes the other.
+//               Coordinates are wrapped into the box at every list build and run
+//               free in between (an atom is at most skin/2 outside), so the image
+//               shifts the build recorded stay valid until the next build.
 //   vel[S]      float4 (vx, vy, vz, 1/m_int)
 //   force[S]    float4 (fx, fy, fz, 0)
 //   ids[S]      int    original atom index (-1 = padding slot)
 //   typ[S]      int    atom type
-//   nlist[S*cap] uint32 (j_slot | type_j << 29), one row per atom, ncount[S];
-//               entries stored in the transposed block order of lpos()
+// Neighbor list, per 64-slot tile t (a tile never straddles two replicas):
+//   jset[t][k]  uint32 the tile's neighbor set: slot | image shift code << 24 | type << 29,
+//               k < jcount[t]; local index k + 1 (local index 0 is a far-away dummy)
+//   nl16[S][cap] uint16 per-atom rows of local indices into the tile's set, in the
+//               block layout of lpos16(); entries past the row length up to the end
+//               of the row's last block are 0 (the dummy), ncount[S] = row length
+// The step kernel stages the tile's set into LDS once per step, each position
+// already shifted to its image, so a pair needs no minimum-image arithmetic.
 // Replicas occupy 64-aligned slot ranges, so a wavefront never straddles two.
 //
 // Control: every replica carries a RepState on the device.  Kernels decide
@@ -21,13 +30,17 @@
 #include <stdint.h>
 
 #define MD_TILE 64
-#define MD_JBITS 29
+#define MD_JBITS 29             // bond_list entries: partner | type << 29
 #define MD_JMASK 0x1FFFFFFFu
+#define MD_SLOTBITS 24          // neighbor-set entries: slot | shift code << 24 | type << 29
+#define MD_SLOTMASK 0xFFFFFFu
+#define MD_SHIFT0 13            // shift code of the unshifted image: (kx+1) + 3 (ky+1) + 9 (kz+1)
 #define MD_KB_REF 8.314e-3      // src/integrator.cu:155 (1000/8.314)
 #define MD_NSTAT 8              // accumulators per replica
 #define MD_CTAB 256             // intervals of the Coulomb force table on [0, rc]
-#define MD_SUBSHIFT 25          // list build: ranks and original indices stay below 2^25 (the slot limit);
-                                // the eighth of the cell an atom sits in is packed above them
+#define MD_SUBSHIFT 25          // list build: ranks and original indices stay below 2^25 (the slot limit is
+                                // lower); the eighth of the cell an atom sits in is packed above them
+#define MD_FAR 1.0e5f           // coordinate of the dummy neighbor (local index 0)
 
 enum { ACC_KE2 = 0, ACC_LJ, ACC_COUL, ACC_BOND, ACC_ANGLE, ACC_VIR };
 enum { MODE_FORCE = 0, MODE_LF = 1, MODE_VV = 2 };
@@ -67,23 +80,21 @@ struct Gate {
     int pad;
 };
 
-// Storage position of logical entry e in a neighbor row read by G lanes per atom.
-// Rows are stored in blocks of 4G entries; inside a block the layout is transposed
-// so that lane l's 16-byte quad holds logical entries l, l+G, l+2G, l+3G.  Each
-// position gather then reads consecutive entries (mostly consecutive slots: a row
-// lists the members of one cell together) across the atom's lanes.
-__host__ __device__ inline int lpos(int e, int G) {
-    return (e & ~(4 * G - 1)) + 4 * (e % G) + ((e / G) & 3);
+// Storage position of logical entry e in a row of 16-bit entries read by G lanes
+// per atom.  Rows are stored in blocks of 8G entries; inside a block lane l's
+// 16-byte load holds logical entries l, l+G, ..., l+7G, so each LDS read of the
+// atom's G lanes covers G consecutive entries (mostly consecutive staged
+// neighbors: a row lists the members of one cell together).
+__host__ __device__ inline int lpos16(int e, int G) {
+    return (e & ~(8 * G - 1)) + 8 * (e % G) + ((e / G) & 7);
 }
-// the same for G = 1 << gs (shifts and masks only)
-__device__ __forceinline__ int lpos_sh(int e, int gs) {
-    const int G = 1 << gs;
-    return (e & ~(4 * G - 1)) | ((e & (G - 1)) << 2) | ((e >> gs) & 3);
-}
-__device__ __forceinline__ uint32_t lpos_u(uint32_t e, int gs) {
+__device__ __forceinline__ uint32_t lpos16_u(uint32_t e, int gs) {
     const uint32_t G = 1u << gs;
-    return (e & ~(4u * G - 1u)) | ((e & (G - 1u)) << 2) | __builtin_amdgcn_ubfe(e, (uint32_t)gs, 2u);
+    return (e & ~(8u * G - 1u)) | ((e & (G - 1u)) << 3) | __builtin_amdgcn_ubfe(e, (uint32_t)gs, 3u);
 }
+
+// image shift (kx, ky, kz) in {-1, 0, 1}^3 of a shift code
+__host__ __device__ inline int shift_code(int kx, int ky, int kz) { return (kx + 1) + 3 * (ky + 1) + 9 * (kz + 1); }
 
 // h at table coordinate x = r * MD_CTAB / rc >= 0 from the cubic Coulomb table in
 // LDS (MD_CTAB intervals and an end entry h(rc); beyond rc the value is finite
@@ -117,6 +128,12 @@ __host__ __device__ inline int cur_buf(const RepState& s) { return (int)((s.step
 // register copy of the state never has to live in scratch)
 __host__ __device__ inline bool stale(const RepState& s) { return ((s.nb & 1) ? s.trip[1] : s.trip[0]) != 0; }
 
+// Position wrapped into the centred box, the reference's x -= L trunc(2x/L)
+// (src/nve_leapfrog.cu:50-55); valid for |x| < 1.5 L.
+__host__ __device__ inline float wrap_coord(float x, float L, float Linv) {
+    return fmaf(-L, truncf(x * 2.0f * Linv), x);
+}
+
 // Thermostat state per replica (FP64).  S is the velocity scale the next
 // position-writing kernel applies; *_end keep the state after the end-of-step
 // half so a run can stop there.
@@ -141,15 +158,17 @@ struct StepArgs {
     float4* force;
     const int* ids;
     const int* typ;
-    const uint32_t* nlist;
+    const uint16_t* nl16;     // [S][cap] rows of local indices (lpos16 layout)
     const int* ncount;
+    const uint32_t* jset;     // [ntiles][jcap] tile neighbor sets
+    const int* jcount;        // [ntiles]
     const float4* pos_build;
     const int* tile_rep;
     const RepParam* rep;
     RepState* st;
     Gate* gate;
     ThState* th;
-    double* part;         // [2][MD_NSTAT][nunits] FP64 partial sums, one slot per run of units
+    double* part;         // [2][MD_NSTAT][ntiles] FP64 partial sums, one slot per tile
                           // (buffer par; the deferred finalize reads buffer par ^ 1)
     RepState* lst;        // [2][R] state handed from one step launch to the next (deferred finalize)
     int vvfused;          // NVE velocity Verlet in leapfrog form (mode MODE_LF): the step kernel ends the
@@ -170,10 +189,11 @@ struct StepArgs {
     const int4* ang_list;       // [N*4] (role | type << 8, a, b, 0)
     const int* nang;            // [N]
     const float4* target;       // [S] target forces in slot order (force matching)
-    int cap;
+    int cap;              // row capacity in entries (a multiple of 128)
+    int jcap;             // neighbor-set capacity per tile
     int ntype;
     int nrep;
-    int nunits;           // S / atoms-per-unit
+    int ntiles;           // S / 64
     int mode;             // MODE_*
     int thermostat;
     int bonded;
@@ -189,6 +209,17 @@ struct StepArgs {
     float skin2q;
     int max_age;          // > 0: the list is also refreshed after this many steps
     float bk2[8], br0[8], ak2[8], at0[8];
+};
+
+// Progress of a run for the host: one small copy per check instead of one per
+// array (tmax holds the builds' longest rows in [0, ntiles) and the largest
+// tile sets in [ntiles, 2 ntiles), since the host last cleared it).
+struct RunReport {
+    long long min_step;   // smallest step counter over the replicas
+    int rowmax;           // longest row of any build
+    int setmax;           // largest tile neighbor set of any build
+    int err;              // non-finite coordinates or an atom beyond the box
+    int pad;
 };
 
 // Streaming kernel of velocity Verlet (begin of step): v = S v + F dt/2m; x += v dt
@@ -234,9 +265,12 @@ struct BuildArgs {
     int* cell_start;      // [C + R]
     const int* cell_rank; // [C] linear cell -> curve rank (per replica block)
     int* slot_of;         // [N]
-    uint32_t* nlist;
+    uint16_t* nl16;
     int* ncount;
+    uint32_t* jset;
+    int* jcount;
     int* tilemax;         // [S/64] longest row per 64-slot tile over the builds since the host cleared it
+    int* jmax;            // [S/64] largest neighbor set per tile over the same builds
     const int4* excl;     // [N] 1-2 partners (-1 empty)
     RepState* st;
     Gate* gate;
@@ -245,11 +279,10 @@ struct BuildArgs {
     int nslot;
     int nrep;
     int cap;
+    int jcap;
     float rl2;
     int has_excl;
-    int build_lanes;      // lanes per atom in k_nl_build (4, 8)
-    int row_g;            // lanes per atom of the step kernel (row layout, lpos)
-    int build_kernel;     // 0: k_nl_build (tile-staged), 1: k_nl_wbuild (per wavefront), 2/3: k_nl_jbuild
+    int row_g;            // lanes per atom of the step kernel (row layout, lpos16)
     int lfgate;           // the step kernel is in leapfrog form: n_run counts every valid replica short of
                           // its target, whatever its phase
 };

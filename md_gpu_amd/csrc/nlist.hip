@@ -8,25 +8,24 @@
 // (RepState.trip, raised by the position update when some atom moved by at
 // least skin/2 since the last build: the skin criterion of
 // src/neighborlist.cu:157-176), so the chain can sit in every graph segment:
-//   k_nl_bin      cell of every atom (cells are ranked along a Hilbert curve),
-//                 arrival rank in the cell by atomic count
+//   k_nl_bin      wrap every coordinate into the box, cell of every atom (cells
+//                 are ranked along a Hilbert curve), arrival rank in the cell
 //   k_nl_scan     exclusive scan of cell counts -> cell_start (one block per replica)
 //   k_nl_scatter  atom -> provisional slot (cell order, arrival order)
 //   k_nl_permute  deterministic order inside a cell (by eighth of the cell, then
-//                 original index);
-//                 gathers pos/vel/force/id/type into scratch in the new order
-//   build         one workgroup per 64-slot tile; the image cells within r_list
-//                 of a bounding box of atoms are listed in LDS and their members
-//                 streamed and tested with the canonical FP32 predicate.  Three
-//                 variants (build_kernel): k_nl_build (LPA lanes per atom, tile
-//                 box, double-buffered block staging), k_nl_wbuild (8 lanes per
-//                 atom, per-wavefront boxes, LDS hit buffers) and k_nl_jbuild
-//                 (a lane per candidate, 16 or 8 atoms per wavefront, one ballot
-//                 per atom and chunk).  Hits are appended in candidate order, so
-//                 rows are deterministic.  Each tile stores its longest row.
+//                 original index); gathers pos/vel/force/id/type into scratch in the
+//                 new order, positions wrapped into the box
+//   k_nl_tbuild   one workgroup per 64-slot tile: the image cells within r_list of
+//                 the tile's bounding box are listed in LDS, their members streamed
+//                 through an LDS ring and tested against the tile's atoms with the
+//                 canonical FP32 predicate.  Candidates that some atom of the tile
+//                 hits form the tile's neighbor set (in candidate order); each atom's
+//                 row holds 16-bit indices into that set, in candidate order, so rows
+//                 are deterministic.  Each tile stores its longest row and set size.
 //   k_nl_commit   per rebuilt replica: new list epoch (or the next step kernel opens it).
 // Bytes per atom per rebuild (algorithmic): read pos/vel/force/id/type 56 B,
-// write them twice (scratch + copy back) 112 B, plus the list 4*K B.
+// write them twice (scratch + copy back) 112 B, plus the rows 2*K B and the
+// tile sets (4 B per set entry, about 4*K/16 B per atom).
 #include "md_device.h"
 
 namespace {
@@ -64,8 +63,10 @@ struct Axis {
 __device__ __forceinline__ Axis make_axis(float bmin, float bmax, float rlp, float halfL, float cinv,
                                           float cedge, int nc) {
     Axis x;
-    const int l = max((int)floorf((bmin + halfL - rlp) * cinv), -2 * nc);
-    const int h = min((int)floorf((bmax + halfL + rlp) * cinv), 3 * nc);
+    // image offsets beyond one box length never hold a candidate (r_list < L/2),
+    // and the bounds keep the walk finite for any input
+    const int l = max((int)floorf((bmin + halfL - rlp) * cinv), -nc);
+    const int h = min((int)floorf((bmax + halfL + rlp) * cinv), 2 * nc - 1);
     x.lo = l;
     x.nr = max(h - l + 1, 0);
     x.nc = nc;
@@ -121,21 +122,24 @@ __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
     const RepParam& p = a.rep[r];
     const float h0 = p.halfL[0], h1 = p.halfL[1], h2 = p.halfL[2];
     const float c0 = p.cinv[0], c1 = p.cinv[1], c2 = p.cinv[2];
+    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
     const int n0 = p.nc[0], n1 = p.nc[1], n2 = p.nc[2], cell_off = p.cell_off, slot_off = p.slot_off;
     loads_issued();
     if (!stale(st)) return;
     if (s == slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
     if (id < 0) return;
-    float4 x = cur_buf(st) ? x1 : x0;
-    int cx = cell_coord(x.x, h0, c0, n0);
-    int cy = cell_coord(x.y, h1, c1, n1);
-    int cz = cell_coord(x.z, h2, c2, n2);
+    const float4 xr = cur_buf(st) ? x1 : x0;
+    // the wrapped position k_nl_permute stores (same operations)
+    const float x = wrap_coord(xr.x, L0, Li0), y = wrap_coord(xr.y, L1, Li1), z = wrap_coord(xr.z, L2, Li2);
+    int cx = cell_coord(x, h0, c0, n0);
+    int cy = cell_coord(y, h1, c1, n1);
+    int cz = cell_coord(z, h2, c2, n2);
     int lin = cx + n0 * (cy + n1 * cz);
     int gc = cell_off + a.cell_rank[cell_off + lin];
     // the atom's eighth of the cell, numbered along a path through adjacent eighths
-    const int sx = (x.x + h0) * c0 - (float)cx >= 0.5f;
-    const int sy = (x.y + h1) * c1 - (float)cy >= 0.5f;
-    const int sz = (x.z + h2) * c2 - (float)cz >= 0.5f;
+    const int sx = (x + h0) * c0 - (float)cx >= 0.5f;
+    const int sy = (y + h1) * c1 - (float)cy >= 0.5f;
+    const int sz = (z + h2) * c2 - (float)cz >= 0.5f;
     const int sub = (0x54672310 >> (4 * (sx | (sy << 1) | (sz << 2)))) & 7;
     a.key[s] = gc;
     a.rank[s] = atomicAdd(&a.cell_count[gc], 1) | (sub << MD_SUBSHIFT);
@@ -216,6 +220,7 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     const RepState st = a.st[r];
     const RepParam& p = a.rep[r];
     const int lim = p.slot_off + p.natom;
+    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
     const int o = min(max(o_raw, 0), a.nslot - 1);
     const int gc = a.key[o];
     const float4 x0 = a.pos[0][o], x1 = a.pos[1][o], v = a.vel[o], f4 = a.force[o];
@@ -223,7 +228,8 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     loads_issued();
     if (!stale(st)) return;
     if (t >= lim) return;
-    const float4 x = cur_buf(st) ? x1 : x0;
+    const float4 xr = cur_buf(st) ? x1 : x0;
+    const float4 x = make_float4(wrap_coord(xr.x, L0, Li0), wrap_coord(xr.y, L1, Li1), wrap_coord(xr.z, L2, Li2), xr.w);
     const int myid = mykey & ((1 << MD_SUBSHIFT) - 1);
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];
     int cnt = 0;
@@ -238,472 +244,46 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     a.slot_of[myid] = f;
 }
 
-// One workgroup per 64-slot tile, LPA lanes per atom (LPA * 64 threads).
-template <bool EXCL, int LPA>
-__global__ __launch_bounds__(64 * LPA) void k_nl_build(BuildArgs a) {
-    constexpr int NT = 64 * LPA;          // threads = candidates staged per chunk
-    constexpr int NW = NT / 64;           // waves
-    constexpr int APW = 64 / LPA;         // atoms per wave
-    constexpr int PER = kTab / NT;        // region cells probed per thread and batch
-    __shared__ float4 s_pos[2][NT];       // (xj, yj, zj, bits of j | tj << 29)
-    __shared__ float4 s_sh[2][NT];        // (-kx L, -ky L, -kz L, bits of jid): image shift of the candidate
-    __shared__ int s_tstart[kTab];
-    __shared__ int s_tpref[kTab + 1];
-    __shared__ int s_tshift[kTab];
+// Tile neighbor-set build: one workgroup of 8 wavefronts per 64-slot tile, 8
+// atoms per wavefront (held as wavefront-uniform values).
+//  1. The image cells within r_list of the tile's bounding box are listed in LDS
+//     (region order, two-pass block scan), in batches of kTab region cells.
+//  2. Their members are streamed 512 per chunk (a thread per candidate); those
+//     whose image lies r_list or farther from the box are dropped, the rest are
+//     appended to an LDS ring in candidate order (stable block compaction).
+//  3. Every 64 ring entries, each wavefront tests the 64 candidates (a lane each)
+//     against its 8 atoms with the canonical predicate; a candidate that hits any
+//     atom of the tile gets the next local index of the tile's set (the OR of the
+//     wavefronts' hit masks goes through LDS, one barrier per 64 candidates), and
+//     each atom's hits are appended to its row at count + (hits in lower lanes).
+// So set and rows follow candidate order and do not depend on scheduling.
+template <bool EXCL>
+__global__ __launch_bounds__(512) void k_nl_tbuild(BuildArgs a) {
+    constexpr int NT = 512, NW = 8, AW = 8, RING = 1024, PER = kTab / NT;
+    __shared__ int s_tstart[kTab];        // first slot of the cell
+    __shared__ int s_tpref[kTab + 1];     // first candidate index of each table row; [ntab] = total
+    __shared__ int s_tcode[kTab];         // image shift code of the cell
+    __shared__ float4 s_cx[RING];         // ring: candidate position (unshifted), w = bits of the set entry
+    __shared__ int s_cid[EXCL ? RING : 1];   // original index of the candidate (exclusions)
+    __shared__ float4 s_shv[27];          // -k L per shift code (canonical displacement (xi - xj) + s)
+    __shared__ float4 s_ax[64];           // the tile's atoms (x, y, z, bits of the id)
+    __shared__ int4 s_ex[EXCL ? 64 : 1];
+    __shared__ unsigned long long s_any[2][NW];
+    __shared__ float s_bb[6];
     __shared__ int s_w[NW];
     __shared__ int s_wmax[NW];
-    __shared__ int s_n[2];               // staged candidates per buffer
-    __shared__ unsigned short s_widx[NW][NT];   // per wave: staged candidates near the wave's box
-    __shared__ float s_bb[NW][6];
-    if (a.gate->n_stale == 0) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
-    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
-    const RepState st = a.st[r];
-    if (!stale(st)) return;
-    const RepParam& p = a.rep[r];
-    const int cb = cur_buf(st);
-    const int ai = wave * APW + lane / LPA;     // atom of this lane within the tile
-    const int sub = lane % LPA;
-    const int slot = tile * 64 + ai;
-    const int id = a.ids_s[slot];
-    const bool active = id >= 0;
-    const float4 xi = a.pos_s[slot];
-    int4 ex = make_int4(-1, -1, -1, -1);
-    if (EXCL && active) ex = a.excl[id];
-
-    // tile bounding box over active atoms
-    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
-    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
-    for (int o = 32; o >= LPA; o >>= 1) {
-        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
-        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
-    }
-    // the wave's own box (its APW atoms) prunes candidates once more before the pair tests
-    const float wlx = lx, wly = ly, wlz = lz, whx = hx, why = hy, whz = hz;
-    if (lane == 0) {
-        s_bb[wave][0] = lx; s_bb[wave][1] = ly; s_bb[wave][2] = lz;
-        s_bb[wave][3] = hx; s_bb[wave][4] = hy; s_bb[wave][5] = hz;
-    }
-    __syncthreads();
-    for (int w = 0; w < NW; w++) {
-        lx = fminf(lx, s_bb[w][0]); ly = fminf(ly, s_bb[w][1]); lz = fminf(lz, s_bb[w][2]);
-        hx = fmaxf(hx, s_bb[w][3]); hy = fmaxf(hy, s_bb[w][4]); hz = fmaxf(hz, s_bb[w][5]);
-    }
-    int cnt = 0;
-    uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
-    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
-    if (lx <= hx) {
-        const float rlp = sqrtf(a.rl2) * 1.0001f + 1e-4f;
-        const float rlp2 = rlp * rlp;
-        // image-cell region covering the tile box grown by r_list.  Coordinates lie in
-        // the box, so a span of more than five box lengths never occurs; the bounds
-        // keep the walk finite for any input.
-        const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
-        const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
-        const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
-        const int nreg = ax.nr * ay.nr * az.nr;
-        const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
-        // region cell c -> (cell start, member count, packed image shift); count 0 when pruned or empty
-        auto probe = [=](int c, int& cs, int& shift) -> int {
-            int kx, ky, kz, wx, wy, wz;
-            const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
-            const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
-            const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
-            const float ddz = az.gap(az.lo + cz, kz, wz);
-            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
-            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
-            cs = a.cell_start[cs_off + rank];
-            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
-            return a.cell_start[cs_off + rank + 1] - cs;
-        };
-        for (int b0 = 0; b0 < nreg; b0 += kTab) {
-            // ---- table of non-empty image cells within r_list of the box (region order);
-            //      pass 1 counts, pass 2 writes (no per-thread arrays)
-            const int c0 = b0 + tid * PER, c1 = min(min(c0 + PER, nreg), b0 + kTab);
-            int n_keep = 0, csum = 0;
-            for (int c = c0; c < c1; c++) {
-                int cs, sh;
-                const int n = probe(c, cs, sh);
-                n_keep += n > 0;
-                csum += n;
-            }
-            int ntab, T;
-            const int off = block_scan<NT>(n_keep, s_w, ntab);
-            const int coff = block_scan<NT>(csum, s_w, T);
-            for (int c = c0, k = off, run = coff; c < c1; c++) {
-                int cs, sh;
-                const int n = probe(c, cs, sh);
-                if (n > 0) {
-                    s_tstart[k] = cs;
-                    s_tpref[k] = run;
-                    s_tshift[k] = sh;
-                    k++;
-                    run += n;
-                }
-            }
-            if (tid == 0) s_tpref[ntab] = T;
-            __syncthreads();
-            // ---- stream the candidates through LDS.  A candidate whose image lies
-            //      r_list or farther from the tile box cannot pass for any atom of the
-            //      tile: it is dropped before staging (stable compaction, so rows keep
-            //      candidate order).
-            auto fetch = [&](int base, float4& rp, float4& rs) -> bool {
-                const int q = base + tid;
-                if (q >= T) return false;
-                int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
-                while (lo_ < hi_) {
-                    const int mid = (lo_ + hi_ + 1) >> 1;
-                    if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
-                }
-                const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
-                const int sh = s_tshift[lo_];
-                rp = a.pos_s[j];
-                // -k L is exact for the |k| <= 1 images that can pass the predicate
-                rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
-                                 -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
-                const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
-                const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
-                const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
-                const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
-                rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
-                return gx * gx + gy * gy + gz * gz < rlp2;
-            };
-            auto stage = [&](int buf, bool keep, const float4& rp, const float4& rs) {
-                int kept;
-                const int at = block_scan<NT>(keep ? 1 : 0, s_w, kept);
-                if (keep) {
-                    s_pos[buf][at] = rp;
-                    s_sh[buf][at] = rs;
-                }
-                if (tid == 0) s_n[buf] = kept;
-            };
-            auto test = [&](int buf, int k, int wn, uint32_t& ent) -> bool {
-                if (k >= wn) return false;
-                const int q = s_widx[wave][k];
-                const float4 pj = s_pos[buf][q];
-                const float4 sj = s_sh[buf][q];
-                // (xi - xj) - k L: the canonical minimum-image displacement for this image
-                const float dx = (xi.x - pj.x) + sj.x;
-                const float dy = (xi.y - pj.y) + sj.y;
-                const float dz = (xi.z - pj.z) + sj.z;
-                const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                ent = (uint32_t)__float_as_int(pj.w);
-                bool ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
-                if (EXCL) {
-                    const int jid = __float_as_int(sj.w);
-                    ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
-                }
-                return ok;
-            };
-            // append in candidate order: rank among this atom's LPA lanes
-            auto append = [&](bool ok, uint32_t ent) {
-                const unsigned long long m = __ballot(ok);
-                const uint32_t gb = (uint32_t)(m >> (lane & ~(LPA - 1))) & (uint32_t)((1ull << LPA) - 1);
-                const int before = __popc(gb & ((1u << sub) - 1u));
-                if (ok && cnt + before < a.cap) rowp[lpos_sh(cnt + before, gs)] = ent;
-                cnt += __popc(gb);
-            };
-            const int nch = (T + NT - 1) / NT;
-            float4 rp = make_float4(0, 0, 0, 0), rs = make_float4(0, 0, 0, 0);
-            if (nch > 0) {
-                const bool keep = fetch(0, rp, rs);
-                stage(0, keep, rp, rs);
-            }
-            __syncthreads();
-            for (int ch = 0; ch < nch; ch++) {
-                const int buf = ch & 1;
-                const bool more = ch + 1 < nch;
-                bool keep_next = false;
-                if (more) keep_next = fetch((ch + 1) * NT, rp, rs);
-                const int n = s_n[buf];
-                // this wave's candidates: staged ones within r_list of the wave's box (in order)
-                int wn = 0;
-                for (int c0 = 0; c0 < n; c0 += 64) {
-                    const int c = c0 + lane;
-                    bool near = false;
-                    if (c < n) {
-                        const float4 pj = s_pos[buf][c];
-                        const float4 sj = s_sh[buf][c];
-                        const float ix = pj.x - sj.x, iy = pj.y - sj.y, iz = pj.z - sj.z;
-                        const float gx = fmaxf(0.0f, fmaxf(wlx - ix, ix - whx));
-                        const float gy = fmaxf(0.0f, fmaxf(wly - iy, iy - why));
-                        const float gz = fmaxf(0.0f, fmaxf(wlz - iz, iz - whz));
-                        near = gx * gx + gy * gy + gz * gz < rlp2;
-                    }
-                    const unsigned long long m = __ballot(near);
-                    if (near) s_widx[wave][wn + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)c;
-                    wn += __popcll(m);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (int u = 0; u < wn; u += 2 * LPA) {
-                    uint32_t e0 = 0, e1 = 0;
-                    const bool ok0 = test(buf, u + sub, wn, e0);
-                    const bool ok1 = test(buf, u + LPA + sub, wn, e1);
-                    append(ok0, e0);
-                    append(ok1, e1);
-                }
-                __syncthreads();
-                if (more) stage(buf ^ 1, keep_next, rp, rs);
-                __syncthreads();
-            }
-        }
-    }
-    if (sub == 0) a.ncount[slot] = active ? cnt : 0;
-    int m = sub == 0 ? cnt : 0;
-    for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    if (lane == 0) s_wmax[wave] = m;
-    // copy the tile back from scratch into the primary arrays (new order)
-    if (tid < 64) {
-        const int s = tile * 64 + tid;
-        const float4 x = a.pos_s[s];
-        a.pos[cb][s] = x;
-        a.pos_build[s] = x;
-        a.vel[s] = a.vel_s[s];
-        a.force[s] = a.force_s[s];
-        a.ids[s] = a.ids_s[s];
-        a.typ[s] = a.typ_s[s];
-    }
-    __syncthreads();
-    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
-        int m = s_wmax[0];
-        for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
-    }
-}
-
-// Per-wave list build: one wavefront per APW = 64/LPA consecutive slots.  Each
-// wave lists the image cells within r_list of its own atoms' bounding box (a
-// wave-private table in LDS), streams their members in chunks of 64 (one
-// candidate per lane; the next chunk's loads are in flight while the current
-// one is tested), drops candidates whose image is r_list or farther from the
-// box, tests the rest with the canonical predicate and appends hits in
-// candidate order.  Hits are gathered per atom in LDS and written to the row
-// in bursts.  Waves never wait for each other until the tile's epilogue.
-template <bool EXCL>
-__global__ __launch_bounds__(512) void k_nl_wbuild(BuildArgs a) {
-    constexpr int LPA = 8, APW = 64 / LPA, NW = 8;   // 8 waves x 8 atoms = one 64-slot tile
-    constexpr int WT = 128;                         // table entries per wave and pass
-    constexpr int HB = 64;                          // buffered hits per atom
-    __shared__ int s_tstart[NW][WT];
-    __shared__ int s_tpref[NW][WT + 1];
-    __shared__ int s_tshift[NW][WT];
-    __shared__ float4 s_cpos[NW][64];               // staged candidates (x, y, z, bits of j | tj << 29)
-    __shared__ float4 s_csh[NW][64];                // (-kx L, -ky L, -kz L, bits of jid)
-    __shared__ uint32_t s_hits[NW][APW][HB];
-    __shared__ int s_wmax[NW];
-    if (a.gate->n_stale == 0) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tile = blockIdx.x;
-    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
-    const RepState st = a.st[r];
-    if (!stale(st)) return;
-    const RepParam& p = a.rep[r];
-    const int cb = cur_buf(st);
-    const int grp = lane / LPA, sub = lane % LPA;
-    const int slot = tile * 64 + wave * APW + grp;
-    const int id = a.ids_s[slot];
-    const bool active = id >= 0;
-    const float4 xi = a.pos_s[slot];
-    int4 ex = make_int4(-1, -1, -1, -1);
-    if (EXCL && active) ex = a.excl[id];
-    const unsigned long long below = (1ull << lane) - 1ull;
-    auto wave_sync = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-
-    // the wave's box over its active atoms
-    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
-    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
-    for (int o = 32; o >= LPA; o >>= 1) {
-        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
-        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
-    }
-    int cnt = 0;        // hits of this lane's atom (all lanes of the group agree)
-    int nbuf = 0;       // buffered, not yet written
-    uint32_t* rowp = a.nlist + (size_t)slot * a.cap;
-    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
-    // write the buffered hits of every atom of the wave to the rows (lanes of a group
-    // write consecutive logical entries)
-    auto flush = [&]() {
-        wave_sync();
-        const int base = cnt - nbuf;
-        for (int k = sub; k < nbuf; k += LPA) {
-            const int e = base + k;
-            if (e < a.cap) rowp[lpos_sh(e, gs)] = s_hits[wave][grp][k];
-        }
-        nbuf = 0;
-        wave_sync();
-    };
-    if (lx <= hx) {
-        const float rlp = sqrtf(a.rl2) * 1.0001f + 1e-4f;
-        const float rlp2 = rlp * rlp;
-        const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
-        const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
-        const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
-        const int nreg = ax.nr * ay.nr * az.nr;
-        const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
-        auto probe = [=](int c, int& cs, int& shift) -> int {
-            int kx, ky, kz, wx, wy, wz;
-            const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
-            const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
-            const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
-            const float ddz = az.gap(az.lo + cz, kz, wz);
-            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
-            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
-            cs = a.cell_start[cs_off + rank];
-            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
-            return a.cell_start[cs_off + rank + 1] - cs;
-        };
-        int c0 = 0;
-        while (c0 < nreg) {
-            // ---- wave table: up to WT non-empty cells, in region order
-            int ntab = 0, T = 0;
-            while (c0 < nreg && ntab <= WT - 64) {
-                const int c = c0 + lane;
-                int cs = 0, sh = 0, n = 0;
-                if (c < nreg) n = probe(c, cs, sh);
-                const unsigned long long m = __ballot(n > 0);
-                int inc = n;   // inclusive scan of counts over the wave
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int t = __shfl_up(inc, o, 64);
-                    if (lane >= o) inc += t;
-                }
-                if (n > 0) {
-                    const int k = ntab + __popcll(m & below);
-                    s_tstart[wave][k] = cs;
-                    s_tpref[wave][k] = T + inc - n;
-                    s_tshift[wave][k] = sh;
-                }
-                ntab += __popcll(m);
-                T += __shfl(inc, 63, 64);
-                c0 += 64;
-            }
-            if (lane == 0) s_tpref[wave][ntab] = T;
-            wave_sync();
-            // ---- stream the table's candidates, 64 per chunk
-            auto fetch = [&](int q, float4& rp, float4& rs) -> bool {
-                if (q >= T) return false;
-                int lo_ = 0, hi_ = ntab - 1;   // last entry with pref <= q
-                while (lo_ < hi_) {
-                    const int mid = (lo_ + hi_ + 1) >> 1;
-                    if (s_tpref[wave][mid] <= q) lo_ = mid; else hi_ = mid - 1;
-                }
-                const int j = s_tstart[wave][lo_] + (q - s_tpref[wave][lo_]);
-                const int sh = s_tshift[wave][lo_];
-                rp = a.pos_s[j];
-                // -k L is exact for the |k| <= 1 images that can pass the predicate
-                rs = make_float4(-(float)((sh & 15) - 8) * p.L[0], -(float)(((sh >> 4) & 15) - 8) * p.L[1],
-                                 -(float)(((sh >> 8) & 15) - 8) * p.L[2], EXCL ? __int_as_float(a.ids_s[j]) : 0.0f);
-                const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
-                const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
-                const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
-                const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
-                rp.w = __int_as_float((int)((uint32_t)j | ((uint32_t)a.typ_s[j] << MD_JBITS)));
-                return gx * gx + gy * gy + gz * gz < rlp2;
-            };
-            float4 rp = make_float4(0, 0, 0, 0), rs = rp;
-            bool keep = fetch(lane, rp, rs);
-            for (int q0 = 0; q0 < T; q0 += 64) {
-                // stage this chunk's survivors (stable compaction), then fetch the next chunk
-                const unsigned long long m = __ballot(keep);
-                if (keep) {
-                    const int k = __popcll(m & below);
-                    s_cpos[wave][k] = rp;
-                    s_csh[wave][k] = rs;
-                }
-                const int wn = __popcll(m);
-                keep = fetch(q0 + 64 + lane, rp, rs);
-                wave_sync();
-                for (int u = 0; u < wn; u += LPA) {
-                    const int q = u + sub;
-                    bool ok = false;
-                    uint32_t ent = 0;
-                    if (q < wn) {
-                        const float4 pj = s_cpos[wave][q];
-                        const float4 sj = s_csh[wave][q];
-                        // (xi - xj) - k L: the canonical minimum-image displacement for this image
-                        const float dx = (xi.x - pj.x) + sj.x;
-                        const float dy = (xi.y - pj.y) + sj.y;
-                        const float dz = (xi.z - pj.z) + sj.z;
-                        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        ent = (uint32_t)__float_as_int(pj.w);
-                        ok = active && rsq < a.rl2 && (int)(ent & MD_JMASK) != slot;
-                        if (EXCL) {
-                            const int jid = __float_as_int(sj.w);
-                            ok = ok && jid != ex.x && jid != ex.y && jid != ex.z && jid != ex.w;
-                        }
-                    }
-                    // rank among this atom's lanes: hits stay in candidate order
-                    const unsigned long long hm = __ballot(ok);
-                    const uint32_t gb = (uint32_t)(hm >> (lane & ~(LPA - 1))) & 0xFFu;
-                    if (ok) s_hits[wave][grp][nbuf + __popc(gb & ((1u << sub) - 1u))] = ent;
-                    const int add = __popc(gb);
-                    cnt += add;
-                    nbuf += add;
-                    // some atom's buffer may not take another LPA hits: write all out
-                    const unsigned long long full = __ballot(nbuf > HB - LPA);
-                    if (full) flush();
-                }
-                wave_sync();
-            }
-        }
-    }
-    if (__ballot(nbuf > 0)) flush();   // wave-uniform
-    if (sub == 0) a.ncount[slot] = active ? cnt : 0;
-    int mx = sub == 0 ? cnt : 0;
-    for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) s_wmax[wave] = mx;
-    __syncthreads();
-    // copy the tile back from scratch into the primary arrays (new order)
-    if (tid < 64) {
-        const int s = tile * 64 + tid;
-        const float4 x = a.pos_s[s];
-        a.pos[cb][s] = x;
-        a.pos_build[s] = x;
-        a.vel[s] = a.vel_s[s];
-        a.force[s] = a.force_s[s];
-        a.ids[s] = a.ids_s[s];
-        a.typ[s] = a.typ_s[s];
-    }
-    __syncthreads();
-    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
-        int m = s_wmax[0];
-        for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
-    }
-}
-
-// Candidate-parallel build: one workgroup (64 / AW wavefronts) per 64-slot tile,
-// AW (8 or 16) atoms per wavefront.  The wavefront lists the image cells within
-// r_list of its atoms' bounding box, streams their members, keeps those whose
-// image lies within r_list of the box (stable compaction into a 128-entry LDS
-// ring) and, per chunk of 64 kept candidates, lets each lane test one
-// candidate against the AW atoms in turn.  The hits of one atom in a chunk are
-// one ballot: they go to count + (hits in lower lanes), so rows are filled in
-// candidate order with coalesced stores and no per-hit LDS traffic.
-template <bool EXCL, int AW>
-__global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
-    constexpr int NW = 64 / AW, WT = 256, RQ = 128;
-    __shared__ float4 s_tcell[NW][WT];  // table row: (-kx L, -ky L, -kz L, bits of (cell start - pref))
-    __shared__ int s_tpref[NW][WT + 1]; // first candidate index of each row; [ntab] = total
-    __shared__ int s_mk[NW][64];        // row-boundary marks of a 64-candidate chunk (tagged)
-    __shared__ float4 s_qp[NW][RQ];     // ring of kept candidates (x, y, z, bits of j | tj << 29)
-    __shared__ float4 s_qs[NW][RQ];     // (-kx L, -ky L, -kz L, bits of jid)
-    __shared__ int4 s_ex[NW][EXCL ? AW : 1];
-    __shared__ int s_wmax[NW];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tile = blockIdx.x;
-    const int slot0 = tile * 64 + wave * AW;
-    // the gate, the replica index and the wavefront's atoms in one round trip
+    const int slot0 = tile * 64;
+    // the gate, the replica index and the tile's atoms in one round trip
     const int n_stale = a.gate->n_stale;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
-    const int id_pre = a.ids_s[slot0 + (lane & (AW - 1))];
-    const float4 x_pre = a.pos_s[slot0 + (lane & (AW - 1))];
+    int id_pre = -1;
+    float4 x_pre = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid < 64) {
+        id_pre = a.ids_s[slot0 + tid];
+        x_pre = a.pos_s[slot0 + tid];
+    }
     loads_issued();
     if (n_stale == 0) return;
     const RepState st = a.st[r];
@@ -711,220 +291,210 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
     if (!stale(st)) return;
     const RepParam& p = a.rep[r];
     const int cb = cur_buf(st);
-    const unsigned long long below = (1ull << lane) - 1ull;
-    s_mk[wave][lane] = 0;   // tags start at 1
-    auto wave_sync = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    bool active = false;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < AW) {
-        const int id = id_pre;
-        active = id >= 0;
-        xi = x_pre;
-        if (EXCL) s_ex[wave][lane] = active ? a.excl[id] : make_int4(-1, -1, -1, -1);
+    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2];
+    if (tid < 27) {
+        const int kx = tid % 3 - 1, ky = (tid / 3) % 3 - 1, kz = tid / 9 - 1;
+        s_shv[tid] = make_float4(-(float)kx * L0, -(float)ky * L1, -(float)kz * L2, 0.0f);
     }
-    // the wavefront's box over its active atoms
-    float lx = active ? xi.x : 3.0e38f, ly = active ? xi.y : 3.0e38f, lz = active ? xi.z : 3.0e38f;
-    float hx = active ? xi.x : -3.0e38f, hy = active ? xi.y : -3.0e38f, hz = active ? xi.z : -3.0e38f;
-    for (int o = 32; o >= 1; o >>= 1) {
-        lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
-        hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
-    }
-    wave_sync();
-    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
-    const int cap = a.cap;
-    const float rl2 = a.rl2;
-    uint32_t* const wrow = a.nlist + (size_t)slot0 * cap;   // row of the wavefront's first atom
-    // the atoms as wavefront-uniform values (scalar registers); an empty slot sits
-    // far outside the box, so no candidate passes the distance test for it
-    float axr[AW], ayr[AW], azr[AW];
-    int cnt[AW];
-    {
-        const float fx = active ? xi.x : 1.0e30f, fy = active ? xi.y : 1.0e30f, fz = active ? xi.z : 1.0e30f;
-#pragma unroll
-        for (int i = 0; i < AW; i++) {
-            axr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fx), i));
-            ayr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fy), i));
-            azr[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fz), i));
-            cnt[i] = 0;
+    if (tid < 64) {
+        const bool act = id_pre >= 0;
+        s_ax[tid] = make_float4(act ? x_pre.x : MD_FAR, act ? x_pre.y : MD_FAR, act ? x_pre.z : MD_FAR,
+                                __int_as_float(id_pre));
+        if (EXCL) s_ex[tid] = act ? a.excl[id_pre] : make_int4(-1, -1, -1, -1);
+        // tile bounding box over the active atoms
+        float lx = act ? x_pre.x : 3.0e38f, ly = act ? x_pre.y : 3.0e38f, lz = act ? x_pre.z : 3.0e38f;
+        float hx = act ? x_pre.x : -3.0e38f, hy = act ? x_pre.y : -3.0e38f, hz = act ? x_pre.z : -3.0e38f;
+        for (int o = 32; o >= 1; o >>= 1) {
+            lx = fminf(lx, __shfl_xor(lx, o, 64)); ly = fminf(ly, __shfl_xor(ly, o, 64)); lz = fminf(lz, __shfl_xor(lz, o, 64));
+            hx = fmaxf(hx, __shfl_xor(hx, o, 64)); hy = fmaxf(hy, __shfl_xor(hy, o, 64)); hz = fmaxf(hz, __shfl_xor(hz, o, 64));
+        }
+        if (tid == 0) {
+            s_bb[0] = lx; s_bb[1] = ly; s_bb[2] = lz;
+            s_bb[3] = hx; s_bb[4] = hy; s_bb[5] = hz;
         }
     }
+    __syncthreads();
+    const float lx = s_bb[0], ly = s_bb[1], lz = s_bb[2], hx = s_bb[3], hy = s_bb[4], hz = s_bb[5];
+    // this wavefront's atoms as wavefront-uniform values; an empty slot sits far
+    // outside the box, so no candidate passes the distance test for it
+    float axr[AW], ayr[AW], azr[AW];
+    int cnt[AW];
+#pragma unroll
+    for (int i = 0; i < AW; i++) {
+        const float4 q = s_ax[wave * AW + i];
+        axr[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.x)));
+        ayr[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.y)));
+        azr[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(q.z)));
+        cnt[i] = 0;
+    }
+    const int gs = __builtin_ctz(a.row_g);   // row layout: lanes per atom of the step kernel, a power of 2
+    const int cap = a.cap, jcap = a.jcap;
+    const float rl2 = a.rl2;
+    uint16_t* const wrow = a.nl16 + (size_t)(slot0 + wave * AW) * cap;   // row of the wavefront's first atom
+    uint32_t* const jset = a.jset + (size_t)tile * jcap;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int jn = 0;     // size of the tile's set so far (workgroup-uniform)
+    int head = 0, tail = 0, nproc = 0;   // ring positions (taken mod RING), sub-chunks processed
 
-    // test ring entries [head, head + n) against the wavefront's atoms
-    auto process = [&](int head, int n) {
-        const int q = (head + lane) & (RQ - 1);
+    // test ring entries [head, head + n) against this wavefront's atoms; assign set indices
+    auto process = [&](int n) {
+        const int q = (head + lane) & (RING - 1);
         const bool valid = lane < n;
-        const float4 pj = s_qp[wave][q];
-        const float4 sj = s_qs[wave][q];
-        const uint32_t ent = __float_as_uint(pj.w);
-        const int js = (int)(ent & MD_JMASK);
-        const int jid = __float_as_int(sj.w);
-        const int jrel = js - slot0;   // the candidate's offset from the wavefront's first atom
+        const float4 cx = s_cx[q];
+        const uint32_t info = __float_as_uint(cx.w);
+        const float4 sv = s_shv[valid ? (info >> MD_SLOTBITS) & 31 : MD_SHIFT0];   // ring data past n is stale
+        const int js = (int)(info & MD_SLOTMASK);
+        const int jid = EXCL ? s_cid[q] : 0;
+        unsigned long long m[AW];
+        unsigned long long any = 0;
 #pragma unroll
         for (int i = 0; i < AW; i++) {
             // (xi - xj) - k L: the canonical minimum-image displacement for this image
-            const float dx = (axr[i] - pj.x) + sj.x;
-            const float dy = (ayr[i] - pj.y) + sj.y;
-            const float dz = (azr[i] - pj.z) + sj.z;
+            const float dx = (axr[i] - cx.x) + sv.x;
+            const float dy = (ayr[i] - cx.y) + sv.y;
+            const float dz = (azr[i] - cx.z) + sv.z;
             const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool ok = valid & (rsq < rl2) & (jrel != i);
+            bool ok = valid & (rsq < rl2) & (js != slot0 + wave * AW + i);
             if (EXCL) {
-                const int4 ex = s_ex[wave][i];
+                const int4 ex = s_ex[wave * AW + i];
                 ok = ok & (jid != ex.x) & (jid != ex.y) & (jid != ex.z) & (jid != ex.w);
             }
-            const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
-            if (m) {
-                const uint32_t e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)cnt[i]));
-                if (ok && e < (uint32_t)cap) wrow[(uint32_t)(i * cap) + lpos_u(e, gs)] = ent;
-                cnt[i] += __popcll(m);
+            m[i] = __builtin_amdgcn_ballot_w64(ok);
+            any |= m[i];
+        }
+        const int par = nproc & 1;
+        if (lane == 0) s_any[par][wave] = any;
+        __syncthreads();
+        any = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) any |= s_any[par][w];
+        const bool mine = (any >> lane) & 1ull;
+        const int li = jn + 1 + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(any >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
+        if (wave == 0 && mine && li - 1 < jcap) jset[li - 1] = info;
+        // a set that outgrew its capacity is rebuilt after the host grows it (the run
+        // is rolled back); until then its rows point past it only at the dummy
+        const uint16_t lis = (uint16_t)(li - 1 < jcap ? li : 0);
+#pragma unroll
+        for (int i = 0; i < AW; i++) {
+            if (m[i]) {
+                const bool hit = (m[i] >> lane) & 1ull;
+                const uint32_t e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[i] >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m[i], (uint32_t)cnt[i]));
+                if (hit && e < (uint32_t)cap) wrow[(uint32_t)(i * cap) + lpos16_u(e, gs)] = lis;
+                cnt[i] += __popcll(m[i]);
             }
         }
+        jn += __popcll(any);
+        head += n;
+        nproc++;
     };
 
     if (lx <= hx) {
         const float rlp = sqrtf(rl2) * 1.0001f + 1e-4f;
         const float rlp2 = rlp * rlp;
+        // image-cell region covering the tile box grown by r_list
         const Axis ax = make_axis(lx, hx, rlp, p.halfL[0], p.cinv[0], p.cedge[0], p.nc[0]);
         const Axis ay = make_axis(ly, hy, rlp, p.halfL[1], p.cinv[1], p.cedge[1], p.nc[1]);
         const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
         const int nreg = ax.nr * ay.nr * az.nr;
         const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
-        // Cell probe without branches around the loads, so the loads of a group of
-        // probes are in flight together: a cell beyond r_list of the box reads
-        // entry 0 and reports no members.
-        auto probe = [=](int c, int& cs, int& shift) -> int {
+        // region cell c -> (cell start, member count, shift code); count 0 when beyond r_list or empty
+        auto probe = [=](int c, int& cs, int& code) -> int {
             int kx, ky, kz, wx, wy, wz;
             const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
             const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
             const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
             const float ddz = az.gap(az.lo + cz, kz, wz);
-            const bool in = c < nreg && ddx * ddx + ddy * ddy + ddz * ddz < rlp2;
-            const int rank = a.cell_rank[in ? cell_off + wx + ncx * (wy + ncy * wz) : cell_off];
+            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
+            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
             cs = a.cell_start[cs_off + rank];
-            const int ce = a.cell_start[cs_off + rank + 1];
-            shift = (kx + 8) | ((ky + 8) << 4) | ((kz + 8) << 8);
-            return in ? ce - cs : 0;
+            code = shift_code(kx, ky, kz);
+            return a.cell_start[cs_off + rank + 1] - cs;
         };
-        // Row of candidate q0 + lane, given row k_lo holding q0: the rows starting
-        // inside the chunk mark their start position, and a lane's row is k_lo plus
-        // the marks at or below it.  Also advances k_lo to the row holding q0 + 64.
-        int ktag = 0;
-        auto locate = [&](int q0, int& k_lo, int ntab) -> int {
-            ++ktag;
-            const int kk = k_lo + 1 + lane;
-            if (kk < ntab) {
-                const int rel = s_tpref[wave][kk] - q0;   // >= 1: row k_lo holds q0
-                if (rel < 64) s_mk[wave][rel] = ktag;
+        for (int b0 = 0; b0 < nreg; b0 += kTab) {
+            // ---- table of non-empty image cells within r_list of the box (region order);
+            //      pass 1 counts, pass 2 writes
+            const int c0 = b0 + tid * PER, c1 = min(min(c0 + PER, nreg), b0 + kTab);
+            int n_keep = 0, csum = 0;
+            for (int c = c0; c < c1; c++) {
+                int cs, cd;
+                const int n = probe(c, cs, cd);
+                n_keep += n > 0;
+                csum += n;
             }
-            wave_sync();
-            const unsigned long long M = __ballot(s_mk[wave][lane] == ktag);
-            const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-            const int k = k_lo + __popcll(M & upto);
-            const int k63 = k_lo + __popcll(M);
-            k_lo = (k63 + 1 < ntab && s_tpref[wave][k63 + 1] <= q0 + 64) ? k63 + 1 : k63;
-            return k;
-        };
-        // candidate q0 + lane: its row, then the loads (consumed one chunk later, so
-        // they overlap the chunk before)
-        struct Cand { float4 p, s; int typ, id, j; bool valid; };
-        auto fetch = [&](int q0, int& k_lo, int ntab, int T) -> Cand {
-            Cand c;
-            const int q = q0 + lane;
-            c.valid = q < T;
-            const int k = locate(q0, k_lo, ntab);
-            c.s = s_tcell[wave][c.valid ? k : 0];
-            c.j = c.valid ? __float_as_int(c.s.w) + q : slot0;
-            c.p = a.pos_s[c.j];
-            c.typ = a.typ_s[c.j];
-            c.id = EXCL ? a.ids_s[c.j] : 0;
-            return c;
-        };
-        int head = 0, tail = 0;   // ring positions (wavefront-uniform, taken mod RQ)
-        int c0 = 0;
-        while (c0 < nreg) {
-            // ---- wavefront table: up to WT non-empty cells, in region order, probed
-            //      in groups of four 64-cell batches
-            int ntab = 0, T = 0;
-            while (c0 < nreg && ntab <= WT - 256) {
-                int cs[4], sh[4], n[4];
-#pragma unroll
-                for (int b = 0; b < 4; b++) n[b] = probe(c0 + 64 * b + lane, cs[b], sh[b]);
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const unsigned long long m = __ballot(n[b] > 0);
-                    int inc = n[b];   // inclusive scan of counts over the wavefront
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int t = __shfl_up(inc, o, 64);
-                        if (lane >= o) inc += t;
-                    }
-                    if (n[b] > 0) {
-                        const int k = ntab + __popcll(m & below);
-                        const int pref = T + inc - n[b];
-                        // -k L is exact for the image offsets of the region
-                        s_tcell[wave][k] = make_float4(-(float)((sh[b] & 15) - 8) * p.L[0],
-                                                       -(float)(((sh[b] >> 4) & 15) - 8) * p.L[1],
-                                                       -(float)(((sh[b] >> 8) & 15) - 8) * p.L[2],
-                                                       __int_as_float(cs[b] - pref));
-                        s_tpref[wave][k] = pref;
-                    }
-                    ntab += __popcll(m);
-                    T += __shfl(inc, 63, 64);
+            int ntab, T;
+            const int off = block_scan<NT>(n_keep, s_w, ntab);
+            const int coff = block_scan<NT>(csum, s_w, T);
+            for (int c = c0, k = off, run = coff; c < c1; c++) {
+                int cs, cd;
+                const int n = probe(c, cs, cd);
+                if (n > 0) {
+                    s_tstart[k] = cs;
+                    s_tpref[k] = run;
+                    s_tcode[k] = cd;
+                    k++;
+                    run += n;
                 }
-                c0 += 256;
             }
-            if (lane == 0) s_tpref[wave][ntab] = T;
-            wave_sync();
-            if (T == 0) continue;
-            // ---- stream the table's candidates, 64 at a time, through the ring
-            int k_lo = 0;
-            Cand nx = fetch(0, k_lo, ntab, T);
-            for (int q0 = 0; q0 < T; q0 += 64) {
-                const Cand cu = nx;
-                if (q0 + 64 < T) nx = fetch(q0 + 64, k_lo, ntab, T);   // wavefront-uniform test
+            if (tid == 0) s_tpref[ntab] = T;
+            __syncthreads();
+            // ---- stream the candidates, NT per chunk, into the ring (stable compaction)
+            for (int base = 0; base < T; base += NT) {
+                const int q = base + tid;
                 bool keep = false;
-                float4 rp = cu.p, rs = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (cu.valid) {
-                    rs = make_float4(cu.s.x, cu.s.y, cu.s.z, __int_as_float(cu.id));
-                    const float ix = rp.x - rs.x, iy = rp.y - rs.y, iz = rp.z - rs.z;   // image position
+                float4 cx = make_float4(0.f, 0.f, 0.f, 0.f);
+                int cid = 0;
+                if (q < T) {
+                    int lo_ = 0, hi_ = ntab - 1;   // last table row with pref <= q
+                    while (lo_ < hi_) {
+                        const int mid = (lo_ + hi_ + 1) >> 1;
+                        if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
+                    }
+                    const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
+                    const int code = s_tcode[lo_];
+                    cx = a.pos_s[j];
+                    if (EXCL) cid = a.ids_s[j];
+                    const float4 sv = s_shv[code];
+                    const float ix = cx.x - sv.x, iy = cx.y - sv.y, iz = cx.z - sv.z;   // image position
                     const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
                     const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
                     const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
-                    rp.w = __int_as_float((int)((uint32_t)cu.j | ((uint32_t)cu.typ << MD_JBITS)));
                     keep = gx * gx + gy * gy + gz * gz < rlp2;
+                    cx.w = __uint_as_float((uint32_t)j | ((uint32_t)code << MD_SLOTBITS) |
+                                           ((uint32_t)a.typ_s[j] << MD_JBITS));
                 }
-                const unsigned long long m = __ballot(keep);
+                int kept;
+                const int at = block_scan<NT>(keep ? 1 : 0, s_w, kept);
                 if (keep) {
-                    const int k = (tail + __popcll(m & below)) & (RQ - 1);
-                    s_qp[wave][k] = rp;
-                    s_qs[wave][k] = rs;
+                    const int k = (tail + at) & (RING - 1);
+                    s_cx[k] = cx;
+                    if (EXCL) s_cid[k] = cid;
                 }
-                tail += __popcll(m);
-                wave_sync();
-                if (tail - head >= 64) {   // at most 127 entries are pending
-                    process(head, 64);
-                    head += 64;
-                    wave_sync();
-                }
+                __syncthreads();
+                tail += kept;
+                while (tail - head >= 64) process(64);   // at most 63 entries stay pending
             }
         }
-        if (tail > head) process(head, tail - head);
+        if (tail > head) process(tail - head);
     }
-    int mine = 0, mx = 0;
+    // row lengths, row padding up to the end of the last block (the dummy, local
+    // index 0), longest row of the wavefront
+    const int blk = 8 << gs;
+    int mx = 0;
 #pragma unroll
     for (int i = 0; i < AW; i++) {
-        if (lane == i) mine = cnt[i];
-        mx = max(mx, cnt[i]);
+        const int c = cnt[i];
+        mx = max(mx, c);
+        const int endb = min((c + blk - 1) / blk * blk, cap);
+        for (int e = c + lane; e < endb; e += 64) wrow[(size_t)i * cap + lpos16(e, blk / 8)] = 0;
+        if (lane == i) a.ncount[slot0 + wave * AW + i] = __float_as_int(s_ax[wave * AW + i].w) >= 0 ? c : 0;
     }
-    if (lane < AW) a.ncount[slot0 + lane] = active ? mine : 0;
     if (lane == 0) s_wmax[wave] = mx;
     __syncthreads();
     // copy the tile back from scratch into the primary arrays (new order)
     if (tid < 64) {
-        const int s = tile * 64 + tid;
+        const int s = slot0 + tid;
         const float4 x = a.pos_s[s];
         a.pos[cb][s] = x;
         a.pos_build[s] = x;
@@ -933,10 +503,12 @@ __global__ __launch_bounds__(64 * (64 / AW)) void k_nl_jbuild(BuildArgs a) {
         a.ids[s] = a.ids_s[s];
         a.typ[s] = a.typ_s[s];
     }
-    if (tid == 0) {   // longest row of the tile; k_nl_commit opens the new list epoch
+    if (tid == 0) {   // longest row and set size of the tile; k_nl_commit opens the new list epoch
         int m = s_wmax[0];
         for (int w = 1; w < NW; w++) m = max(m, s_wmax[w]);
-        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maximum since the host cleared it
+        a.tilemax[tile] = max(a.tilemax[tile], m);   // running maxima since the host cleared them
+        a.jmax[tile] = max(a.jmax[tile], jn);
+        a.jcount[tile] = min(jn, jcap);
     }
 }
 
@@ -987,6 +559,59 @@ __global__ void k_force_trip(RepState* st, Gate* g, int nrep) {
     }
 }
 
+// ---------------------------------------------------------------- invariants
+// Consistency of the sorted atom order and the lists, checked on the device
+// (test support, no counterpart in the reference).  Counts into bad[]:
+//   0: an active slot s with slot_of[ids[s]] != s (the order is not a bijection)
+//   1: a row entry outside the tile's set (local index 0 or beyond jcount)
+//   2: a row pair (i, j) whose image distance at the build positions is not
+//      below r_list, or j = i
+//   3: a set entry whose slot is not an active slot of the tile's replica
+//   4: a row longer than the capacity or a set larger than its capacity
+//   5: a row block padding entry that is not the dummy
+// One thread per atom slot; the caller has just built the lists (pos_build).
+__global__ __launch_bounds__(256) void k_check_lists(BuildArgs a, int row_g, int* bad) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nslot) return;
+    const int tile = s >> 6;
+    const int r = a.tile_rep[tile];
+    const RepParam& p = a.rep[r];
+    const int id = a.ids[s];
+    if (id < 0) return;
+    if (id >= 0 && a.slot_of[id] != s) atomicAdd(&bad[0], 1);
+    const int nn = a.ncount[s];
+    const int jn = a.jcount[tile];
+    if (nn > a.cap || a.jmax[tile] > a.jcap) atomicAdd(&bad[4], 1);
+    const float4 xi = a.pos_build[s];
+    const uint16_t* row = a.nl16 + (size_t)s * a.cap;
+    const uint32_t* js = a.jset + (size_t)tile * a.jcap;
+    const int n = min(nn, a.cap);
+    for (int e = 0; e < n; e++) {
+        const int li = row[lpos16(e, row_g)];
+        if (li < 1 || li > jn) {
+            atomicAdd(&bad[1], 1);
+            continue;
+        }
+        const uint32_t info = js[li - 1];
+        const int j = (int)(info & MD_SLOTMASK), code = (int)((info >> MD_SLOTBITS) & 31);
+        if (j < p.slot_off || j >= p.slot_off + p.nslot || a.ids[j] < 0) {
+            atomicAdd(&bad[3], 1);
+            continue;
+        }
+        const float4 xj = a.pos_build[j];
+        const int kx = code % 3 - 1, ky = (code / 3) % 3 - 1, kz = code / 9 - 1;
+        const float dx = (xi.x - xj.x) + (-(float)kx * p.L[0]);
+        const float dy = (xi.y - xj.y) + (-(float)ky * p.L[1]);
+        const float dz = (xi.z - xj.z) + (-(float)kz * p.L[2]);
+        const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(rsq < a.rl2) || j == s) atomicAdd(&bad[2], 1);
+    }
+    const int blk = 8 * row_g;
+    const int endb = min((nn + blk - 1) / blk * blk, a.cap);
+    for (int e = n; e < endb; e++)
+        if (row[lpos16(e, row_g)] != 0) atomicAdd(&bad[5], 1);
+}
+
 // ---------------------------------------------------------------- launchers
 
 // commit = 0: a deferred-finalize block follows and its first step kernel opens the new lists
@@ -997,29 +622,18 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStrea
     hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
     const dim3 ntiles(a.nslot / 64);
-    if (a.build_kernel == 2) {
-        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true, 16>), ntiles, dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_nl_jbuild<false, 16>), ntiles, dim3(256), 0, st, a);
-    } else if (a.build_kernel == 3) {
-        if (a.has_excl) hipLaunchKernelGGL((k_nl_jbuild<true, 8>), ntiles, dim3(512), 0, st, a);
-        else hipLaunchKernelGGL((k_nl_jbuild<false, 8>), ntiles, dim3(512), 0, st, a);
-    } else if (a.build_kernel == 1) {
-        if (a.has_excl) hipLaunchKernelGGL((k_nl_wbuild<true>), ntiles, dim3(512), 0, st, a);
-        else hipLaunchKernelGGL((k_nl_wbuild<false>), ntiles, dim3(512), 0, st, a);
-    } else {
-        switch ((a.has_excl ? 100 : 0) + a.build_lanes) {
-        case 4:   hipLaunchKernelGGL((k_nl_build<false, 4>), ntiles, dim3(256), 0, st, a); break;
-        case 8:   hipLaunchKernelGGL((k_nl_build<false, 8>), ntiles, dim3(512), 0, st, a); break;
-        case 104: hipLaunchKernelGGL((k_nl_build<true, 4>), ntiles, dim3(256), 0, st, a); break;
-        case 108: hipLaunchKernelGGL((k_nl_build<true, 8>), ntiles, dim3(512), 0, st, a); break;
-        default:  return hipErrorInvalidValue;
-        }
-    }
+    if (a.has_excl) hipLaunchKernelGGL((k_nl_tbuild<true>), ntiles, dim3(512), 0, st, a);
+    else hipLaunchKernelGGL((k_nl_tbuild<false>), ntiles, dim3(512), 0, st, a);
     if (commit) hipLaunchKernelGGL(k_nl_commit, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 extern "C" hipError_t md_launch_force_trip(RepState* s, Gate* g, int nrep, hipStream_t st) {
     hipLaunchKernelGGL(k_force_trip, dim3(1), dim3(256), 0, st, s, g, nrep);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_check_lists(const BuildArgs& a, int row_g, int* bad, hipStream_t st) {
+    hipLaunchKernelGGL(k_check_lists, dim3((a.nslot + 255) / 256), dim3(256), 0, st, a, row_g, bad);
     return hipGetLastError();
 }

@@ -1,6 +1,6 @@
 // step.hip -- the fused MD step: pair forces (LJ + Wolf / reaction-field /
 // damped Coulomb), harmonic bonds and angles, and the integrator epilogue,
-// in one persistent launch per step.
+// in one launch per step.
 //
 // Reference arithmetic restated (FP32, same operation order):
 //   pair loop          src/pair_forces.cu:151-200 (Wolf), :60-126 (RF)
@@ -13,31 +13,40 @@
 //   Nose-Hoover        MTK chain-of-one Trotter half steps (Martyna, Tuckerman,
 //                      Klein, Mol. Phys. 87 (1996) 1117), no reference code exists
 //
-// Work decomposition: G lanes cooperate on one atom (each strides over its
-// neighbor row), 64/G atoms per wavefront, 4 wavefronts per workgroup, and a
-// persistent grid walks contiguous ranges of 256/G-atom units.  Per-atom sums
-// are reduced across the G lanes with DPP row moves; KE / energies /
-// virial and the skin-check flag are reduced per workgroup into FP64 partial
-// slots (plain stores).  k_step_finalize, launched next, reduces a replica's
-// slots and computes its temperature, thermostat update and statistics record,
-// folds the skin-check flag into the list state and advances the step counter.
-// No device-scope atomics on shared addresses in the hot kernel: with a
-// thousand workgroups they serialise behind each other.
+// Work decomposition: one workgroup per 64-slot tile, G lanes per atom (64 G
+// threads).  The workgroup stages the tile's neighbor set into LDS (the
+// positions of the recorded images, md_rows.h), then each atom's G lanes walk
+// its row of 16-bit local indices (the LDS-staged tiles of
+// src/pair_forces_tiles.cu:98-155, restricted to the atoms that are
+// neighbors).  Per-atom sums are reduced across the G lanes with DPP row
+// moves; KE / energies / virial and the skin-check flag are reduced per
+// workgroup into FP64 partial slots, one per tile (plain stores).
+// k_step_finalize, launched next, reduces a replica's slots and computes its
+// temperature, thermostat update and statistics record, folds the skin-check
+// flag into the list state and advances the step counter.  No device-scope
+// atomics on shared addresses in the hot kernel: with thousands of workgroups
+// they serialise behind each other.
 // A replica is skipped when its list is stale or its run target is reached,
 // so a captured segment can be replayed with no host decision.
+//
+// Positions run unwrapped between list builds (the build wraps them), so the
+// drift x += v dt has no wrap here; an atom is at most skin/2 outside the box
+// before the skin check forces a build.
 //
 // Algorithmic work per ordered pair (FMA = 2, add/mul/rsqrt/rcp/exp2/rint = 1):
 // LJ + Wolf 65 flop, LJ + RF 45, LJ 36, Wolf only 60 (src/pair_forces.cu:151-200).
 #include <hip/hip_ext.h>
 
-#include "md_device.h"
+#include <algorithm>
+#include <type_traits>
+
+#include "md_rows.h"
 
 namespace {
 
 __device__ __forceinline__ float minimg(float d, float L, float Linv) {
     return fmaf(-L, rintf(d * Linv), d);
 }
-
 
 // MTK chain-of-one half step: returns the velocity scale, updates xi, vxi, K.
 __device__ double nh_half(double& xi, double& vxi, double& K, double nf, double kT0, double Q, double dt) {
@@ -106,8 +115,6 @@ __device__ __forceinline__ void lf_gates(const StepArgs& a, const RepState& o, c
 // Runs once per replica and step-kernel launch, by thread 0 of the replica's
 // workgroup in k_step_finalize; v = the replica's reduced partial sums, tn = the
 // skin check tripped during the position update.
-// p, st0 and th are register copies loaded at kernel start (their loads overlap
-// the partial-sum loads); the updated state is stored back at the end.
 __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, const RepState& st0, ThState th,
                                  long long kstat, bool en, const double* v, bool tn) {
     const double dt = a.dt, tau = a.tau;
@@ -171,29 +178,6 @@ __device__ void finalize_replica(const StepArgs& a, int r, const RepParam& p, co
     }
 }
 
-// The partial slots of a replica in one step-kernel launch (grid workgroups;
-// slot u holds the run of the workgroup that started it: the replica's first
-// unit ru0 and the first unit u0(b) = floor(b * nunits / grid) of every
-// workgroup b whose range starts inside the replica, so the slots are ru0 and
-// u0(b) for b in [ceil((ru0 + 1) grid / nunits), ceil(ru1 grid / nunits));
-// with grid >= nunits every unit starts a range).
-struct SlotList {
-    int ru0, cnt, b_lo;
-    bool every;
-    long long n, g;
-    __device__ SlotList(int ru0_, int ru1, int grid, int nunits)
-        : ru0(ru0_), cnt(ru1 - ru0_), b_lo(0), every(grid >= nunits), n(nunits), g(grid) {
-        if (!every) {
-            b_lo = (int)(((long long)(ru0 + 1) * g + n - 1) / n);
-            const int b_hi = (int)(((long long)ru1 * g + n - 1) / n);
-            cnt = 1 + max(0, b_hi - b_lo);
-        }
-    }
-    __device__ int operator()(int k) const {   // the k-th slot, 0 <= k < cnt
-        return (every || k == 0) ? ru0 + k : (int)(((long long)(b_lo + k - 1) * n) / g);
-    }
-};
-
 // x from another lane of the row by a DPP pattern (both 32-bit halves).
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
@@ -208,25 +192,23 @@ __device__ __forceinline__ float dpp_f(float x) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
 }
 
-// Sum of x over the G lanes of each aligned lane group (G = 4, 8, 16 or 32), the
+// Sum of x over the G lanes of each aligned lane group (G = 4, 8 or 16), the
 // same bits in every lane of the group: pairs and quads by quad permutations,
-// then mirrors within 8 and 16 lanes, then the two 16-lane halves of 32.
+// then mirrors within 8 and 16 lanes.
 template <int G>
 __device__ __forceinline__ float group_sum(float x) {
-    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lanes per atom");
+    static_assert(G == 4 || G == 8 || G == 16, "lanes per atom");
     x += dpp_f<0xB1>(x);                          // quad_perm [1, 0, 3, 2]
     x += dpp_f<0x4E>(x);                          // quad_perm [2, 3, 0, 1]
     if constexpr (G >= 8) x += dpp_f<0x141>(x);   // row_half_mirror
     if constexpr (G >= 16) x += dpp_f<0x140>(x);  // row_mirror
-    if constexpr (G >= 32) x += __shfl_xor(x, 16, 64);
     return x;
 }
 
 // Sum of x over the 64 lanes of the wavefront (every lane active), the same bits
 // in every lane: pairs, quads (quad permutations), halves of 16-lane rows and
 // rows (mirrors; addition is commutative, so both partners get the same sum),
-// then (row 0 + row 1) + (row 2 + row 3) from lanes 0, 16, 32 and 48.  DPP moves
-// instead of LDS permutes: no LDS round trip per level.
+// then (row 0 + row 1) + (row 2 + row 3) from lanes 0, 16, 32 and 48.
 __device__ __forceinline__ double wave_sum_d(double x) {
     x += dpp_d<0xB1>(x);    // quad_perm [1, 0, 3, 2]
     x += dpp_d<0x4E>(x);    // quad_perm [2, 3, 0, 1]
@@ -243,11 +225,11 @@ __device__ __forceinline__ double wave_sum_d(double x) {
     return (r[0] + r[1]) + (r[2] + r[3]);
 }
 
-// Totals over the 256 threads of the workgroup of per-thread sums v (the slots
-// k = tid, tid + 256, ... of a SlotList, added in that order); every thread gets
-// the totals.  The order of the additions is fixed, so every caller gets the same
-// bits.  ALL = false: only KE and the trip flag.
-template <bool ALL>
+// Totals over the NW wavefronts of the workgroup of per-thread sums v; every
+// thread gets the totals.  The order of the additions is fixed, so every caller
+// with the same workgroup size gets the same bits.  ALL = false: only KE and
+// the trip flag.
+template <int NW, bool ALL>
 __device__ __forceinline__ void reduce_slots(double (*s_red)[MD_NSTAT], double* v) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -261,58 +243,33 @@ __device__ __forceinline__ void reduce_slots(double (*s_red)[MD_NSTAT], double* 
         for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = v[k];
     }
     __syncthreads();
+    // thread k adds quantity k over the wavefronts in order; the totals go to row 0
+    double t = 0.0;
+    if (threadIdx.x < MD_NSTAT) {
+        t = s_red[0][threadIdx.x];
+        for (int w = 1; w < NW; w++) t += s_red[w][threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x < MD_NSTAT) s_red[0][threadIdx.x] = t;
+    __syncthreads();
 #pragma unroll
-    for (int k = 0; k < MD_NSTAT; k++) v[k] = s_red[0][k] + s_red[1][k] + s_red[2][k] + s_red[3][k];
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = s_red[0][k];
 }
 
-// Sums of a replica's partial slots in one buffer of part ([MD_NSTAT][nunits]:
-// one plane per quantity, so that ALL = false, which reads only KE and the trip
-// flag, touches 16 bytes a slot; every workgroup of a deferred finalize reads
-// them all).
-template <bool ALL>
-__device__ __forceinline__ void sum_slots(const double* part, int ru0, int ru1, int grid, int nunits,
+// Sums of a replica's partial slots (its tiles t0 .. t1-1) in one buffer of part
+// ([MD_NSTAT][ntiles]: one plane per quantity, so ALL = false reads 16 bytes a
+// tile).  Thread k adds tiles t0 + k, t0 + k + blockDim, ... in that order.
+template <int NW, bool ALL>
+__device__ __forceinline__ void sum_tiles(const double* part, int t0, int t1, int ntiles,
                                           double (*s_red)[MD_NSTAT], double* v) {
-    const SlotList sl(ru0, ru1, grid, nunits);
 #pragma unroll
     for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
-    for (int k = (int)threadIdx.x; k < sl.cnt; k += blockDim.x) {
-        const int u = sl(k);
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) {
 #pragma unroll
         for (int j = 0; j < MD_NSTAT; j++)
-            if (ALL || j == ACC_KE2 || j == MD_NSTAT - 1) v[j] += part[(size_t)j * nunits + u];
+            if (ALL || j == ACC_KE2 || j == MD_NSTAT - 1) v[j] += part[(size_t)j * ntiles + t];
     }
-    reduce_slots<ALL>(s_red, v);
-}
-
-// sum_slots<true> for the finalize kernel: each thread's slots are read four at
-// a time, all loads in flight before the first addition (the additions keep the
-// order of sum_slots, so the bits are the same).
-__device__ __forceinline__ void sum_slots_all4(const double* part, int ru0, int ru1, int grid, int nunits,
-                                               double (*s_red)[MD_NSTAT], double* v) {
-    const SlotList sl(ru0, ru1, grid, nunits);
-#pragma unroll
-    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
-    int k = (int)threadIdx.x;
-    for (; k + 3 * (int)blockDim.x < sl.cnt; k += 4 * (int)blockDim.x) {
-        double x[4][MD_NSTAT];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int u = sl(k + q * (int)blockDim.x);
-#pragma unroll
-            for (int j = 0; j < MD_NSTAT; j++) x[q][j] = part[(size_t)j * nunits + u];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-#pragma unroll
-            for (int j = 0; j < MD_NSTAT; j++) v[j] += x[q][j];
-        }
-    }
-    for (; k < sl.cnt; k += blockDim.x) {
-        const int u = sl(k);
-#pragma unroll
-        for (int j = 0; j < MD_NSTAT; j++) v[j] += part[(size_t)j * nunits + u];
-    }
-    reduce_slots<true>(s_red, v);
+    reduce_slots<NW, ALL>(s_red, v);
 }
 
 // One pair: force factor (F_i += ff * d) and, with EN, the pair energies.
@@ -349,44 +306,126 @@ __device__ __forceinline__ float pair_ff(const StepArgs& a, const float4* s_ct, 
     return ff;
 }
 
-// Pair loop of one atom over its neighbor row.  Rows are read as quads of
-// entries (16-byte loads; the capacity is a multiple of 4G, so rows and blocks
-// are 16-byte aligned).  Block b holds logical entries 4Gb .. 4Gb+4G-1 in the
-// transposed order of lpos(): lane `sub` loads quad sub of every block and gets
-// logical entries 4Gb + sub + G u (u = 0..3).  So the G lanes of an atom read
-// G*16 contiguous bytes of the row, every gather instruction covers G
-// consecutive entries, and each lane has four independent gathers in flight.
-// The gathers of block b + 1 are issued before block b computes (the row quads
-// two blocks ahead), so their latency hides behind a block of arithmetic.
-// Every list entry holds a slot index (the list is zeroed when allocated and
-// builds write only slot indices), so the entries past the row length in the
-// last block gather some atom in bounds; their pairs are dropped.
-__device__ __forceinline__ void gather4(const float4* __restrict__ pin, const uint4 e4, float4* pj) {
-    const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
-#pragma unroll
-    for (int u = 0; u < 4; u++)   // 32-bit byte offset from the position array (scalar base + vector offset)
-        pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(pin) + ((ev[u] & MD_JMASK) << 4));
+// LJ-only pairs: r^-2 from one reciprocal, no square root.
+__device__ __forceinline__ float pair_ff_lj(float rsq, const float4 lj) {
+    const float ri2 = __builtin_amdgcn_rcpf(rsq);
+    const float ri6 = ri2 * ri2 * ri2;
+    return fmaf(lj.x, ri6, -lj.y) * (ri6 * ri2);   // lj1 r^-14 - lj2 r^-8
 }
 
-template <int G, int COUL, bool EN, bool TAIL>
-__device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* s_lj, const float4* s_ct, const uint4 e4,
-                                          const float4* pj, int base, int nn, const float4 xi, float qi, float L0,
-                                          float L1, float L2, float Li0, float Li1, float Li2, float& fx, float& fy,
-                                          float& fz, double& elj, double& eco, double& evi) {
-    const uint32_t ev[4] = {e4.x, e4.y, e4.z, e4.w};
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        // type_j * 16 bytes: slots stay below 2^(MD_JBITS - 4), so the bits of j above
-        // the shift are zero (enforced when the system is set)
-        const float4 lj = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_lj) + (ev[u] >> (MD_JBITS - 4)));
-        const float dx = minimg(xi.x - pj[u].x, L0, Li0);
-        const float dy = minimg(xi.y - pj[u].y, L1, Li1);
-        const float dz = minimg(xi.z - pj[u].z, L2, Li2);
+}  // namespace
+
+// ENMODE 0: no energies (compiled out); 1: energies where requested (runtime, per replica)
+template <int G, int COUL, int ENMODE>
+__global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
+    constexpr int NW = G;                 // wavefronts: 64 atoms x G lanes
+    extern __shared__ float4 s_j[];       // [jcap + 1] the tile's staged neighbor set
+    __shared__ float4 s_lj[64];
+    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
+    __shared__ float4 s_sh[27];
+    __shared__ double s_red[NW][MD_NSTAT];
+    static_assert(MD_CTAB == 256, "the table is loaded by the first 256 threads");
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sub = tid & (G - 1), ai = tid / G;
+    const int tile = xcd_block(blockIdx.x, gridDim.x);   // tiles contiguous per XCD
+    const int slot = tile * 64 + ai;
+    // One memory round trip before any work: the gate, the tables, the replica's
+    // state and constants, the set size and the atom's own data.
+    const Gate gate = *a.gate;
+    const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
+    const float4 lj_v = a.ljtab[tid & 63];
+    float4 ct_v = make_float4(0.f, 0.f, 0.f, 0.f), ct_e = ct_v;
+    if (COUL == 1 || COUL == 2) {
+        if (tid < 256) ct_v = a.ctab[tid];
+        ct_e = a.ctab[MD_CTAB];
+    }
+    const int jn = a.jcount[tile];
+    const int id = a.ids[slot], tyi = a.typ[slot], nci = a.ncount[slot];
+    RepState s = a.lag ? a.lst[(size_t)(a.par ^ 1) * a.nrep + r] : a.st[r];
+    const RepParam& pr = a.rep[r];
+    const int so = pr.slot_off, ns = pr.nslot;
+    const float nf = pr.nf, T0 = pr.T0;
+    const float L0 = pr.L[0], L1 = pr.L[1], L2 = pr.L[2];
+    const float Li0 = pr.Linv[0], Li1 = pr.Linv[1], Li2 = pr.Linv[2];
+    const double c_S = a.th[r].S;
+    loads_issued();
+    if (tid < 64) s_lj[tid] = lj_v;
+    if (COUL == 1 || COUL == 2) {
+        if (tid < 256) s_ct[tid] = ct_v;
+        if (tid == 0) s_ct[MD_CTAB] = ct_e;
+    }
+    stage_shift_table(s_sh, L0, L1, L2);
+    // nothing can step (and no list was just rebuilt for this block to open): idle launch
+    if (a.mode == MODE_LF && gate.n_run == 0 && !(a.open && gate.n_stale != 0)) return;
+    if (a.mode == MODE_VV && gate.n_force == 0) return;
+    const int t0 = so / 64, t1 = (so + ns) / 64;
+    const bool leader = tile == t0;   // the workgroup of the replica's first tile
+    double lamd = 1.0;
+    if (a.lag) {
+        // Deferred finalize of the previous launch (same block): every workgroup sums
+        // the replica's KE and trip flag from the previous partial slots and advances
+        // the hand-over state; the leader also writes the statistics record.  Same
+        // arithmetic as k_step_finalize.
+        lamd = s.lam;
+        if (s.ran) {
+            const bool en_prev = a.energy_every > 0 && (s.step % a.energy_every) == 0;
+            double v[MD_NSTAT];
+            const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.ntiles;
+            if (leader && en_prev && a.stats) sum_tiles<NW, true>(prev, t0, t1, a.ntiles, s_red, v);
+            else sum_tiles<NW, false>(prev, t0, t1, a.ntiles, s_red, v);
+            const long long kst = s.step - s.start;
+            double T;
+            lf_done(a, nf, T0, s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
+            if (leader && tid == 0 && a.stats)
+                put_record(a, r, kst, en_prev, v, 0.5 * v[ACC_KE2], T, a.th[r].xi, a.th[r].vxi, 0.0);
+        }
+    } else {
+        if (a.open && stale(s)) {   // rebuilt by the block's chain: the new list epoch opens
+            s.nb += 1;
+            s.age = 0;
+        }
+        if (a.mode == MODE_LF) lamd = c_S;
+    }
+    bool run_r;
+    if (a.mode == MODE_FORCE) run_r = !stale(s);
+    else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
+    else run_r = s.phase == 1 && !stale(s);
+    if (a.lagmode && leader && tid == 0) {   // hand-over to the next launch
+        RepState o = s;
+        o.ran = run_r ? 1 : 0;
+        o.lam = lamd;
+        a.lst[(size_t)a.par * a.nrep + r] = o;
+    }
+    if (!run_r) return;   // workgroup-uniform: one replica per tile
+    const bool en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
+                                                           : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
+    const int cb = cur_buf(s);
+    const float4* __restrict__ pin = a.pos[cb];
+    float4* pout = a.pos[cb ^ 1];
+    const float lam = (float)lamd;
+    const bool drift = !a.vvfused || s.step + 1 < s.target;
+    const float4 xi = pin[slot];
+    const bool active = id >= 0;
+    const int nn = active ? min(nci, a.cap) : 0;
+    const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(a.nl16 + (size_t)slot * a.cap);
+    __syncthreads();   // shift table visible
+    stage_tile_set(s_j, s_sh, pin, a.jset + (size_t)tile * a.jcap, min(jn, a.jcap));
+    __syncthreads();
+
+    const float qi = xi.w * a.qfac * a.qq_scale;
+    const float4* s_lji = s_lj + (active ? tyi : 0) * 8;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    double elj = 0.0, eco = 0.0, evi = 0.0;
+    auto pair = [&](const float4& pj, auto en_tag) {
+        constexpr bool EN = decltype(en_tag)::value;
+        const float4 lj = s_lji[staged_type(pj)];
+        const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
         const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         float ulj = 0.0f, ucoul = 0.0f;
-        float ff = pair_ff<COUL, EN>(a, s_ct, qi, pj[u].w, rsq, lj, ulj, ucoul);
-        // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2); entries past the row length
-        const bool use = (!TAIL || base + G * u < nn) && !(rsq > a.rc2);
+        float ff;
+        if (COUL == 3 && !EN) ff = pair_ff_lj(rsq, lj);
+        else ff = pair_ff<COUL, EN>(a, s_ct, qi, pj.w, rsq, lj, ulj, ucoul);
+        const bool use = !(rsq > a.rc2);   // cutoff as src/pair_forces.cu:195 (zero for rsq > rc2)
         ff = use ? ff : 0.0f;
         fx = fmaf(ff, dx, fx);
         fy = fmaf(ff, dy, fy);
@@ -396,441 +435,172 @@ __device__ __forceinline__ void pair_quad(const StepArgs& a, const float4* s_lj,
             eco += (double)ucoul;
             evi += (double)(ff * rsq);
         }
-    }
-}
-
-template <int G, int COUL, bool EN>
-__device__ __forceinline__ void pair_row(const StepArgs& a, const float4* __restrict__ pin, const float4* s_lj,
-                                         const float4* s_ct,
-                                         const uint32_t* __restrict__ row, const uint4 q0, int nn, int sub,
-                                         const float4 xi, float qi, float L0, float L1, float L2, float Li0,
-                                         float Li1, float Li2, float& fx, float& fy, float& fz, double& elj,
-                                         double& eco, double& evi) {
-    const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(row);
-    const int nb = (int)(((uint32_t)nn + 4 * G - 1) / (4 * G));   // blocks; all but the last are full
-    if (nb == 0) return;
-    uint4 e0 = q0;   // the row's first quad, loaded by the caller
-    float4 p0[4];
-    gather4(pin, e0, p0);
-    uint4 e1 = row4[min(1, nb - 1) * G + sub];
-    auto block = [&](int b) {
-        float4 p1[4];
-        gather4(pin, e1, p1);                               // block b + 1
-        const uint4 e2 = row4[min(b + 2, nb - 1) * G + sub];   // row quad of block b + 2
-        pair_quad<G, COUL, EN, false>(a, s_lj, s_ct, e0, p0, 0, nn, xi, qi, L0, L1, L2, Li0, Li1, Li2, fx, fy, fz,
-                                      elj, eco, evi);
-        e0 = e1;
-        e1 = e2;
-#pragma unroll
-        for (int u = 0; u < 4; u++) p0[u] = p1[u];
     };
-    int b = 0;
-    if (EN) {   // energy steps: more live registers, no unrolling
-        for (; b < nb - 1; b++) block(b);
-    } else {    // two blocks per iteration: no register copies of the pipelined values
-#pragma unroll 2
-        for (; b < nb - 1; b++) block(b);
-    }
-    pair_quad<G, COUL, EN, true>(a, s_lj, s_ct, e0, p0, 4 * G * (nb - 1) + sub, nn, xi, qi, L0, L1, L2, Li0, Li1, Li2,
-                                 fx, fy, fz, elj, eco, evi);
-}
+    if (ENMODE != 0 && en)
+        walk_row<G>(s_j, row4, nn, sub, [&](const float4& pj) { pair(pj, std::true_type{}); });
+    else
+        walk_row<G>(s_j, row4, nn, sub, [&](const float4& pj) { pair(pj, std::false_type{}); });
 
-}  // namespace
-
-// ENMODE 0: no energies (compiled out); 1: energies where requested (runtime, per replica)
-template <int G, int COUL, int ENMODE>
-__global__ __launch_bounds__(256, 4) void k_step(const StepArgs a) {
-    __shared__ float4 s_lj[64];
-    __shared__ float4 s_ct[(COUL == 1 || COUL == 2) ? MD_CTAB + 1 : 1];
-    static_assert(MD_CTAB == 256, "the table is loaded by the 256 threads of the workgroup");
-    __shared__ double s_red[4][MD_NSTAT];
-    constexpr int APW = 64 / G;        // atoms per wavefront
-    constexpr int APB = 4 * APW;       // atoms per unit (workgroup pass)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int sub = lane & (G - 1), grp = lane / G;
-    const int lb = xcd_block(blockIdx.x, gridDim.x);   // logical workgroup: contiguous per XCD
-    const int u0 = (int)(((long long)lb * a.nunits) / gridDim.x);
-    const int u1 = (int)(((long long)(lb + 1) * a.nunits) / gridDim.x);
-    // One memory round trip before any work: the gate counters, the LDS tables,
-    // the first unit's per-atom data (none of it depends on the replica state; both
-    // position buffers are read) and, with one replica, its state.
-    // The table loads go to registers first and to LDS after every other load of
-    // the prologue is issued: an LDS store waits for its data, and loads placed
-    // after it would start a round trip of their own.
-    const Gate gate = *a.gate;
-    const float4 lj_v = a.ljtab[tid & 63];
-    float4 ct_v = make_float4(0.f, 0.f, 0.f, 0.f), ct_e = ct_v;
-    if (COUL == 1 || COUL == 2) {   // 256 threads, MD_CTAB + 1 entries
-        ct_v = a.ctab[tid];
-        ct_e = a.ctab[MD_CTAB];
-    }
-    const int fslot = u0 * APB + wave * APW + grp;
-    int f_id = -1, f_typ = 0, f_nc = 0;
-    float4 f_x0 = make_float4(0.f, 0.f, 0.f, 0.f), f_x1 = f_x0;
-    uint4 f_q = make_uint4(0, 0, 0, 0);
-    if (ENMODE == 0 && u0 < u1) {   // (the energy variants have no registers to spare)
-        f_id = a.ids[fslot];
-        f_typ = a.typ[fslot];
-        f_nc = a.ncount[fslot];
-        f_x0 = a.pos[0][fslot];
-        f_x1 = a.pos[1][fslot];
-        f_q = reinterpret_cast<const uint4*>(a.nlist + (size_t)fslot * a.cap)[sub];
-    }
-    const bool one = a.nrep == 1 && u0 < u1;
-    RepState s_one;
-    if (one) s_one = a.lag ? a.lst[a.par ^ 1] : a.st[0];
-    // With one replica its box and thermostat constants (wavefront-uniform) are
-    // loaded here as well; the unit loop would load them per replica change, each
-    // after the one before.
-    float g_L0 = 0.f, g_L1 = 0.f, g_L2 = 0.f, g_Li0 = 0.f, g_Li1 = 0.f, g_Li2 = 0.f, g_nf = 1.f, g_T0 = 0.f;
-    double g_S = 1.0;
-    if (one) {
-        const RepParam& p0 = a.rep[0];
-        g_L0 = p0.L[0]; g_L1 = p0.L[1]; g_L2 = p0.L[2];
-        g_Li0 = p0.Linv[0]; g_Li1 = p0.Linv[1]; g_Li2 = p0.Linv[2];
-        g_nf = p0.nf;
-        g_T0 = p0.T0;
-        g_S = a.th[0].S;
-    }
-    // With one replica and a deferred finalize, the previous launch's KE and trip
-    // flags: each thread's first four slots are loaded in the same round trip (the
-    // workgroup that writes the statistics record sums the full slots instead; the
-    // energy variants sum after the idle test).
-    const bool one_lag = one && a.lag;
-    const bool one_all = one_lag && u0 == 0 && a.stats;
-    const bool one_pre = ENMODE == 0 && one_lag && !one_all;
-    const double* prev_one = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.nunits;
-    const SlotList sl_one(0, a.nunits, gridDim.x, a.nunits);
-    double kpre[4], tpre[4];
-    if (one_pre) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {   // unconditional loads (a clamped slot): no wait here
-            const int u = sl_one(min(tid + 256 * j, sl_one.cnt - 1));
-            kpre[j] = prev_one[(size_t)ACC_KE2 * a.nunits + u];
-            tpre[j] = prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u];
+    // bonded terms (lanes of the atom's group share them)
+    double ebo = 0.0, ean = 0.0, bvi = 0.0;
+    if (a.bonded && active) {
+        const int nb = a.nbond[id];
+        for (int b = sub; b < nb; b += G) {          // src/bonded_forces.cu:50-84
+            const uint32_t e = a.bond_list[(size_t)id * 4 + b];
+            const int js = a.slot_of[e & MD_JMASK];
+            const int bt = (int)(e >> MD_JBITS);
+            const float4 pj = pin[js];
+            const float dx = minimg(xi.x - pj.x, L0, Li0);
+            const float dy = minimg(xi.y - pj.y, L1, Li1);
+            const float dz = minimg(xi.z - pj.z, L2, Li2);
+            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float kk = a.bk2[bt], r0 = a.br0[bt];
+            const float fa = kk * (r0 * __builtin_amdgcn_rsqf(rsq) - 1.0f);
+            fx = fmaf(fa, dx, fx);
+            fy = fmaf(fa, dy, fy);
+            fz = fmaf(fa, dz, fz);
+            if (ENMODE != 0 && en) {
+                const float dr = sqrtf(rsq) - r0;
+                ebo += 0.25 * (double)kk * (double)dr * (double)dr;   // each bond is seen from both atoms
+                bvi += 0.5 * (double)(fa * rsq);
+            }
+        }
+        const int na = a.nang[id];
+        for (int q = sub; q < na; q += G) {          // src/bonded_forces.cu:119-188
+            const int4 e = a.ang_list[(size_t)id * 4 + q];
+            const int role = e.x & 0xff, at = e.x >> 8;
+            const float4 pA = pin[a.slot_of[e.y]];
+            const float4 pk = pin[a.slot_of[e.z]];
+            float4 pi = xi, pc = pA;
+            if (role == 1) { pi = pA; pc = xi; }
+            float ax = minimg(pi.x - pc.x, L0, Li0), ay = minimg(pi.y - pc.y, L1, Li1), az = minimg(pi.z - pc.z, L2, Li2);
+            float ai_ = __builtin_amdgcn_rsqf(fmaf(az, az, fmaf(ay, ay, ax * ax)));
+            float bx = minimg(pk.x - pc.x, L0, Li0), by = minimg(pk.y - pc.y, L1, Li1), bz = minimg(pk.z - pc.z, L2, Li2);
+            float bi = __builtin_amdgcn_rsqf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+            const float ab = fmaf(az, bz, fmaf(ay, by, ax * bx));
+            float cang = ab * ai_ * bi;
+            cang = fminf(fmaxf(cang, -0.9999999f), 0.9999999f);
+            const float th = acosf(cang);
+            const float fk = a.ak2[at] * (th - a.at0[at]) * __builtin_amdgcn_rsqf(1.0f - cang * cang) * ai_ * bi;
+            const float ca = ab * ai_ * ai_;
+            float gx = bx - ca * ax, gy = by - ca * ay, gz = bz - ca * az;
+            if (role == 1) {
+                const float cbb = ab * bi * bi;
+                gx = cbb * bx - ax - gx;
+                gy = cbb * by - ay - gy;
+                gz = cbb * bz - az - gz;
+            }
+            fx = fmaf(fk, gx, fx);
+            fy = fmaf(fk, gy, fy);
+            fz = fmaf(fk, gz, fz);
+            if (ENMODE != 0 && en) {
+                const double d = (double)th - (double)a.at0[at];
+                ean += (0.5 / 3.0) * (double)a.ak2[at] * d * d;   // each angle is seen from 3 atoms
+            }
         }
     }
-    if (tid < 64) s_lj[tid] = lj_v;
-    if (COUL == 1 || COUL == 2) {
-        s_ct[tid] = ct_v;
-        if (tid == 0) s_ct[MD_CTAB] = ct_e;
-    }
-    // nothing can step (and no list was just rebuilt for this block to open): idle launch
-    if (a.mode == MODE_LF && gate.n_run == 0 && !(a.open && gate.n_stale != 0)) return;
-    if (a.mode == MODE_VV && gate.n_force == 0) return;
-    // With one replica and a deferred finalize, the previous launch's partial sums
-    // (the additions in the order of sum_slots).
-    double v_one[MD_NSTAT];
-    if (one_lag) {
-        if (one_all) {
-            sum_slots<true>(prev_one, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
-        } else if (!one_pre) {
-            sum_slots<false>(prev_one, 0, a.nunits, gridDim.x, a.nunits, s_red, v_one);
-        } else {
-#pragma unroll
-            for (int k = 0; k < MD_NSTAT; k++) v_one[k] = 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (tid + 256 * j < sl_one.cnt) {
-                    v_one[ACC_KE2] += kpre[j];
-                    v_one[MD_NSTAT - 1] += tpre[j];
-                }
-            }
-            for (int k = tid + 1024; k < sl_one.cnt; k += 256) {
-                const int u = sl_one(k);
-                v_one[ACC_KE2] += prev_one[(size_t)ACC_KE2 * a.nunits + u];
-                v_one[MD_NSTAT - 1] += prev_one[(size_t)(MD_NSTAT - 1) * a.nunits + u];
-            }
-            reduce_slots<false>(s_red, v_one);
-        }
-    }
-    __syncthreads();
-    int cur_r = -1, run_u0 = 0;
-    bool run_r = false, en = false;
-    const float4* pin = nullptr;
-    float4* pout = nullptr;
-    float L0 = 0, L1 = 0, L2 = 0, Li0 = 0, Li1 = 0, Li2 = 0, lam = 1.0f;
-    bool drift = true;   // leapfrog form: the positions advance (velocity Verlet: not after the run's last step)
-    double dke = 0.0, dlj = 0.0, dco = 0.0, dbo = 0.0, dan = 0.0, dvi = 0.0;
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    double dke = 0.0;
     int trip = 0;
-
-    // The workgroup's partial sums of a run of units of one replica go to the slot of
-    // the run's first unit (plain stores; k_step_finalize, the next kernel, reduces the
-    // replica's slots).  No atomics: same-address device atomics from every
-    // workgroup would serialise behind each other.
-    auto flush = [&]() {
-        double vals[MD_NSTAT] = {dke, dlj, dco, dbo, dan, dvi, 0.0, 0.0};
-        for (int k = 0; k < MD_NSTAT - 2; k++) vals[k] = wave_sum_d(vals[k]);
-        const unsigned long long tb = __ballot(trip != 0);
-        if (lane == 0) {
-            for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = vals[k];
-            s_red[wave][MD_NSTAT - 1] = tb ? 1.0 : 0.0;
-        }
-        __syncthreads();
-        if (tid < MD_NSTAT)
-            a.part[((size_t)a.par * MD_NSTAT + tid) * a.nunits + run_u0] =
-                s_red[0][tid] + s_red[1][tid] + s_red[2][tid] + s_red[3][tid];
-        __syncthreads();
-        dke = dlj = dco = dbo = dan = dvi = 0.0;
-        trip = 0;
-    };
-
-    for (int u = u0; u < u1; u++) {
-        const int slot0 = u * APB;
-        const int r = one ? 0 : __builtin_amdgcn_readfirstlane(a.tile_rep[slot0 >> 6]);
-        if (r != cur_r) {
-            if (cur_r >= 0 && run_r) flush();
-            cur_r = r;
-            run_u0 = u;
-            // the replica's parameters and state, wavefront-uniform, in one round trip
-            // (with one replica they were loaded in the prologue)
-            float c_nf = g_nf, c_T0 = g_T0, c_L0 = g_L0, c_L1 = g_L1, c_L2 = g_L2;
-            float c_Li0 = g_Li0, c_Li1 = g_Li1, c_Li2 = g_Li2;
-            double c_S = g_S;
-            int c_so = 0, c_ns = 0;
-            RepState s = s_one;
-            if (!one) {
-                const RepParam& pr = a.rep[r];
-                c_so = pr.slot_off;
-                c_ns = pr.nslot;
-                c_nf = pr.nf;
-                c_T0 = pr.T0;
-                c_L0 = pr.L[0]; c_L1 = pr.L[1]; c_L2 = pr.L[2];
-                c_Li0 = pr.Linv[0]; c_Li1 = pr.Linv[1]; c_Li2 = pr.Linv[2];
-                c_S = a.th[r].S;
-                s = a.lag ? a.lst[(size_t)(a.par ^ 1) * a.nrep + r] : a.st[r];
-                loads_issued();
-            }
-            const int ru0 = c_so / APB;
-            const bool leader = u == ru0;   // the workgroup holding the replica's first unit
-            double lamd = 1.0;
-            if (a.lag) {
-                // Deferred finalize of the previous launch (same block, same grid): every
-                // workgroup sums the replica's KE and trip flag from the previous partial
-                // slots and advances the hand-over state; the leader also writes the
-                // statistics record.  Same arithmetic as k_step_finalize.
-                lamd = s.lam;
-                if (s.ran) {
-                    const bool en_prev = a.energy_every > 0 && (s.step % a.energy_every) == 0;
-                    double v[MD_NSTAT];
-                    if (one) {
-#pragma unroll
-                        for (int k = 0; k < MD_NSTAT; k++) v[k] = v_one[k];
-                    } else {
-                        const int ru1 = (c_so + c_ns) / APB;
-                        const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.nunits;
-                        if (leader && en_prev && a.stats) sum_slots<true>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
-                        else sum_slots<false>(prev, ru0, ru1, gridDim.x, a.nunits, s_red, v);
-                    }
-                    const long long kst = s.step - s.start;
-                    double T;
-                    lf_done(a, c_nf, c_T0, s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
-                    if (leader && tid == 0 && a.stats)
-                        put_record(a, r, kst, en_prev, v, 0.5 * v[ACC_KE2], T, a.th[r].xi, a.th[r].vxi, 0.0);
-                }
-            } else {
-                if (a.open && stale(s)) {   // rebuilt by the block's chain: the new list epoch opens
-                    s.nb += 1;
-                    s.age = 0;
-                }
-                if (a.mode == MODE_LF) lamd = c_S;
-            }
-            if (a.mode == MODE_FORCE) run_r = !stale(s);
-            else if (a.mode == MODE_LF) run_r = !stale(s) && s.step < s.target;
-            else run_r = s.phase == 1 && !stale(s);
-            if (a.lagmode && leader && tid == 0) {   // hand-over to the next launch
-                RepState o = s;
-                o.ran = run_r ? 1 : 0;
-                o.lam = lamd;
-                a.lst[(size_t)a.par * a.nrep + r] = o;
-            }
-            if (!run_r) continue;
-            en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
-                                                        : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
-            const int cb = cur_buf(s);
-            pin = a.pos[cb];
-            pout = a.pos[cb ^ 1];
-            L0 = c_L0; L1 = c_L1; L2 = c_L2;
-            Li0 = c_Li0; Li1 = c_Li1; Li2 = c_Li2;
-            lam = (float)lamd;
-            drift = !a.vvfused || s.step + 1 < s.target;
-        }
-        if (!run_r) continue;
-        const int slot = slot0 + wave * APW + grp;
-        const uint32_t* row = a.nlist + (size_t)slot * a.cap;
-        int id, tyi, nci;
-        float4 xi;
-        uint4 q0;
-        if (ENMODE == 0 && u == u0) {   // loaded in the prologue
-            id = f_id; tyi = f_typ; nci = f_nc;
-            xi = pin == a.pos[0] ? f_x0 : f_x1;
-            q0 = f_q;
-        } else {
-            id = a.ids[slot]; tyi = a.typ[slot]; nci = a.ncount[slot];
-            xi = pin[slot];
-            q0 = reinterpret_cast<const uint4*>(row)[sub];
-        }
-        const bool active = id >= 0;
-        const int ti = active ? tyi : 0;
-        const int nn = active ? min(nci, a.cap) : 0;
-        const float qi = xi.w * a.qfac * a.qq_scale;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        double elj = 0.0, eco = 0.0, evi = 0.0;
-        if (ENMODE != 0 && en)
-            pair_row<G, COUL, true>(a, pin, s_lj + ti * 8, s_ct, row, q0, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
-                                    fx, fy, fz, elj, eco, evi);
-        else
-            pair_row<G, COUL, false>(a, pin, s_lj + ti * 8, s_ct, row, q0, nn, sub, xi, qi, L0, L1, L2, Li0, Li1, Li2,
-                                     fx, fy, fz, elj, eco, evi);
-        // bonded terms (lanes of the atom's group share them)
-        double ebo = 0.0, ean = 0.0, bvi = 0.0;
-        if (a.bonded && active) {
-            const int nb = a.nbond[id];
-            for (int b = sub; b < nb; b += G) {          // src/bonded_forces.cu:50-84
-                const uint32_t e = a.bond_list[(size_t)id * 4 + b];
-                const int js = a.slot_of[e & MD_JMASK];
-                const int bt = (int)(e >> MD_JBITS);
-                const float4 pj = pin[js];
-                const float dx = minimg(xi.x - pj.x, L0, Li0);
-                const float dy = minimg(xi.y - pj.y, L1, Li1);
-                const float dz = minimg(xi.z - pj.z, L2, Li2);
-                const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const float kk = a.bk2[bt], r0 = a.br0[bt];
-                const float fa = kk * (r0 * __builtin_amdgcn_rsqf(rsq) - 1.0f);
-                fx = fmaf(fa, dx, fx);
-                fy = fmaf(fa, dy, fy);
-                fz = fmaf(fa, dz, fz);
-                if (ENMODE != 0 && en) {
-                    const float dr = sqrtf(rsq) - r0;
-                    ebo += 0.25 * (double)kk * (double)dr * (double)dr;   // each bond is seen from both atoms
-                    bvi += 0.5 * (double)(fa * rsq);
-                }
-            }
-            const int na = a.nang[id];
-            for (int q = sub; q < na; q += G) {          // src/bonded_forces.cu:119-188
-                const int4 e = a.ang_list[(size_t)id * 4 + q];
-                const int role = e.x & 0xff, at = e.x >> 8;
-                const float4 pA = pin[a.slot_of[e.y]];
-                const float4 pk = pin[a.slot_of[e.z]];
-                float4 pi = xi, pc = pA;
-                if (role == 1) { pi = pA; pc = xi; }
-                float ax = minimg(pi.x - pc.x, L0, Li0), ay = minimg(pi.y - pc.y, L1, Li1), az = minimg(pi.z - pc.z, L2, Li2);
-                float ai = __builtin_amdgcn_rsqf(fmaf(az, az, fmaf(ay, ay, ax * ax)));
-                float bx = minimg(pk.x - pc.x, L0, Li0), by = minimg(pk.y - pc.y, L1, Li1), bz = minimg(pk.z - pc.z, L2, Li2);
-                float bi = __builtin_amdgcn_rsqf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
-                const float ab = fmaf(az, bz, fmaf(ay, by, ax * bx));
-                float cang = ab * ai * bi;
-                cang = fminf(fmaxf(cang, -0.9999999f), 0.9999999f);
-                const float th = acosf(cang);
-                const float fk = a.ak2[at] * (th - a.at0[at]) * __builtin_amdgcn_rsqf(1.0f - cang * cang) * ai * bi;
-                const float ca = ab * ai * ai;
-                float gx = bx - ca * ax, gy = by - ca * ay, gz = bz - ca * az;
-                if (role == 1) {
-                    const float cb = ab * bi * bi;
-                    gx = cb * bx - ax - gx;
-                    gy = cb * by - ay - gy;
-                    gz = cb * bz - az - gz;
-                }
-                fx = fmaf(fk, gx, fx);
-                fy = fmaf(fk, gy, fy);
-                fz = fmaf(fk, gz, fz);
-                if (ENMODE != 0 && en) {
-                    const double d = (double)th - (double)a.at0[at];
-                    ean += (0.5 / 3.0) * (double)a.ak2[at] * d * d;   // each angle is seen from 3 atoms
-                }
-            }
-        }
-        fx = group_sum<G>(fx);
-        fy = group_sum<G>(fy);
-        fz = group_sum<G>(fz);
-        if (ENMODE != 0 && en) {
-            dlj += elj;
-            dco += eco;
-            dvi += evi + 2.0 * bvi;
-            dbo += ebo;
-            dan += ean;
-        }
-        if (active && sub == 0) {
-            a.force[slot] = make_float4(fx, fy, fz, 0.0f);
-            if (a.mode == MODE_LF) {
-                float4 v = a.vel[slot];
-                const float4 pb = a.pos_build[slot];   // for the skin check: same round trip as v
-                if (a.vvfused) {
-                    // velocity Verlet, NVE: v(t) = v(t - dt/2) + F dt/2m ends the step (its kinetic
-                    // energy is the step's); v(t + dt/2) = v(t) + F dt/2m begins the next.  The same
-                    // operations as the kick kernel and the force-half epilogue below.
-                    const float h = 0.5f * a.dt * v.w;
-                    v.x = fmaf(fx, h, v.x);
-                    v.y = fmaf(fy, h, v.y);
-                    v.z = fmaf(fz, h, v.z);
-                    dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
-                    if (drift) {
-                        v.x = fmaf(fx, h, v.x * lam);
-                        v.y = fmaf(fy, h, v.y * lam);
-                        v.z = fmaf(fz, h, v.z * lam);
-                    }
-                } else {
-                    const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
-                    v.x = fmaf(fx, massif, v.x) * lam;
-                    v.y = fmaf(fy, massif, v.y) * lam;
-                    v.z = fmaf(fz, massif, v.z) * lam;
-                    dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
-                }
-                a.vel[slot] = v;
-                if (drift) {
-                    float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
-                    x = fmaf(-L0, truncf(x * 2.0f * Li0), x);   // src/nve_leapfrog.cu:50-55
-                    y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
-                    z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
-                    pout[slot] = make_float4(x, y, z, xi.w);
-                    const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
-                    if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
-                    if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
-                }
-            } else if (a.mode == MODE_VV) {
-                float4 v = a.vel[slot];
+    if (active && sub == 0) {
+        a.force[slot] = make_float4(fx, fy, fz, 0.0f);
+        if (a.mode == MODE_LF) {
+            float4 v = a.vel[slot];
+            const float4 pb = a.pos_build[slot];   // for the skin check: same round trip as v
+            if (a.vvfused) {
+                // velocity Verlet, NVE: v(t) = v(t - dt/2) + F dt/2m ends the step (its kinetic
+                // energy is the step's); v(t + dt/2) = v(t) + F dt/2m begins the next.  The same
+                // operations as the kick kernel and the force-half epilogue below.
                 const float h = 0.5f * a.dt * v.w;
                 v.x = fmaf(fx, h, v.x);
                 v.y = fmaf(fy, h, v.y);
                 v.z = fmaf(fz, h, v.z);
-                a.vel[slot] = v;
-                dke += ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+                dke = ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
+                if (drift) {
+                    v.x = fmaf(fx, h, v.x * lam);
+                    v.y = fmaf(fy, h, v.y * lam);
+                    v.z = fmaf(fz, h, v.z * lam);
+                }
+            } else {
+                const float massif = a.dt * v.w;          // dt / m_int (src/nve_leapfrog.cu:38)
+                v.x = fmaf(fx, massif, v.x) * lam;
+                v.y = fmaf(fy, massif, v.y) * lam;
+                v.z = fmaf(fz, massif, v.z) * lam;
+                dke = ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
             }
+            a.vel[slot] = v;
+            if (drift) {
+                const float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
+                pout[slot] = make_float4(x, y, z, xi.w);
+                const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
+                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
+                if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
+            }
+        } else if (a.mode == MODE_VV) {
+            float4 v = a.vel[slot];
+            const float h = 0.5f * a.dt * v.w;
+            v.x = fmaf(fx, h, v.x);
+            v.y = fmaf(fy, h, v.y);
+            v.z = fmaf(fz, h, v.z);
+            a.vel[slot] = v;
+            dke = ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z) / (double)v.w;
         }
     }
-    if (cur_r >= 0 && run_r) flush();
+    // the workgroup's partial sums go to the tile's slot (plain stores; the next
+    // kernel reduces the replica's slots)
+    double vals[MD_NSTAT] = {dke, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ENMODE != 0 && en) {
+        vals[ACC_LJ] = elj;
+        vals[ACC_COUL] = eco;
+        vals[ACC_BOND] = ebo;
+        vals[ACC_ANGLE] = ean;
+        vals[ACC_VIR] = evi + 2.0 * bvi;
+    }
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT - 2; k++) {
+        if (ENMODE == 0 && k != ACC_KE2) continue;
+        vals[k] = wave_sum_d(vals[k]);
+    }
+    const unsigned long long tb = __ballot(trip != 0);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < MD_NSTAT; k++) s_red[wave][k] = vals[k];
+        s_red[wave][MD_NSTAT - 1] = tb ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (tid < MD_NSTAT) {
+        double t = s_red[0][tid];
+#pragma unroll
+        for (int w = 1; w < NW; w++) t += s_red[w][tid];
+        a.part[((size_t)a.par * MD_NSTAT + tid) * a.ntiles + tile] = t;
+    }
 }
 
-// After k_step (same grid): one workgroup per replica sums the partial slots
-// the step kernel's workgroups wrote for the replica and finalizes it (only the
-// slots written in this launch are read, so none has to be cleared).  After a
-// leapfrog block with deferred finalize (a.lagmode) the replica's state comes
-// from the last launch's hand-over copy lst[par]: this kernel finalizes that
-// launch's step, writes the state back and settles the gate counters for the
-// whole block.
-__global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb, int grid) {
-    __shared__ double s_red[4][MD_NSTAT];
+// After k_step: one workgroup per replica (64 G threads, the step kernel's
+// workgroup size, so the slot sums have the deferred finalize's bits) sums the
+// partial slots of the replica's tiles and finalizes it.  After a leapfrog block
+// with deferred finalize (a.lagmode) the replica's state comes from the last
+// launch's hand-over copy lst[par]: this kernel finalizes that launch's step,
+// writes the state back and settles the gate counters for the whole block.
+template <int G>
+__global__ __launch_bounds__(64 * G) void k_step_finalize(const StepArgs a) {
+    constexpr int NW = G;
+    __shared__ double s_red[NW][MD_NSTAT];
     const int r = blockIdx.x;
-    // Every load is issued before any is consumed: with one replica its units are
-    // all units, so the slot loads need no layout load first.
-    const double* part = a.part + (size_t)a.par * MD_NSTAT * a.nunits;
+    const double* part = a.part + (size_t)a.par * MD_NSTAT * a.ntiles;
     const RepState s = a.lagmode ? a.lst[(size_t)a.par * a.nrep + r] : a.st[r];
-    RepParam p;
+    const RepParam p = a.rep[r];
     ThState th;
     RepState s_old;
     if (threadIdx.x == 0) {
-        p = a.rep[r];
         th = a.th[r];
         if (a.lagmode) s_old = a.st[r];
     }
-    int ru0 = 0, ru1 = a.nunits;
-    if (a.nrep > 1) {
-        const int so = a.rep[r].slot_off, ns = a.rep[r].nslot;
-        ru0 = so / apb;
-        ru1 = (so + ns) / apb;
-    }
+    const int t0 = p.slot_off / 64, t1 = (p.slot_off + p.nslot) / 64;
     double v[MD_NSTAT];
-    sum_slots_all4(part, ru0, ru1, grid, a.nunits, s_red, v);   // used only if the replica ran
+    sum_tiles<NW, true>(part, t0, t1, a.ntiles, s_red, v);   // used only if the replica ran
     if (a.lagmode) {
         // No launch of the block passed the gate when none could step, and lst[par]
         // was not written: nothing to do.  (A replica that could step keeps
@@ -869,11 +639,11 @@ __global__ __launch_bounds__(256) void k_step_finalize(const StepArgs a, int apb
     }
 }
 
-// Velocity-Verlet begin-of-step kernel: v = S v + F dt/2m; x += v dt; wrap;
-// skin check.  With apply_drift = 0 it only applies the end-of-step
-// thermostat scale (run end).  Each wavefront stores its 64-slot tile's skin
-// check flag; k_kick_finalize folds them into the list state and moves the
-// replica to the force phase.
+// Velocity-Verlet begin-of-step kernel: v = S v + F dt/2m; x += v dt; skin
+// check (no wrap: positions run free between list builds).  With
+// apply_drift = 0 it only applies the end-of-step thermostat scale (run end).
+// Each wavefront stores its 64-slot tile's skin check flag; k_kick_finalize
+// folds them into the list state and moves the replica to the force phase.
 // Two round trips: the gate, the replica index and every per-atom value (both
 // position buffers) first, the replica's state and constants second.
 __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
@@ -910,10 +680,7 @@ __global__ __launch_bounds__(256) void k_vv_kick(const KickArgs a) {
         v.z = fmaf(f.z, h, v.z * S);
         a.vel[s] = v;
         const float4 xi = cb ? x1 : x0;
-        float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
-        x = fmaf(-L0, truncf(x * 2.0f * Li0), x);
-        y = fmaf(-L1, truncf(y * 2.0f * Li1), y);
-        z = fmaf(-L2, truncf(z * 2.0f * Li2), z);
+        const float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
         a.pos[cb ^ 1][s] = make_float4(x, y, z, xi.w);
         const float ddx = minimg(x - pb.x, L0, Li0);
         const float ddy = minimg(y - pb.y, L1, Li1);
@@ -1022,48 +789,110 @@ __global__ void k_set_target(RepState* st, Gate* g, int nrep, long long nsteps) 
     }
 }
 
+// Run report: one workgroup reduces the replicas' step counters and the
+// builds' maxima (see RunReport).
+__global__ __launch_bounds__(256) void k_report(const RepState* st, int nrep, const int* tmax, int ntiles,
+                                                const int* err, RunReport* out) {
+    __shared__ long long s_step[4];
+    __shared__ int s_row[4], s_set[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    long long mn = 0x7fffffffffffffffLL;
+    int row = 0, set = 0;
+    for (int r = tid; r < nrep; r += 256) mn = min(mn, st[r].step);
+    for (int t = tid; t < ntiles; t += 256) {
+        row = max(row, tmax[t]);
+        set = max(set, tmax[ntiles + t]);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        mn = min(mn, (long long)__shfl_xor(mn, o, 64));
+        row = max(row, __shfl_xor(row, o, 64));
+        set = max(set, __shfl_xor(set, o, 64));
+    }
+    if (lane == 0) {
+        s_step[wave] = mn;
+        s_row[wave] = row;
+        s_set[wave] = set;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        RunReport rep;
+        rep.min_step = min(min(s_step[0], s_step[1]), min(s_step[2], s_step[3]));
+        rep.rowmax = max(max(s_row[0], s_row[1]), max(s_row[2], s_row[3]));
+        rep.setmax = max(max(s_set[0], s_set[1]), max(s_set[2], s_set[3]));
+        rep.err = *err;
+        rep.pad = 0;
+        *out = rep;
+    }
+}
+
+// Copy of the engine state a run may have to be rolled back to (one launch;
+// the caller passes matching buffer pairs).  nbytes per pair, 16-byte units.
+struct CopyPlan {
+    const float4* src[12];
+    float4* dst[12];
+    long long n16[12];   // 16-byte units per pair
+    int npair;
+};
+__global__ __launch_bounds__(256) void k_copy_plan(const CopyPlan c) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int k = 0; k < c.npair; k++) {
+        const float4* __restrict__ s = c.src[k];
+        float4* __restrict__ d = c.dst[k];
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < c.n16[k]; i += stride) d[i] = s[i];
+    }
+}
+
 // ---------------------------------------------------------------- launchers
 
 // With ev0/ev1 the launch records the kernel's own start and end in the two events
 // (hipExtLaunchKernelGGL), so their elapsed time is the kernel's execution time.
 template <typename K>
-static void launch_k(K kern, const StepArgs& a, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-    if (ev0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, ev0, ev1, 0, a);
-    else hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, a);
+static void launch_k(K kern, const StepArgs& a, int grid, int threads, size_t shmem, hipStream_t st, hipEvent_t ev0,
+                     hipEvent_t ev1) {
+    if (ev0) hipExtLaunchKernelGGL(kern, dim3(grid), dim3(threads), shmem, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), shmem, st, a);
 }
 
 template <int G, int EM>
-static void launch_ge(const StepArgs& a, int coul, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+static void launch_ge(const StepArgs& a, int coul, size_t shmem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    const int grid = a.ntiles;
     switch (coul) {
-    case 0: launch_k(k_step<G, 0, EM>, a, grid, st, ev0, ev1); break;
-    case 1: launch_k(k_step<G, 1, EM>, a, grid, st, ev0, ev1); break;
-    case 2: launch_k(k_step<G, 2, EM>, a, grid, st, ev0, ev1); break;
-    default: launch_k(k_step<G, 3, EM>, a, grid, st, ev0, ev1); break;
+    case 0: launch_k(k_step<G, 0, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    case 1: launch_k(k_step<G, 1, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    case 2: launch_k(k_step<G, 2, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    default: launch_k(k_step<G, 3, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
     }
 }
 
 template <int G>
-static void launch_g(const StepArgs& a, int coul, int grid, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+static void launch_g(const StepArgs& a, int coul, size_t shmem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const bool em = (a.mode == MODE_FORCE) ? a.force_energy != 0 : a.energy_every > 0;
-    if (em) launch_ge<G, 1>(a, coul, grid, st, ev0, ev1);
-    else launch_ge<G, 0>(a, coul, grid, st, ev0, ev1);
+    if (em) launch_ge<G, 1>(a, coul, shmem, st, ev0, ev1);
+    else launch_ge<G, 0>(a, coul, shmem, st, ev0, ev1);
 }
 
-extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, int grid, hipStream_t st,
-                                     hipEvent_t ev0, hipEvent_t ev1) {
+// dynamic LDS of the step kernel: the staged set and its dummy
+static size_t step_shmem(const StepArgs& a) { return (size_t)(a.jcap + 1) * sizeof(float4); }
+
+extern "C" hipError_t md_launch_step(const StepArgs& a, int lanes, int coul, hipStream_t st, hipEvent_t ev0,
+                                     hipEvent_t ev1) {
+    const size_t sh = step_shmem(a);
     switch (lanes) {
-    case 4: launch_g<4>(a, coul, grid, st, ev0, ev1); break;
-    case 8: launch_g<8>(a, coul, grid, st, ev0, ev1); break;
-    case 16: launch_g<16>(a, coul, grid, st, ev0, ev1); break;
-    case 32: launch_g<32>(a, coul, grid, st, ev0, ev1); break;
+    case 4: launch_g<4>(a, coul, sh, st, ev0, ev1); break;
+    case 8: launch_g<8>(a, coul, sh, st, ev0, ev1); break;
+    case 16: launch_g<16>(a, coul, sh, st, ev0, ev1); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, int grid, hipStream_t st) {
-    if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_step_finalize, dim3(a.nrep), dim3(256), 0, st, a, 256 / lanes, grid);
+extern "C" hipError_t md_launch_step_finalize(const StepArgs& a, int lanes, hipStream_t st) {
+    switch (lanes) {
+    case 4: hipLaunchKernelGGL(k_step_finalize<4>, dim3(a.nrep), dim3(256), 0, st, a); break;
+    case 8: hipLaunchKernelGGL(k_step_finalize<8>, dim3(a.nrep), dim3(512), 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_step_finalize<16>, dim3(a.nrep), dim3(1024), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1084,5 +913,30 @@ extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int
 
 extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long long nsteps, hipStream_t st) {
     hipLaunchKernelGGL(k_set_target, dim3(1), dim3(256), 0, st, s, g, nrep, nsteps);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
+                                       RunReport* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, stream, st, nrep, tmax, ntiles, err, out);
+    return hipGetLastError();
+}
+
+// copies: npair (src, dst, bytes) with 16-byte aligned sizes
+extern "C" hipError_t md_launch_copies(int npair, const void* const* src, void* const* dst, const long long* bytes,
+                                       hipStream_t st) {
+    if (npair < 0 || npair > 12) return hipErrorInvalidValue;
+    CopyPlan c{};
+    long long most = 0;
+    for (int k = 0; k < npair; k++) {
+        if (bytes[k] % 16) return hipErrorInvalidValue;
+        c.src[k] = static_cast<const float4*>(src[k]);
+        c.dst[k] = static_cast<float4*>(dst[k]);
+        c.n16[k] = bytes[k] / 16;
+        most = std::max(most, c.n16[k]);
+    }
+    c.npair = npair;
+    const int grid = (int)std::max(1LL, std::min(2048LL, (most + 255) / 256));
+    hipLaunchKernelGGL(k_copy_plan, dim3(grid), dim3(256), 0, st, c);
     return hipGetLastError();
 }

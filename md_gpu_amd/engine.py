@@ -78,6 +78,7 @@ def lib():
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
+            "md_check_lists": ([P, P], C.c_int),
             "md_get_info": ([P, P, P, P, P, P], C.c_int), "md_set_list_capacity": ([P, C.c_int], C.c_int),
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
             "md_autotune": ([P, C.c_int, P], C.c_int), "md_timing_enable": ([P, C.c_int], C.c_int),
@@ -97,7 +98,8 @@ def lib():
 def exported_symbols() -> List[str]:
     return ["md_last_error", "md_version", "md_create", "md_destroy", "md_set_forcefield", "md_set_integrator",
             "md_set_system", "md_set_topology", "md_upload_state", "md_download_state", "md_compute_forces",
-            "md_run", "md_get_neighbors", "md_get_thresholds", "md_get_info", "md_set_launch", "md_get_launch", "md_autotune",
+            "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
+            "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
             "md_set_list_capacity"]
 
@@ -229,6 +231,14 @@ class Engine:
         nbrs = np.full((self.N, cap), -1, np.int32)
         _check(lib().md_get_neighbors(self._h, int(cap), _ptr(counts), _ptr(nbrs)))
         return counts, nbrs
+
+    def check_lists(self):
+        """Invariant violation counts of the device lists (md_check_lists): a dict
+        of six counters, all zero when the sorted order and the lists are sound."""
+        c = np.zeros(6, np.int32)
+        _check(lib().md_check_lists(self._h, _ptr(c)))
+        keys = ["order", "row_index", "row_distance", "set_entry", "capacity", "padding"]
+        return dict(zip(keys, c.tolist()))
 
     def thresholds(self):
         a = np.zeros(3, np.float32)

@@ -20,7 +20,7 @@ import sys
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--launch", default="lanes=8,grid=1024,cell_div=2,check_every=8,build_lanes=8,build_kernel=3,"
+    ap.add_argument("--launch", default="lanes=8,cell_div=2,check_every=8,"
                                         "refresh=1,defer=1,submit=1")
     ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe"])
     ap.add_argument("--steps", type=int, default=60)

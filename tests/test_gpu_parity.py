@@ -57,13 +57,14 @@ def oracle_forces(orc, s):
     return orc.forces(s)
 
 
-@pytest.mark.parametrize("build_kernel", [0, 1, 2, 3])
+@pytest.mark.parametrize("lanes,cell_div", [(4, 1), (8, 2), (16, 3)])
 @pytest.mark.parametrize("name", list(CASES))
-def test_neighbor_sets_bit_exact(E, orc, name, build_kernel):
+def test_neighbor_sets_bit_exact(E, orc, name, lanes, cell_div):
     s = CASES[name]
-    say("neighbors", name, s.n, "build kernel", build_kernel)
+    say("neighbors", name, s.n, "lanes", lanes, "cell_div", cell_div)
     e = E.Engine.from_system(s)
-    e.set_launch(build_kernel=build_kernel)
+    e.set_launch(lanes=lanes, cell_div=cell_div)
+    assert all(v == 0 for v in e.check_lists().values()), e.check_lists()
     pos, _, _ = e.download()
     rc2, rl2, sk = e.thresholds()
     orc_rc2, orc_rl2, orc_sk, _ = orc.thresholds(s.ff.rc, s.ff.skin, s.box)
@@ -287,15 +288,15 @@ def test_logged_reference_forces_nacl_rf(E, orc):
 
 
 LAUNCH_VARIANTS = [
-    dict(submit=1, refresh=0, check_every=4, build_kernel=0, build_lanes=4, lanes=4),
-    dict(submit=0, refresh=1, check_every=4, build_kernel=0, build_lanes=8, lanes=8),
-    dict(submit=1, refresh=1, check_every=6, build_kernel=1, lanes=32),
-    dict(submit=0, refresh=0, check_every=12, build_kernel=1, lanes=16, cell_div=3),
-    dict(submit=0, refresh=0, check_every=8, build_kernel=1, lanes=8, cell_div=1),
-    dict(submit=0, refresh=1, check_every=8, build_kernel=2, lanes=4, cell_div=2, defer=1),
-    dict(submit=1, refresh=0, check_every=16, build_kernel=2, lanes=16, cell_div=3),
-    dict(submit=0, refresh=1, check_every=6, build_kernel=3, lanes=8, cell_div=1, defer=1),
-    dict(submit=1, refresh=0, check_every=12, build_kernel=3, lanes=16, defer=1),
+    dict(submit=1, refresh=0, check_every=4, lanes=4),
+    dict(submit=0, refresh=1, check_every=4, lanes=8),
+    dict(submit=1, refresh=1, check_every=6, lanes=16),
+    dict(submit=0, refresh=0, check_every=12, lanes=16, cell_div=3),
+    dict(submit=0, refresh=0, check_every=8, lanes=8, cell_div=1),
+    dict(submit=0, refresh=1, check_every=8, lanes=4, cell_div=2, defer=1),
+    dict(submit=1, refresh=0, check_every=16, lanes=16, cell_div=3, defer=0),
+    dict(submit=0, refresh=1, check_every=6, lanes=8, cell_div=1, defer=1),
+    dict(submit=1, refresh=0, check_every=12, lanes=4, defer=1),
 ]
 
 

@@ -27,21 +27,26 @@ def pairs_within(orc, pos, box, r2):
 
 
 @pytest.mark.parametrize("nsites,cfg", [(16, dict(check_every=8)),
-                                        (32, dict(check_every=16, lanes=8, build_kernel=0, build_lanes=8)),
-                                        (32, dict(check_every=16, lanes=8, build_kernel=1)),
-                                        (32, dict(check_every=8, lanes=8, build_kernel=2)),
-                                        (32, dict(check_every=8, lanes=16, build_kernel=3)),
+                                        (32, dict(check_every=16, lanes=8)),
+                                        (32, dict(check_every=16, lanes=4, cell_div=1)),
+                                        (32, dict(check_every=8, lanes=8, cell_div=3)),
+                                        (32, dict(check_every=8, lanes=16)),
                                         (32, dict(check_every=4, refresh=1, submit=1))])
 def test_lists_valid_after_in_run_rebuilds(E, orc, nsites, cfg):
-    """After a run with many device-side rebuilds, every pair within rc of the
-    current positions is in the list, and no row holds a pair beyond
+    """After a run with many device-side rebuilds, the device's sorted order and
+    lists pass the invariant check (order a bijection, every listed pair within
+    r_list at the build positions, sets and padding sound), every pair within rc
+    of the current positions is in the list, and no row holds a pair beyond
     rc + 2 skin (the list is never older than a skin/2 move per atom)."""
     s = S.nacl(nsites=nsites)
     say("lists after run", s.n, cfg)
     e = E.Engine.from_system(s)
     e.set_launch(**cfg)
     reb0 = e.info()["rebuilds"]
-    e.run(120, energy_every=0, stats=False)
+    for _ in range(3):
+        e.run(40, energy_every=0, stats=False)
+        bad = e.check_lists()
+        assert all(v == 0 for v in bad.values()), bad
     assert e.info()["rebuilds"] - reb0 >= 3
     pos, _, _ = e.download()
     counts, nbrs = e.neighbors()
@@ -64,7 +69,7 @@ def test_runs_reproducible(E):
     runs = []
     for _ in range(2):
         e = E.Engine.from_system(s)
-        e.set_launch(lanes=8, grid=2048, check_every=16, build_lanes=8)
+        e.set_launch(lanes=8, check_every=16)
         st = e.run(150, energy_every=10)
         pos, vel, _ = e.download()
         runs.append((st["temp"][:, 0].copy(), pos, e.info()["rebuilds"]))
@@ -105,18 +110,16 @@ def test_deferred_finalize_bit_identical(E, cfg):
             assert np.array_equal(a[k], b[k])
 
 
-@pytest.mark.parametrize("grid", [0, 16])
-def test_deferred_finalize_bit_identical_batched(E, grid):
+@pytest.mark.parametrize("lanes", [4, 16])
+def test_deferred_finalize_bit_identical_batched(E, lanes):
     """The same for six Ar-Xe replicas of uneven size: every workgroup sums the
-    slots of its replicas, and with a grid of 16 workgroups (fewer than the
-    units) workgroups span several replicas and the slots of a replica are
-    those of the workgroups that start inside it."""
+    slots of its replica's tiles."""
     reps = S.arxe_replicas(nrep=6, seed=6, nmax=1024)
-    say("deferred finalize batched", [x.n for x in reps], "grid", grid)
+    say("deferred finalize batched", [x.n for x in reps], "lanes", lanes)
     out = []
     for defer in (0, 1):
         e = E.Engine.from_system(reps)
-        e.set_launch(defer=defer, grid=grid, check_every=8, refresh=1)
+        e.set_launch(defer=defer, lanes=lanes, check_every=8, refresh=1)
         st = e.run(120, energy_every=5)
         pos, vel, frc = e.download()
         out.append((st, pos, vel, frc, e.info()["rebuilds"]))
@@ -139,7 +142,7 @@ def _nve_vv(s):
 @pytest.mark.parametrize("name,cfg", [("nacl", dict(check_every=8)),
                                       ("nacl", dict(check_every=6, defer=0, submit=1)),
                                       ("nacl", dict(check_every=4, lanes=16)),
-                                      ("argon", dict(check_every=16, lanes=32))])
+                                      ("argon", dict(check_every=16, lanes=16))])
 def test_fused_vv_bit_identical(E, name, cfg):
     """NVE velocity Verlet in leapfrog form (one step kernel per step that ends
     the step and begins the next, vv_fuse=1) follows the same trajectory bit for
@@ -202,7 +205,7 @@ def test_nve_conservation_nacl32768(E):
     s.thermostat = "none"
     say("nve nacl32768")
     e = E.Engine.from_system(s)
-    e.set_launch(lanes=8, grid=2048, check_every=16, build_lanes=8)
+    e.set_launch(lanes=8, check_every=16)
     e.run(100, energy_every=0, stats=False)
     st = e.run(1000, energy_every=50)
     m = st["has_energy"][:, 0] == 1
@@ -228,7 +231,7 @@ def test_invalid_calls_raise_and_leave_the_engine_usable(E):
         e.run(10)                                   # no force field, no system
     e.set_forcefield(s.ff)
     e.set_integrator(s.integrator, s.thermostat, s.dt, s.tau)
-    n = 1 << 25                                     # slot limit: 2^25 slots
+    n = (1 << 24) + 1                               # slot limit: 2^24 slots
     natom = np.array([n], np.int32)
     box = np.full(3, 1000.0, np.float32)
     T0 = np.array([100.0], np.float32)
@@ -239,7 +242,8 @@ def test_invalid_calls_raise_and_leave_the_engine_usable(E):
     assert rc == -4, rc                              # capacity
     del typ, q, m
     e.set_systems(s)
-    for bad in (dict(lanes=3), dict(cell_div=4), dict(check_every=0), dict(build_kernel=7), dict(defer=2)):
+    for bad in (dict(lanes=3), dict(lanes=32), dict(cell_div=4), dict(check_every=0), dict(build_kernel=1),
+                dict(defer=2)):
         with pytest.raises(E.MDError):
             e.set_launch(**bad)
     pos = s.pos.copy()
