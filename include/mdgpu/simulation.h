@@ -6,7 +6,9 @@
  *
  * The data model re-declares the reference structs (src/simulation.h:50-130)
  * field for field with plain POD vector types (md_float4 / md_uint2 /
- * md_uint4 carry the same x,y,z,w members as CUDA's float4 / uint2 / uint4).
+ * md_uint4 carry the same x,y,z,w members and the same 16 / 8 / 16-byte
+ * alignment as CUDA's float4 / uint2 / uint4), so a caller that allocates the
+ * structs itself, as src/simulation.cu:40-131 does, passes them unchanged.
  * Units and conversions are the reference's: mass x 1e4 (src/simulation.cu:177),
  * lj1 = 48 eps sig^12 / lj2 = 24 eps sig^6 (:191,194), bnd1 = 2k (:227),
  * ang1 = 2k and ang2 in radians (:242-243).
@@ -23,6 +25,11 @@
  * Optional trailing .inp lines (after "rdel:"), ignored by older files:
  *   integrator: leapfrog | vverlet
  *   thermostat: none | berendsen | nosehoover  <tau_fs>
+ * SimulationInit keeps these choices in library-side state tied to the integr
+ * it returns (the reference's integr has no fields for them, and its tcons is
+ * never set, src/integrator.cu:68).  An integr the library did not create runs
+ * the reference's own scheme: leapfrog, NVE for enstype 0, Berendsen with the
+ * reference's tco = 0.01/fs (tau 100 fs, src/integrator.cu:146) for enstype 1.
  */
 #ifndef MDGPU_SIMULATION_H
 #define MDGPU_SIMULATION_H
@@ -31,9 +38,9 @@
 #define NUM_PARTICLE_TYPES 8
 #define NUM_BOND_TYPES 8
 
-typedef struct { float x, y, z, w; } md_float4;
-typedef struct { unsigned int x, y; } md_uint2;
-typedef struct { unsigned int x, y, z, w; } md_uint4;
+typedef struct __attribute__((aligned(16))) { float x, y, z, w; } md_float4;
+typedef struct __attribute__((aligned(8))) { unsigned int x, y; } md_uint2;
+typedef struct __attribute__((aligned(16))) { unsigned int x, y, z, w; } md_uint4;
 
 typedef struct {            /* src/simulation.h:51-59 */
     int natom;
@@ -91,12 +98,12 @@ typedef struct {            /* src/simulation.h:102-106 */
     md_float4 energ_bonded;
 } thermodyn;
 
-typedef struct {            /* src/simulation.h:108-122 (+ integrator choice) */
+typedef struct {            /* src/simulation.h:108-122 */
     char* ensemble;
-    int enstype;            /* 0 NVE, 1 NVT (2 NPT is rejected) */
+    int enstype;            /* 0 NVE, 1 NVT, 2 NPT */
     float dt;
     float temper_0;
-    float tcons;            /* thermostat time constant, fs (reference hard-codes 100) */
+    float tcons;            /* not read by the run (the reference never sets it) */
     float press_0;
     float pcons;
     float temper;
@@ -105,8 +112,6 @@ typedef struct {            /* src/simulation.h:108-122 (+ integrator choice) */
     float* Ke_total;
     float* Wi_total;
     thermodyn* therm;
-    int integrator;         /* MD_LEAPFROG (reference default) or MD_VVERLET */
-    int thermostat;         /* MD_TH_*: NVT defaults to Berendsen */
 } integr;
 
 typedef struct {            /* src/simulation.h:124-130 */

@@ -58,6 +58,15 @@ def build(verbose: bool = False) -> str:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"athom link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # test program of the drop-in boundary: a caller built against the reference's
+    # struct layout (host code only, g++), linked against the library
+    caller_src = os.path.join(ROOT, "tests", "ref_layout_caller.cpp")
+    if os.path.exists(caller_src):
+        cmd = ["g++", "-O1", "-std=c++17", caller_src, "-o", os.path.join(BIN, "ref_layout_caller"), "-L", HERE,
+               "-lmdgpu", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"caller link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -210,6 +210,7 @@ struct md_engine {
     DBuf<uint16_t> nl16;           // [S][cap] rows of local indices into the tile sets
     DBuf<uint32_t> jset;           // [S/64][jcap] tile neighbor sets
     DBuf<int> ncount, jcount;
+    DBuf<uint8_t> rperm;           // [S] per tile: atoms by decreasing row length
     DBuf<int> tmax;                // [2][S/64] builds' longest row / largest set per tile (running maxima)
     DBuf<int> tflag;               // [S/64] skin-check flag per tile (kick)
     DBuf<RunReport> report;        // run progress for the host
@@ -456,7 +457,7 @@ BuildArgs build_args(md_engine* e) {
     b.pos_build = e->pos_build.p;
     b.key = e->key.p; b.rank = e->rank.p; b.tmp = e->tmp.p; b.tmpid = e->tmpid.p;
     b.cell_count = e->cell_count.p; b.cell_start = e->cell_start.p; b.cell_rank = e->cell_rank.p;
-    b.slot_of = e->slot_of.p; b.nl16 = e->nl16.p; b.ncount = e->ncount.p;
+    b.slot_of = e->slot_of.p; b.nl16 = e->nl16.p; b.ncount = e->ncount.p; b.rperm = e->rperm.p;
     b.jset = e->jset.p; b.jcount = e->jcount.p;
     b.tilemax = e->tmax.p; b.jmax = e->tmax.p + e->ntiles();
     b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p;
@@ -471,6 +472,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     StepArgs a{};
     a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
     a.ids = e->ids.p; a.typ = e->typ.p; a.nl16 = e->nl16.p; a.ncount = e->ncount.p; a.pos_build = e->pos_build.p;
+    a.rperm = e->rperm.p;
     a.jset = e->jset.p; a.jcount = e->jcount.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
     a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0; a.open = 0;
@@ -674,7 +676,7 @@ int md_create(md_engine** out, int device) {
     }
     // the step and force-matching kernels hold the staged set and the dummy
     // (16 bytes each) next to at most 8 KB of static LDS
-    e->jcap_max = (int)std::min<size_t>(65535, (lds - 8192) / 16 - 1) / 64 * 64;
+    e->jcap_max = (int)std::min<size_t>(MD_JMAX, (lds - 8192) / 16 - 1) / 64 * 64;
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return fail(MD_E_HIP, "hipStreamCreate failed");
@@ -823,6 +825,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
                     &e->snap_ids, &e->snap_typ})
         CHECK(b->alloc(Sz));
     CHECK(e->jcount.alloc(S / MD_TILE));
+    CHECK(e->rperm.alloc(Sz));
     CHECK(e->slot_of.alloc(N));
     CHECK(e->snap_slot_of.alloc(N));
     CHECK(e->rep.alloc(nrep));
@@ -1232,7 +1235,7 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
         const int t = s / MD_TILE;
         int c = std::min(std::min(nc[s], cap), e->cap);
         for (int k = 0; k < c; k++) {
-            const int li = L[(size_t)s * e->cap + lpos16(k, e->lanes)];
+            const int li = L[(size_t)s * e->cap + lpos16(k, e->lanes)] >> 4;   // entries are byte offsets
             if (li < 1 || li > jc[t]) return fail(MD_E_STATE, "row of slot %d holds set index %d outside 1..%d", s, li, jc[t]);
             nbrs[(size_t)i * cap + k] = ids[J[(size_t)t * e->jcap + li - 1] & MD_SLOTMASK];
         }

@@ -14,9 +14,13 @@
 // Neighbor list, per 64-slot tile t (a tile never straddles two replicas):
 //   jset[t][k]  uint32 the tile's neighbor set: slot | image shift code << 24 | type << 29,
 //               k < jcount[t]; local index k + 1 (local index 0 is a far-away dummy)
-//   nl16[S][cap] uint16 per-atom rows of local indices into the tile's set, in the
-//               block layout of lpos16(); entries past the row length up to the end
+//   nl16[S][cap] uint16 per-atom rows: local index into the tile's set times 16 (the
+//               byte offset of the staged entry; sets hold at most MD_JMAX entries), in
+//               the block layout of lpos16(); entries past the row length up to the end
 //               of the row's last block are 0 (the dummy), ncount[S] = row length
+//   rperm[S]    uint8 per tile: the tile's atoms (0..63) by decreasing row length; the
+//               step kernel gives lane group g the atom rperm[g], so the atoms of a
+//               wavefront have rows of similar length
 // The step kernel stages the tile's set into LDS once per step, each position
 // already shifted to its image, so a pair needs no minimum-image arithmetic.
 // Replicas occupy 64-aligned slot ranges, so a wavefront never straddles two.
@@ -41,6 +45,7 @@
 #define MD_SUBSHIFT 25          // list build: ranks and original indices stay below 2^25 (the slot limit is
                                 // lower); the eighth of the cell an atom sits in is packed above them
 #define MD_FAR 1.0e5f           // coordinate of the dummy neighbor (local index 0)
+#define MD_JMAX 4095            // largest tile neighbor set (a row entry is the local index times 16)
 
 enum { ACC_KE2 = 0, ACC_LJ, ACC_COUL, ACC_BOND, ACC_ANGLE, ACC_VIR };
 enum { MODE_FORCE = 0, MODE_LF = 1, MODE_VV = 2 };
@@ -158,8 +163,9 @@ struct StepArgs {
     float4* force;
     const int* ids;
     const int* typ;
-    const uint16_t* nl16;     // [S][cap] rows of local indices (lpos16 layout)
+    const uint16_t* nl16;     // [S][cap] rows of local indices x 16 (lpos16 layout)
     const int* ncount;
+    const uint8_t* rperm;     // [S] per tile: atoms by decreasing row length
     const uint32_t* jset;     // [ntiles][jcap] tile neighbor sets
     const int* jcount;        // [ntiles]
     const float4* pos_build;
@@ -267,6 +273,7 @@ struct BuildArgs {
     int* slot_of;         // [N]
     uint16_t* nl16;
     int* ncount;
+    uint8_t* rperm;
     uint32_t* jset;
     int* jcount;
     int* tilemax;         // [S/64] longest row per 64-slot tile over the builds since the host cleared it

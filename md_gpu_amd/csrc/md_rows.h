@@ -4,12 +4,14 @@
 // A workgroup owns one 64-slot tile.  At its start it stages the tile's
 // neighbor set (md_device.h) into LDS: entry k of jset becomes local index
 // k + 1 with the position of the recorded image, x_j + k L (the shift table
-// s_sh holds k L per shift code), and the type of j packed into the three low
-// mantissa bits of the staged charge (q_j changes by at most 7 ulp; the type
-// selects the LJ row).  Local index 0 is a dummy far outside every box, the
-// padding of the rows' last blocks, so no entry needs a length test: its pair
-// lies beyond the cutoff.  The pair displacement is then x_i - x_j(image):
-// three subtractions, no minimum image.
+// s_sh holds k L per shift code), and in w the type of j packed into the three
+// low mantissa bits of the staged charge (q_j changes by at most 7 ulp; the type
+// selects the LJ row), or for systems without charges the type itself as a
+// float.  Local index 0 is a dummy far outside every box, the padding of the
+// rows' last blocks, so no entry needs a length test: its pair lies beyond the
+// cutoff.  The pair displacement is then x_i - x_j(image): three subtractions,
+// no minimum image.  Row entries are byte offsets into the staged set (local
+// index times 16), so an entry addresses LDS without a shift.
 #pragma once
 #include "md_device.h"
 
@@ -22,9 +24,18 @@ __device__ __forceinline__ void stage_shift_table(float4* s_sh, float L0, float 
     }
 }
 
+// w of a staged neighbor: the charge with the type in its low mantissa bits, or
+// (CHARGED = false) the type as a float
+template <bool CHARGED>
+__device__ __forceinline__ float staged_w(float q, uint32_t e) {
+    if (CHARGED) return __uint_as_float((__float_as_uint(q) & ~7u) | (e >> MD_JBITS));
+    return (float)(e >> MD_JBITS);
+}
+
 // The tile's set into s_j[1 .. jn]; s_j[0] = the dummy.  pin = the replica's
 // current position buffer.  s_sh must be visible (barrier) before the call;
 // the caller places a barrier after it.
+template <bool CHARGED = true>
 __device__ __forceinline__ void stage_tile_set(float4* s_j, const float4* s_sh, const float4* __restrict__ pin,
                                                const uint32_t* __restrict__ jset, int jn) {
     const int nth = (int)blockDim.x;
@@ -40,16 +51,14 @@ __device__ __forceinline__ void stage_tile_set(float4* s_j, const float4* s_sh, 
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const float4 sv = s_sh[(e[u] >> MD_SLOTBITS) & 31];
-            s_j[k + u * nth + 1] = make_float4(x[u].x + sv.x, x[u].y + sv.y, x[u].z + sv.z,
-                                               __uint_as_float((__float_as_uint(x[u].w) & ~7u) | (e[u] >> MD_JBITS)));
+            s_j[k + u * nth + 1] = make_float4(x[u].x + sv.x, x[u].y + sv.y, x[u].z + sv.z, staged_w<CHARGED>(x[u].w, e[u]));
         }
     }
     for (; k < jn; k += nth) {
         const uint32_t e = jset[k];
         const float4 x = pin[e & MD_SLOTMASK];
         const float4 sv = s_sh[(e >> MD_SLOTBITS) & 31];
-        s_j[k + 1] = make_float4(x.x + sv.x, x.y + sv.y, x.z + sv.z,
-                                 __uint_as_float((__float_as_uint(x.w) & ~7u) | (e >> MD_JBITS)));
+        s_j[k + 1] = make_float4(x.x + sv.x, x.y + sv.y, x.z + sv.z, staged_w<CHARGED>(x.w, e));
     }
     if (threadIdx.x == 0) s_j[0] = make_float4(MD_FAR, MD_FAR, MD_FAR, 0.0f);
 }
@@ -60,23 +69,30 @@ __device__ __forceinline__ int staged_type(const float4& p) { return (int)(__flo
 // Row walk: lane `sub` of the atom's G lanes loads 16-byte blocks of its row
 // (8 local indices: logical entries 8Gb + sub + G u, u = 0..7), two blocks ahead
 // of the one it computes, reads the 8 staged neighbors from LDS and calls
-// f(pj) for each.  The row length nn is a multiple of nothing: the last block's
-// entries past nn are the dummy.
+// f(pj) for each.  Entries of the last block past the row length nn are the
+// dummy.  e0, e1: the row's blocks 0 and 1 (block 1 only read if nb > 1).
 template <int G, class F>
-__device__ __forceinline__ void walk_row(const float4* s_j, const uint4* __restrict__ row4, int nn, int sub, F&& f) {
+__device__ __forceinline__ void walk_row_from(const float4* s_j, const uint4* __restrict__ row4, int nn, int sub,
+                                              uint4 e0, uint4 e1, F&& f) {
     const int nb = (int)(((uint32_t)nn + 8 * G - 1) / (8 * G));
-    if (nb == 0) return;
-    uint4 e0 = row4[sub];
-    uint4 e1 = row4[min(1, nb - 1) * G + sub];
     for (int b = 0; b < nb; b++) {
         const uint4 e2 = row4[min(b + 2, nb - 1) * G + sub];
         const uint32_t w[4] = {e0.x, e0.y, e0.z, e0.w};
         float4 pj[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) pj[u] = s_j[(u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xFFFFu)];
+        for (int u = 0; u < 8; u++)   // entries are byte offsets of the staged neighbors
+            pj[u] = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_j) +
+                                                     ((u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xFFFFu)));
 #pragma unroll
         for (int u = 0; u < 8; u++) f(pj[u]);
         e0 = e1;
         e1 = e2;
     }
+}
+
+template <int G, class F>
+__device__ __forceinline__ void walk_row(const float4* s_j, const uint4* __restrict__ row4, int nn, int sub, F&& f) {
+    const int nb = (int)(((uint32_t)nn + 8 * G - 1) / (8 * G));
+    if (nb == 0) return;
+    walk_row_from<G>(s_j, row4, nn, sub, row4[sub], row4[min(1, nb - 1) * G + sub], f);
 }

@@ -272,6 +272,7 @@ __global__ __launch_bounds__(512) void k_nl_tbuild(BuildArgs a) {
     __shared__ float s_bb[6];
     __shared__ int s_w[NW];
     __shared__ int s_wmax[NW];
+    __shared__ int s_cnt[64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
     const int slot0 = tile * 64;
@@ -373,8 +374,9 @@ __global__ __launch_bounds__(512) void k_nl_tbuild(BuildArgs a) {
                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)any, 0u));
         if (wave == 0 && mine && li - 1 < jcap) jset[li - 1] = info;
         // a set that outgrew its capacity is rebuilt after the host grows it (the run
-        // is rolled back); until then its rows point past it only at the dummy
-        const uint16_t lis = (uint16_t)(li - 1 < jcap ? li : 0);
+        // is rolled back); until then its rows point past it only at the dummy.  Row
+        // entries are byte offsets of the staged entry (li x 16, li <= MD_JMAX).
+        const uint16_t lis = (uint16_t)(li - 1 < jcap ? li << 4 : 0);
 #pragma unroll
         for (int i = 0; i < AW; i++) {
             if (m[i]) {
@@ -488,10 +490,25 @@ __global__ __launch_bounds__(512) void k_nl_tbuild(BuildArgs a) {
         mx = max(mx, c);
         const int endb = min((c + blk - 1) / blk * blk, cap);
         for (int e = c + lane; e < endb; e += 64) wrow[(size_t)i * cap + lpos16(e, blk / 8)] = 0;
-        if (lane == i) a.ncount[slot0 + wave * AW + i] = __float_as_int(s_ax[wave * AW + i].w) >= 0 ? c : 0;
+        if (lane == i) {
+            const int cc = __float_as_int(s_ax[wave * AW + i].w) >= 0 ? c : 0;
+            a.ncount[slot0 + wave * AW + i] = cc;
+            s_cnt[wave * AW + i] = cc;
+        }
     }
     if (lane == 0) s_wmax[wave] = mx;
     __syncthreads();
+    // the tile's atoms by decreasing row length (ties by slot): the step kernel's
+    // wavefronts then hold atoms whose rows end in the same block
+    if (tid < 64) {
+        const int c = s_cnt[tid];
+        int rank = 0;
+        for (int k = 0; k < 64; k++) {
+            const int d = s_cnt[k];
+            rank += (d > c) || (d == c && k < tid);
+        }
+        a.rperm[slot0 + rank] = (uint8_t)tid;
+    }
     // copy the tile back from scratch into the primary arrays (new order)
     if (tid < 64) {
         const int s = slot0 + tid;
@@ -587,8 +604,9 @@ __global__ __launch_bounds__(256) void k_check_lists(BuildArgs a, int row_g, int
     const uint32_t* js = a.jset + (size_t)tile * a.jcap;
     const int n = min(nn, a.cap);
     for (int e = 0; e < n; e++) {
-        const int li = row[lpos16(e, row_g)];
-        if (li < 1 || li > jn) {
+        const int off = row[lpos16(e, row_g)];
+        const int li = off >> 4;
+        if ((off & 15) || li < 1 || li > jn) {
             atomicAdd(&bad[1], 1);
             continue;
         }

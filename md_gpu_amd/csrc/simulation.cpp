@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,24 @@ namespace {
 
 int g_status = MD_OK;
 const float kPiRef = 3.1415927f;   // M_Pi of src/simulation.h:37
+
+// Integrator choice of an integr made by SimulationInit (the optional .inp keys).
+// The reference's integr has no fields for it, so it is kept here, keyed by the
+// struct's address, and dropped by SimulationFinish.
+struct RunOptions {
+    int integrator = MD_LEAPFROG;   // the reference loop runs leapfrog (vver = 0, src/runGPU.cu:71)
+    int thermostat = -1;            // -1: the ensemble's default
+    float tau = 100.0f;             // reference tco = 0.01 /fs (src/integrator.cu:146)
+};
+std::mutex g_opt_mu;
+std::map<const integr*, RunOptions> g_opts;
+
+RunOptions options_of(const integr* in, bool* found) {
+    std::lock_guard<std::mutex> lk(g_opt_mu);
+    auto it = g_opts.find(in);
+    *found = it != g_opts.end();
+    return *found ? it->second : RunOptions{};
+}
 
 bool rd(int got, int want, const char* what, const std::string& file) {
     if (got != want) {
@@ -235,15 +255,13 @@ simulation* SimulationInit(char* name_sim) {
     strcpy(sim->simname, name_sim);
     params* pars = alloc<params>(1);
     integr* integ = alloc<integr>(1);
+    RunOptions opt;
     char saux[256];
     integ->ensemble = alloc<char>(16);
-    integ->tcons = 100.0f;                 // reference tco = 0.01 /fs (src/integrator.cu:146)
-    integ->integrator = MD_LEAPFROG;       // the reference loop runs leapfrog (vver = 0, src/runGPU.cu:71)
-    integ->thermostat = MD_TH_NONE;
+    integ->tcons = 100.0f;
     bool ok = rd(fscanf(fin, "%255s %15s", saux, integ->ensemble), 2, "ensemble", file);
     if (ok && strcmp(integ->ensemble, "nvt") == 0) {
         integ->enstype = 1;
-        integ->thermostat = MD_TH_BERENDSEN;
         ok = rd(fscanf(fin, "%255s %f", saux, &integ->temper_0), 2, "temperature", file);
     } else if (ok && strcmp(integ->ensemble, "npt") == 0) {
         integ->enstype = 2;
@@ -271,13 +289,14 @@ simulation* SimulationInit(char* name_sim) {
     while (ok && fscanf(fin, "%255s", saux) == 1) {
         char val[64];
         if (strcmp(saux, "integrator:") == 0 && fscanf(fin, "%63s", val) == 1) {
-            if (strcmp(val, "vverlet") == 0) integ->integrator = MD_VVERLET;
-            else if (strcmp(val, "leapfrog") == 0) integ->integrator = MD_LEAPFROG;
+            if (strcmp(val, "vverlet") == 0) opt.integrator = MD_VVERLET;
+            else if (strcmp(val, "leapfrog") == 0) opt.integrator = MD_LEAPFROG;
             else { fprintf(stderr, "md_gpu: %s: unknown integrator %s\n", file.c_str(), val); ok = false; }
-        } else if (strcmp(saux, "thermostat:") == 0 && fscanf(fin, "%63s %f", val, &integ->tcons) == 2) {
-            if (strcmp(val, "none") == 0) integ->thermostat = MD_TH_NONE;
-            else if (strcmp(val, "berendsen") == 0) integ->thermostat = MD_TH_BERENDSEN;
-            else if (strcmp(val, "nosehoover") == 0) integ->thermostat = MD_TH_NOSEHOOVER;
+        } else if (strcmp(saux, "thermostat:") == 0 && fscanf(fin, "%63s %f", val, &opt.tau) == 2) {
+            integ->tcons = opt.tau;
+            if (strcmp(val, "none") == 0) opt.thermostat = MD_TH_NONE;
+            else if (strcmp(val, "berendsen") == 0) opt.thermostat = MD_TH_BERENDSEN;
+            else if (strcmp(val, "nosehoover") == 0) opt.thermostat = MD_TH_NOSEHOOVER;
             else { fprintf(stderr, "md_gpu: %s: unknown thermostat %s\n", file.c_str(), val); ok = false; }
         } else {
             fprintf(stderr, "md_gpu: %s: unknown key %s\n", file.c_str(), saux);
@@ -285,7 +304,6 @@ simulation* SimulationInit(char* name_sim) {
         }
     }
     fclose(fin);
-    if (ok && integ->enstype == 0) integ->thermostat = MD_TH_NONE;
     if (ok) {
         sim->fld = FFieldInput(name_sim);
         ok = sim->fld != nullptr;
@@ -304,6 +322,10 @@ simulation* SimulationInit(char* name_sim) {
     }
     integ->temper = integ->temper_0;
     integ->Ke_sum = alloc<float>(1);
+    {
+        std::lock_guard<std::mutex> lk(g_opt_mu);
+        g_opts[integ] = opt;
+    }
     return sim;
 }
 
@@ -347,11 +369,15 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
     ff.nangle_types = fld->tangle;
     for (int i = 0; i < fld->tangle; i++) { ff.angle_k2[i] = fld->ang1[i]; ff.angle_t0[i] = fld->ang2[i]; }
     if ((rc = md_set_forcefield(e, &ff))) return die(e, rc);
+    // integrator and thermostat: the options SimulationInit recorded for this integr,
+    // else the reference's scheme (tcons is not read: the reference never sets it)
+    bool own = false;   // the integr was made by SimulationInit (its Ke_sum is a host buffer)
+    const RunOptions opt = options_of(integ, &own);
     md_integrator in;
-    in.integrator = integ->integrator;
-    in.thermostat = integ->enstype == 0 ? MD_TH_NONE : integ->thermostat;
+    in.integrator = opt.integrator;
+    in.thermostat = integ->enstype == 0 ? MD_TH_NONE : (opt.thermostat >= 0 ? opt.thermostat : MD_TH_BERENDSEN);
     in.dt = integ->dt;
-    in.tau = integ->tcons > 0 ? integ->tcons : 100.0f;
+    in.tau = opt.tau > 0 ? opt.tau : 100.0f;
     if ((rc = md_set_integrator(e, &in))) return die(e, rc);
     const int N = conf->natom;
     std::vector<int> type(N);
@@ -392,8 +418,9 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
         if (last % prn == 0) {
             const md_step_stats& s = st[k - 1];
             integ->temper = (float)s.temp;
-            integ->Ke_sum[0] = (float)s.ke;
-            printf("%i\t%f\t%f\n", last, integ->temper, integ->Ke_sum[0]);   // src/prints.cu:39
+            const float ke = (float)s.ke;
+            if (own) integ->Ke_sum[0] = ke;   // a caller's integr may hold no host buffer here
+            printf("%i\t%f\t%f\n", last, integ->temper, ke);   // src/prints.cu:39
             if ((rc = md_download_state(e, nullptr, vel.data(), frc.data()))) return die(e, rc);
             double fs[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
             for (int i = 0; i < N; i++)
@@ -433,7 +460,15 @@ void SimulationFinish(simulation* sim) {
     ConfigFinish(sim->conf, sim->fld, sim->simname);
     FFieldFinish(sim->fld);
     free(sim->pars);
-    if (sim->integ) { free(sim->integ->ensemble); free(sim->integ->Ke_sum); free(sim->integ); }
+    if (sim->integ) {
+        {
+            std::lock_guard<std::mutex> lk(g_opt_mu);
+            g_opts.erase(sim->integ);
+        }
+        free(sim->integ->ensemble);
+        free(sim->integ->Ke_sum);
+        free(sim->integ);
+    }
     free(sim->simname);
     free(sim);
 }

@@ -316,8 +316,11 @@ __device__ __forceinline__ float pair_ff_lj(float rsq, const float4 lj) {
 }  // namespace
 
 // ENMODE 0: no energies (compiled out); 1: energies where requested (runtime, per replica)
-template <int G, int COUL, int ENMODE>
-__global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
+// NT2: at most two atom types, so the LJ parameters of a pair are one of the
+// two entries of the atom's table row, held in registers and picked by a select
+// (no LDS read per pair); otherwise the row is read from LDS per pair.
+template <int G, int COUL, int ENMODE, bool NT2>
+__global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepArgs a) {
     constexpr int NW = G;                 // wavefronts: 64 atoms x G lanes
     extern __shared__ float4 s_j[];       // [jcap + 1] the tile's staged neighbor set
     __shared__ float4 s_lj[64];
@@ -326,11 +329,18 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
     __shared__ double s_red[NW][MD_NSTAT];
     static_assert(MD_CTAB == 256, "the table is loaded by the first 256 threads");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int sub = tid & (G - 1), ai = tid / G;
+    const int sub = tid & (G - 1);
     const int tile = xcd_block(blockIdx.x, gridDim.x);   // tiles contiguous per XCD
-    const int slot = tile * 64 + ai;
-    // One memory round trip before any work: the gate, the tables, the replica's
-    // state and constants, the set size and the atom's own data.
+    // lane group tid / G takes the atom of that rank in the tile's row-length order
+    const int slot = tile * 64 + a.rperm[tile * 64 + tid / G];
+    // Loads that do not depend on the replica's state, issued before any is used:
+    // the gate, the tables, the replica's constants and state, the tile's set
+    // entries (JPRE per thread; entries past the set are loaded and not used:
+    // every entry of the buffer is some slot), the atom's data in both position
+    // buffers, its row's first two blocks, and its velocity and build position
+    // (epilogue).
+    // (LJ-only systems have sparser sets and need the registers for occupancy)
+    constexpr int NTH = 64 * G, JPRE = (COUL == 3 ? 16 : 32) / G;
     const Gate gate = *a.gate;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
     const float4 lj_v = a.ljtab[tid & 63];
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
         if (tid < 256) ct_v = a.ctab[tid];
         ct_e = a.ctab[MD_CTAB];
     }
-    const int jn = a.jcount[tile];
+    const int jn = min(a.jcount[tile], a.jcap);
     const int id = a.ids[slot], tyi = a.typ[slot], nci = a.ncount[slot];
     RepState s = a.lag ? a.lst[(size_t)(a.par ^ 1) * a.nrep + r] : a.st[r];
     const RepParam& pr = a.rep[r];
@@ -348,6 +358,19 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
     const float L0 = pr.L[0], L1 = pr.L[1], L2 = pr.L[2];
     const float Li0 = pr.Linv[0], Li1 = pr.Linv[1], Li2 = pr.Linv[2];
     const double c_S = a.th[r].S;
+    const uint32_t* __restrict__ jt = a.jset + (size_t)tile * a.jcap;
+    uint32_t je[JPRE];
+#pragma unroll
+    for (int u = 0; u < JPRE; u++) je[u] = jt[min(tid + u * NTH, a.jcap - 1)];
+    const float4 xa = a.pos[0][slot], xb = a.pos[1][slot];
+    const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(a.nl16 + (size_t)slot * a.cap);
+    const uint4 q0 = row4[sub];
+    const uint4 q1 = row4[(a.cap >= 16 * G ? G : 0) + sub];   // block 1 when the capacity holds one
+    float4 v_e = make_float4(0.f, 0.f, 0.f, 0.f), pb_e = v_e;
+    if (sub == 0) {
+        v_e = a.vel[slot];
+        pb_e = a.pos_build[slot];
+    }
     loads_issued();
     if (tid < 64) s_lj[tid] = lj_v;
     if (COUL == 1 || COUL == 2) {
@@ -360,6 +383,54 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
     if (a.mode == MODE_VV && gate.n_force == 0) return;
     const int t0 = so / 64, t1 = (so + ns) / 64;
     const bool leader = tile == t0;   // the workgroup of the replica's first tile
+    // The position buffer this step reads: the deferred finalize below advances the
+    // step counter (and ends a fused velocity-Verlet run's phase) first.
+    const bool lagsum = a.lag && s.ran;
+    RepState sp = s;
+    if (lagsum) {
+        sp.step += 1;
+        if (a.vvfused && sp.step >= sp.target) sp.phase = 0;
+    }
+    const float4* __restrict__ pin = a.pos[cur_buf(sp)];
+    // Round trip 2: the neighbors' positions and, for the deferred finalize, the
+    // previous launch's slots of the replica (added in the order of sum_tiles).
+    float4 xg[JPRE];
+#pragma unroll
+    for (int u = 0; u < JPRE; u++) xg[u] = pin[(tid + u * NTH < jn) ? (je[u] & MD_SLOTMASK) : 0u];
+    const bool en_prev = lagsum && a.energy_every > 0 && (s.step % a.energy_every) == 0;
+    const bool sum_all = leader && en_prev && a.stats;
+    double v[MD_NSTAT];
+#pragma unroll
+    for (int k = 0; k < MD_NSTAT; k++) v[k] = 0.0;
+    if (lagsum) {
+        const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.ntiles;
+        for (int t = t0 + tid; t < t1; t += NTH) {
+#pragma unroll
+            for (int j = 0; j < MD_NSTAT; j++)
+                if (sum_all || j == ACC_KE2 || j == MD_NSTAT - 1) v[j] += prev[(size_t)j * a.ntiles + t];
+        }
+    }
+    // the staged set: the image of each neighbor (x + k L per shift code, the
+    // table s_sh), in w the type of j (packed into the charge's low mantissa bits,
+    // or as a float without charges); local index 0 = the dummy
+    constexpr bool CHG = COUL != 3;
+    __syncthreads();   // s_sh visible
+#pragma unroll
+    for (int u = 0; u < JPRE; u++) {
+        const int k = tid + u * NTH;
+        if (k < jn) {
+            const uint32_t e = je[u];
+            const float4 sv = s_sh[(e >> MD_SLOTBITS) & 31u];
+            s_j[k + 1] = make_float4(xg[u].x + sv.x, xg[u].y + sv.y, xg[u].z + sv.z, staged_w<CHG>(xg[u].w, e));
+        }
+    }
+    for (int k = tid + JPRE * NTH; k < jn; k += NTH) {   // sets beyond JPRE entries per thread
+        const uint32_t e = jt[k];
+        const float4 x = pin[e & MD_SLOTMASK];
+        const float4 sv = s_sh[(e >> MD_SLOTBITS) & 31u];
+        s_j[k + 1] = make_float4(x.x + sv.x, x.y + sv.y, x.z + sv.z, staged_w<CHG>(x.w, e));
+    }
+    if (tid == 0) s_j[0] = make_float4(MD_FAR, MD_FAR, MD_FAR, 0.0f);
     double lamd = 1.0;
     if (a.lag) {
         // Deferred finalize of the previous launch (same block): every workgroup sums
@@ -367,12 +438,9 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
         // the hand-over state; the leader also writes the statistics record.  Same
         // arithmetic as k_step_finalize.
         lamd = s.lam;
-        if (s.ran) {
-            const bool en_prev = a.energy_every > 0 && (s.step % a.energy_every) == 0;
-            double v[MD_NSTAT];
-            const double* prev = a.part + (size_t)(a.par ^ 1) * MD_NSTAT * a.ntiles;
-            if (leader && en_prev && a.stats) sum_tiles<NW, true>(prev, t0, t1, a.ntiles, s_red, v);
-            else sum_tiles<NW, false>(prev, t0, t1, a.ntiles, s_red, v);
+        if (lagsum) {
+            if (sum_all) reduce_slots<NW, true>(s_red, v);
+            else reduce_slots<NW, false>(s_red, v);
             const long long kst = s.step - s.start;
             double T;
             lf_done(a, nf, T0, s, v[ACC_KE2], v[MD_NSTAT - 1] != 0.0, T, lamd);
@@ -400,25 +468,31 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
     const bool en = ENMODE != 0 && ((a.mode == MODE_FORCE) ? (a.force_energy != 0)
                                                            : (a.energy_every > 0 && (s.step % a.energy_every) == 0));
     const int cb = cur_buf(s);
-    const float4* __restrict__ pin = a.pos[cb];
     float4* pout = a.pos[cb ^ 1];
     const float lam = (float)lamd;
     const bool drift = !a.vvfused || s.step + 1 < s.target;
-    const float4 xi = pin[slot];
+    const float4 xi = cb ? xb : xa;
     const bool active = id >= 0;
     const int nn = active ? min(nci, a.cap) : 0;
-    const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(a.nl16 + (size_t)slot * a.cap);
-    __syncthreads();   // shift table visible
-    stage_tile_set(s_j, s_sh, pin, a.jset + (size_t)tile * a.jcap, min(jn, a.jcap));
-    __syncthreads();
+    __syncthreads();   // tables and the staged set visible
 
     const float qi = xi.w * a.qfac * a.qq_scale;
     const float4* s_lji = s_lj + (active ? tyi : 0) * 8;
+    const float4 lj_a = s_lji[0], lj_b = s_lji[1];
+    const float4 lj_d = make_float4(lj_b.x - lj_a.x, lj_b.y - lj_a.y, lj_b.z - lj_a.z, 0.0f);
     float fx = 0.f, fy = 0.f, fz = 0.f;
     double elj = 0.0, eco = 0.0, evi = 0.0;
     auto pair = [&](const float4& pj, auto en_tag) {
         constexpr bool EN = decltype(en_tag)::value;
-        const float4 lj = s_lji[staged_type(pj)];
+        float4 lj;
+        if (NT2 && !CHG) {   // w = type (0 or 1) as a float: the row entry by interpolation
+            lj = make_float4(fmaf(pj.w, lj_d.x, lj_a.x), fmaf(pj.w, lj_d.y, lj_a.y), fmaf(pj.w, lj_d.z, lj_a.z), 0.0f);
+        } else if (NT2) {
+            const bool tb = (__float_as_uint(pj.w) & 1u) != 0;   // type 1 (types are 0 or 1)
+            lj = make_float4(tb ? lj_b.x : lj_a.x, tb ? lj_b.y : lj_a.y, tb ? lj_b.z : lj_a.z, 0.0f);
+        } else {
+            lj = s_lji[CHG ? staged_type(pj) : (int)pj.w];
+        }
         const float dx = xi.x - pj.x, dy = xi.y - pj.y, dz = xi.z - pj.z;
         const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         float ulj = 0.0f, ucoul = 0.0f;
@@ -437,9 +511,9 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
         }
     };
     if (ENMODE != 0 && en)
-        walk_row<G>(s_j, row4, nn, sub, [&](const float4& pj) { pair(pj, std::true_type{}); });
+        walk_row_from<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::true_type{}); });
     else
-        walk_row<G>(s_j, row4, nn, sub, [&](const float4& pj) { pair(pj, std::false_type{}); });
+        walk_row_from<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::false_type{}); });
 
     // bonded terms (lanes of the atom's group share them)
     double ebo = 0.0, ean = 0.0, bvi = 0.0;
@@ -507,8 +581,8 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
     if (active && sub == 0) {
         a.force[slot] = make_float4(fx, fy, fz, 0.0f);
         if (a.mode == MODE_LF) {
-            float4 v = a.vel[slot];
-            const float4 pb = a.pos_build[slot];   // for the skin check: same round trip as v
+            float4 v = v_e;                 // loaded in the first round trip
+            const float4 pb = pb_e;         // for the skin check
             if (a.vvfused) {
                 // velocity Verlet, NVE: v(t) = v(t - dt/2) + F dt/2m ends the step (its kinetic
                 // energy is the step's); v(t + dt/2) = v(t) + F dt/2m begins the next.  The same
@@ -539,7 +613,7 @@ __global__ __launch_bounds__(64 * G, 4) void k_step(const StepArgs a) {
                 if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
             }
         } else if (a.mode == MODE_VV) {
-            float4 v = a.vel[slot];
+            float4 v = v_e;
             const float h = 0.5f * a.dt * v.w;
             v.x = fmaf(fx, h, v.x);
             v.y = fmaf(fy, h, v.y);
@@ -853,22 +927,28 @@ static void launch_k(K kern, const StepArgs& a, int grid, int threads, size_t sh
     else hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), shmem, st, a);
 }
 
-template <int G, int EM>
+template <int G, int EM, bool NT2>
 static void launch_ge(const StepArgs& a, int coul, size_t shmem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const int grid = a.ntiles;
     switch (coul) {
-    case 0: launch_k(k_step<G, 0, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
-    case 1: launch_k(k_step<G, 1, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
-    case 2: launch_k(k_step<G, 2, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
-    default: launch_k(k_step<G, 3, EM>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    case 0: launch_k(k_step<G, 0, EM, NT2>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    case 1: launch_k(k_step<G, 1, EM, NT2>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    case 2: launch_k(k_step<G, 2, EM, NT2>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
+    default: launch_k(k_step<G, 3, EM, NT2>, a, grid, 64 * G, shmem, st, ev0, ev1); break;
     }
 }
 
 template <int G>
 static void launch_g(const StepArgs& a, int coul, size_t shmem, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const bool em = (a.mode == MODE_FORCE) ? a.force_energy != 0 : a.energy_every > 0;
-    if (em) launch_ge<G, 1>(a, coul, shmem, st, ev0, ev1);
-    else launch_ge<G, 0>(a, coul, shmem, st, ev0, ev1);
+    const bool nt2 = a.ntype <= 2;
+    if (em) {
+        if (nt2) launch_ge<G, 1, true>(a, coul, shmem, st, ev0, ev1);
+        else launch_ge<G, 1, false>(a, coul, shmem, st, ev0, ev1);
+    } else {
+        if (nt2) launch_ge<G, 0, true>(a, coul, shmem, st, ev0, ev1);
+        else launch_ge<G, 0, false>(a, coul, shmem, st, ev0, ev1);
+    }
 }
 
 // dynamic LDS of the step kernel: the staged set and its dummy

@@ -10,18 +10,24 @@ Counter collection serialises every dispatch, so this driver keeps the run
 short, needs no autotuning and reports (rather than raises) a failed run.
 With --timed every step is launched directly with HIP events around it (the
 same timing path bench.py uses for kernel durations).
+
+The JSON line ends with a checksum of the final state (FP64 sums of the
+downloaded positions and velocities, and their hex digest) and the device
+list-invariant counters.  The engine is deterministic, so a run under a
+profiler is trustworthy only if its checksum equals that of the same command
+run without one.
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import sys
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--launch", default="lanes=8,cell_div=2,check_every=8,"
-                                        "refresh=1,defer=1,submit=1")
+    ap.add_argument("--launch", default="lanes=8,cell_div=2,check_every=8,refresh=1,defer=1,submit=1")
     ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe"])
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=0)
@@ -38,11 +44,19 @@ def main() -> int:
         if args.timed:
             e.timing(True)
         e.run(args.steps, stats=False)
-        out = dict(ok=True, system=args.system, launch=e.get_launch(), steps=args.steps)
+        out = dict(ok=True, system=args.system, launch=e.get_launch(), steps=args.steps,
+                   rebuilds=e.info()["rebuilds"])
         if args.timed:
             out["timing"] = e.timing_get()
     except E.MDError as x:
         out = dict(ok=False, error=str(x))
+    try:
+        pos, vel, _ = e.download()
+        out["checksum"] = dict(pos=float(pos.astype("f8").sum()), vel=float(vel.astype("f8").sum()),
+                               digest=hashlib.sha1(pos.tobytes() + vel.tobytes()).hexdigest()[:16])
+        out["invariants"] = e.check_lists()
+    except E.MDError as x:
+        out["checksum"] = dict(error=str(x))
     print(json.dumps(out), flush=True)
     e.close()
     return 0

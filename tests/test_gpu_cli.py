@@ -1,6 +1,7 @@
 """The athom tool (reference CLI, src/athom.cu:30-47) end to end on the GPU:
 inputs written in the reference formats, statistics lines and the written
 restart compared with the FP64 oracle started from the same file state."""
+import dataclasses
 import os
 import subprocess
 import sys
@@ -61,3 +62,50 @@ def test_athom_reports_bad_input(tmp_path):
     name = str(tmp_path / "none")
     r = subprocess.run([ATHOM, name], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+CALLER = os.path.join(ROOT, "md_gpu_amd", "bin", "ref_layout_caller")
+
+
+@pytest.mark.parametrize("ens", ["nvt", "nve"])
+def test_simulationrun_cuda_with_reference_structs(tmp_path, orc, ens):
+    """SimulationRun_CUDA called by a program that allocates the reference's own
+    structs (its layout, malloc'd integr whose tcons and buffers are garbage,
+    src/simulation.cu:40-131): NaCl 4096, leapfrog with the reference's
+    Berendsen tau (nvt) or NVE, t = 0..timrun inclusive; the returned
+    configuration follows the oracle."""
+    s = S.nacl(nsites=16)
+    s.thermostat = "berendsen" if ens == "nvt" else "none"
+    timrun, timprn = 19, 10
+    ff = s.ff
+    inp = tmp_path / "in.txt"
+    with open(inp, "w") as f:
+        f.write(f"{s.n} {ff.ntype}\n{s.box[0]:.9g} {s.box[1]:.9g} {s.box[2]:.9g}\n")
+        f.write(f"{ens} {s.T0:.9g} {s.dt:.9g} {timrun} {timprn}\n")
+        f.write(f"{ff.rc:.9g} 1 {ff.alpha:.9g} {ff.skin:.9g} 1000\n")
+        f.write(" ".join(f"{m:.9g}" for m in ff.masses) + "\n")
+        f.write(" ".join(f"{np.float32(v):.9g}" for v in np.asarray(ff.lj1).reshape(-1)) + "\n")
+        f.write(" ".join(f"{np.float32(v):.9g}" for v in np.asarray(ff.lj2).reshape(-1)) + "\n")
+        for i in range(s.n):
+            p, v = s.pos[i], s.vel[i]
+            f.write(f"{s.type[i] + 1} {s.charge[i]:.9g} {p[0]:.9g} {p[1]:.9g} {p[2]:.9g} "
+                    f"{v[0]:.9g} {v[1]:.9g} {v[2]:.9g}\n")
+    out = tmp_path / "out.txt"
+    print("[case] ref layout caller", ens, file=sys.stderr, flush=True)
+    r = subprocess.run([CALLER, str(inp), str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = np.loadtxt(out)
+    t = s.copy()
+    t.ff = dataclasses.replace(ff, lj1=np.asarray(ff.lj1, np.float32).astype(np.float64),
+                               lj2=np.asarray(ff.lj2, np.float32).astype(np.float64))
+    ref = orc.md_run(t, timrun + 1)
+    d = got[:, :3] - ref["pos"]
+    d -= s.box * np.rint(d / s.box)
+    print("[case]   max |dx|", np.abs(d).max(), file=sys.stderr, flush=True)
+    assert np.abs(d).max() < 2e-4
+    dv = got[:, 3:] - ref["vel"]
+    assert np.sqrt((dv ** 2).sum() / (ref["vel"] ** 2).sum()) < 1e-4
+    rows = stat_lines(r.stdout)
+    assert [x[0] for x in rows] == [0, 10]   # t % timprn == 0 (src/runGPU.cu:98)
+    for step, T, ke in rows:
+        assert T == pytest.approx(ref["temp"][step], rel=2e-4), step
