@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X MD step engine.
 
-One JSON line on stdout.  The headline `value` is timesteps/s of the molten
-NaCl workload (32,768 ions, Wolf alpha 0.2 rc 12 A, NVT leapfrog + Berendsen
-tau 0.1 ps, dt 2 fs, FP32 forces), inputs resident in HBM, hipGraph segments.
-The other workloads (argon 864 NVE-VV, water 196,608 atoms NVT-VV Nose-Hoover,
-512 batched Ar-Xe replicas, NaCl force matching) are reported under
+One JSON line on stdout.  The headline `value` is timesteps/s of the largest
+workload: 512 independent Ar-Xe LJ replicas (log-uniform 256..8192 atoms,
+1.29 M atoms in all, per-replica T 80..300 K, rc 10.25 A, NVT leapfrog +
+Berendsen, dt 2 fs, FP32 forces) advanced together, one step-kernel launch
+per step, inputs resident in HBM, hipGraph segments.  The other workloads
+(water 196,608 atoms NVT-VV Nose-Hoover, molten NaCl 32,768 ions NVT-LF
+Berendsen, argon 864 NVE-VV, NaCl force matching) are reported under
 "workloads" with the same fields.
 
 Roofline of the dominant kernel (the fused step kernel): algorithmic FP32
 flops per launch = f_pair * P (P = unordered pairs within rc at the timed
 configuration, f_pair counted from src/pair_forces.cu:151-200: LJ+Wolf 65,
 Wolf only 60, LJ 36) divided by its mean launch time measured with HIP events
-on the engine's stream, against 157.3 TFLOP/s.
+on the engine's stream, against 157.3 TFLOP/s.  `traffic` is the HBM bytes per
+launch of the same kernel from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+under profiles/.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
 """
@@ -71,6 +75,16 @@ def time_run(e, nsteps, energy_every=0):
     return time.perf_counter() - t0
 
 
+def traffic_of(workload):
+    """HBM bytes per step-kernel launch from the counter summary in profiles/ (or None)."""
+    path = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        tr = json.load(f)
+    return tr.get("bytes_per_launch"), os.path.relpath(path, ROOT)
+
+
 def kernel_time(e, nsteps):
     """Mean duration of the fused step kernel and of a rebuild, HIP events on the engine stream."""
     e.timing(True)
@@ -81,7 +95,7 @@ def kernel_time(e, nsteps):
 
 
 def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=None, force_check=None,
-                  launch=None, calibrate=False):
+                  launch=None, calibrate=False, long_steps=0):
     from md_gpu_amd import engine as E
     log(f"{name}: setting up")
     e = E.Engine.from_system(systems)
@@ -103,10 +117,14 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
     pos1, _, _ = e.download()
     flops1 = f_pair_fn(pos1)
     flops = 0.5 * (flops0 + flops1)
-    ktime_steps = min(steps, 2000)
+    long_ms = None
+    if long_steps:   # the same engine over a long run: per-step time in steady state
+        long_ms = 1e3 * time_run(e, long_steps) / long_steps
+    ktime_steps = max(20, min(steps, 2000))
     kstep_ms, kreb_ms, traw = kernel_time(e, ktime_steps)
     sps = steps / wall
     info = e.info()
+    traffic, tsrc = traffic_of(name)
     out = {
         "value": sps, "unit": "timesteps/s", "ms_per_step": 1e3 * wall / steps,
         "ns_per_day": sps * dt * 0.0864, "steps": steps, "warmup": warmup,
@@ -116,9 +134,12 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
         "rebuild_chain_ms_per_step": traw["rebuild_ms"] / max(ktime_steps, 1),
         "roofline": {"bound": "valu", "achieved": flops / (kstep_ms * 1e-3) / 1e12, "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": flops / (kstep_ms * 1e-3) / 1e12 / PEAK_FP32,
-                     "traffic": None, "algorithmic_flop_per_launch": flops},
+                     "traffic": traffic, "traffic_source": tsrc, "algorithmic_flop_per_launch": flops},
         "step_roofline_frac": flops / (1e-3 * 1e3 * wall / steps) / 1e12 / PEAK_FP32,
     }
+    if long_ms is not None:
+        out["long_run"] = {"steps": long_steps, "ms_per_step": long_ms, "value": 1e3 / long_ms,
+                           "short_over_long": (1e3 * wall / steps) / long_ms}
     if calibrate:
         nbytes = 64 << 20
         cms = e.copy_bench(nbytes, 5)
@@ -177,72 +198,7 @@ def main():
     want_cpu = not args.no_cpu
     tune = not args.no_autotune
 
-    # ---- headline: NaCl 32,768 ions, NVT leapfrog Berendsen ----
-    nacl = S.nacl()
-
-    def f_nacl(pos):
-        return 65.0 * count_pairs(pos, nacl.box, nacl.ff.rc)[0]
-
-    def cpu_nacl(e, s):
-        if not want_cpu:
-            return None
-        pos, vel, _ = e.download()
-        t = s.copy()
-        t.pos, t.vel = pos, vel
-        n, sps, cores = oracle_cpu_steps(t, 12.0)
-        return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
-                "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
-
-    res = step_workload("nacl32768", nacl, f_nacl, args.steps, args.warmup, tune, nacl.dt, cpu_nacl,
-                        (lambda e, s: force_error(e, s)) if want_cpu else None, launch, calibrate=True)
-    line = {
-        "metric": METRIC, "value": res["value"], "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "nacl32768_wolf_nvt_leapfrog_berendsen", "atoms": nacl.n, "box_A": float(nacl.box[0]),
-                   "rc_A": nacl.ff.rc, "alpha": nacl.ff.alpha, "skin_A": nacl.ff.skin, "dt_fs": nacl.dt,
-                   "T_K": nacl.T0, "tau_fs": nacl.tau},
-        "ns_per_day": res["ns_per_day"], "force_rel_rms": res.get("force_rel_rms"),
-        "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline"),
-        "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_per_step": res["rebuild_chain_ms_per_step"],
-        "rebuilds_per_step": res["rebuilds_per_step"], "launch": res["launch"],
-        "rebuild_ms": res.get("rebuild_ms"), "hbm_copy_GBps": res.get("hbm_copy_GBps"),
-    }
-    traffic = os.path.join(ROOT, "profiles", "step_kernel_nacl32768_traffic.json")
-    if os.path.exists(traffic):
-        with open(traffic) as f:
-            tr = json.load(f)
-        line["roofline"]["traffic"] = tr.get("bytes_per_launch")
-        line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
-        line["roofline"]["valu_busy_counters"] = tr.get("valu_busy")
-    work = {}
-    if not args.quick:
-        work = extra_workloads(args, want_cpu, tune)
-    line["workloads"] = work
-    print(json.dumps(line), flush=True)
-
-
-def extra_workloads(args, want_cpu, tune):
-    from md_gpu_amd import systems as S
-    out = {}
-    # argon 864, NVE velocity Verlet (the 1000-step case, timed over 5000 steps: a
-    # step takes microseconds)
-    ar = S.argon()
-    out["argon864_nve_vv"] = step_workload(
-        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 5000, 500, tune, ar.dt,
-        (lambda e, s: cpu_small(e, s, 5.0)) if want_cpu else None)
-    # water 196,608 atoms, NVT velocity Verlet + Nose-Hoover
-    wa = S.water()
-    isO = (wa.type == 0)
-
-    def f_water(p):
-        n, noo = count_pairs(p, wa.box, wa.ff.rc, wa.type, lambda a, b: (a == 0) & (b == 0))
-        return 60.0 * n + 5.0 * noo
-    out["water196608_nvt_vv_nh"] = step_workload(
-        "water196608", wa, f_water, 1000, 200, tune, wa.dt,
-        (lambda e, s: cpu_small(e, s, 10.0)) if want_cpu else None)
-    del isO
-    # 512 batched Ar-Xe replicas, one launch per step
+    # ---- headline: 512 batched Ar-Xe replicas, NVT leapfrog Berendsen ----
     reps = S.arxe_replicas()
 
     def f_batch(p):
@@ -251,9 +207,82 @@ def extra_workloads(args, want_cpu, tune):
             tot += count_pairs(p[off:off + r.n], r.box, r.ff.rc)[0]
             off += r.n
         return 36.0 * tot
-    out["arxe512_batched_nvt_lf"] = step_workload("arxe512", reps, f_batch, 500, 50, tune, reps[0].dt,
-                                                  (lambda e, s: cpu_batch(e, s, 10.0)) if want_cpu else None)
-    out["arxe512_batched_nvt_lf"]["atoms"] = int(sum(r.n for r in reps))
+
+    res = step_workload("arxe512", reps, f_batch, args.steps, args.warmup, tune, reps[0].dt,
+                        (lambda e, s: cpu_batch(e, s, 10.0)) if want_cpu else None,
+                        (lambda e, s: batch_force_error(e, s)) if want_cpu else None, launch, calibrate=True,
+                        long_steps=0 if args.quick else 500)
+    natoms = int(sum(r.n for r in reps))
+    line = {
+        "metric": METRIC, "value": res["value"], "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "arxe512_batched_nvt_leapfrog_berendsen", "replicas": len(reps), "atoms": natoms,
+                   "atoms_min": int(min(r.n for r in reps)), "atoms_max": int(max(r.n for r in reps)),
+                   "rc_A": reps[0].ff.rc, "skin_A": reps[0].ff.skin, "dt_fs": reps[0].dt, "T_K": "80..300",
+                   "tau_fs": reps[0].tau},
+        "ns_per_day": res["ns_per_day"], "force_rel_rms": res.get("force_rel_rms"),
+        "roofline": res["roofline"], "cpu_baseline": res.get("cpu_baseline"),
+        "step_kernel_ms": res["step_kernel_ms"], "rebuild_chain_ms_per_step": res["rebuild_chain_ms_per_step"],
+        "rebuilds_per_step": res["rebuilds_per_step"], "launch": res["launch"],
+        "rebuild_ms": res.get("rebuild_ms"), "hbm_copy_GBps": res.get("hbm_copy_GBps"),
+        "long_run": res.get("long_run"),
+    }
+    work = {}
+    if not args.quick:
+        work = extra_workloads(args, want_cpu, tune)
+    line["workloads"] = work
+    print(json.dumps(line), flush=True)
+
+
+def batch_force_error(e, reps, nsample=8):
+    """Relative RMS force error against the FP64 oracle over a seeded sample of replicas."""
+    import oracle
+    pos, _, _ = e.download()
+    F, _, _ = e.compute_forces()
+    offs = np.concatenate([[0], np.cumsum([r.n for r in reps])])
+    rng = np.random.default_rng(1)
+    num = den = 0.0
+    for k in rng.choice(len(reps), nsample, replace=False):
+        t = reps[k].copy()
+        t.pos = pos[offs[k]:offs[k + 1]]
+        Fo, _, _ = oracle.forces(t)
+        num += float(((F[offs[k]:offs[k + 1]] - Fo) ** 2).sum())
+        den += float((Fo ** 2).sum())
+    return float(np.sqrt(num / den))
+
+
+def extra_workloads(args, want_cpu, tune):
+    from md_gpu_amd import systems as S
+    out = {}
+    # water 196,608 atoms, NVT velocity Verlet + Nose-Hoover
+    wa = S.water()
+
+    def f_water(p):
+        n, noo = count_pairs(p, wa.box, wa.ff.rc, wa.type, lambda a, b: (a == 0) & (b == 0))
+        return 60.0 * n + 5.0 * noo
+    out["water196608_nvt_vv_nh"] = step_workload(
+        "water196608", wa, f_water, 1000, 200, tune, wa.dt,
+        (lambda e, s: cpu_small(e, s, 10.0)) if want_cpu else None)
+    # molten NaCl 32,768 ions, NVT leapfrog Berendsen
+    nacl = S.nacl()
+
+    def cpu_nacl(e, s):
+        pos, vel, _ = e.download()
+        t = s.copy()
+        t.pos, t.vel = pos, vel
+        n, sps, cores = oracle_cpu_steps(t, 12.0)
+        return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
+                "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
+    out["nacl32768_wolf_nvt_lf"] = step_workload(
+        "nacl32768", nacl, lambda p: 65.0 * count_pairs(p, nacl.box, nacl.ff.rc)[0], 5000, 300, tune, nacl.dt,
+        cpu_nacl if want_cpu else None, (lambda e, s: force_error(e, s)) if want_cpu else None)
+    # argon 864, NVE velocity Verlet (the 1000-step case, timed over 5000 steps: a
+    # step takes microseconds)
+    ar = S.argon()
+    out["argon864_nve_vv"] = step_workload(
+        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 5000, 500, tune, ar.dt,
+        (lambda e, s: cpu_small(e, s, 5.0)) if want_cpu else None)
     out["nacl4096x64_force_matching"] = fm_workload(want_cpu)
     return out
 
