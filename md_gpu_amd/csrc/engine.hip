@@ -37,9 +37,9 @@ extern "C" hipError_t md_launch_kick(const KickArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_thermo_fix(ThState* th, const RepParam* rep, int nrep, int thermostat, float dt,
                                            int mode, hipStream_t st);
 extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                           hipStream_t st);
+                                           hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 extern "C" hipError_t md_launch_fm_fused(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                         hipStream_t st);
+                                         hipStream_t st, hipEvent_t ev0, hipEvent_t ev1);
 extern "C" hipError_t md_launch_copies(int npair, const void* const* src, void* const* dst, const long long* bytes,
                                        hipStream_t st);
 extern "C" hipError_t md_launch_check_lists(const BuildArgs& a, int row_g, int* bad, hipStream_t st);
@@ -226,7 +226,7 @@ struct md_engine {
     DBuf<double> part, fm_acc;     // part: [2][MD_NSTAT][S/64] partial sums of the step kernel, one per tile
     DBuf<double> fm_part;          // [S/64][2 + 2 nt^2] per-tile slots of the force-matching kernels
     DBuf<float4> fm_ljtab;         // LJ table of the trial parameters (force matching)
-    std::vector<float4> fm_ljtab_h;    // its host staging copy (alive until the next evaluation)
+    float4 fm_ljtab_h[64];         // its host staging copy (alive until the next evaluation)
     DBuf<RepState> lst;            // [2][R] hand-over state of leapfrog blocks with deferred finalize
     DBuf<int> err;
     long long reb_base = 0;        // list builds of states uploaded before the current one
@@ -1569,40 +1569,44 @@ int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* t
     return MD_OK;
 }
 
-int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qscale, double* loss, double* g_lj1,
-                    double* g_lj2, double* g_qscale) {
-    CHECK(check_ready(e, true));
-    if (!e->fm_nframes) return fail(MD_E_STATE, "call md_fm_set_frames first");
-    if (!lj1 || !lj2) return fail(MD_E_INVALID, "null LJ tables");
+// One loss-and-gradient evaluation with the given device buffers: ljtab_d (64
+// float4, the trial table), part_d (ntiles x nacc doubles, per-tile slots) and
+// acc_d (nacc doubles, the sums); nacc = 2 + 2 ntk^2.  Every slot of part_d and
+// acc_d is written before it is read, so nothing needs clearing.
+static int fm_evaluate(md_engine* e, const float* lj1, const float* lj2, float qscale, float4* ljtab_d, double* part_d,
+                       double* acc_d, double* loss, double* g_lj1, double* g_lj2, double* g_qscale) {
     const int nt = e->ff.ntype;
-    if (nt > 4) return fail(MD_E_CAPACITY, "force matching supports at most 4 atom types");
     const int ntk = nt == 1 ? 1 : (nt == 2 ? 2 : 4);
-    HIPCHECK(hipSetDevice(e->dev));
     // trial LJ table in its own buffer: the engine's table is left alone, and the
     // upload is ordered on the stream (one synchronisation per evaluation, at the end)
-    if (!e->fm_ljtab.p) CHECK(e->fm_ljtab.alloc(64));
-    e->fm_ljtab_h.resize(64);
     HIPCHECK(hipStreamSynchronize(e->st));   // the staging copy of the previous evaluation is free
-    make_ljtab(e, lj1, lj2, e->fm_ljtab_h.data());
-    HIPCHECK(hipMemcpyAsync(e->fm_ljtab.p, e->fm_ljtab_h.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
+    make_ljtab(e, lj1, lj2, e->fm_ljtab_h);
+    HIPCHECK(hipMemcpyAsync(ljtab_d, e->fm_ljtab_h, 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
     StepArgs a = step_args(e, MODE_FORCE, 0);
-    a.ljtab = e->fm_ljtab.p;
+    a.ljtab = ljtab_d;
     a.stats = nullptr;
     const int coul = e->coul_kernel;
-    const size_t nslot = (size_t)e->ntiles() * (2 + 2 * ntk * ntk);
-    if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
+    hipEvent_t ev0 = e->timing ? ev_get(e, 0) : nullptr, ev1 = e->timing ? ev_get(e, 1) : nullptr;
+    if (e->timing && (!ev0 || !ev1)) return fail(MD_E_HIP, "hipEventCreate failed");
     if (e->fm_fused && !e->bonded) {   // one pass: force and gradient sums together
         a.qq_scale = qscale;
-        HIPCHECK(md_launch_fm_fused(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->st));
+        HIPCHECK(md_launch_fm_fused(a, part_d, acc_d, e->lanes, coul, ntk, e->st, ev0, ev1));
     } else {                            // force pass (bonded terms included), then the adjoint walk
         a.qq_scale = qscale * qscale;
         HIPCHECK(md_launch_step(a, e->lanes, coul, e->st, nullptr, nullptr));
         a.qq_scale = qscale;
-        HIPCHECK(md_launch_fm_adjoint(a, e->fm_part.p, e->fm_acc.p, e->lanes, coul, ntk, e->st));
+        HIPCHECK(md_launch_fm_adjoint(a, part_d, acc_d, e->lanes, coul, ntk, e->st, ev0, ev1));
     }
-    std::vector<double> g(2 + 2 * ntk * ntk);
-    HIPCHECK(hipMemcpyAsync(g.data(), e->fm_acc.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, e->st));
+    double g[2 + 2 * 16];
+    const int nacc = 2 + 2 * ntk * ntk;
+    HIPCHECK(hipMemcpyAsync(g, acc_d, nacc * sizeof(double), hipMemcpyDeviceToHost, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
+    if (e->timing) {   // the per-tile kernel's duration counts as this evaluation's "step"
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        e->t_step_ms += ms;
+        e->n_step++;
+    }
     const double M = 3.0 * (double)e->N;
     if (loss) *loss = g[0] / M;
     if (g_qscale) *g_qscale = 2.0 / M * g[1];
@@ -1612,6 +1616,30 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
             if (g_lj2) g_lj2[i * nt + j] = 2.0 / M * g[2 + ntk * ntk + i * ntk + j];
         }
     return MD_OK;
+}
+
+static int fm_check(md_engine* e, const float* lj1, const float* lj2) {
+    CHECK(check_ready(e, true));
+    if (!e->fm_nframes) return fail(MD_E_STATE, "call md_fm_set_frames first");
+    if (!lj1 || !lj2) return fail(MD_E_INVALID, "null LJ tables");
+    if (e->ff.ntype > 4) return fail(MD_E_CAPACITY, "force matching supports at most 4 atom types");
+    HIPCHECK(hipSetDevice(e->dev));
+    return MD_OK;
+}
+
+static int fm_nacc(const md_engine* e) {
+    const int nt = e->ff.ntype;
+    const int ntk = nt == 1 ? 1 : (nt == 2 ? 2 : 4);
+    return 2 + 2 * ntk * ntk;
+}
+
+int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qscale, double* loss, double* g_lj1,
+                    double* g_lj2, double* g_qscale) {
+    CHECK(fm_check(e, lj1, lj2));
+    if (!e->fm_ljtab.p) CHECK(e->fm_ljtab.alloc(64));
+    const size_t nslot = (size_t)e->ntiles() * fm_nacc(e);
+    if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
+    return fm_evaluate(e, lj1, lj2, qscale, e->fm_ljtab.p, e->fm_part.p, e->fm_acc.p, loss, g_lj1, g_lj2, g_qscale);
 }
 
 }  // extern "C"

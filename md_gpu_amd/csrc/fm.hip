@@ -282,13 +282,18 @@ static hipError_t launch_fm_nt(const StepArgs& a, double* part, int lanes, int c
     return hipGetLastError();
 }
 
+// ev0 / ev1 (optional): events recorded around the per-tile kernel alone
 template <bool FUSED>
-static hipError_t launch_fm(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt, hipStream_t st) {
+static hipError_t launch_fm(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt, hipStream_t st,
+                            hipEvent_t ev0, hipEvent_t ev1) {
     hipError_t e = hipErrorInvalidValue;
+    if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+    e = hipErrorInvalidValue;
     if (nt == 1) e = launch_fm_nt<FUSED, 1>(a, part, lanes, coul, st);
     if (nt == 2) e = launch_fm_nt<FUSED, 2>(a, part, lanes, coul, st);
     if (nt == 4) e = launch_fm_nt<FUSED, 4>(a, part, lanes, coul, st);
     if (e != hipSuccess) return e;
+    if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_fm_reduce, dim3(1), dim3(256), 0, st, (const double*)part, a.ntiles, 2 + 2 * nt * nt, out);
     return hipGetLastError();
 }
@@ -298,11 +303,11 @@ static hipError_t launch_fm(const StepArgs& a, double* part, double* out, int la
 // ntype must be 1, 2 or 4 (the host rounds up; unused type rows stay zero)
 // part: [ntiles][2 + 2 nt^2] scratch; out: [2 + 2 nt^2] sums (loss, qscale, lj1, lj2 terms)
 extern "C" hipError_t md_launch_fm_fused(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                         hipStream_t st) {
-    return launch_fm<true>(a, part, out, lanes, coul, nt, st);
+                                         hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    return launch_fm<true>(a, part, out, lanes, coul, nt, st, ev0, ev1);
 }
 
 extern "C" hipError_t md_launch_fm_adjoint(const StepArgs& a, double* part, double* out, int lanes, int coul, int nt,
-                                           hipStream_t st) {
-    return launch_fm<false>(a, part, out, lanes, coul, nt, st);
+                                           hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    return launch_fm<false>(a, part, out, lanes, coul, nt, st, ev0, ev1);
 }

@@ -8,13 +8,15 @@ the chain rule to (eps, sigma) is applied here.
 
 PyTorch is used only for the fitting driver: FMLoss is a
 torch.autograd.Function whose forward calls the C ABI (md_fm_loss_grad) and
-whose backward returns the adjoint gradient.
+whose backward returns the adjoint gradient.  ForceMatch needs an engine
+object with fm_set_frames / fm_loss_grad (md_gpu_amd.engine.Engine).
 """
 from __future__ import annotations
 
 from typing import Tuple
 
 import numpy as np
+import torch
 
 from .systems import System, eps_sig_from_lj
 
@@ -65,34 +67,33 @@ class ForceMatch:
         return loss, grad
 
 
-def _torch_function():
-    import torch
+class FMLoss(torch.autograd.Function):
+    """loss(theta) with the GPU adjoint gradient.
 
-    class FMLoss(torch.autograd.Function):
-        """loss(theta) with the GPU adjoint gradient (theta: float64 tensor of ForceMatch.nparam)."""
+    theta: float64 tensor of ForceMatch.nparam entries.  forward calls the C
+    ABI once (md_fm_loss_grad: loss and gradient in the same pass); backward
+    returns grad_output times that gradient, so no second evaluation is needed.
+    """
 
-        @staticmethod
-        def forward(ctx, theta, fm):
-            loss, grad = fm.loss_grad(theta.detach().cpu().numpy())
-            ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype, device=theta.device))
-            return torch.as_tensor(loss, dtype=theta.dtype, device=theta.device)
+    @staticmethod
+    def forward(ctx, theta, fm):
+        loss, grad = fm.loss_grad(theta.detach().cpu().numpy())
+        ctx.save_for_backward(torch.as_tensor(grad, dtype=theta.dtype, device=theta.device))
+        return torch.as_tensor(loss, dtype=theta.dtype, device=theta.device)
 
-        @staticmethod
-        def backward(ctx, gout):
-            (grad,) = ctx.saved_tensors
-            return gout * grad, None
-
-    return FMLoss
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        return gout * grad, None
 
 
 def fm_loss(theta, fm: ForceMatch):
     """Differentiable loss for torch: fm_loss(theta_tensor, fm).backward()."""
-    return _torch_function().apply(theta, fm)
+    return FMLoss.apply(theta, fm)
 
 
 def fit(fm: ForceMatch, theta0=None, steps: int = 100, lr: float = 1e-3, log_every: int = 0):
     """Adam on theta (relative parameterisation theta = theta0 * (1 + u)); returns (theta, losses)."""
-    import torch
     t0 = torch.as_tensor(fm.theta0() if theta0 is None else theta0, dtype=torch.float64)
     u = torch.zeros_like(t0, requires_grad=True)
     opt = torch.optim.Adam([u], lr=lr)
@@ -102,7 +103,39 @@ def fit(fm: ForceMatch, theta0=None, steps: int = 100, lr: float = 1e-3, log_eve
         loss = fm_loss(t0 * (1.0 + u), fm)
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(loss.detach().item())
         if log_every and it % log_every == 0:
             print(f"fit {it}: loss {losses[-1]:.6e}", flush=True)
     return (t0 * (1.0 + u)).detach().numpy(), losses
+
+
+def nacl_problem(engine_cls, nframes: int = 64, nsites: int = 16, seed: int = 9, equil: int = 200, stride: int = 10):
+    """The NaCl force-matching problem: frames sampled every `stride` steps of a
+    GPU trajectory of the base model (nsites^3 ions, after `equil` steps), and
+    target forces of the perturbed model (eps, sigma and the charge scale each
+    times 1 + U(-5 %, +5 %), seeded).  Returns (engine, ForceMatch, base,
+    frames, targets, theta_target); the engine holds the frames."""
+    import dataclasses
+    from . import systems as S
+    base = S.nacl(nsites=nsites)
+    e = engine_cls.from_system(base)
+    e.run(equil, stats=False)
+    frames = []
+    for _ in range(nframes):
+        e.run(stride, stats=False)
+        frames.append(e.download()[0].copy())
+    frames = np.stack(frames)
+    tgt_ff, tgt_q = S.perturb_nacl_ff(base.ff, seed=seed)
+    t_sys = dataclasses.replace(base.copy(), ff=tgt_ff)
+    t_sys.charge = base.charge * tgt_q
+    e2 = engine_cls.from_system(t_sys)
+    tgt = []
+    for f in frames:
+        e2.upload(f)
+        tgt.append(e2.compute_forces()[0])
+    e2.close()
+    tgt = np.stack(tgt)
+    fm = ForceMatch(e, base, frames, tgt)
+    eps, sig = eps_sig_from_lj(tgt_ff.lj1, tgt_ff.lj2)
+    theta_t = np.concatenate([eps[fm.iu], sig[fm.iu], [tgt_q]])
+    return e, fm, base, frames, tgt, theta_t

@@ -1,9 +1,11 @@
 """Short fixed-launch run of one benchmark workload for hardware-counter
 collection and kernel traces: molten NaCl (32,768 ions, the default), argon
-(864 atoms, NVE velocity Verlet), water (196,608 atoms, NVT velocity Verlet)
-or the 512 batched Ar-Xe replicas.
+(864 atoms, NVE velocity Verlet), water (196,608 atoms, NVT velocity Verlet),
+the 512 batched Ar-Xe replicas, force matching (NaCl 4,096 ions x 64 frames,
+--steps loss-and-gradient evaluations) or a plain 64 MiB device copy (the
+calibration of the HBM byte counters).
 
-    python -m md_gpu_amd.profrun [--system nacl|argon|water|arxe]
+    python -m md_gpu_amd.profrun [--system nacl|argon|water|arxe|fm|copy]
                                  [--launch k=v,...] [--steps N] [--timed]
 
 Counter collection serialises every dispatch, so this driver keeps the run
@@ -24,17 +26,27 @@ import hashlib
 import json
 import sys
 
+import numpy as np
+
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--launch", default="lanes=8,cell_div=2,check_every=8,refresh=1,defer=1,submit=1")
-    ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe"])
+    ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe", "fm", "copy"])
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--timed", action="store_true")
     args = ap.parse_args()
     from md_gpu_amd import engine as E, systems as S
     cfg = dict((k, int(v)) for k, v in (kv.split("=") for kv in args.launch.split(",") if kv))
+    if args.system == "copy":
+        e = E.Engine.from_system(S.argon(ncell=4))
+        ms = e.copy_bench(64 << 20, 6)
+        print(json.dumps(dict(ok=True, system="copy", bytes=64 << 20, reps=6, ms=ms)), flush=True)
+        e.close()
+        return 0
+    if args.system == "fm":
+        return fm_main(args, cfg)
     build = {"nacl": S.nacl, "argon": S.argon, "water": S.water, "arxe": S.arxe_replicas}[args.system]
     e = E.Engine.from_system(build())
     e.set_launch(**cfg)
@@ -57,6 +69,27 @@ def main() -> int:
         out["invariants"] = e.check_lists()
     except E.MDError as x:
         out["checksum"] = dict(error=str(x))
+    print(json.dumps(out), flush=True)
+    e.close()
+    return 0
+
+
+def fm_main(args, cfg) -> int:
+    """Force matching: --steps evaluations of loss and gradient at the base
+    parameters; the checksum is the loss and gradient themselves."""
+    from md_gpu_amd import engine as E
+    from md_gpu_amd.fit import nacl_problem
+    e, fm, base, frames, tgt, _ = nacl_problem(E.Engine, nframes=64)
+    e.set_launch(**cfg)
+    th = fm.theta0()
+    if args.warmup:
+        fm.loss_grad(th)
+    if args.timed:
+        e.timing(True)
+    for _ in range(args.steps):
+        loss, g = fm.loss_grad(th)
+    out = dict(ok=True, system="fm", launch=e.get_launch(), evals=args.steps, loss=loss, grad=[float(x) for x in g])
+    out["checksum"] = dict(digest=hashlib.sha1(np.float64([loss, *g]).tobytes()).hexdigest()[:16])
     print(json.dumps(out), flush=True)
     e.close()
     return 0

@@ -1,0 +1,85 @@
+"""Force matching on the GPU at the benchmark's size: NaCl 4,096 ions x 64
+frames (frames sampled from a GPU trajectory, targets = forces of the model
+with eps, sigma and the charge scale perturbed by up to 5 %).
+
+- loss and gradient of all 64 frames against the FP64 oracle;
+- the GPU gradient against central finite differences (h = 1e-4 relative) of
+  the GPU loss itself;
+- one Adam step through the torch wrapper (fm_loss / FMLoss).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    from md_gpu_amd import engine as E
+    from md_gpu_amd.fit import nacl_problem
+    say("fm problem nacl 4096 x 64 frames")
+    e, fm, base, frames, tgt, theta_t = nacl_problem(E.Engine, nframes=64)
+    yield e, fm, base, frames, tgt, theta_t
+    e.close()
+
+
+def test_all_64_frames_match_oracle(problem, orc):
+    e, fm, base, frames, tgt, _ = problem
+    say("fm 64 frames vs oracle")
+    loss, g1, g2, gq = e.fm_loss_grad(base.ff.lj1, base.ff.lj2, 1.0)
+    lo, o1, o2, oq = orc.fm_loss_grad(frames, base.type, base.charge, base.box, base.ff, 1.0, tgt)
+    say("  loss", loss, lo, "gq", gq, oq)
+    assert abs(loss - lo) <= 1e-5 * lo
+    assert abs(gq - oq) <= 1e-5 * abs(oq)
+    np.testing.assert_allclose(g1, o1, rtol=1e-5, atol=1e-5 * np.abs(o1).max())
+    np.testing.assert_allclose(g2, o2, rtol=1e-5, atol=1e-5 * np.abs(o2).max())
+
+
+def test_gradient_matches_gpu_central_differences(problem):
+    """d loss / d theta (adjoint kernel + chain rule) vs (L(theta + h e_k) -
+    L(theta - h e_k)) / 2h of the GPU loss, h = 1e-4 theta_k.  The central
+    difference carries an O(h^2) term of relative size ~2e-6 (sigma^12) and the
+    FP32 forces' rounding; the bar is 1e-4 of the largest component."""
+    _, fm, *_ = problem
+    th = fm.theta0() * (1.0 + 0.02 * np.cos(np.arange(7)))   # away from the base and the target
+    loss, g = fm.loss_grad(th)
+    say("fm central FD, loss", loss)
+    for k in range(fm.nparam):
+        h = 1e-4 * abs(th[k])
+        tp, tm = th.copy(), th.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (fm.loss_grad(tp)[0] - fm.loss_grad(tm)[0]) / (2 * h)
+        say("  k", k, "adjoint", g[k], "fd", fd)
+        assert abs(g[k] - fd) <= 1e-4 * np.abs(g).max(), (k, g[k], fd)
+
+
+def test_loss_small_at_target_parameters(problem):
+    _, fm, _, _, _, theta_t = problem
+    l0, _ = fm.loss_grad(fm.theta0())
+    lt, gt = fm.loss_grad(theta_t)
+    say("fm loss at base / target", l0, lt)
+    assert lt < 1e-8 * l0
+
+
+def test_adam_step_through_torch_wrapper(problem):
+    import torch
+    from md_gpu_amd.fit import FMLoss, fit
+    _, fm, *_ = problem
+    th = torch.tensor(fm.theta0(), dtype=torch.float64, requires_grad=True)
+    loss = FMLoss.apply(th, fm)
+    loss.backward()
+    ref_loss, ref_g = fm.loss_grad(fm.theta0())
+    say("fm torch", loss.item(), ref_loss)
+    assert loss.item() == ref_loss                 # the GPU evaluation is deterministic
+    np.testing.assert_array_equal(th.grad.numpy(), ref_g)
+    theta, losses = fit(fm, steps=3, lr=5e-3)
+    say("fm adam losses", losses)
+    assert losses[-1] < losses[0]
+    assert np.all(np.isfinite(theta))
