@@ -205,6 +205,24 @@ int md_fm_set_frames(md_engine* e, int nframes, const float* pos, const float* t
 int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qscale,
                     double* loss, double* g_lj1, double* g_lj2, double* g_qscale);
 
+/* ---- no-allocation variant (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory at evaluation time (code
+ * under HIP graph capture, tight fitting loops, hosts that forbid hipMalloc):
+ * the same evaluation and results as md_fm_loss_grad, but the per-tile
+ * partial sums, the trial LJ table and the final sums live in one fixed
+ * MD_FM_SCRATCH_BYTES device array of the library (one copy per GPU, held
+ * until the process ends, its address resolved once per device by
+ * md_create).  Nothing is allocated on the device or by a library routine.
+ * Returns MD_E_CAPACITY when ntiles x (2 + 2 k^2) x 8 bytes (k = 1, 2 or 4 for
+ * 1, 2 or 3-4 atom types; ntiles = the frames' 64-atom tiles) exceed the array
+ * less its first 2 KiB: 13,081 tiles for two types, e.g. 204 frames of 4,096
+ * ions.  Every call on one GPU shares that array and leaves its contents
+ * behind (each call writes every slot it reads): calls that overlap on one
+ * GPU, from several engines or threads, race. */
+#define MD_FM_SCRATCH_BYTES (1 << 20)
+int md_fm_loss_grad_noalloc(md_engine* e, const float* lj1, const float* lj2, float qscale,
+                            double* loss, double* g_lj1, double* g_lj2, double* g_qscale);
+
 #ifdef __cplusplus
 }
 #endif

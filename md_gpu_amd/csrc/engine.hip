@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -650,6 +651,31 @@ int read_states(md_engine* e, std::vector<RepState>& s) {
 }  // namespace
 
 // ======================================================================= C ABI
+// ---- fixed scratch of md_fm_loss_grad_noalloc ----------------------------
+// Byte layout: [0, 512) the sums (2 + 2 k^2 <= 34 doubles), [512, 1536) the
+// trial LJ table (64 float4), [2048, MD_FM_SCRATCH_BYTES) the per-tile slots.
+// Host code never passes `scratch` itself to a kernel or a copy: its device
+// address is resolved per device with hipGetSymbolAddress (kept in
+// g_scratch_ptr) at md_create, with that device current.
+constexpr size_t kScratchAcc = 0, kScratchLj = 512, kScratchPart = 2048;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[MD_FM_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device must be `dev`
+static int scratch_setup(int dev) {
+    if (dev >= kMaxDevices) return MD_OK;   // md_fm_loss_grad_noalloc then reports the device as unsupported
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return MD_OK;
+    void* p = nullptr;
+    hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !p)
+        return fail(MD_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(r));
+    g_scratch_ptr[dev] = p;
+    return MD_OK;
+}
+
 extern "C" {
 
 const char* md_last_error(void) { return g_err.c_str(); }
@@ -683,8 +709,16 @@ int md_create(md_engine** out, int device) {
     }
     if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) ||
         e->report.alloc(1) || e->badcnt.alloc(8)) {
-        delete e;
+        md_destroy(e);
         return MD_E_HIP;
+    }
+    if (!ev_get(e, 1)) {   // events 0 and 1 exist from here on: timing allocates no events later
+        md_destroy(e);
+        return fail(MD_E_HIP, "hipEventCreate failed");
+    }
+    if (int rc = scratch_setup(device)) {
+        md_destroy(e);
+        return rc;
     }
     *out = e;
     return MD_OK;
@@ -1640,6 +1674,21 @@ int md_fm_loss_grad(md_engine* e, const float* lj1, const float* lj2, float qsca
     const size_t nslot = (size_t)e->ntiles() * fm_nacc(e);
     if (e->fm_part.n < nslot) CHECK(e->fm_part.alloc(nslot));
     return fm_evaluate(e, lj1, lj2, qscale, e->fm_ljtab.p, e->fm_part.p, e->fm_acc.p, loss, g_lj1, g_lj2, g_qscale);
+}
+
+int md_fm_loss_grad_noalloc(md_engine* e, const float* lj1, const float* lj2, float qscale, double* loss,
+                            double* g_lj1, double* g_lj2, double* g_qscale) {
+    CHECK(fm_check(e, lj1, lj2));
+    void* base = e->dev < kMaxDevices ? g_scratch_ptr[e->dev] : nullptr;
+    if (!base) return fail(MD_E_STATE, "no scratch address for device %d", e->dev);
+    const size_t need = (size_t)e->ntiles() * fm_nacc(e) * sizeof(double);
+    if (need > MD_FM_SCRATCH_BYTES - kScratchPart)
+        return fail(MD_E_CAPACITY, "force matching over %d tiles needs %zu bytes of per-tile sums; the fixed scratch holds %zu",
+                    e->ntiles(), need, (size_t)(MD_FM_SCRATCH_BYTES - kScratchPart));
+    char* b = static_cast<char*>(base);
+    return fm_evaluate(e, lj1, lj2, qscale, reinterpret_cast<float4*>(b + kScratchLj),
+                       reinterpret_cast<double*>(b + kScratchPart), reinterpret_cast<double*>(b + kScratchAcc), loss,
+                       g_lj1, g_lj2, g_qscale);
 }
 
 }  // extern "C"

@@ -86,6 +86,7 @@ def lib():
             "md_rebuild_bench": ([P, C.c_int, P], C.c_int),
             "md_fm_set_frames": ([P, C.c_int, P, P], C.c_int),
             "md_fm_loss_grad": ([P, P, P, C.c_float, P, P, P, P], C.c_int),
+            "md_fm_loss_grad_noalloc": ([P, P, P, C.c_float, P, P, P, P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -101,7 +102,7 @@ def exported_symbols() -> List[str]:
             "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
-            "md_set_list_capacity"]
+            "md_set_list_capacity", "md_fm_loss_grad_noalloc"]
 
 
 def _check(rc: int):
@@ -303,14 +304,16 @@ class Engine:
         self.nrep = nf
         self.N = frames.shape[0] * frames.shape[1]
 
-    def fm_loss_grad(self, lj1: np.ndarray, lj2: np.ndarray, qscale: float):
+    def fm_loss_grad(self, lj1: np.ndarray, lj2: np.ndarray, qscale: float, noalloc: bool = False):
+        """(loss, g_lj1, g_lj2, g_qscale); noalloc=True goes through
+        md_fm_loss_grad_noalloc (the library's fixed device scratch)."""
         nt = lj1.shape[0]
         a1 = np.ascontiguousarray(lj1, np.float32)
         a2 = np.ascontiguousarray(lj2, np.float32)
         loss = C.c_double(); gq = C.c_double()
         g1 = np.zeros((nt, nt)); g2 = np.zeros((nt, nt))
-        _check(lib().md_fm_loss_grad(self._h, _ptr(a1), _ptr(a2), C.c_float(qscale), C.byref(loss), _ptr(g1), _ptr(g2),
-                                     C.byref(gq)))
+        fn = lib().md_fm_loss_grad_noalloc if noalloc else lib().md_fm_loss_grad
+        _check(fn(self._h, _ptr(a1), _ptr(a2), C.c_float(qscale), C.byref(loss), _ptr(g1), _ptr(g2), C.byref(gq)))
         return loss.value, g1, g2, gq.value
 
 
