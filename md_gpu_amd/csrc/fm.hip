@@ -244,19 +244,18 @@ __global__ __launch_bounds__(64 * G) void k_fm_fused(const StepArgs a, double* p
     fm_flush<NW, NACC>(acc, s_red, part, tile);
 }
 
-// One workgroup: out[k] = sum over the per-tile slots.
+// Workgroup k: out[k] = sum over the per-tile slots of accumulator k (fixed
+// order: deterministic).
 __global__ __launch_bounds__(256) void k_fm_reduce(const double* part, int nslot, int nacc, double* out) {
     __shared__ double s_red[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < nacc; k++) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nslot; b += blockDim.x) v += part[(size_t)b * nacc + k];
-        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) s_red[wave] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) out[k] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-        __syncthreads();
-    }
+    const int k = blockIdx.x;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nslot; b += blockDim.x) v += part[(size_t)b * nacc + k];
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[k] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
 }
 
 template <bool FUSED, int G, int NT>
@@ -294,7 +293,8 @@ static hipError_t launch_fm(const StepArgs& a, double* part, double* out, int la
     if (nt == 4) e = launch_fm_nt<FUSED, 4>(a, part, lanes, coul, st);
     if (e != hipSuccess) return e;
     if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_fm_reduce, dim3(1), dim3(256), 0, st, (const double*)part, a.ntiles, 2 + 2 * nt * nt, out);
+    hipLaunchKernelGGL(k_fm_reduce, dim3(2 + 2 * nt * nt), dim3(256), 0, st, (const double*)part, a.ntiles,
+                       2 + 2 * nt * nt, out);
     return hipGetLastError();
 }
 

@@ -11,12 +11,15 @@
            bytes (--calib-bytes, 64 MiB by default) each way, and FETCH_SIZE /
            WRITE_SIZE read for it give the scale from counter KiB to bytes
 --kernel   substring of the kernel whose traffic goes into --json
+--only     print the tables of these kernels only (substrings)
 
 Printed: a markdown table per kernel of the per-dispatch means of every
 counter, the derived HBM bytes, FP32 flop executed (64 lanes x (ADD + MUL +
-2 FMA + TRANS) wave instructions) and VALU busy (SQ_ACTIVE_INST_VALU x 4 over
-SIMDs x cycles; GRBM_GUI_ACTIVE is summed over the 8 XCDs of the device and is
-divided by 8 first), plus the trace statistics.
+2 FMA + TRANS) wave instructions) and VALU busy (SQ_ACTIVE_INST_VALU, in
+units of 4 cycles per SIMD, x 4 over SIMDs x cycles; GRBM_GUI_ACTIVE is summed
+over the 8 XCDs of the device and is divided by 8 first; ~1 means the VALU
+issues every cycle, and small excesses over 1 come from the two counters being
+read in different runs), plus the trace statistics.
 """
 from __future__ import annotations
 
@@ -92,8 +95,6 @@ def derive(cnt, fscale, wscale):
         out["fp32_flop"] = 64.0 * (m[f32[0]] + m[f32[1]] + 2.0 * m[f32[2]] + m[f32[3]])
     if "SQ_ACTIVE_INST_VALU" in m and m.get("GRBM_GUI_ACTIVE"):
         out["valu_busy"] = m["SQ_ACTIVE_INST_VALU"] * 4.0 / (N_SIMD * m["GRBM_GUI_ACTIVE"] / N_XCD)
-    if "SQ_INSTS_VALU" in m and "fp32_flop" in out:
-        out["fp32_share_of_valu"] = out["fp32_flop"] / 64.0 / max(m["SQ_INSTS_VALU"], 1.0)
     if "SQ_INSTS_LDS" in m and "SQ_LDS_BANK_CONFLICT" in m:
         out["lds_conflict_cycles_per_lds_inst"] = m["SQ_LDS_BANK_CONFLICT"] / max(m["SQ_INSTS_LDS"], 1.0)
     return m, out
@@ -126,6 +127,7 @@ def main(argv=None) -> int:
     ap.add_argument("--launch", default=None)
     ap.add_argument("--json", default=None)
     ap.add_argument("--min-dispatches", type=int, default=2)
+    ap.add_argument("--only", nargs="*", default=None, help="print only kernels whose name contains one of these")
     args = ap.parse_args(argv)
 
     fscale, wscale = calibration(args.calib, args.calib_bytes) if args.calib else (None, None)
@@ -136,6 +138,8 @@ def main(argv=None) -> int:
     for k in sorted(vals, key=lambda k: -sum(dur.get(k, [0]))):
         ndisp = max(len(v) for v in vals[k].values())
         if ndisp < args.min_dispatches or "__amd_rocclr" in k:
+            continue
+        if args.only and not any(o in k for o in args.only) and args.kernel not in k:
             continue
         m, d = derive(vals[k], fscale, wscale)
         us = sum(dur[k]) / len(dur[k]) / 1e3 if dur.get(k) else None
@@ -168,7 +172,7 @@ def main(argv=None) -> int:
                "dispatches": ndisp, "bytes_per_launch": d["bytes"], "fetch_bytes": d["fetch_bytes"],
                "write_bytes": d["write_bytes"], "fetch_scale": fscale, "write_scale": wscale,
                "duration_us_under_counters": us}
-        for c in ("fp32_flop", "valu_busy", "fp32_share_of_valu", "lds_conflict_cycles_per_lds_inst"):
+        for c in ("fp32_flop", "valu_busy", "lds_conflict_cycles_per_lds_inst"):
             if c in d:
                 out[c] = d[c]
         with open(args.json, "w") as f:
