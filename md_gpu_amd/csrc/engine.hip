@@ -1479,7 +1479,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
         for (int sub : {0, 1})
             for (int rf : {0, 1})
                 for (int df : {1, 0})
-                for (int ce : {6, 8, 12, 16, 24, 32}) {
+                for (int ce : {3, 4, 6, 8, 12, 16, 24, 32}) {
                     if (df == 0 && !lf) continue;   // deferral applies to leapfrog-form steps only
                     md_launch_config c = base;
                     c.cell_div = sh.cell_div;

@@ -104,28 +104,30 @@ __device__ int block_scan(int v, int* s_w, int& total) {
 
 }  // namespace
 
-// The short rebuild kernels issue every load that does not depend on another
-// (the gate counter included) in one round trip, and the loads that depend on
-// the replica index in the next: an idle launch costs about the same, a working
-// one saves round trips.  A wavefront's 64 slots are one tile, so the replica
-// index is wavefront-uniform and its state and parameters are scalar loads.
+// The short rebuild kernels first read what says whether their wavefront has
+// work (the gate counter, the tile's replica and that replica's state: scalar
+// loads, a wavefront's 64 slots being one tile of one replica), and only the
+// wavefronts of replicas being rebuilt go on to their slots' data.  In a
+// batched run most replicas are not rebuilt at a given check, and their slots'
+// data (tens of MB over all replicas) is then never read.
 __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
-    const int id = a.ids[s];
-    const float4 x0 = a.pos[0][s], x1 = a.pos[1][s];
     loads_issued();
     if (n_stale == 0) return;
     const RepState st = a.st[r];
+    loads_issued();
+    if (!stale(st)) return;
+    const int id = a.ids[s];
+    const float4 x0 = a.pos[0][s], x1 = a.pos[1][s];
     const RepParam& p = a.rep[r];
     const float h0 = p.halfL[0], h1 = p.halfL[1], h2 = p.halfL[2];
     const float c0 = p.cinv[0], c1 = p.cinv[1], c2 = p.cinv[2];
     const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
     const int n0 = p.nc[0], n1 = p.nc[1], n2 = p.nc[2], cell_off = p.cell_off, slot_off = p.slot_off;
     loads_issued();
-    if (!stale(st)) return;
     if (s == slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
     if (id < 0) return;
     const float4 xr = cur_buf(st) ? x1 : x0;
@@ -190,14 +192,15 @@ __global__ __launch_bounds__(256) void k_nl_scatter(BuildArgs a) {
     if (s >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[s >> 6]);
-    const int id = a.ids[s];
-    const int gc = a.key[s];
-    const int rk = a.rank[s];
     loads_issued();
     if (n_stale == 0) return;
     const RepState st = a.st[r];   // whole (no load that waits for nb)
     loads_issued();
     if (!stale(st)) return;
+    const int id = a.ids[s];
+    const int gc = a.key[s];
+    const int rk = a.rank[s];
+    loads_issued();
     if (id < 0) return;
     int t = a.cell_start[gc + r] + (rk & ((1 << MD_SUBSHIFT) - 1));
     a.tmp[t] = s;
@@ -209,26 +212,23 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     if (t >= a.nslot) return;
     const int n_stale = a.gate->n_stale;
     const int r = __builtin_amdgcn_readfirstlane(a.tile_rep[t >> 6]);
-    const int o_raw = a.tmp[t];
-    const int mykey = a.tmpid[t];
     loads_issued();
     if (n_stale == 0) return;
-    // Second round trip: the replica state and parameters and the atom at slot o.
-    // tmp holds a slot of this replica for every t below slot_off + natom; above
-    // that, and for a replica that is not rebuilt, o is only clamped into range
-    // and the loaded values are not used.
     const RepState st = a.st[r];
-    const RepParam& p = a.rep[r];
-    const int lim = p.slot_off + p.natom;
-    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
-    const int o = min(max(o_raw, 0), a.nslot - 1);
-    const int gc = a.key[o];
-    const float4 x0 = a.pos[0][o], x1 = a.pos[1][o], v = a.vel[o], f4 = a.force[o];
-    const int ty = a.typ[o];
     loads_issued();
     if (!stale(st)) return;
+    // tmp holds a slot of this replica for every t below slot_off + natom
+    const RepParam& p = a.rep[r];
+    const int lim = p.slot_off + p.natom;
     if (t >= lim) return;
-    const float4 xr = cur_buf(st) ? x1 : x0;
+    const int o = a.tmp[t];
+    const int mykey = a.tmpid[t];
+    const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
+    loads_issued();
+    const int gc = a.key[o];
+    const float4 xr = cur_buf(st) ? a.pos[1][o] : a.pos[0][o];
+    const float4 v = a.vel[o], f4 = a.force[o];
+    const int ty = a.typ[o];
     const float4 x = make_float4(wrap_coord(xr.x, L0, Li0), wrap_coord(xr.y, L1, Li1), wrap_coord(xr.z, L2, Li2), xr.w);
     const int myid = mykey & ((1 << MD_SUBSHIFT) - 1);
     int cs = a.cell_start[gc + r], ce = a.cell_start[gc + r + 1];

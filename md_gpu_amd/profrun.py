@@ -25,6 +25,7 @@ import argparse
 import hashlib
 import json
 import sys
+import time
 
 import numpy as np
 
@@ -55,9 +56,13 @@ def main() -> int:
             e.run(args.warmup, stats=False)
         if args.timed:
             e.timing(True)
+        reb0 = e.info()["rebuilds"]
+        t0 = time.perf_counter()
         e.run(args.steps, stats=False)
+        wall = time.perf_counter() - t0
         out = dict(ok=True, system=args.system, launch=e.get_launch(), steps=args.steps,
-                   rebuilds=e.info()["rebuilds"])
+                   rebuilds=e.info()["rebuilds"], run_rebuilds=e.info()["rebuilds"] - reb0,
+                   wall_ms_per_step=1e3 * wall / args.steps)
         if args.timed:
             out["timing"] = e.timing_get()
     except E.MDError as x:
