@@ -48,6 +48,7 @@ extern "C" {
 enum { MD_COUL_RF = 0, MD_COUL_WOLF = 1, MD_COUL_DAMPED = 2, MD_COUL_NONE = 3 };
 enum { MD_LEAPFROG = 0, MD_VVERLET = 1 };
 enum { MD_TH_NONE = 0, MD_TH_BERENDSEN = 1, MD_TH_NOSEHOOVER = 2 };
+enum { MD_BARO_NONE = 0, MD_BARO_BERENDSEN = 1 };
 
 typedef struct {
     int   ntype;                         /* 1..MD_MAX_TYPES */
@@ -72,6 +73,16 @@ typedef struct {
     int   thermostat;   /* MD_TH_*; leapfrog supports NONE and BERENDSEN */
     float dt;           /* fs */
     float tau;          /* fs; Berendsen time constant (reference tco = 0.01/fs -> 100 fs) or NH period */
+    /* Pressure coupling (leapfrog only; the reference's NPT ensemble, enstype 2,
+     * src/simulation.cu:71-75, whose kernel src/npt_leapfrog_ber.cu:28-123 never
+     * scales the box): MD_BARO_BERENDSEN scales positions and box by
+     * mu = cbrt(1 - compress dt / tau_p (P0 - P)) after every step, P = (sum m v^2
+     * + W) / (3 V) from the step's kinetic energy and virial W (pairs and bonds).
+     * Energies and virial are then computed at every step. */
+    int   barostat;     /* MD_BARO_NONE (0) or MD_BARO_BERENDSEN */
+    float press0;       /* P0, bar */
+    float tau_p;        /* fs */
+    float compress;     /* isothermal compressibility, 1/bar */
 } md_integrator;
 
 /* Per-step, per-replica statistics written by md_run (FP64 throughout). */
@@ -85,6 +96,7 @@ typedef struct {
     double virial;      /* sum over pairs and bonds of r.F, valid when has_energy */
     double conserved;   /* ke + pe (+ Nose-Hoover terms), valid when has_energy */
     double xi, vxi;     /* Nose-Hoover thermostat position / velocity */
+    double volume;      /* box volume of the step's forces, A^3 */
     int    has_energy;
     int    pad;
 } md_step_stats;
@@ -141,6 +153,12 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs);
  *   [5] row padding that is not the empty entry */
 #define MD_NCHECK 6
 int md_check_lists(md_engine* e, int* counts);
+
+/* Current box of every replica (3 floats each): the box of md_set_system, or
+ * after a run with pressure coupling the coupled box (src/simulation.h:54
+ * config.box, which the reference would have had to update for NPT,
+ * src/integrator.cu:60-61). */
+int md_get_box(md_engine* e, float* box);
 
 /* Canonical FP32 thresholds the kernels use (for the parity checker):
  * rc2 = rc*rc, rl2 = (float)((skin + sqrt(rc2))^2), skin2q = skin*skin/4. */

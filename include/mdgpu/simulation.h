@@ -20,11 +20,16 @@
  * the force / momentum sums (src/prints.cu:117-118).
  * Behaviour changed on purpose: errors are reported with a message and
  * SimulationInit returns NULL (the reference read unchecked files); NPT
- * (never implemented in the reference integrator) is rejected.
+ * (enstype 2, read but never integrated by the reference: its kernel
+ * src/npt_leapfrog_ber.cu:28-123 is only called from commented-out code,
+ * src/integrator.cu:157-164) runs leapfrog with Berendsen temperature and
+ * pressure coupling to press_0 (bar), and the coupled box is written back to
+ * the config.
  *
  * Optional trailing .inp lines (after "rdel:"), ignored by older files:
  *   integrator: leapfrog | vverlet
  *   thermostat: none | berendsen | nosehoover  <tau_fs>
+ *   barostat: berendsen <tau_p_fs> <compressibility_per_bar>   (NPT; defaults 1000 fs, 4.5e-5)
  * SimulationInit keeps these choices in library-side state tied to the integr
  * it returns (the reference's integr has no fields for them, and its tcons is
  * never set, src/integrator.cu:68).  An integr the library did not create runs

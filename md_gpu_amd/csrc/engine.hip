@@ -219,7 +219,7 @@ struct md_engine {
     DBuf<int4> excl, ang_list;
     DBuf<uint32_t> bond_list;
     DBuf<int> nbond, nang;
-    DBuf<RepParam> rep;
+    DBuf<RepParam> rep, snap_rep;
     DBuf<int> tile_rep;
     DBuf<RepState> rst;
     DBuf<Gate> gate;
@@ -462,7 +462,7 @@ BuildArgs build_args(md_engine* e) {
     b.jset = e->jset.p; b.jcount = e->jcount.p;
     b.tilemax = e->tmax.p; b.jmax = e->tmax.p + e->ntiles();
     b.excl = e->excl.p; b.st = e->rst.p; b.gate = e->gate.p;
-    b.tile_rep = e->tile_rep.p; b.rep = e->rep.p;
+    b.tile_rep = e->tile_rep.p; b.rep = e->rep.p; b.repw = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.jcap = e->jcap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.row_g = e->lanes;
     b.lfgate = lf_form(e) ? 1 : 0;
@@ -491,6 +491,12 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.wolf = e->wolf; a.reacf = e->reacf; a.ecut = e->ecut; a.log2rc = e->log2rc; a.qq_scale = 1.0f;
     a.dt = e->in.dt; a.tau = e->in.tau; a.skin2q = e->skin2q;
     a.max_age = e->refresh ? e->check_every : 0;
+    a.baro = (mode == MODE_LF && e->in.barostat == MD_BARO_BERENDSEN) ? 1 : 0;
+    if (a.baro) a.energy_every = 1;   // the pressure needs the virial of every step
+    a.p0 = e->in.press0;
+    a.baro_k = (float)((double)e->in.compress * e->in.dt / (double)(e->in.tau_p > 0 ? e->in.tau_p : 1.0f));
+    a.rl = std::sqrt(e->rl2);
+    a.rc = e->ff.rc;
     for (int k = 0; k < 8; k++) {
         a.bk2[k] = e->ff.bond_k2[k]; a.br0[k] = e->ff.bond_r0[k];
         a.ak2[k] = e->ff.angle_k2[k]; a.at0[k] = e->ff.angle_t0[k];
@@ -579,6 +585,15 @@ int enqueue_step(md_engine* e, int energy_every, TimedSpan* ts, bool chain_mid =
 
 // Copy of the state a run may be rolled back to (restore = false), or back
 // (restore = true): one copy kernel for all arrays.
+// Pressure coupling changes the boxes on the device: copy them to the host
+// parameters (wrapping at download, cell set-up and later uploads use them).
+int sync_box(md_engine* e) {
+    HIPCHECK(hipMemcpy(e->rep_h.data(), e->rep.p, e->R * sizeof(RepParam), hipMemcpyDeviceToHost));
+    for (int r = 0; r < e->R; r++)
+        for (int d = 0; d < 3; d++) e->box[3 * r + d] = e->rep_h[r].L[d];
+    return MD_OK;
+}
+
 int snapshot(md_engine* e, bool restore) {
     struct Pair { void* a; void* b; size_t n; };
     const Pair list[] = {
@@ -586,16 +601,18 @@ int snapshot(md_engine* e, bool restore) {
         {e->snap_vel.p, e->vel.p, e->vel.bytes()}, {e->snap_force.p, e->force.p, e->force.bytes()},
         {e->snap_ids.p, e->ids.p, e->ids.bytes()}, {e->snap_typ.p, e->typ.p, e->typ.bytes()},
         {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()}, {e->snap_th.p, e->th.p, e->th.bytes()},
-        {e->snap_rst.p, e->rst.p, e->rst.bytes()}, {e->snap_gate.p, e->gate.p, e->gate.bytes()}};
-    const void* src[10];
-    void* dst[10];
-    long long nb[10];
-    for (int k = 0; k < 10; k++) {
+        {e->snap_rst.p, e->rst.p, e->rst.bytes()}, {e->snap_gate.p, e->gate.p, e->gate.bytes()},
+        {e->snap_rep.p, e->rep.p, e->rep.bytes()}};   // the box, under pressure coupling
+    constexpr int np = sizeof(list) / sizeof(list[0]);
+    const void* src[np];
+    void* dst[np];
+    long long nb[np];
+    for (int k = 0; k < np; k++) {
         src[k] = restore ? list[k].a : list[k].b;
         dst[k] = restore ? list[k].b : list[k].a;
         nb[k] = (long long)list[k].n;
     }
-    HIPCHECK(md_launch_copies(10, src, dst, nb, e->st));
+    HIPCHECK(md_launch_copies(np, src, dst, nb, e->st));
     return MD_OK;
 }
 
@@ -763,6 +780,11 @@ int md_set_integrator(md_engine* e, const md_integrator* in) {
         return fail(MD_E_INVALID, "Nose-Hoover needs the velocity-Verlet integrator");
     if (!(in->dt > 0)) return fail(MD_E_INVALID, "dt must be > 0");
     if (in->thermostat != MD_TH_NONE && !(in->tau > 0)) return fail(MD_E_INVALID, "tau must be > 0");
+    if (in->barostat != MD_BARO_NONE && in->barostat != MD_BARO_BERENDSEN) return fail(MD_E_INVALID, "unknown barostat");
+    if (in->barostat == MD_BARO_BERENDSEN) {
+        if (in->integrator != MD_LEAPFROG) return fail(MD_E_INVALID, "pressure coupling needs the leapfrog integrator");
+        if (!(in->tau_p > 0) || !(in->compress >= 0)) return fail(MD_E_INVALID, "tau_p must be > 0 and compress >= 0");
+    }
     if (e->has_state && e->in.integrator != in->integrator)
         return fail(MD_E_STATE, "switching the integrator needs a new upload (velocity convention differs)");
     // A new velocity-Verlet thermostat starts from the current velocities (leapfrog-form NVE
@@ -839,6 +861,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
             p.L[d] = box[3 * r + d];
             p.Linv[d] = (float)(1.0 / (double)box[3 * r + d]);         // src/simulation.cu:290-292
             p.halfL[d] = 0.5f * box[3 * r + d];
+            p.Lb[d] = box[3 * r + d];
         }
         p.slot_off = e->slot_off[r];
         p.nslot = (natom[r] + MD_TILE - 1) / MD_TILE * MD_TILE;
@@ -863,6 +886,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     CHECK(e->slot_of.alloc(N));
     CHECK(e->snap_slot_of.alloc(N));
     CHECK(e->rep.alloc(nrep));
+    CHECK(e->snap_rep.alloc(nrep));
     CHECK(e->tile_rep.alloc(S / MD_TILE));
     CHECK(e->rst.alloc(nrep));
     CHECK(e->snap_rst.alloc(nrep));
@@ -1078,7 +1102,8 @@ namespace {
 int enqueue_segment(md_engine* e, int energy_every, int blocks) {
     const int ce = std::max(1, e->check_every);
     for (int b = 0; b < blocks; b++) {
-        if (e->defer && lf_form(e)) {   // the block's first step kernel opens the rebuilt lists
+        if (e->defer && lf_form(e) && !e->in.barostat) {   // the block's first step kernel opens the rebuilt lists
+            // (pressure coupling scales the box in the finalize kernel after every step)
             CHECK(enqueue_rebuild(e, nullptr, false));
             CHECK(enqueue_lf_block(e, energy_every, ce));
         } else {
@@ -1210,6 +1235,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
             CHECK(full_rebuild(e));
             continue;
         }
+        if (rr.err == 2) return fail(MD_E_UNSTABLE, "the coupled box shrank below twice the list radius (steps %lld..%lld)", e->step, target);
         if (rr.err) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)", e->step, target);
         if (rr.min_step != target) {
             std::vector<RepState> rs;
@@ -1238,11 +1264,12 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
             md_step_stats& o = stats[k];
             o.ke = s.ke; o.temp = s.temp; o.pe_lj = s.pe_lj; o.pe_coul = s.pe_coul; o.pe_bond = s.pe_bond;
             o.pe_angle = s.pe_angle; o.virial = s.virial; o.conserved = s.conserved; o.xi = s.xi;
-            o.vxi = s.vxi; o.has_energy = s.has_energy; o.pad = 0;
+            o.vxi = s.vxi; o.volume = s.volume; o.has_energy = s.has_energy; o.pad = 0;
         }
     }
     HIPCHECK(hipStreamSynchronize(e->st));
     e->step = target;
+    if (e->in.barostat) CHECK(sync_box(e));
     return MD_OK;
 }
 
@@ -1276,6 +1303,16 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
     }
     if (worst > cap) return fail(MD_E_CAPACITY, "rows need %d entries, cap is %d", worst, cap);
     if (worst > e->cap) return fail(MD_E_CAPACITY, "device list overflowed (%d > %d)", worst, e->cap);
+    return MD_OK;
+}
+
+int md_get_box(md_engine* e, float* box) {
+    if (!e || !box) return fail(MD_E_INVALID, "null argument");
+    if (!e->has_sys) return fail(MD_E_STATE, "system not set");
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    CHECK(sync_box(e));
+    for (int k = 0; k < 3 * e->R; k++) box[k] = e->box[k];
     return MD_OK;
 }
 

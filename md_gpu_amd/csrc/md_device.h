@@ -40,6 +40,7 @@
 #define MD_SLOTMASK 0xFFFFFFu
 #define MD_SHIFT0 13            // shift code of the unshifted image: (kx+1) + 3 (ky+1) + 9 (kz+1)
 #define MD_KB_REF 8.314e-3      // src/integrator.cu:155 (1000/8.314)
+#define MD_BAR_PER_KJMOL_A3 16605.390671738467   // 1 kJ/mol/A^3 in bar
 #define MD_NSTAT 8              // accumulators per replica
 #define MD_CTAB 256             // intervals of the Coulomb force table on [0, rc]
 #define MD_SUBSHIFT 25          // list build: ranks and original indices stay below 2^25 (the slot limit is
@@ -61,6 +62,8 @@ struct RepParam {
     float T0;
     float nf;          // 3N - 3
     double kT0, Q;     // Nose-Hoover: kB T0, N_f kB T0 tau^2
+    float Lb[3];       // box at the replica's last list build (pressure coupling scales L)
+    float pad_;
 };
 
 // Per-replica control state (device).
@@ -153,7 +156,7 @@ struct ThState {
 
 // Step statistics record (mirrors md_step_stats in md_api.h)
 struct StatRec {
-    double ke, temp, pe_lj, pe_coul, pe_bond, pe_angle, virial, conserved, xi, vxi;
+    double ke, temp, pe_lj, pe_coul, pe_bond, pe_angle, virial, conserved, xi, vxi, volume;
     int has_energy, pad;
 };
 
@@ -170,7 +173,7 @@ struct StepArgs {
     const int* jcount;        // [ntiles]
     const float4* pos_build;
     const int* tile_rep;
-    const RepParam* rep;
+    RepParam* rep;        // the box changes under pressure coupling (k_step_finalize)
     RepState* st;
     Gate* gate;
     ThState* th;
@@ -215,6 +218,12 @@ struct StepArgs {
     float skin2q;
     int max_age;          // > 0: the list is also refreshed after this many steps
     float bk2[8], br0[8], ak2[8], at0[8];
+    // Berendsen pressure coupling (leapfrog, finalize kernel): baro != 0 scales
+    // positions and box by mu = cbrt(1 - baro_k (P0 - P)) after each step
+    int baro;
+    float p0;             // bar
+    float baro_k;         // compress dt / tau_p, 1/bar
+    float rl, rc;         // list radius and cutoff (the skin check under a scaled box)
 };
 
 // Progress of a run for the host: one small copy per check instead of one per
@@ -255,6 +264,7 @@ struct BuildArgs {
     float4* force;
     int* ids;
     int* typ;
+    RepParam* repw;       // the same parameters, written: Lb at the build
     // scratch, new order
     float4* pos_s;
     float4* vel_s;

@@ -128,7 +128,12 @@ __global__ __launch_bounds__(256) void k_nl_bin(BuildArgs a) {
     const float L0 = p.L[0], L1 = p.L[1], L2 = p.L[2], Li0 = p.Linv[0], Li1 = p.Linv[1], Li2 = p.Linv[2];
     const int n0 = p.nc[0], n1 = p.nc[1], n2 = p.nc[2], cell_off = p.cell_off, slot_off = p.slot_off;
     loads_issued();
-    if (s == slot_off) a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
+    if (s == slot_off) {
+        a.st[r].trip[(st.nb + 1) & 1] = 0;   // the buffer the next epoch will use
+        a.repw[r].Lb[0] = L0;                 // the box of this build (the skin check under pressure coupling)
+        a.repw[r].Lb[1] = L1;
+        a.repw[r].Lb[2] = L2;
+    }
     if (id < 0) return;
     const float4 xr = cur_buf(st) ? x1 : x0;
     // the wrapped position k_nl_permute stores (same operations)
@@ -618,9 +623,10 @@ __global__ __launch_bounds__(256) void k_check_lists(BuildArgs a, int row_g, int
         }
         const float4 xj = a.pos_build[j];
         const int kx = code % 3 - 1, ky = (code / 3) % 3 - 1, kz = code / 9 - 1;
-        const float dx = (xi.x - xj.x) + (-(float)kx * p.L[0]);
-        const float dy = (xi.y - xj.y) + (-(float)ky * p.L[1]);
-        const float dz = (xi.z - xj.z) + (-(float)kz * p.L[2]);
+        // the image shift of the build's box (a coupled box has moved on since)
+        const float dx = (xi.x - xj.x) + (-(float)kx * p.Lb[0]);
+        const float dy = (xi.y - xj.y) + (-(float)ky * p.Lb[1]);
+        const float dz = (xi.z - xj.z) + (-(float)kz * p.Lb[2]);
         const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         if (!(rsq < a.rl2) || j == s) atomicAdd(&bad[2], 1);
     }

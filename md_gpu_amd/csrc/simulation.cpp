@@ -27,6 +27,8 @@ struct RunOptions {
     int integrator = MD_LEAPFROG;   // the reference loop runs leapfrog (vver = 0, src/runGPU.cu:71)
     int thermostat = -1;            // -1: the ensemble's default
     float tau = 100.0f;             // reference tco = 0.01 /fs (src/integrator.cu:146)
+    float tau_p = 1000.0f;          // NPT: Berendsen pressure coupling time, fs
+    float compress = 4.5e-5f;       // NPT: isothermal compressibility, 1/bar (water)
 };
 std::mutex g_opt_mu;
 std::map<const integr*, RunOptions> g_opts;
@@ -298,6 +300,14 @@ simulation* SimulationInit(char* name_sim) {
             else if (strcmp(val, "berendsen") == 0) opt.thermostat = MD_TH_BERENDSEN;
             else if (strcmp(val, "nosehoover") == 0) opt.thermostat = MD_TH_NOSEHOOVER;
             else { fprintf(stderr, "md_gpu: %s: unknown thermostat %s\n", file.c_str(), val); ok = false; }
+        } else if (strcmp(saux, "barostat:") == 0 && fscanf(fin, "%63s %f %f", val, &opt.tau_p, &opt.compress) == 3) {
+            if (strcmp(val, "berendsen") != 0) {
+                fprintf(stderr, "md_gpu: %s: unknown barostat %s\n", file.c_str(), val);
+                ok = false;
+            } else if (!(opt.tau_p > 0) || !(opt.compress >= 0)) {
+                fprintf(stderr, "md_gpu: %s: barostat needs tau_p > 0 and compressibility >= 0\n", file.c_str());
+                ok = false;
+            }
         } else {
             fprintf(stderr, "md_gpu: %s: unknown key %s\n", file.c_str(), saux);
             ok = false;
@@ -337,11 +347,6 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
         g_status = rc;
         return rc;
     };
-    if (integ->enstype == 2) {
-        fprintf(stderr, "md_gpu error: NPT is not implemented (the reference never integrates it)\n");
-        g_status = MD_E_INVALID;
-        return MD_E_INVALID;
-    }
     md_engine* e = nullptr;
     int rc = md_create(&e, 0);
     if (rc) return die(nullptr, rc);
@@ -378,6 +383,11 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
     in.thermostat = integ->enstype == 0 ? MD_TH_NONE : (opt.thermostat >= 0 ? opt.thermostat : MD_TH_BERENDSEN);
     in.dt = integ->dt;
     in.tau = opt.tau > 0 ? opt.tau : 100.0f;
+    // NPT (enstype 2, src/simulation.cu:71-75): Berendsen pressure coupling to press_0 (bar)
+    in.barostat = integ->enstype == 2 ? MD_BARO_BERENDSEN : MD_BARO_NONE;
+    in.press0 = integ->enstype == 2 ? integ->press_0 : 1.0f;
+    in.tau_p = opt.tau_p;
+    in.compress = opt.compress;
     if ((rc = md_set_integrator(e, &in))) return die(e, rc);
     const int N = conf->natom;
     std::vector<int> type(N);
@@ -435,6 +445,12 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
         }
     }
     if ((rc = md_download_state(e, pos.data(), vel.data(), frc.data()))) return die(e, rc);
+    if (in.barostat != MD_BARO_NONE) {   // the coupled box goes back to the caller's config
+        float b[3];
+        if ((rc = md_get_box(e, b))) return die(e, rc);
+        conf->box.x = b[0]; conf->box.y = b[1]; conf->box.z = b[2];
+        conf->boxi.x = 1.0f / b[0]; conf->boxi.y = 1.0f / b[1]; conf->boxi.z = 1.0f / b[2];
+    }
     for (int i = 0; i < N; i++) {
         conf->pos[i].x = pos[3 * i]; conf->pos[i].y = pos[3 * i + 1]; conf->pos[i].z = pos[3 * i + 2];
         conf->vel[i].x = vel[3 * i]; conf->vel[i].y = vel[3 * i + 1]; conf->vel[i].z = vel[3 * i + 2];

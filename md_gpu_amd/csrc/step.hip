@@ -92,6 +92,8 @@ __device__ __forceinline__ void put_record(const StepArgs& a, int r, long long k
     rec.temp = temp;
     rec.xi = xi;
     rec.vxi = vxi;
+    const RepParam& pb = a.rep[r];   // the box of the step's forces (a coupled box is scaled after the record)
+    rec.volume = (double)pb.L[0] * (double)pb.L[1] * (double)pb.L[2];
     rec.has_energy = en ? 1 : 0;
     rec.pad = 0;
     rec.pe_lj = 0.5 * v[ACC_LJ];
@@ -608,8 +610,20 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
             if (drift) {
                 const float x = fmaf(v.x, a.dt, xi.x), y = fmaf(v.y, a.dt, xi.y), z = fmaf(v.z, a.dt, xi.z);
                 pout[slot] = make_float4(x, y, z, xi.w);
-                const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
-                if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
+                if (!a.baro) {
+                    const float ddx = minimg(x - pb.x, L0, Li0), ddy = minimg(y - pb.y, L1, Li1), ddz = minimg(z - pb.z, L2, Li2);
+                    if (fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= a.skin2q) trip = 1;   // src/neighborlist.cu:174
+                } else {
+                    // Coupled box: since the build every pair vector was scaled by m = L / Lb
+                    // besides the atoms' own moves d = x - m x_build, so a pair's distance is
+                    // at least m r_list - |d_i| - |d_j|: the list holds while |d| stays below
+                    // (min(m) r_list - r_c) / 2 (skin / 2 when m = 1).
+                    const float m0 = L0 / pr.Lb[0], m1 = L1 / pr.Lb[1], m2 = L2 / pr.Lb[2];
+                    const float ddx = minimg(x - m0 * pb.x, L0, Li0), ddy = minimg(y - m1 * pb.y, L1, Li1),
+                                ddz = minimg(z - m2 * pb.z, L2, Li2);
+                    const float h = 0.5f * (fminf(m0, fminf(m1, m2)) * a.rl - a.rc);
+                    if (!(h > 0.0f) || fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)) >= h * h) trip = 1;
+                }
                 if (!(fabsf(x) <= L0 && fabsf(y) <= L1 && fabsf(z) <= L2)) *a.err = 1;
             }
         } else if (a.mode == MODE_VV) {
@@ -710,6 +724,45 @@ __global__ __launch_bounds__(64 * G) void k_step_finalize(const StepArgs a) {
         const bool en = (a.mode == MODE_FORCE) ? (a.force_energy != 0)
                                                : (a.energy_every > 0 && (s.step % a.energy_every) == 0);
         finalize_replica(a, r, p, s, th, kstat, en, v, v[MD_NSTAT - 1] != 0.0);
+    }
+    if (a.baro && a.mode == MODE_LF) {
+        // Berendsen pressure coupling (Berendsen et al., J. Chem. Phys. 81, 3684
+        // (1984)), isotropic: P = (sum m v^2 + W) / (3 V) from the step's kinetic
+        // energy and virial (every step computes energies under coupling), then the
+        // positions just written and the box are scaled by mu = cbrt(1 - k (P0 - P)).
+        __shared__ float s_mu;
+        if (threadIdx.x == 0) {
+            float mu = 1.0f;
+            if (run) {
+                const double V = (double)p.L[0] * (double)p.L[1] * (double)p.L[2];
+                const double P = (v[ACC_KE2] + 0.5 * v[ACC_VIR]) / (3.0 * V) * MD_BAR_PER_KJMOL_A3;
+                const double m3 = 1.0 - (double)a.baro_k * ((double)a.p0 - P);
+                mu = (float)cbrt(fmax(m3, 0.125));   // at most a halving of the box edge per step
+                RepParam& w = a.rep[r];
+                for (int d = 0; d < 3; d++) {
+                    const float L = (float)((double)p.L[d] * (double)mu);
+                    w.L[d] = L;
+                    w.Linv[d] = (float)(1.0 / (double)L);   // as the host sets it
+                    w.halfL[d] = 0.5f * L;
+                    w.cinv[d] = (float)p.nc[d] / L;
+                    w.cedge[d] = L / (float)p.nc[d];
+                    if (!(a.rl < 0.5f * L)) *a.err = 2;   // the list radius must stay below half the box
+                }
+            }
+            s_mu = mu;
+        }
+        __syncthreads();
+        const float mu = s_mu;
+        if (mu != 1.0f) {
+            float4* x = a.pos[cur_buf(s) ^ 1] + p.slot_off;   // the buffer this launch wrote
+            for (int k = threadIdx.x; k < p.nslot; k += blockDim.x) {
+                float4 q = x[k];
+                q.x *= mu;
+                q.y *= mu;
+                q.z *= mu;
+                x[k] = q;
+            }
+        }
     }
 }
 

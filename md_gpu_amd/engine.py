@@ -50,12 +50,16 @@ class _FF(C.Structure):
 
 
 class _Integ(C.Structure):
-    _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_float), ("tau", C.c_float)]
+    _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_float), ("tau", C.c_float),
+                ("barostat", C.c_int), ("press0", C.c_float), ("tau_p", C.c_float), ("compress", C.c_float)]
+
+
+BAROSTATS = {"none": 0, "berendsen": 1}
 
 
 STAT_DTYPE = np.dtype([("ke", "f8"), ("temp", "f8"), ("pe_lj", "f8"), ("pe_coul", "f8"), ("pe_bond", "f8"),
                        ("pe_angle", "f8"), ("virial", "f8"), ("conserved", "f8"), ("xi", "f8"), ("vxi", "f8"),
-                       ("has_energy", "i4"), ("pad", "i4")])
+                       ("volume", "f8"), ("has_energy", "i4"), ("pad", "i4")])
 
 _lib = None
 
@@ -78,6 +82,7 @@ def lib():
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
+            "md_get_box": ([P, P], C.c_int),
             "md_check_lists": ([P, P], C.c_int),
             "md_get_info": ([P, P, P, P, P, P], C.c_int), "md_set_list_capacity": ([P, C.c_int], C.c_int),
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
@@ -102,7 +107,7 @@ def exported_symbols() -> List[str]:
             "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
-            "md_set_list_capacity", "md_fm_loss_grad_noalloc"]
+            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box"]
 
 
 def _check(rc: int):
@@ -162,8 +167,10 @@ class Engine:
         s = forcefield_struct(ff)
         _check(lib().md_set_forcefield(self._h, C.byref(s)))
 
-    def set_integrator(self, integrator: str, thermostat: str, dt: float, tau: float = 100.0):
-        s = _Integ(INTEGRATORS[integrator], THERMOSTATS[thermostat], float(dt), float(tau))
+    def set_integrator(self, integrator: str, thermostat: str, dt: float, tau: float = 100.0,
+                       barostat: str = "none", P0: float = 1.0, tau_p: float = 1000.0, compress: float = 4.5e-5):
+        s = _Integ(INTEGRATORS[integrator], THERMOSTATS[thermostat], float(dt), float(tau), BAROSTATS[barostat],
+                   float(P0), float(tau_p), float(compress))
         _check(lib().md_set_integrator(self._h, C.byref(s)))
 
     def set_systems(self, systems: Union[System, Sequence[System]]):
@@ -172,7 +179,7 @@ class Engine:
             systems = [systems]
         s0 = systems[0]
         self.set_forcefield(s0.ff)
-        self.set_integrator(s0.integrator, s0.thermostat, s0.dt, s0.tau)
+        self.set_integrator(s0.integrator, s0.thermostat, s0.dt, s0.tau, s0.barostat, s0.P0, s0.tau_p, s0.compress)
         natom = np.array([s.n for s in systems], np.int32)
         box = np.concatenate([np.asarray(s.box, np.float32) for s in systems])
         T0 = np.array([s.T0 for s in systems], np.float32)
@@ -240,6 +247,12 @@ class Engine:
         _check(lib().md_check_lists(self._h, _ptr(c)))
         keys = ["order", "row_index", "row_distance", "set_entry", "capacity", "padding"]
         return dict(zip(keys, c.tolist()))
+
+    def box(self) -> np.ndarray:
+        """(nrep, 3) current boxes (changed by pressure coupling)."""
+        b = np.zeros(3 * max(self.nrep, 1), np.float32)
+        _check(lib().md_get_box(self._h, _ptr(b)))
+        return b.reshape(-1, 3)
 
     def thresholds(self):
         a = np.zeros(3, np.float32)

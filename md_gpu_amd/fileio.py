@@ -21,12 +21,14 @@ from .systems import COUL_DAMPED, COUL_NONE, COUL_RF, COUL_WOLF, ForceField, Sys
 def write_inputs(s: System, name: str, timrun: int = 100, timprn: int = 10, eps_rf: float = 78.0):
     """Write <name>.inp, <name>.fld, <name>.cfg for System s."""
     ff = s.ff
-    ens = "nve" if s.thermostat == "none" else "nvt"
+    ens = "npt" if s.barostat != "none" else ("nve" if s.thermostat == "none" else "nvt")
     lk = {COUL_RF: 0, COUL_WOLF: 1, COUL_DAMPED: 2, COUL_NONE: 1}[ff.coulomb]
     with open(name + ".inp", "w") as f:
         f.write(f"ensemble: {ens}\n")
         if ens == "nvt":
             f.write(f"temper: {s.T0:.6f}\n")
+        elif ens == "npt":   # temperature and pressure (bar), src/simulation.cu:71-75
+            f.write(f"temper_press: {s.T0:.6f} {s.P0:.6f}\n")
         f.write(f"times: {timrun} {timrun} {timprn} {s.dt}\n")
         f.write(f"cutoff: {ff.rc} {lk}\n")
         if lk == 0:
@@ -39,6 +41,8 @@ def write_inputs(s: System, name: str, timrun: int = 100, timprn: int = 10, eps_
         f.write("rdel: 0.02\n")
         f.write(f"integrator: {s.integrator}\n")
         f.write(f"thermostat: {s.thermostat} {s.tau}\n")
+        if ens == "npt":
+            f.write(f"barostat: {s.barostat} {s.tau_p} {s.compress}\n")
     nt = ff.ntype
     counts = np.bincount(s.type, minlength=nt)
     qtype = [float(s.charge[s.type == t][0]) if counts[t] else 0.0 for t in range(nt)]
@@ -147,11 +151,11 @@ def read_system(name: str) -> System:
     tok = open(name + ".inp").read().split()
     p = 1
     ens = tok[p]; p += 1
-    T0 = 300.0
+    T0, P0 = 300.0, 1.0
     if ens == "nvt":
         T0 = float(tok[p + 1]); p += 2
     elif ens == "npt":
-        T0 = float(tok[p + 1]); p += 3
+        T0, P0 = float(tok[p + 1]), float(tok[p + 2]); p += 3
     dt = float(tok[p + 4]); p += 5
     rc = float(tok[p + 1]); lk = int(tok[p + 2]); p += 3
     alpha, eps_rf = 0.0, 1.0
@@ -164,12 +168,15 @@ def read_system(name: str) -> System:
     p += 2
     skin = float(tok[p + 1]); p += 3
     p += 2
-    integ, thermo, tau = "leapfrog", ("berendsen" if ens == "nvt" else "none"), 100.0
+    integ, thermo, tau = "leapfrog", ("berendsen" if ens in ("nvt", "npt") else "none"), 100.0
+    baro, tau_p, compress = ("berendsen" if ens == "npt" else "none"), 1000.0, 4.5e-5
     while p < len(tok):
         if tok[p] == "integrator:":
             integ = tok[p + 1]; p += 2
         elif tok[p] == "thermostat:":
             thermo, tau = tok[p + 1], float(tok[p + 2]); p += 3
+        elif tok[p] == "barostat:":
+            baro, tau_p, compress = tok[p + 1], float(tok[p + 2]), float(tok[p + 3]); p += 4
         else:
             raise ValueError(f"unknown key {tok[p]}")
     fld = read_fld(name + ".fld")
@@ -181,4 +188,5 @@ def read_system(name: str) -> System:
     if vel is None:
         vel = np.zeros_like(pos)
     return System(os.path.basename(name), box, pos, vel, typ, q, ff, bonds=fld["bonds"], angles=fld["angles"],
-                  integrator=integ, thermostat=thermo if ens != "nve" else "none", dt=dt, T0=T0, tau=tau)
+                  integrator=integ, thermostat=thermo if ens != "nve" else "none", dt=dt, T0=T0, tau=tau,
+                  barostat=baro if ens == "npt" else "none", P0=P0, tau_p=tau_p, compress=compress)

@@ -80,6 +80,10 @@ class System:
     dt: float = 1.0
     T0: float = 300.0
     tau: float = 100.0            # fs; Berendsen tco = 0.01 /fs in src/integrator.cu:146
+    barostat: str = "none"        # "berendsen": isotropic Berendsen pressure coupling (leapfrog)
+    P0: float = 1.0               # bar
+    tau_p: float = 1000.0         # fs
+    compress: float = 4.5e-5      # isothermal compressibility, 1/bar
 
     @property
     def n(self) -> int:

@@ -28,7 +28,8 @@ class OrcFF(C.Structure):
 
 class OrcMD(C.Structure):
     _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_double), ("T0", C.c_double),
-                ("tau", C.c_double), ("skin", C.c_double)]
+                ("tau", C.c_double), ("skin", C.c_double), ("barostat", C.c_int), ("P0", C.c_double),
+                ("tau_p", C.c_double), ("compress", C.c_double)]
 
 
 def build():
@@ -56,7 +57,7 @@ def lib():
         L.orc_bond_forces.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P, P]
         L.orc_angle_forces.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P]
         L.orc_md_run.argtypes = [C.c_int, P, P, P, P, P, P, P, P, C.c_int, P, C.c_int, P, P, C.c_int,
-                                 P, P, P, P, P]
+                                 P, P, P, P, P, P, P]
         L.orc_md_run.restype = C.c_int
         L.orc_fm_loss_grad.argtypes = [C.c_int, C.c_int, P, P, P, P, P, C.c_double, P, P, P, P]
         L.orc_fm_loss_grad.restype = C.c_double
@@ -186,15 +187,21 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     md.thermostat = {"none": 0, "berendsen": 1, "nosehoover": 2}[sys_.thermostat]
     md.dt, md.T0, md.tau = float(sys_.dt), float(sys_.T0), float(sys_.tau)
     md.skin = float(sys_.ff.skin if skin is None else skin)
+    md.barostat = {"none": 0, "berendsen": 1}[getattr(sys_, "barostat", "none")]
+    md.P0 = float(getattr(sys_, "P0", 1.0))
+    md.tau_p = float(getattr(sys_, "tau_p", 1000.0))
+    md.compress = float(getattr(sys_, "compress", 4.5e-5))
     b = np.ascontiguousarray(sys_.bonds, np.int32)
     a = np.ascontiguousarray(sys_.angles, np.int32)
     ex = np.ascontiguousarray(sys_.exclusions(), np.int32)
     ke = np.zeros(nsteps); pe = np.zeros(nsteps); T = np.zeros(nsteps); H = np.zeros(nsteps)
     ts = np.zeros(2) if thermo_state is None else np.ascontiguousarray(thermo_state, np.float64).copy()
+    vol = np.zeros(nsteps)
+    box_end = np.zeros(3)
     nreb = lib().orc_md_run(n, _p(p), _p(v), _p(typ), _p(q), _p(m), _p(box), C.byref(o), C.byref(md),
                             b.shape[0], _p(b), a.shape[0], _p(a), _p(ex), nsteps, _p(ke), _p(pe), _p(T), _p(H),
-                            _p(ts))
-    return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb)
+                            _p(ts), _p(vol), _p(box_end))
+    return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb, volume=vol, box=box_end)
 
 
 def fm_loss_grad(frames, type_, charge, box, ff, qscale, targets, lj1=None, lj2=None):

@@ -23,6 +23,8 @@
  *   orc_bond_forces           FD of k(r-r0)^2, sum F = 0
  *   orc_angle_forces          FD of k(theta-theta0)^2, sum F = 0, zero torque
  *   orc_md_run                constant-force parabola (LF/VV), free flight,
+ *                             barostat: zero compressibility = NVT bit for bit,
+ *                             ideal-gas volume N_f kT0 / (3 P0), pressure response,
  *                             harmonic-dimer period, VV time reversibility,
  *                             NVE drift bound, Berendsen relaxation rate,
  *                             Nose-Hoover conserved quantity and <T>
@@ -448,13 +450,15 @@ bool md_need_rebuild(const MDSys& s, const double* pos, double skin) {
 }
 
 // Forces (FP64 positions), returns potential energy.
-double md_forces(MDSys& s, const double* pos, double* F) {
+// Forces and potential energy; *vir (if given) = W, the sum over pairs and
+// bonds of F.r (each pair once).
+double md_forces(MDSys& s, const double* pos, double* F, double* vir = nullptr) {
     const orc_ff* ff = s.ff;
     int n = s.n, nt = ff->ntype;
     CoulConst cc = coul_const(ff);
     double rc2 = ff->rc * ff->rc;
-    double epot = 0.0;
-#pragma omp parallel for schedule(dynamic, 32) reduction(+ : epot)
+    double epot = 0.0, w = 0.0;
+#pragma omp parallel for schedule(dynamic, 32) reduction(+ : epot, w)
     for (int i = 0; i < n; i++) {
         double f[3] = {0, 0, 0};
         for (int k = 0; k < s.counts[i]; k++) {
@@ -474,6 +478,7 @@ double md_forces(MDSys& s, const double* pos, double* F) {
             double ffv = qq * fr + lj_force(ff->lj1[tp], ff->lj2[tp], rsq);
             for (int q = 0; q < 3; q++) f[q] += ffv * d[q];
             epot += 0.5 * (qq * ur + lj_energy(ff->lj1[tp], ff->lj2[tp], rsq) - lj_energy(ff->lj1[tp], ff->lj2[tp], rc2));
+            w += 0.5 * ffv * rsq;
         }
         for (int q = 0; q < 3; q++) F[3 * (size_t)i + q] = f[q];
     }
@@ -489,6 +494,7 @@ double md_forces(MDSys& s, const double* pos, double* F) {
         double fa = ff->bk2[t] * (ff->br0[t] / r - 1.0);
         for (int q = 0; q < 3; q++) { F[3 * (size_t)i + q] += fa * d[q]; F[3 * (size_t)j + q] -= fa * d[q]; }
         epot += 0.5 * ff->bk2[t] * (r - ff->br0[t]) * (r - ff->br0[t]);
+        w += fa * r * r;
     }
     for (int a = 0; a < s.na; a++) {
         int i = s.angles[4 * a], j = s.angles[4 * a + 1], k = s.angles[4 * a + 2], t = s.angles[4 * a + 3];
@@ -514,6 +520,7 @@ double md_forces(MDSys& s, const double* pos, double* F) {
         }
         epot += 0.5 * ff->ak2[t] * (th - ff->at0[t]) * (th - ff->at0[t]);
     }
+    if (vir) *vir = w;
     return epot;
 }
 
@@ -537,7 +544,7 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
                           const double* mass, const float* box3, const orc_ff* ff, const orc_md* md,
                           int nb, const int* bonds, int na, const int* angles, const int* excl,
                           int nsteps, double* ke, double* pe, double* temp, double* conserved,
-                          double* thermo_state) {
+                          double* thermo_state, double* volume, double* box_final) {
     MDSys s;
     s.n = n; s.type = type; s.charge = charge; s.mass = mass;
     for (int d = 0; d < 3; d++) s.L[d] = box3[d];
@@ -549,8 +556,10 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
     std::vector<double> im(n);
     for (int i = 0; i < n; i++) im[i] = 1.0 / (1e4 * mass[i]);
     md_build(s, pos);
-    double U = md_forces(s, pos, F.data());
+    double W = 0.0;
+    double U = md_forces(s, pos, F.data(), &W);
     int rebuilds = 1;
+    const bool baro = md->barostat == ORC_BARO_BERENDSEN && md->integrator == ORC_INT_LEAPFROG;
     // thermostat state
     double xi = thermo_state ? thermo_state[0] : 0.0, vxi = thermo_state ? thermo_state[1] : 0.0;
     const double kT0 = KB_REF * md->T0;
@@ -576,8 +585,20 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
             // i.e. of the forces this step used
             ke[step] = 0.5 * k2; pe[step] = U; temp[step] = T_prev;
             conserved[step] = 0.5 * k2 + U;
-            if (md_need_rebuild(s, pos, md->skin)) { md_build(s, pos); rebuilds++; }
-            U = md_forces(s, pos, F.data());
+            if (volume) volume[step] = s.L[0] * s.L[1] * s.L[2];
+            if (baro) {   // Berendsen barostat: P of this step, then positions and box scaled
+                const double V = s.L[0] * s.L[1] * s.L[2];
+                const double P = (k2 + W) / (3.0 * V) * ORC_BAR_PER_KJMOL_A3;
+                const double mu = std::cbrt(1.0 - md->compress * dt / md->tau_p * (md->P0 - P));
+                for (size_t k = 0; k < 3 * (size_t)n; k++) pos[k] *= mu;
+                for (int d = 0; d < 3; d++) s.L[d] *= mu;
+                md_build(s, pos);   // the scaled pair distances: a list per step
+                rebuilds++;
+            } else if (md_need_rebuild(s, pos, md->skin)) {
+                md_build(s, pos);
+                rebuilds++;
+            }
+            U = md_forces(s, pos, F.data(), &W);
         } else {
             // velocity Verlet (textbook; the reference's vverlet_nve_pre double-kicks, src/nve_vverlet.cu:49-54)
             auto nh_half = [&](double& K) {   // MTK chain-of-one half step (Martyna, Tuckerman, Klein 1996)
@@ -617,6 +638,8 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
         }
     }
     if (thermo_state) { thermo_state[0] = xi; thermo_state[1] = vxi; }
+    if (box_final)
+        for (int d = 0; d < 3; d++) box_final[d] = s.L[d];
     return rebuilds;
 }
 

@@ -95,6 +95,16 @@ void orc_angle_forces(int n, const float* pos, const float* box3, const orc_ff* 
 /* ---- MD trajectories in FP64 (positions and velocities double) ---- */
 enum { ORC_INT_LEAPFROG = 0, ORC_INT_VVERLET = 1 };
 enum { ORC_TH_NONE = 0, ORC_TH_BERENDSEN = 1, ORC_TH_NOSEHOOVER = 2 };
+/* Berendsen barostat (Berendsen et al., J. Chem. Phys. 81, 3684 (1984)), isotropic:
+ * after each leapfrog step P = (sum m v^2 + W) / (3 V) with W = sum over pairs and
+ * bonds of F.r at the forces of the step (angles carry no virial, as in the
+ * reference's bonded kernels), mu = cbrt(1 - compress dt / tau_p (P0 - P)),
+ * positions and box scaled by mu.  The reference declares press_0 and pcons
+ * (src/simulation.h:113-116) and an NPT kernel without box scaling
+ * (src/npt_leapfrog_ber.cu:28-123, its call commented out at
+ * src/integrator.cu:157-164). */
+enum { ORC_BARO_NONE = 0, ORC_BARO_BERENDSEN = 1 };
+#define ORC_BAR_PER_KJMOL_A3 16605.390671738467   /* 1 kJ/mol/A^3 in bar */
 
 typedef struct {
     int    integrator;      /* ORC_INT_* */
@@ -103,6 +113,10 @@ typedef struct {
     double T0;              /* K */
     double tau;             /* fs (Berendsen tau = 1/tco, src/integrator.cu:146; NH period) */
     double skin;            /* A */
+    int    barostat;        /* ORC_BARO_*: leapfrog only */
+    double P0;              /* bar */
+    double tau_p;           /* fs */
+    double compress;        /* isothermal compressibility, 1/bar */
 } orc_md;
 
 /* Stats per step (length nsteps): ke, pe (lj+coul+bond+angle), temperature,
@@ -114,7 +128,9 @@ int orc_md_run(int n, double* pos, double* vel, const int* type, const double* c
                const double* mass, const float* box3, const orc_ff* ff, const orc_md* md,
                int nb, const int* bonds, int na, const int* angles, const int* excl,
                int nsteps, double* ke, double* pe, double* temp, double* conserved,
-               double* thermo_state /* [xi, v_xi] in/out, may be NULL */);
+               double* thermo_state /* [xi, v_xi] in/out, may be NULL */,
+               double* volume /* per step: the box volume of the step's forces, may be NULL */,
+               double* box_final /* [3] the box after the run, may be NULL */);
 
 /* ---- force matching (config: NaCl, 7 parameters) ----
  * loss = mean over frames*atoms*3 of (F(theta) - Ftarget)^2 with charges
