@@ -104,7 +104,7 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
         e.set_launch(**launch)
         log(f"{name}: launch {e.get_launch()}")
     elif autotune:
-        tuned = e.autotune(192)
+        tuned = e.autotune(steps)   # tuned for runs of the timed run's length
         log(f"{name}: autotuned {e.get_launch()} ({tuned:.4f} ms/step)")
     pos0, _, _ = e.download()
     flops0 = f_pair_fn(pos0)

@@ -197,8 +197,12 @@ typedef struct md_launch_config {
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);
-/* Time candidate launch configurations on the current system (nsteps each)
- * and keep the fastest.  best_ms_per_step may be NULL. */
+/* Time candidate launch configurations on the current system and keep the
+ * fastest.  nsteps is the length of the runs the caller will make (md_run
+ * calls): each trial times a sequence of runs of that length (at least 384
+ * steps in all), so a caller of short runs gets the configuration that is
+ * fastest for short runs.  The physical state is restored after every trial.
+ * best_ms_per_step may be NULL. */
 int md_autotune(md_engine* e, int nsteps, double* best_ms_per_step);
 
 /* Kernel timing with HIP events recorded on the engine's stream around every

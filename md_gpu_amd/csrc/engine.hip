@@ -1488,25 +1488,32 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     std::sort(shapes.begin(), shapes.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
     shapes.resize(std::min<size_t>(shapes.size(), 3));
     // stage 2, whole runs (best of two): shape x rebuild-check interval x age refresh x
-    // submission mode x deferred finalize
+    // submission mode x deferred finalize.  A trial times runs of nsteps steps each (the
+    // length of the caller's md_run calls: a short run pays its first rebuild chain and
+    // waits for the slowest replica, which long runs amortise), at least 4 segments' worth.
     double best_run = 1e30;
-    const int kr = std::max(nsteps, 4 * kSegment);
+    const long long total = std::max<long long>(nsteps, 4LL * kSegment);
     md_launch_config base = best;
     const bool lf = lf_form(e);
-    // whole-run time of configuration c: the best of `reps` timed runs of n steps
-    auto time_run = [&](const md_launch_config& c, int n, int reps, double& per) -> int {
+    // per-step time of configuration c: the best of `reps` trials, each a sequence of
+    // md_run(nsteps) calls covering `steps` steps after a warm-up run
+    auto time_run = [&](const md_launch_config& c, long long steps, int reps, double& per) -> int {
         CHECK(md_set_launch(e, &c));
         per = 1e30;
         for (int rep = 0; rep < reps; rep++) {
             CHECK(restore());
-            CHECK(md_run(e, std::min(n, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
+            CHECK(md_run(e, std::min(nsteps, kSegment), 0, nullptr));   // warm up (graph capture, queue ratio)
+            long long done = 0;
             HIPCHECK(hipEventRecord(t0, e->st));
-            CHECK(md_run(e, n, 0, nullptr));
+            while (done < steps) {
+                CHECK(md_run(e, nsteps, 0, nullptr));
+                done += nsteps;
+            }
             HIPCHECK(hipEventRecord(t1, e->st));
             HIPCHECK(hipEventSynchronize(t1));
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, t0, t1));
-            per = std::min(per, (double)ms / n);
+            per = std::min(per, (double)ms / (double)done);
         }
         return MD_OK;
     };
@@ -1526,7 +1533,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
                     c.refresh = rf;
                     c.defer = df;
                     double per = 1e30;
-                    CHECK(time_run(c, kr, 2, per));
+                    CHECK(time_run(c, total, 2, per));
                     trials.push_back({per, c});
                 }
     // stage 3: the four fastest again, longer and three times each (the minimum
@@ -1535,7 +1542,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     trials.resize(std::min<size_t>(trials.size(), 4));
     for (const Trial& t : trials) {
         double per = 1e30;
-        CHECK(time_run(t.c, 2 * kr, 3, per));
+        CHECK(time_run(t.c, 2 * total, 3, per));
         if (per < best_run) {
             best_run = per;
             best = t.c;
