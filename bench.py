@@ -262,7 +262,7 @@ def extra_workloads(args, want_cpu, tune):
         n, noo = count_pairs(p, wa.box, wa.ff.rc, wa.type, lambda a, b: (a == 0) & (b == 0))
         return 60.0 * n + 5.0 * noo
     out["water196608_nvt_vv_nh"] = step_workload(
-        "water196608", wa, f_water, 1000, 200, tune, wa.dt,
+        "water196608_nvt_vv_nh", wa, f_water, 1000, 200, tune, wa.dt,
         (lambda e, s: cpu_small(e, s, 10.0)) if want_cpu else None)
     # molten NaCl 32,768 ions, NVT leapfrog Berendsen
     nacl = S.nacl()
@@ -275,13 +275,14 @@ def extra_workloads(args, want_cpu, tune):
         return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
                 "sample": f"{n} steps of the same 32,768-ion NaCl state, FP64 oracle (OpenMP, {cpu_info()})"}
     out["nacl32768_wolf_nvt_lf"] = step_workload(
-        "nacl32768", nacl, lambda p: 65.0 * count_pairs(p, nacl.box, nacl.ff.rc)[0], 5000, 300, tune, nacl.dt,
+        "nacl32768_wolf_nvt_lf", nacl, lambda p: 65.0 * count_pairs(p, nacl.box, nacl.ff.rc)[0], 5000, 300, tune,
+        nacl.dt,
         cpu_nacl if want_cpu else None, (lambda e, s: force_error(e, s)) if want_cpu else None)
     # argon 864, NVE velocity Verlet (the 1000-step case, timed over 5000 steps: a
     # step takes microseconds)
     ar = S.argon()
     out["argon864_nve_vv"] = step_workload(
-        "argon864", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 5000, 500, tune, ar.dt,
+        "argon864_nve_vv", ar, lambda p: 36.0 * count_pairs(p, ar.box, ar.ff.rc)[0], 5000, 500, tune, ar.dt,
         (lambda e, s: cpu_small(e, s, 5.0)) if want_cpu else None)
     out["nacl4096x64_force_matching"] = fm_workload(want_cpu)
     return out

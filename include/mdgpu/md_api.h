@@ -154,6 +154,12 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs);
 #define MD_NCHECK 6
 int md_check_lists(md_engine* e, int* counts);
 
+/* Sums over all atoms of the current forces and of the momentum m v, FP64
+ * (the force and momentum lines of the reference's statistics print,
+ * src/prints.cu:117-118), reduced on the device: 2 x 3 doubles come back.
+ * Either pointer may be NULL. */
+int md_get_sums(md_engine* e, double* force_sum, double* momentum_sum);
+
 /* Current box of every replica (3 floats each): the box of md_set_system, or
  * after a run with pressure coupling the coupled box (src/simulation.h:54
  * config.box, which the reference would have had to update for NPT,

@@ -44,6 +44,8 @@ extern "C" hipError_t md_launch_fm_fused(const StepArgs& a, double* part, double
 extern "C" hipError_t md_launch_copies(int npair, const void* const* src, void* const* dst, const long long* bytes,
                                        hipStream_t st);
 extern "C" hipError_t md_launch_check_lists(const BuildArgs& a, int row_g, int* bad, hipStream_t st);
+extern "C" hipError_t md_launch_state_sums(const float4* vel, const float4* force, const int* ids, int nslot, double* out,
+                                           hipStream_t st);
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream);
 
@@ -220,6 +222,7 @@ struct md_engine {
     DBuf<uint32_t> bond_list;
     DBuf<int> nbond, nang;
     DBuf<RepParam> rep, snap_rep;
+    DBuf<double> sums;             // [6] force and momentum sums (md_get_sums)
     DBuf<int> tile_rep;
     DBuf<RepState> rst;
     DBuf<Gate> gate;
@@ -725,7 +728,7 @@ int md_create(md_engine** out, int device) {
         return fail(MD_E_HIP, "hipStreamCreate failed");
     }
     if (e->ljtab.alloc(64) || e->ctab.alloc(MD_CTAB + 1) || e->err.alloc(1) || e->fm_acc.alloc(2 + 2 * 64) ||
-        e->report.alloc(1) || e->badcnt.alloc(8)) {
+        e->report.alloc(1) || e->badcnt.alloc(8) || e->sums.alloc(6)) {
         md_destroy(e);
         return MD_E_HIP;
     }
@@ -1303,6 +1306,20 @@ int md_get_neighbors(md_engine* e, int cap, int* counts, int* nbrs) {
     }
     if (worst > cap) return fail(MD_E_CAPACITY, "rows need %d entries, cap is %d", worst, cap);
     if (worst > e->cap) return fail(MD_E_CAPACITY, "device list overflowed (%d > %d)", worst, e->cap);
+    return MD_OK;
+}
+
+int md_get_sums(md_engine* e, double* force_sum, double* momentum_sum) {
+    CHECK(check_ready(e, true));
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(md_launch_state_sums(e->vel.p, e->force.p, e->ids.p, e->S, e->sums.p, e->st));
+    double h[6];
+    HIPCHECK(hipMemcpyAsync(h, e->sums.p, sizeof h, hipMemcpyDeviceToHost, e->st));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    for (int d = 0; d < 3; d++) {
+        if (force_sum) force_sum[d] = h[d];
+        if (momentum_sum) momentum_sum[d] = h[3 + d];
+    }
     return MD_OK;
 }
 

@@ -431,13 +431,8 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
             const float ke = (float)s.ke;
             if (own) integ->Ke_sum[0] = ke;   // a caller's integr may hold no host buffer here
             printf("%i\t%f\t%f\n", last, integ->temper, ke);   // src/prints.cu:39
-            if ((rc = md_download_state(e, nullptr, vel.data(), frc.data()))) return die(e, rc);
-            double fs[3] = {0, 0, 0}, ps[3] = {0, 0, 0};
-            for (int i = 0; i < N; i++)
-                for (int d = 0; d < 3; d++) {
-                    fs[d] += frc[3 * i + d];
-                    ps[d] += (double)vel[3 * i + d] * conf->vel[i].w;
-                }
+            double fs[3], ps[3];   // reduced on the device: no state download per print
+            if ((rc = md_get_sums(e, fs, ps))) return die(e, rc);
             printf("force: %f %f %f\n", fs[0], fs[1], fs[2]);       // src/prints.cu:117-118
             printf("moment: %f %f %f\n", ps[0], ps[1], ps[2]);
             if (s.has_energy)

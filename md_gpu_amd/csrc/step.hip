@@ -1049,6 +1049,36 @@ extern "C" hipError_t md_launch_set_target(RepState* s, Gate* g, int nrep, long 
     return hipGetLastError();
 }
 
+// Sums over every atom of the force and of the momentum m_int v (vel.w holds
+// 1 / m_int): one workgroup, each thread a strided share of the slots, then a
+// tree in LDS (fixed order: the same bits every call).
+__global__ __launch_bounds__(1024) void k_state_sums(const float4* __restrict__ vel, const float4* __restrict__ force,
+                                                     const int* __restrict__ ids, int nslot, double* out) {
+    __shared__ double s[6][1024];
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = threadIdx.x; k < nslot; k += blockDim.x) {
+        if (ids[k] < 0) continue;
+        const float4 f = force[k], v = vel[k];
+        const double m = 1.0 / (double)v.w;
+        a[0] += f.x; a[1] += f.y; a[2] += f.z;
+        a[3] += m * v.x; a[4] += m * v.y; a[5] += m * v.z;
+    }
+    for (int c = 0; c < 6; c++) s[c][threadIdx.x] = a[c];
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int c = 0; c < 6; c++) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+extern "C" hipError_t md_launch_state_sums(const float4* vel, const float4* force, const int* ids, int nslot, double* out,
+                                           hipStream_t st) {
+    hipLaunchKernelGGL(k_state_sums, dim3(1), dim3(1024), 0, st, vel, force, ids, nslot, out);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream) {
     hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, stream, st, nrep, tmax, ntiles, err, out);

@@ -82,7 +82,7 @@ def lib():
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
-            "md_get_box": ([P, P], C.c_int),
+            "md_get_box": ([P, P], C.c_int), "md_get_sums": ([P, P, P], C.c_int),
             "md_check_lists": ([P, P], C.c_int),
             "md_get_info": ([P, P, P, P, P, P], C.c_int), "md_set_list_capacity": ([P, C.c_int], C.c_int),
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
@@ -107,7 +107,7 @@ def exported_symbols() -> List[str]:
             "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
-            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box"]
+            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums"]
 
 
 def _check(rc: int):
@@ -247,6 +247,13 @@ class Engine:
         _check(lib().md_check_lists(self._h, _ptr(c)))
         keys = ["order", "row_index", "row_distance", "set_entry", "capacity", "padding"]
         return dict(zip(keys, c.tolist()))
+
+    def sums(self):
+        """(force sum (3,), momentum sum (3,)) over all atoms, FP64, reduced on the device."""
+        f = np.zeros(3)
+        p = np.zeros(3)
+        _check(lib().md_get_sums(self._h, _ptr(f), _ptr(p)))
+        return f, p
 
     def box(self) -> np.ndarray:
         """(nrep, 3) current boxes (changed by pressure coupling)."""
