@@ -480,6 +480,7 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.jset = e->jset.p; a.jcount = e->jcount.p;
     a.tile_rep = e->tile_rep.p; a.rep = e->rep.p; a.st = e->rst.p; a.gate = e->gate.p; a.th = e->th.p; a.part = e->part.p;
     a.lst = e->lst.p; a.lagmode = 0; a.lag = 0; a.par = 0; a.open = 0;
+    a.prefilter = e->R > 1 ? 1 : 0;   // cleared for deferred-finalize blocks (every tile takes part)
     a.vvfused = vv_fused(e) ? 1 : 0;
     a.err = e->err.p; a.stats = e->stats.p; a.stat_cap = e->stat_cap;
     a.ljtab = e->ljtab.p;
@@ -563,6 +564,7 @@ int enqueue_lf_block(md_engine* e, int energy_every, int n) {
     StepArgs a = step_args(e, MODE_LF, energy_every);
     a.lagmode = 1;
     a.open = 1;
+    a.prefilter = 0;
     for (int k = 0; k < n; k++) {
         a.lag = k > 0 ? 1 : 0;
         a.par = k & 1;

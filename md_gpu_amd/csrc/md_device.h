@@ -187,6 +187,8 @@ struct StepArgs {
     int open;             // the block's rebuild chain had no commit: the first step kernel (lag = 0)
                           // opens the new list epoch of every stale (just rebuilt) replica
     int lag;              // the previous launch was a step kernel of the same block: finalize it first
+    int prefilter;        // many replicas and no deferred finalize: each tile first reads its replica's
+                          // state and leaves if the replica does not step in this launch
     int par;              // buffer parity of this launch
     int* err;             // [1] non-finite coordinates
     StatRec* stats;       // [stat_cap][R] or null

@@ -333,6 +333,19 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = tid & (G - 1);
     const int tile = xcd_block(blockIdx.x, gridDim.x);   // tiles contiguous per XCD
+    if (a.prefilter) {
+        // Many replicas, no deferred finalize: a tile whose replica does not step in
+        // this launch (its list waits for the next rebuild chain, or it reached its
+        // target) leaves after two scalar loads, before the speculative loads below
+        // (about 16 KB per tile) are issued.
+        const int r0 = __builtin_amdgcn_readfirstlane(a.tile_rep[tile]);
+        const RepState s0 = a.st[r0];
+        bool go;
+        if (a.mode == MODE_FORCE) go = !stale(s0);
+        else if (a.mode == MODE_LF) go = !stale(s0) && s0.step < s0.target;
+        else go = s0.phase == 1 && !stale(s0);
+        if (!go) return;   // workgroup-uniform
+    }
     // lane group tid / G takes the atom of that rank in the tile's row-length order
     const int slot = tile * 64 + a.rperm[tile * 64 + tid / G];
     // Loads that do not depend on the replica's state, issued before any is used:
@@ -394,6 +407,16 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
         if (a.vvfused && sp.step >= sp.target) sp.phase = 0;
     }
     const float4* __restrict__ pin = a.pos[cur_buf(sp)];
+    if (!a.lagmode) {
+        // No launch of this schedule hands state on: a tile whose replica does not
+        // step now (its list is stale and waits for the next rebuild chain, or it
+        // reached its target) leaves before gathering and staging its set.
+        bool go;
+        if (a.mode == MODE_FORCE) go = !stale(s);
+        else if (a.mode == MODE_LF) go = !stale(s) && s.step < s.target;
+        else go = s.phase == 1 && !stale(s);
+        if (!go) return;   // workgroup-uniform: one replica per tile
+    }
     // Round trip 2: the neighbors' positions and, for the deferred finalize, the
     // previous launch's slots of the replica (added in the order of sum_tiles).
     float4 xg[JPRE];
