@@ -200,6 +200,11 @@ typedef struct md_launch_config {
                           next (leapfrog form, so defer applies); 0 = kick, force and finalize kernels */
     int fm_fused;      /* force matching without bonded terms: 1 = one pass computes the force and the
                           gradient sums; 0 = a force pass, then the adjoint pass */
+    int persist;       /* small single systems (one replica of at most 4,096 atoms, Lennard-Jones only,
+                          no bonded terms, leapfrog or NVE velocity Verlet, no pressure coupling):
+                          1 = every step of a run in one launch of a few workgroups that meet at a
+                          grid barrier each step (1024 threads each, `lanes` per atom); other systems
+                          ignore it */
 } md_launch_config;
 int md_set_launch(md_engine* e, const md_launch_config* cfg);
 int md_get_launch(md_engine* e, md_launch_config* cfg);
@@ -211,8 +216,16 @@ int md_get_launch(md_engine* e, md_launch_config* cfg);
  * best_ms_per_step may be NULL. */
 int md_autotune(md_engine* e, int nsteps, double* best_ms_per_step);
 
+/* The persistent kernel (md_launch_config.persist; no counterpart in the
+ * reference): runs made on it, replays of a run after a lane's row outgrew its
+ * length (the run restarts from its first step with rows twice as long), and
+ * the current row length per lane. */
+int md_persist_info(md_engine* e, long long* runs, long long* replays, int* row_len);
+
 /* Kernel timing with HIP events recorded on the engine's stream around every
- * launch of the pair/step kernel during md_run.  enable resets the counters. */
+ * launch of the pair/step kernel during md_run (the persistent kernel, which
+ * runs all steps of a run in one launch, counts as one launch per step).
+ * enable resets the counters. */
 int md_timing_enable(md_engine* e, int enable);
 int md_timing_get(md_engine* e, double* step_kernel_ms, long long* step_kernel_launches,
                   double* rebuild_ms, long long* rebuild_count);

@@ -46,6 +46,9 @@ extern "C" hipError_t md_launch_copies(int npair, const void* const* src, void* 
 extern "C" hipError_t md_launch_check_lists(const BuildArgs& a, int row_g, int* bad, hipStream_t st);
 extern "C" hipError_t md_launch_state_sums(const float4* vel, const float4* force, const int* ids, int nslot, double* out,
                                            hipStream_t st);
+extern "C" hipError_t md_launch_persist(const PersistArgs& a, int lanes, size_t shmem, hipStream_t st, hipEvent_t ev0,
+                                        hipEvent_t ev1);
+extern "C" hipError_t md_launch_persist_clear(unsigned* flags, hipStream_t st);
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream);
 
@@ -252,6 +255,14 @@ struct md_engine {
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
     int fm_fused = 1;              // force matching without bonded terms: one pass for loss and gradient
+    int persist = 0;               // small single systems: all steps of a run in one launch (persist.hip)
+    bool pk_off = false;           // the persistent kernel's rows do not fit its LDS for this system
+    int pk_cap = 0;                // its row entries per lane (0: from the density and pk_req)
+    int pk_req = 0;                // list capacity the caller set (entries per atom): bounds the lane rows
+    long long pk_runs = 0, pk_replays = 0;   // runs on the persistent kernel, replays after a row overflow
+    size_t lds_max = 65536;        // LDS per workgroup on this device
+    DBuf<unsigned> pk_flags;       // [2][MD_PK_WG] barrier flags
+    DBuf<double> pk_pub;           // [2][MD_PK_WG][4] partial sums published at the barriers
     static constexpr double kQueueRatio0 = 1.15;
     double queue_ratio = kQueueRatio0;   // launched / completed steps, learned per launch configuration
     int fm_nframes = 0;
@@ -722,6 +733,7 @@ int md_create(md_engine** out, int device) {
         e->arch = prop.gcnArchName;
         lds = prop.sharedMemPerBlock;
     }
+    e->lds_max = lds;
     // the step and force-matching kernels hold the staged set and the dummy
     // (16 bytes each) next to at most 8 KB of static LDS
     e->jcap_max = (int)std::min<size_t>(MD_JMAX, (lds - 8192) / 16 - 1) / 64 * 64;
@@ -1166,6 +1178,114 @@ int launch_blocks(md_engine* e, int energy_every, long long nblocks) {
     return MD_OK;
 }
 
+// ---- the persistent kernel for small single systems (persist.hip) ----------
+// One replica of at most MD_PK_SLOTS slots, Lennard-Jones only (no charges,
+// bonded terms or exclusions), leapfrog (NVE or Berendsen) or NVE velocity
+// Verlet, no pressure coupling, and r_list below half the box (the kernel's
+// pairs take the minimum image).
+bool persist_eligible(const md_engine* e) {
+    if (e->pk_off || e->R != 1 || e->S > MD_PK_SLOTS) return false;
+    if (e->bonded || e->has_excl || e->coul_kernel != MD_COUL_NONE || e->ff.ntype > 8) return false;
+    if (e->in.barostat != MD_BARO_NONE) return false;
+    const bool lf = e->in.integrator == MD_LEAPFROG && e->in.thermostat != MD_TH_NOSEHOOVER;
+    const bool vv = e->in.integrator == MD_VVERLET && e->in.thermostat == MD_TH_NONE;
+    if (!lf && !vv) return false;
+    const float rl = std::sqrt(e->rl2);
+    for (int d = 0; d < 3; d++)
+        if (!(2.0f * rl < e->box[d])) return false;
+    return true;
+}
+bool persist_ok(const md_engine* e) { return e->persist && persist_eligible(e); }
+
+// Dynamic LDS of a launch: staged positions and types (20 B per slot) and the
+// lane rows (2 B per entry and thread); at least half the CU's 160 KB, so that
+// no two of the workgroups share a CU.
+size_t persist_lds(const md_engine* e, int cap_l) {
+    const size_t need = (size_t)e->S * 20 + (size_t)cap_l * MD_PK_THREADS * 2;
+    return std::max(need, (size_t)81920);
+}
+
+// Row entries per lane: 1.5 x the mean count of neighbors within r_list over
+// the atom's lanes, plus 6, or less when the caller set a smaller list
+// capacity (md_set_list_capacity: entries per atom over the lanes, plus 2).  A
+// longer row is caught and the run replayed with rows twice as long.
+int persist_cap(const md_engine* e) {
+    if (e->pk_cap > 0) return e->pk_cap;
+    const double V = (double)e->box[0] * e->box[1] * e->box[2];
+    const double rl = std::sqrt((double)e->rl2);
+    const double n_rl = e->N / V * 4.0 / 3.0 * kPi * rl * rl * rl;
+    int c = (int)std::ceil(1.5 * n_rl / e->lanes) + 6;
+    if (e->pk_req > 0) c = std::min(c, (e->pk_req + e->lanes - 1) / e->lanes + 2);
+    return (c + 1) & ~1;
+}
+
+// One run on the persistent kernel.  *use_tiles = 1 (state unchanged) when the
+// rows do not fit the kernel's LDS: the caller runs the tile path instead.
+int run_persist(md_engine* e, int nsteps, int energy_every, bool want_stats, int* use_tiles) {
+    *use_tiles = 0;
+    const int G = e->lanes, apw = MD_PK_THREADS / G;
+    const int nwg = (e->S + apw - 1) / apw;
+    if (nwg > MD_PK_WG) {   // (S <= MD_PK_SLOTS and apw >= 64 keep nwg <= 64)
+        *use_tiles = 1;
+        return MD_OK;
+    }
+    if (!e->pk_flags.p) CHECK(e->pk_flags.alloc(2 * MD_PK_WG));
+    if (!e->pk_pub.p) CHECK(e->pk_pub.alloc(2 * MD_PK_WG * 4));
+    const size_t static_lds = 2048;   // the kernel's own tables and sums (1.4 KB)
+    for (;;) {
+        const int cap_l = persist_cap(e);
+        if (persist_lds(e, cap_l) + static_lds > e->lds_max) {
+            e->pk_off = true;   // rows this long do not fit: the tile path from now on
+            *use_tiles = 1;
+            return MD_OK;
+        }
+        CHECK(snapshot(e, false));
+        HIPCHECK(md_launch_persist_clear(e->pk_flags.p, e->st));
+        HIPCHECK(hipMemsetAsync(e->err.p, 0, sizeof(int), e->st));
+        PersistArgs a{};
+        a.pos[0] = e->pos[0].p; a.pos[1] = e->pos[1].p; a.vel = e->vel.p; a.force = e->force.p;
+        a.ids = e->ids.p; a.typ = e->typ.p; a.ljtab = e->ljtab.p;
+        a.st = e->rst.p; a.th = e->th.p; a.rep = e->rep.p;
+        a.flags = e->pk_flags.p; a.pub = e->pk_pub.p; a.err = e->err.p;
+        a.stats = want_stats ? e->stats.p : nullptr; a.stat_cap = e->stat_cap;
+        a.nslot = e->S; a.nwg = nwg; a.nsteps = nsteps; a.energy_every = energy_every;
+        a.leapfrog = e->in.integrator == MD_LEAPFROG ? 1 : 0;
+        a.thermostat = e->in.thermostat; a.ntype = e->ff.ntype; a.cap_l = cap_l;
+        a.dt = e->in.dt; a.tau = e->in.tau; a.rc2 = e->rc2; a.rl2 = e->rl2; a.skin2q = e->skin2q;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (e->timing) {
+            ev0 = ev_get(e, 2);
+            ev1 = ev_get(e, 3);
+            if (!ev0 || !ev1) return fail(MD_E_HIP, "hipEventCreate failed");
+        }
+        HIPCHECK(md_launch_persist(a, G, persist_lds(e, cap_l), e->st, ev0, ev1));
+        int err = 0;
+        HIPCHECK(hipMemcpyAsync(&err, e->err.p, sizeof(int), hipMemcpyDeviceToHost, e->st));
+        HIPCHECK(hipStreamSynchronize(e->st));
+        if (err & MD_PK_ERR_WAIT) return fail(MD_E_HIP, "persistent kernel: a grid barrier wait timed out");
+        if (err & MD_PK_ERR_ROWS) {   // a row outgrew its length: replay the run with longer rows
+            CHECK(snapshot(e, true));
+            HIPCHECK(hipStreamSynchronize(e->st));
+            e->pk_cap = 2 * cap_l;
+            e->pk_replays++;
+            continue;
+        }
+        if (err) return fail(MD_E_UNSTABLE, "non-finite coordinates or an atom beyond the box (steps %lld..%lld)",
+                             e->step, e->step + nsteps);
+        if (e->timing) {   // counted per step: one launch runs them all
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            e->t_step_ms += ms;
+            e->n_step += nsteps;
+        }
+        e->pk_runs++;
+        break;
+    }
+    // the tile path's lists were built for earlier positions
+    HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
+    return MD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1179,8 +1299,14 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     const bool vv = e->in.integrator == MD_VVERLET, fused = vv_fused(e);
     if (stats) CHECK(ensure_stats(e, nsteps));
     const long long target = e->step + nsteps;
+    bool tiles = true;   // the tile path (else the persistent kernel ran the whole run)
+    if (persist_ok(e)) {
+        int use_tiles = 0;
+        CHECK(run_persist(e, nsteps, energy_every, stats != nullptr, &use_tiles));
+        tiles = use_tiles != 0;
+    }
     int retries = 0;
-    for (;;) {   // one attempt; repeated only after a list-capacity overflow
+    while (tiles) {   // one attempt; repeated only after a list-capacity overflow
         CHECK(snapshot(e, false));
         CHECK(clear_rowmax(e));
         HIPCHECK(hipMemsetAsync(e->err.p, 0, sizeof(int), e->st));
@@ -1256,7 +1382,7 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
         }
         break;
     }
-    if (vv && !fused) {   // (leapfrog form: the last step kernel left full-step velocities)
+    if (tiles && vv && !fused) {   // (leapfrog form: the last step kernel left full-step velocities)
         HIPCHECK(md_launch_kick(kick_args(e, 0), e->st));
         HIPCHECK(md_launch_thermo_fix(e->th.p, e->rep.p, e->R, e->in.thermostat, e->in.dt, 1, e->st));
     }
@@ -1368,6 +1494,9 @@ int md_set_list_capacity(md_engine* e, int cap) {
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     CHECK(alloc_list(e, cap, e->jcap));
+    e->pk_req = cap;
+    e->pk_cap = 0;
+    e->pk_off = false;
     HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));   // every list is rebuilt when next needed
     HIPCHECK(hipStreamSynchronize(e->st));
     return MD_OK;
@@ -1389,6 +1518,7 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     if (c->defer != 0 && c->defer != 1) return fail(MD_E_INVALID, "defer must be 0 or 1");
     if (c->vv_fuse != 0 && c->vv_fuse != 1) return fail(MD_E_INVALID, "vv_fuse must be 0 or 1");
     if (c->fm_fused != 0 && c->fm_fused != 1) return fail(MD_E_INVALID, "fm_fused must be 0 or 1");
+    if (c->persist != 0 && c->persist != 1) return fail(MD_E_INVALID, "persist must be 0 or 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
     // the row layout (lpos16) follows the step kernel's lanes
@@ -1403,6 +1533,11 @@ int md_set_launch(md_engine* e, const md_launch_config* c) {
     e->defer = c->defer;
     e->vv_fuse = c->vv_fuse;
     e->fm_fused = c->fm_fused;
+    if (c->persist != e->persist || relayout) {   // lane rows are sized per lanes per atom
+        e->pk_off = false;
+        e->pk_cap = 0;
+    }
+    e->persist = c->persist;
     e->queue_ratio = md_engine::kQueueRatio0;   // what a step costs in launches depends on the configuration
     e->drop_graphs();
     if (c->cell_div != e->cell_div) {
@@ -1430,6 +1565,15 @@ int md_get_launch(md_engine* e, md_launch_config* c) {
     c->defer = e->defer;
     c->vv_fuse = e->vv_fuse;
     c->fm_fused = e->fm_fused;
+    c->persist = e->persist;
+    return MD_OK;
+}
+
+int md_persist_info(md_engine* e, long long* runs, long long* replays, int* row_len) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (runs) *runs = e->pk_runs;
+    if (replays) *replays = e->pk_replays;
+    if (row_len) *row_len = (e->has_sys && e->has_ff) ? persist_cap(e) : 0;
     return MD_OK;
 }
 
@@ -1484,6 +1628,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             md_launch_config c = best;
             c.cell_div = d;
             c.lanes = l;
+            c.persist = 0;
             if (md_set_launch(e, &c) != MD_OK) continue;   // e.g. a tile set beyond the LDS
             CHECK(restore());
             HIPCHECK(hipEventRecord(t0, e->st));
@@ -1513,6 +1658,7 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     double best_run = 1e30;
     const long long total = std::max<long long>(nsteps, 4LL * kSegment);
     md_launch_config base = best;
+    base.persist = 0;
     const bool lf = lf_form(e);
     // per-step time of configuration c: the best of `reps` trials, each a sequence of
     // md_run(nsteps) calls covering `steps` steps after a warm-up run
@@ -1555,6 +1701,17 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
                     CHECK(time_run(c, total, 2, per));
                     trials.push_back({per, c});
                 }
+    // the persistent kernel (small single systems), one trial per lanes per atom
+    if (persist_eligible(e))
+        for (int l : {4, 8, 16}) {
+            md_launch_config c = base;
+            c.cell_div = shapes[0].cell_div;
+            c.lanes = l;
+            c.persist = 1;
+            double per = 1e30;
+            CHECK(time_run(c, total, 2, per));
+            trials.push_back({per, c});
+        }
     // stage 3: the four fastest again, longer and three times each (the minimum
     // over many short noisy trials favours a lucky one)
     std::sort(trials.begin(), trials.end(), [](const Trial& x, const Trial& y) { return x.ms < y.ms; });

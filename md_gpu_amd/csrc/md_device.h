@@ -260,6 +260,41 @@ struct KickArgs {
     int lfgate;           // the step kernel is in leapfrog form: a kicked replica stays counted in n_run
 };
 
+// Persistent kernel for small single systems (persist.hip): all steps of a run
+// in one launch of at most MD_PK_WG workgroups of MD_PK_THREADS threads.
+#define MD_PK_SLOTS 4096          // largest system (slots): the staged positions fill at most 64 KB of LDS
+#define MD_PK_WG 64               // most workgroups (one wavefront polls every barrier flag)
+#define MD_PK_THREADS 1024
+#define MD_PK_TIMEOUT 50000000ull // barrier wait limit, 100 MHz wall-clock ticks (0.5 s)
+#define MD_PK_ERR_ROWS 4          // err bit: a lane's row outgrew cap_l (the run is replayed with longer rows)
+#define MD_PK_ERR_WAIT 8          // err bit: a barrier wait timed out
+
+struct PersistArgs {
+    float4* pos[2];
+    float4* vel;
+    float4* force;
+    const int* ids;
+    const int* typ;
+    const float4* ljtab;  // [8*8] (lj1, lj2, LJ energy shift, 0)
+    RepState* st;         // replica 0
+    ThState* th;
+    const RepParam* rep;
+    unsigned* flags;      // [2][MD_PK_WG] barrier flags, zero at launch
+    double* pub;          // [2][MD_PK_WG][4] partial sums published at the barriers (2 KE, LJ, virial)
+    int* err;             // 1: an atom left the box; MD_PK_ERR_*
+    StatRec* stats;       // [stat_cap] or null
+    int stat_cap;
+    int nslot;
+    int nwg;              // ceil(nslot / (MD_PK_THREADS / lanes))
+    long long nsteps;
+    int energy_every;
+    int leapfrog;         // 1: leapfrog (thermostat none or Berendsen); 0: NVE velocity Verlet
+    int thermostat;
+    int ntype;
+    int cap_l;            // row entries per lane
+    float dt, tau, rc2, rl2, skin2q;
+};
+
 struct BuildArgs {
     float4* pos[2];       // current buffer per replica: pos[cur_buf(st[r])]
     float4* vel;

@@ -28,7 +28,7 @@ class LaunchConfig(C.Structure):
     _fields_ = [("lanes", C.c_int), ("grid", C.c_int), ("cell_div", C.c_int), ("check_every", C.c_int),
                 ("build_lanes", C.c_int), ("submit", C.c_int), ("refresh", C.c_int),
                 ("build_kernel", C.c_int), ("defer", C.c_int), ("vv_fuse", C.c_int),
-                ("fm_fused", C.c_int)]
+                ("fm_fused", C.c_int), ("persist", C.c_int)]
 
     @classmethod
     def names(cls):
@@ -88,7 +88,7 @@ def lib():
             "md_set_launch": ([P, P], C.c_int), "md_get_launch": ([P, P], C.c_int),
             "md_autotune": ([P, C.c_int, P], C.c_int), "md_timing_enable": ([P, C.c_int], C.c_int),
             "md_timing_get": ([P, P, P, P, P], C.c_int), "md_copy_bench": ([P, C.c_longlong, C.c_int, P], C.c_int),
-            "md_rebuild_bench": ([P, C.c_int, P], C.c_int),
+            "md_rebuild_bench": ([P, C.c_int, P], C.c_int), "md_persist_info": ([P, P, P, P], C.c_int),
             "md_fm_set_frames": ([P, C.c_int, P, P], C.c_int),
             "md_fm_loss_grad": ([P, P, P, C.c_float, P, P, P, P], C.c_int),
             "md_fm_loss_grad_noalloc": ([P, P, P, C.c_float, P, P, P, P], C.c_int),
@@ -107,7 +107,7 @@ def exported_symbols() -> List[str]:
             "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
-            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums"]
+            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums", "md_persist_info"]
 
 
 def _check(rc: int):
@@ -295,6 +295,13 @@ class Engine:
         best = C.c_double()
         _check(lib().md_autotune(self._h, nsteps, C.byref(best)))
         return best.value
+
+    def persist_info(self):
+        """Runs made on the persistent small-system kernel, replays after a lane row
+        overflow, and the current row length per lane (md_persist_info)."""
+        runs = C.c_longlong(); rep = C.c_longlong(); rl = C.c_int()
+        _check(lib().md_persist_info(self._h, C.byref(runs), C.byref(rep), C.byref(rl)))
+        return dict(runs=runs.value, replays=rep.value, row_len=rl.value)
 
     def timing(self, enable: bool):
         _check(lib().md_timing_enable(self._h, 1 if enable else 0))
