@@ -34,6 +34,8 @@
 // host reports the error.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "md_device.h"
 #include "md_reduce.h"
 
@@ -98,7 +100,8 @@ __global__ __launch_bounds__(MD_PK_THREADS) void k_persist(const PersistArgs a) 
     extern __shared__ float4 s_dyn[];
     float4* s_pos = s_dyn;                                                 // [nslot] x, y, z, type (-1: padding)
     float* s_tyf = reinterpret_cast<float*>(s_dyn + a.nslot);              // [nslot] type as a float
-    uint16_t* s_rows = reinterpret_cast<uint16_t*>(s_tyf + a.nslot);       // [cap_l][NTH] lane rows
+    // lane rows: entry p of thread t in the 16-bit half p & 1 of word (p >> 1) NTH + t
+    uint16_t* s_rows = reinterpret_cast<uint16_t*>(s_tyf + a.nslot);       // [cap_l / 2][NTH] words
     __shared__ float4 s_lj[64];
     __shared__ double s_red[NW][3];
     __shared__ double s_lam;         // Berendsen factor of the current step
@@ -127,6 +130,7 @@ __global__ __launch_bounds__(MD_PK_THREADS) void k_persist(const PersistArgs a) 
     const double dtd = a.dt, taud = a.tau;
     double lamd = a.leapfrog ? a.th[0].S : 1.0;   // leapfrog: the factor of the run's first step
     const bool need_lam = a.leapfrog && a.thermostat == 1;
+    const bool need_ke = need_lam || a.stats != nullptr;   // the published sums have a reader
     int b = 0;                                      // barriers passed
     unsigned bits = 0;
     if (!a.leapfrog) {
@@ -200,40 +204,57 @@ __global__ __launch_bounds__(MD_PK_THREADS) void k_persist(const PersistArgs a) 
                 const unsigned gm = (unsigned)(__ballot(hit) >> g0) & ((1u << G) - 1u);
                 if (hit) {
                     const int e = n + __builtin_popcount(gm & ((1u << sub) - 1u));
-                    if (e / G < a.cap_l) s_rows[(e / G) * NTH + (tid - sub) + (e % G)] = (uint16_t)j;
+                    const int q = e / G;
+                    if (q < a.cap_l) s_rows[((q >> 1) * NTH + (tid - sub) + (e % G)) * 2 + (q & 1)] = (uint16_t)j;
                 }
                 n += __builtin_popcount(gm);
             }
             cnt = n > sub ? (n - sub + G - 1) / G : 0;
+            // an odd row ends in half a word: its other half names the atom itself (a
+            // valid slot whose pair the walk masks)
+            if ((cnt & 1) && cnt < a.cap_l) s_rows[((cnt >> 1) * NTH + tid) * 2 + 1] = (uint16_t)slot;
             if ((n + G - 1) / G > a.cap_l) mybits |= FL_ROWS;
             nbuild++;
         }
         const bool en = a.energy_every > 0 && ((step0 + k) % a.energy_every) == 0;
-        // pair forces of the own atom (this lane's share of its row)
+        // pair forces of the own atom (this lane's share of its row), two entries per
+        // LDS word so that both neighbors' loads are in flight together
         float fx = 0.f, fy = 0.f, fz = 0.f;
         double elj = 0.0, evi = 0.0;
         const int nr = min(cnt, a.cap_l);
         const float4* s_lji = s_lj + ty * 8;
-        for (int e = 0; e < nr; e++) {
-            const float4 pj = s_pos[s_rows[e * NTH + tid]];
-            const float dx = minimg(x.x - pj.x, L0, Li0);
-            const float dy = minimg(x.y - pj.y, L1, Li1);
-            const float dz = minimg(x.z - pj.z, L2, Li2);
-            const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float4 lj = NT1 ? lj1 : s_lji[(int)pj.w];
-            float ff = pair_ff_lj(rsq, lj);
-            const bool use = !(rsq > a.rc2);   // src/pair_forces.cu:195
-            ff = use ? ff : 0.0f;
-            fx = fmaf(ff, dx, fx);
-            fy = fmaf(ff, dy, fy);
-            fz = fmaf(ff, dz, fz);
-            if (en && use) {
-                const float ri2 = __builtin_amdgcn_rcpf(rsq);
-                const float ri6 = ri2 * ri2 * ri2;
-                elj += (double)(fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z);
-                evi += (double)(ff * rsq);
+        const uint32_t* s_w = reinterpret_cast<const uint32_t*>(s_rows) + tid;
+        auto walk = [&](auto en_tag) {
+            constexpr bool EN = decltype(en_tag)::value;
+            auto pair = [&](const float4& pj, bool ok) {
+                const float dx = minimg(x.x - pj.x, L0, Li0);
+                const float dy = minimg(x.y - pj.y, L1, Li1);
+                const float dz = minimg(x.z - pj.z, L2, Li2);
+                const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float4 lj = NT1 ? lj1 : s_lji[(int)pj.w];
+                float ff = pair_ff_lj(rsq, lj);
+                const bool use = ok && !(rsq > a.rc2);   // src/pair_forces.cu:195
+                ff = use ? ff : 0.0f;
+                fx = fmaf(ff, dx, fx);
+                fy = fmaf(ff, dy, fy);
+                fz = fmaf(ff, dz, fz);
+                if (EN && use) {
+                    const float ri2 = __builtin_amdgcn_rcpf(rsq);
+                    const float ri6 = ri2 * ri2 * ri2;
+                    elj += (double)(fmaf(lj.x * (1.0f / 12.0f) * ri6, ri6, -lj.y * (1.0f / 6.0f) * ri6) - lj.z);
+                    evi += (double)(ff * rsq);
+                }
+            };
+            for (int e = 0; e < nr; e += 2) {
+                const uint32_t w = s_w[(e >> 1) * NTH];
+                const float4 p0 = s_pos[w & 0xffffu];
+                const float4 p1 = s_pos[w >> 16];
+                pair(p0, true);
+                pair(p1, e + 1 < nr);
             }
-        }
+        };
+        if (en) walk(std::true_type{});
+        else walk(std::false_type{});
         fx = group_sum<G>(fx);
         fy = group_sum<G>(fy);
         fz = group_sum<G>(fz);
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(MD_PK_THREADS) void k_persist(const PersistArgs a) 
             if (sub != 0) dke = 0.0;
         }
         // the workgroup's partial sums and flag bits
-        const double q0 = wave_sum_d(dke);
+        const double q0 = need_ke ? wave_sum_d(dke) : 0.0;
         const double q1 = en ? wave_sum_d(elj) : 0.0;
         const double q2 = en ? wave_sum_d(evi) : 0.0;
         const unsigned wb = (__ballot((mybits & FL_TRIP) != 0) ? FL_TRIP : 0u) |
@@ -319,7 +340,7 @@ __global__ __launch_bounds__(MD_PK_THREADS) void k_persist(const PersistArgs a) 
         }
         __syncthreads();
         unsigned wgbits = s_wgb;
-        if (tid < 3) {
+        if (tid < 3 && need_ke) {
             double t = s_red[0][tid];
             for (int w = 1; w < NW; w++) t += s_red[w][tid];
             a.pub[((size_t)(b & 1) * MD_PK_WG + blockIdx.x) * 4 + tid] = t;

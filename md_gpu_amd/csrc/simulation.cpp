@@ -413,6 +413,12 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
     }
     if ((rc = md_set_topology(e, fld->nbonds, bonds.data(), fld->nangle, angles.data()))) return die(e, rc);
     if ((rc = md_upload_state(e, pos.data(), vel.data()))) return die(e, rc);
+    // small single Lennard-Jones systems run every step of an md_run call in one
+    // launch (the engine ignores the option for any other system)
+    md_launch_config lc;
+    if ((rc = md_get_launch(e, &lc))) return die(e, rc);
+    lc.persist = 1;
+    if ((rc = md_set_launch(e, &lc))) return die(e, rc);
 
     printf("Starting the main loop ...\n");
     const int prn = pars->timprn > 0 ? pars->timprn : 1;

@@ -32,11 +32,15 @@ def stat_lines(out):
     return rows
 
 
-@pytest.mark.parametrize("kind", ["nacl_lf_berendsen", "water_vv_nh"])
+@pytest.mark.parametrize("kind", ["nacl_lf_berendsen", "water_vv_nh", "argon_vv_nve"])
 def test_athom_matches_oracle(tmp_path, orc, kind):
+    """argon_vv_nve runs on the persistent small-system kernel (athom enables it)."""
     if kind == "nacl_lf_berendsen":
         s = S.nacl(nsites=12)
         timrun, timprn = 20, 5
+    elif kind == "argon_vv_nve":
+        s = S.argon()
+        timrun, timprn = 40, 10
     else:
         s = S.water(nmol=512)
         timrun, timprn = 20, 5
