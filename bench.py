@@ -76,13 +76,14 @@ def time_run(e, nsteps, energy_every=0):
 
 
 def traffic_of(workload, kernel="step_kernel"):
-    """HBM bytes per launch of the workload's dominant kernel from the counter summary in profiles/ (or None)."""
+    """HBM bytes per launch of the workload's dominant kernel from the counter summary in profiles/ (or None).
+    The persistent kernel runs a whole run in one launch: its bytes are given per step."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
         tr = json.load(f)
-    return tr.get("bytes_per_launch"), os.path.relpath(path, ROOT)
+    return tr.get("bytes_per_step", tr.get("bytes_per_launch")), os.path.relpath(path, ROOT)
 
 
 def kernel_time(e, nsteps):
@@ -124,7 +125,8 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
     kstep_ms, kreb_ms, traw = kernel_time(e, ktime_steps)
     sps = steps / wall
     info = e.info()
-    traffic, tsrc = traffic_of(name)
+    persistent = e.persist_info()["runs"] > 0   # small single system on the persistent kernel
+    traffic, tsrc = traffic_of(name, kernel="persist_kernel" if persistent else "step_kernel")
     out = {
         "value": sps, "unit": "timesteps/s", "ms_per_step": 1e3 * wall / steps,
         "ns_per_day": sps * dt * 0.0864, "steps": steps, "warmup": warmup,
@@ -134,7 +136,9 @@ def step_workload(name, systems, f_pair_fn, steps, warmup, autotune, dt, cpu_fn=
         "rebuild_chain_ms_per_step": traw["rebuild_ms"] / max(ktime_steps, 1),
         "roofline": {"bound": "valu", "achieved": flops / (kstep_ms * 1e-3) / 1e12, "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": flops / (kstep_ms * 1e-3) / 1e12 / PEAK_FP32,
-                     "traffic": traffic, "traffic_source": tsrc, "algorithmic_flop_per_launch": flops},
+                     "traffic": traffic, "traffic_source": tsrc, "algorithmic_flop_per_launch": flops,
+                     "per": "step of the persistent kernel (one launch runs the whole run)" if persistent
+                     else "launch (one step)"},
         "step_roofline_frac": flops / (1e-3 * 1e3 * wall / steps) / 1e12 / PEAK_FP32,
     }
     if long_ms is not None:
