@@ -241,7 +241,9 @@ def water(nmol: int = 65536, seed: int = 3, T: float = 300.0, density: float = 1
                   tau=100.0)
 
 
-def arxe_ff(skin: float = 1.0) -> ForceField:
+def arxe_ff(skin: float = 0.3 * 4.10) -> ForceField:
+    """Ar-Xe Lennard-Jones with Lorentz-Berthelot mixing, cutoff 2.5 sigma_max and
+    skin 0.3 sigma_max (the skin rule of the argon configuration, 0.3 sigma)."""
     sig = np.array([3.405, 4.10])
     eps = np.array([119.8, 221.0]) * KB_KJ
     S = 0.5 * (sig[:, None] + sig[None, :])
