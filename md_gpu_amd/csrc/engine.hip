@@ -610,19 +610,26 @@ int sync_box(md_engine* e) {
     return MD_OK;
 }
 
+// Between runs every replica is at step e->step with phase 0, so its positions
+// are in pos[e->step & 1] (the other buffer is overwritten by the next step) and
+// only velocity Verlet reads the force array at a run's start (first half
+// kick): the snapshot holds what a replay from the run's start reads.
 int snapshot(md_engine* e, bool restore) {
     struct Pair { void* a; void* b; size_t n; };
-    const Pair list[] = {
-        {e->snap_pos0.p, e->pos[0].p, e->pos[0].bytes()}, {e->snap_pos1.p, e->pos[1].p, e->pos[1].bytes()},
-        {e->snap_vel.p, e->vel.p, e->vel.bytes()}, {e->snap_force.p, e->force.p, e->force.bytes()},
+    const int cb = (int)(e->step & 1);
+    DBuf<float4>& sp = cb ? e->snap_pos1 : e->snap_pos0;
+    std::vector<Pair> list = {
+        {sp.p, e->pos[cb].p, e->pos[cb].bytes()},
+        {e->snap_vel.p, e->vel.p, e->vel.bytes()},
         {e->snap_ids.p, e->ids.p, e->ids.bytes()}, {e->snap_typ.p, e->typ.p, e->typ.bytes()},
         {e->snap_slot_of.p, e->slot_of.p, e->slot_of.bytes()}, {e->snap_th.p, e->th.p, e->th.bytes()},
         {e->snap_rst.p, e->rst.p, e->rst.bytes()}, {e->snap_gate.p, e->gate.p, e->gate.bytes()},
         {e->snap_rep.p, e->rep.p, e->rep.bytes()}};   // the box, under pressure coupling
-    constexpr int np = sizeof(list) / sizeof(list[0]);
-    const void* src[np];
-    void* dst[np];
-    long long nb[np];
+    if (e->in.integrator == MD_VVERLET) list.push_back({e->snap_force.p, e->force.p, e->force.bytes()});
+    const int np = (int)list.size();
+    const void* src[12];
+    void* dst[12];
+    long long nb[12];
     for (int k = 0; k < np; k++) {
         src[k] = restore ? list[k].a : list[k].b;
         dst[k] = restore ? list[k].b : list[k].a;

@@ -884,17 +884,29 @@ __global__ void k_set_target(RepState* st, Gate* g, int nrep, long long nsteps) 
 
 // Run report: one workgroup reduces the replicas' step counters and the
 // builds' maxima (see RunReport).
-__global__ __launch_bounds__(256) void k_report(const RepState* st, int nrep, const int* tmax, int ntiles,
-                                                const int* err, RunReport* out) {
-    __shared__ long long s_step[4];
-    __shared__ int s_row[4], s_set[4];
+__global__ __launch_bounds__(1024) void k_report(const RepState* st, int nrep, const int* tmax, int ntiles,
+                                                 const int* err, RunReport* out) {
+    constexpr int NT = 1024, NW = NT / 64, U = 4;
+    __shared__ long long s_step[NW];
+    __shared__ int s_row[NW], s_set[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     long long mn = 0x7fffffffffffffffLL;
     int row = 0, set = 0;
-    for (int r = tid; r < nrep; r += 256) mn = min(mn, st[r].step);
-    for (int t = tid; t < ntiles; t += 256) {
-        row = max(row, tmax[t]);
-        set = max(set, tmax[ntiles + t]);
+    for (int r = tid; r < nrep; r += NT) mn = min(mn, st[r].step);
+    // U independent loads of each array in flight per thread
+    for (int t0 = tid; t0 < ntiles; t0 += U * NT) {
+        int rv[U], sv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * NT;
+            rv[u] = t < ntiles ? tmax[t] : 0;
+            sv[u] = t < ntiles ? tmax[ntiles + t] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            row = max(row, rv[u]);
+            set = max(set, sv[u]);
+        }
     }
     for (int o = 32; o >= 1; o >>= 1) {
         mn = min(mn, (long long)__shfl_xor(mn, o, 64));
@@ -909,9 +921,14 @@ __global__ __launch_bounds__(256) void k_report(const RepState* st, int nrep, co
     __syncthreads();
     if (tid == 0) {
         RunReport rep;
-        rep.min_step = min(min(s_step[0], s_step[1]), min(s_step[2], s_step[3]));
-        rep.rowmax = max(max(s_row[0], s_row[1]), max(s_row[2], s_row[3]));
-        rep.setmax = max(max(s_set[0], s_set[1]), max(s_set[2], s_set[3]));
+        rep.min_step = s_step[0];
+        rep.rowmax = s_row[0];
+        rep.setmax = s_set[0];
+        for (int w = 1; w < NW; w++) {
+            rep.min_step = min(rep.min_step, s_step[w]);
+            rep.rowmax = max(rep.rowmax, s_row[w]);
+            rep.setmax = max(rep.setmax, s_set[w]);
+        }
         rep.err = *err;
         rep.pad = 0;
         *out = rep;
@@ -1047,7 +1064,7 @@ extern "C" hipError_t md_launch_state_sums(const float4* vel, const float4* forc
 
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, stream, st, nrep, tmax, ntiles, err, out);
+    hipLaunchKernelGGL(k_report, dim3(1), dim3(1024), 0, stream, st, nrep, tmax, ntiles, err, out);
     return hipGetLastError();
 }
 

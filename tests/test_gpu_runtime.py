@@ -1,6 +1,7 @@
 """GPU checks of state that only exists during runs: neighbor lists rebuilt
 inside replayed graph segments, run-to-run reproducibility, and energy
 conservation at the benchmark size."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -256,19 +257,28 @@ def test_invalid_calls_raise_and_leave_the_engine_usable(E):
     e.close()
 
 
-def test_list_overflow_rolls_back_and_replays(E):
+@pytest.mark.parametrize("kind", ["nacl_lf", "nacl_vv_nve", "water_vv_nh"])
+def test_list_overflow_rolls_back_and_replays(E, kind):
     """A list capacity below the rows' length (the reference's nmaxlist too
     small, which it does not guard) makes the run roll back to its start, grow
-    the capacity and replay: the result equals a run that never overflowed."""
-    s = S.nacl(nsites=16)
-    say("list overflow replay", s.n)
+    the capacity and replay: the result equals a run that never overflowed.
+    Leapfrog, fused NVE velocity Verlet and velocity Verlet + Nose-Hoover (the
+    velocity-Verlet runs also restore the forces their first half kick reads)."""
+    if kind == "nacl_lf":
+        s = S.nacl(nsites=16)
+    elif kind == "nacl_vv_nve":
+        s = dataclasses.replace(S.nacl(nsites=16), integrator="vverlet", thermostat="none")
+    else:
+        s = S.water(nmol=512)                 # rows hold ~560 entries
+    say("list overflow replay", kind, s.n)
     out = []
     for small in (False, True):
         e = E.Engine.from_system(s)
         e.set_launch(check_every=8)
-        if small:
-            e.set_list_capacity(128)          # NaCl rows hold ~410 entries
-            assert e.info()["cap"] == 128
+        e.run(1, stats=False)                 # the run below starts at an odd step (the other position buffer)
+        # both engines rebuild at the same positions (same atom order); NaCl rows hold ~410 entries
+        e.set_list_capacity(128 if small else 1024)
+        assert e.info()["cap"] == (128 if small else 1024)
         st = e.run(60, energy_every=20)
         pos, vel, frc = e.download()
         out.append((st, pos, vel, frc, e.info()["cap"]))
