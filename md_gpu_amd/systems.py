@@ -84,6 +84,10 @@ class System:
     P0: float = 1.0               # bar
     tau_p: float = 1000.0         # fs
     compress: float = 4.5e-5      # isothermal compressibility, 1/bar
+    # rigid 3-site molecules (leapfrog only): atom indices (O, H1, H2) per molecule, held
+    # at the distances rigid_d = (d_OH, d_HH) by SETTLE after every step
+    rigid: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    rigid_d: tuple = (1.0, 1.6330)
 
     @property
     def n(self) -> int:
@@ -95,26 +99,37 @@ class System:
 
     def exclusions(self) -> np.ndarray:
         """(n, 4) int32 1-2 bonded partners, -1 = empty (src/neighborlist.cu:64-99,
-        filled correctly: the reference's slot test at :89-94 reads excl[i])."""
+        filled correctly: the reference's slot test at :89-94 reads excl[i]); the
+        atoms of a rigid molecule exclude each other."""
         ex = np.full((self.n, 4), -1, np.int32)
         fill = np.zeros(self.n, np.int32)
-        for i, j, _ in self.bonds:
+        pairs = [(int(i), int(j)) for i, j, _ in self.bonds]
+        for o, h1, h2 in self.rigid:
+            pairs += [(int(o), int(h1)), (int(o), int(h2)), (int(h1), int(h2))]
+        for i, j in pairs:
             for a, b in ((i, j), (j, i)):
+                if b in ex[a, :fill[a]]:
+                    continue
                 if fill[a] >= 4:
-                    raise ValueError(f"atom {a} has more than 4 bonded partners")
+                    raise ValueError(f"atom {a} has more than 4 excluded partners")
                 ex[a, fill[a]] = b
                 fill[a] += 1
         return ex
 
+    def nf(self) -> int:
+        """Degrees of freedom of the temperature: 3N - 3 (src/integrator.cu:155), less
+        three per rigid molecule."""
+        return 3 * self.n - 3 - 3 * int(self.rigid.shape[0])
+
     def temperature(self, vel: Optional[np.ndarray] = None) -> float:
         v = self.vel if vel is None else vel
         k2 = float(np.sum(1e4 * self.mass[:, None] * v.astype(np.float64) ** 2))
-        return k2 / ((3 * self.n - 3) * KB_REF)
+        return k2 / (self.nf() * KB_REF)
 
     def copy(self) -> "System":
         return dataclasses.replace(self, box=self.box.copy(), pos=self.pos.copy(), vel=self.vel.copy(),
                                    type=self.type.copy(), charge=self.charge.copy(),
-                                   bonds=self.bonds.copy(), angles=self.angles.copy())
+                                   bonds=self.bonds.copy(), angles=self.angles.copy(), rigid=self.rigid.copy())
 
 
 # ---------------------------------------------------------------- helpers
@@ -204,11 +219,15 @@ def nacl(nsites: int = 32, seed: int = 2, T: float = 1200.0, a: float = 5.64, ff
                   integrator="leapfrog", thermostat="berendsen", dt=2.0, T0=T, tau=100.0)
 
 
-def water(nmol: int = 65536, seed: int = 3, T: float = 300.0, density: float = 1.0) -> System:
+def water(nmol: int = 65536, seed: int = 3, T: float = 300.0, density: float = 1.0, rigid: bool = False) -> System:
     """Flexible 3-site water at density g/cm^3, O on a cubic lattice (random
     subset of sites), random orientations; harmonic O-H and H-O-H, LJ on O only,
     charges O -0.82 / H +0.41, Wolf alpha 0.25 rc 10, NVT velocity Verlet
-    Nose-Hoover 300 K, dt 0.5 fs.  Exclusions are 1-2 only (reference semantics)."""
+    Nose-Hoover 300 K, dt 0.5 fs.  Exclusions are 1-2 only (reference semantics).
+
+    rigid=True: the same molecules held rigid by SETTLE (O-H 1 A, H-O-H 109.47
+    degrees, so H-H 1.633 A), no bonds or angles, the three atoms of a molecule
+    excluding each other; NVT leapfrog Berendsen (tau 100 fs), dt 2 fs."""
     m_o, m_h = 15.9994, 1.0079
     mw = m_o + 2 * m_h
     nd = density * 6.0221415e23 / mw * 1e-24          # molecules / A^3
@@ -236,6 +255,11 @@ def water(nmol: int = 65536, seed: int = 3, T: float = 300.0, density: float = 1
     angles = np.stack([base + 1, base, base + 2, np.zeros_like(base)], 1).astype(np.int32)
     vel = maxwell_boltzmann(rng, ff.masses[typ], T)
     box = np.full(3, L, np.float32)
+    if rigid:
+        mol = np.stack([base, base + 1, base + 2], 1).astype(np.int32)
+        return System("water_rigid", box, wrap_centered(pos, L).astype(np.float32), vel.astype(np.float32), typ, q,
+                      ff, integrator="leapfrog", thermostat="berendsen", dt=2.0, T0=T, tau=100.0, rigid=mol,
+                      rigid_d=(r0, 2.0 * r0 * math.sin(th0 / 2)))
     return System("water", box, wrap_centered(pos, L).astype(np.float32), vel.astype(np.float32), typ, q, ff,
                   bonds=bonds, angles=angles, integrator="vverlet", thermostat="nosehoover", dt=0.5, T0=T,
                   tau=100.0)

@@ -29,7 +29,8 @@ class OrcFF(C.Structure):
 class OrcMD(C.Structure):
     _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_double), ("T0", C.c_double),
                 ("tau", C.c_double), ("skin", C.c_double), ("barostat", C.c_int), ("P0", C.c_double),
-                ("tau_p", C.c_double), ("compress", C.c_double)]
+                ("tau_p", C.c_double), ("compress", C.c_double), ("nrigid", C.c_int), ("rigid", C.c_void_p),
+                ("d_oh", C.c_double), ("d_hh", C.c_double)]
 
 
 def build():
@@ -60,6 +61,10 @@ def lib():
                                  P, P, P, P, P, P, P]
         L.orc_md_run.restype = C.c_int
         L.orc_fm_loss_grad.argtypes = [C.c_int, C.c_int, P, P, P, P, P, C.c_double, P, P, P, P]
+        L.orc_settle.argtypes = [C.c_int, P, P, P, C.c_double, C.c_double, P, P]
+        L.orc_settle.restype = None
+        L.orc_shake.argtypes = [C.c_int, P, P, P, C.c_double, C.c_double, P, P, C.c_double, C.c_int]
+        L.orc_shake.restype = C.c_int
         L.orc_fm_loss_grad.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
         _lib = L
@@ -191,6 +196,10 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     md.P0 = float(getattr(sys_, "P0", 1.0))
     md.tau_p = float(getattr(sys_, "tau_p", 1000.0))
     md.compress = float(getattr(sys_, "compress", 4.5e-5))
+    rig = np.ascontiguousarray(getattr(sys_, "rigid", np.zeros((0, 3), np.int32)), np.int32)
+    md.nrigid = int(rig.shape[0])
+    md.rigid = rig.ctypes.data if rig.shape[0] else None
+    md.d_oh, md.d_hh = (float(x) for x in getattr(sys_, "rigid_d", (1.0, 1.0)))
     b = np.ascontiguousarray(sys_.bonds, np.int32)
     a = np.ascontiguousarray(sys_.angles, np.int32)
     ex = np.ascontiguousarray(sys_.exclusions(), np.int32)
@@ -201,7 +210,32 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     nreb = lib().orc_md_run(n, _p(p), _p(v), _p(typ), _p(q), _p(m), _p(box), C.byref(o), C.byref(md),
                             b.shape[0], _p(b), a.shape[0], _p(a), _p(ex), nsteps, _p(ke), _p(pe), _p(T), _p(H),
                             _p(ts), _p(vol), _p(box_end))
+    if nreb < 0:
+        raise ValueError("the oracle runs rigid molecules with leapfrog only")
     return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb, volume=vol, box=box_end)
+
+
+def settle(mol, mass, box, d_oh, d_hh, x0, x1):
+    """SETTLE: x1 (n, 3) constrained from the positions x0 before the step; returns a new array."""
+    mol = np.ascontiguousarray(mol, np.int32)
+    m = np.ascontiguousarray(mass, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    a0 = np.ascontiguousarray(x0, np.float64)
+    a1 = np.ascontiguousarray(x1, np.float64).copy()
+    lib().orc_settle(mol.shape[0], _p(mol), _p(m), _p(b), float(d_oh), float(d_hh), _p(a0), _p(a1))
+    return a1
+
+
+def shake(mol, mass, box, d_oh, d_hh, x0, x1, tol=1e-14, maxit=10000):
+    """SHAKE on the same constraints: (constrained copy of x1, most iterations used or -1)."""
+    mol = np.ascontiguousarray(mol, np.int32)
+    m = np.ascontiguousarray(mass, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    a0 = np.ascontiguousarray(x0, np.float64)
+    a1 = np.ascontiguousarray(x1, np.float64).copy()
+    it = lib().orc_shake(mol.shape[0], _p(mol), _p(m), _p(b), float(d_oh), float(d_hh), _p(a0), _p(a1),
+                         float(tol), int(maxit))
+    return a1, it
 
 
 def fm_loss_grad(frames, type_, charge, box, ff, qscale, targets, lj1=None, lj2=None):

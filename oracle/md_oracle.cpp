@@ -30,6 +30,11 @@
  *                             Nose-Hoover conserved quantity and <T>
  *   orc_fm_loss_grad          central finite differences of the loss (h = 1e-4
  *                             relative), zero gradient at the target parameters
+ *   orc_settle                exact O-H / H-H distances, agreement with orc_shake
+ *                             (an independent iterative method), the centre of mass
+ *                             kept, identity on positions already satisfying the
+ *                             constraints after a rigid motion; in orc_md_run: NVE
+ *                             energy conservation of rigid water
  * The FP32 device intrinsics of the reference (rsqrtf, __fdividef, exp2f, acosf
  * on compute capability 1.1, src/pair_forces.cu:175-181, src/bonded_forces.cu:165)
  * have no reference test vectors: parity at that boundary is unpinned and is
@@ -382,6 +387,163 @@ void orc_angle_forces(int n, const float* pos, const float* box3, const orc_ff* 
 
 }  // extern "C"
 
+// ---------------- rigid 3-site water: SETTLE and SHAKE ----------------
+namespace {
+
+// Canonical triangle about the centre of mass: O at (0, ra, 0), H at (-+rc, -rb, 0)
+// with ra + rb the height of the triangle and mA ra = 2 mB rb.
+struct SettleGeom { double mA, mB, M, ra, rb, rc; };
+
+SettleGeom settle_geom(double mA, double mB, double d_oh, double d_hh) {
+    SettleGeom g;
+    g.mA = mA;
+    g.mB = mB;
+    g.M = mA + 2.0 * mB;
+    g.rc = 0.5 * d_hh;
+    const double h = std::sqrt(d_oh * d_oh - g.rc * g.rc);
+    g.ra = 2.0 * mB * h / g.M;
+    g.rb = h - g.ra;
+    return g;
+}
+
+inline double mimg(double d, double L) { return d - L * std::rint(d / L); }
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline void unit3(double* a) {
+    const double r = std::sqrt(dot3(a, a));
+    for (int d = 0; d < 3; d++) a[d] /= r;
+}
+
+// One molecule.  B0, C0: old H - O vectors; B1, C1: new H - O vectors (the new O
+// at the origin).  Out: the corrections of O, H1, H2.
+//  1. centre of mass of the new positions; the frame ez = unit(B0 x C0) (normal to
+//     the old plane), ex = unit(a1 x ez), ey = ez x ex;
+//  2. the corrections lie in the old plane, so each atom keeps its ez coordinate:
+//     the canonical triangle is tilted by phi (about ex) and psi (about ey) to match
+//     them: sin phi = z_a1 / ra, sin psi = (z_b1 - z_c1) / (2 rc cos phi);
+//  3. the turn theta about ez makes the in-plane corrections combinations of the
+//     old bond vectors: alpha sin(theta) ... the paper's eqs. (A17)-(A20):
+//     sin theta = (alpha gamma - beta sqrt(alpha^2 + beta^2 - gamma^2)) / (alpha^2 + beta^2).
+void settle1(const SettleGeom& g, const double* B0, const double* C0, const double* B1, const double* C1,
+             double* dA, double* dB, double* dC) {
+    double a1[3], b1[3], c1[3];
+    for (int d = 0; d < 3; d++) {
+        const double com = g.mB * (B1[d] + C1[d]) / g.M;
+        a1[d] = -com;
+        b1[d] = B1[d] - com;
+        c1[d] = C1[d] - com;
+    }
+    double ez[3], ex[3], ey[3];
+    cross3(B0, C0, ez);
+    unit3(ez);
+    cross3(a1, ez, ex);
+    unit3(ex);
+    cross3(ez, ex, ey);
+    const double xb0 = dot3(B0, ex), yb0 = dot3(B0, ey), xc0 = dot3(C0, ex), yc0 = dot3(C0, ey);
+    const double za1 = dot3(a1, ez);
+    const double xb1 = dot3(b1, ex), yb1 = dot3(b1, ey), zb1 = dot3(b1, ez);
+    const double xc1 = dot3(c1, ex), yc1 = dot3(c1, ey), zc1 = dot3(c1, ez);
+    const double sinphi = za1 / g.ra;
+    const double cosphi = std::sqrt(std::max(0.0, 1.0 - sinphi * sinphi));
+    const double sinpsi = (zb1 - zc1) / (2.0 * g.rc * cosphi);
+    const double cospsi = std::sqrt(std::max(0.0, 1.0 - sinpsi * sinpsi));
+    const double ya2 = g.ra * cosphi;
+    const double xb2 = -g.rc * cospsi;
+    const double t1 = -g.rb * cosphi, t2 = g.rc * sinpsi * sinphi;
+    const double yb2 = t1 - t2, yc2 = t1 + t2;
+    const double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+    const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+    const double gamma = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+    const double ab2 = alpha * alpha + beta * beta;
+    const double sinth = (alpha * gamma - beta * std::sqrt(std::max(0.0, ab2 - gamma * gamma))) / ab2;
+    const double costh = std::sqrt(std::max(0.0, 1.0 - sinth * sinth));
+    const double a3[3] = {-ya2 * sinth, ya2 * costh, za1};
+    const double b3[3] = {xb2 * costh - yb2 * sinth, xb2 * sinth + yb2 * costh, zb1};
+    const double c3[3] = {-xb2 * costh - yc2 * sinth, -xb2 * sinth + yc2 * costh, zc1};
+    for (int d = 0; d < 3; d++) {
+        dA[d] = a3[0] * ex[d] + a3[1] * ey[d] + a3[2] * ez[d] - a1[d];
+        dB[d] = b3[0] * ex[d] + b3[1] * ey[d] + b3[2] * ez[d] - b1[d];
+        dC[d] = c3[0] * ex[d] + c3[1] * ey[d] + c3[2] * ez[d] - c1[d];
+    }
+}
+
+// the O -> H vectors of molecule m at positions x (minimum image)
+void oh_vectors(const int* mol, int m, const double* x, const double* L, double* B, double* C) {
+    const int a = mol[3 * m], b = mol[3 * m + 1], c = mol[3 * m + 2];
+    for (int d = 0; d < 3; d++) {
+        B[d] = mimg(x[3 * (size_t)b + d] - x[3 * (size_t)a + d], L[d]);
+        C[d] = mimg(x[3 * (size_t)c + d] - x[3 * (size_t)a + d], L[d]);
+    }
+}
+
+}  // namespace
+
+extern "C" void orc_settle(int nmol, const int* mol, const double* mass, const float* box3, double d_oh,
+                           double d_hh, const double* x0, double* x1) {
+    const double L[3] = {box3[0], box3[1], box3[2]};
+#pragma omp parallel for schedule(static)
+    for (int m = 0; m < nmol; m++) {
+        const int a = mol[3 * m], b = mol[3 * m + 1], c = mol[3 * m + 2];
+        const SettleGeom g = settle_geom(mass[a], mass[b], d_oh, d_hh);
+        double B0[3], C0[3], B1[3], C1[3], dA[3], dB[3], dC[3];
+        oh_vectors(mol, m, x0, L, B0, C0);
+        oh_vectors(mol, m, x1, L, B1, C1);
+        settle1(g, B0, C0, B1, C1, dA, dB, dC);
+        for (int d = 0; d < 3; d++) {
+            x1[3 * (size_t)a + d] += dA[d];
+            x1[3 * (size_t)b + d] += dB[d];
+            x1[3 * (size_t)c + d] += dC[d];
+        }
+    }
+}
+
+extern "C" int orc_shake(int nmol, const int* mol, const double* mass, const float* box3, double d_oh,
+                         double d_hh, const double* x0, double* x1, double tol, int maxit) {
+    const double L[3] = {box3[0], box3[1], box3[2]};
+    int worst = 0;
+#pragma omp parallel for schedule(static) reduction(max : worst)
+    for (int m = 0; m < nmol; m++) {
+        const int id[3] = {mol[3 * m], mol[3 * m + 1], mol[3 * m + 2]};
+        double X0[3][3] = {{0, 0, 0}}, X[3][3] = {{0, 0, 0}}, Xs[3][3];
+        oh_vectors(mol, m, x0, L, X0[1], X0[2]);
+        oh_vectors(mol, m, x1, L, X[1], X[2]);
+        std::memcpy(Xs, X, sizeof X);
+        const double im[3] = {1.0 / mass[id[0]], 1.0 / mass[id[1]], 1.0 / mass[id[2]]};
+        const int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+        const double d2[3] = {d_oh * d_oh, d_oh * d_oh, d_hh * d_hh};
+        int used = -1;
+        for (int it = 1; it <= maxit && used < 0; it++) {
+            bool done = true;
+            for (int k = 0; k < 3; k++) {
+                const int i = pi[k], j = pj[k];
+                double s[3], r0[3];
+                for (int d = 0; d < 3; d++) {
+                    s[d] = X[i][d] - X[j][d];
+                    r0[d] = X0[i][d] - X0[j][d];
+                }
+                const double diff = d2[k] - dot3(s, s);
+                if (std::fabs(diff) > tol * d2[k]) {
+                    done = false;
+                    const double gk = diff / (2.0 * dot3(s, r0) * (im[i] + im[j]));
+                    for (int d = 0; d < 3; d++) {
+                        X[i][d] += gk * r0[d] * im[i];
+                        X[j][d] -= gk * r0[d] * im[j];
+                    }
+                }
+            }
+            if (done) used = it;
+        }
+        for (int a = 0; a < 3; a++)
+            for (int d = 0; d < 3; d++) x1[3 * (size_t)id[a] + d] += X[a][d] - Xs[a][d];
+        worst = std::max(worst, used < 0 ? (1 << 30) : used);
+    }
+    return worst >= (1 << 30) ? -1 : worst;
+}
+
 // ---------------- FP64 MD driver ----------------
 namespace {
 
@@ -551,7 +713,10 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
     s.ff = ff; s.nb = nb; s.na = na; s.bonds = bonds; s.angles = angles; s.excl = excl;
     s.rl = ff->rc + md->skin;
     const double dt = md->dt;
-    const double Nf = 3.0 * n - 3.0;                 // src/integrator.cu:155
+    // src/integrator.cu:155, less the three constraints of each rigid molecule
+    const double Nf = 3.0 * n - 3.0 - 3.0 * (md->nrigid > 0 ? md->nrigid : 0);
+    if (md->nrigid > 0 && md->integrator != ORC_INT_LEAPFROG) return -1;   // SETTLE with leapfrog only
+    std::vector<double> x_old(md->nrigid > 0 ? 3 * (size_t)n : 0);
     std::vector<double> F(3 * (size_t)n);
     std::vector<double> im(n);
     for (int i = 0; i < n; i++) im[i] = 1.0 / (1e4 * mass[i]);
@@ -572,12 +737,21 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
             double lam = 1.0;
             if (md->thermostat == ORC_TH_BERENDSEN)   // src/integrator.cu:146-147
                 lam = std::sqrt(1.0 + dt / md->tau * (md->T0 / T_prev - 1.0));
+            if (md->nrigid > 0) std::memcpy(x_old.data(), pos, 3 * (size_t)n * sizeof(double));
             for (int i = 0; i < n; i++)
                 for (int d = 0; d < 3; d++) {
                     size_t k = 3 * (size_t)i + d;
                     vel[k] = (vel[k] + F[k] * dt * im[i]) * lam;   // src/nvt_leapfrog_ber.cu:39-41
                     pos[k] += vel[k] * dt;
                 }
+            if (md->nrigid > 0) {   // constrained positions; the velocities they imply
+                orc_settle(md->nrigid, md->rigid, mass, box3, md->d_oh, md->d_hh, x_old.data(), pos);
+                for (int m = 0; m < md->nrigid; m++)
+                    for (int a = 0; a < 3; a++) {
+                        const size_t k = 3 * (size_t)md->rigid[3 * m + a];
+                        for (int d = 0; d < 3; d++) vel[k + d] = (pos[k + d] - x_old[k + d]) / dt;
+                    }
+            }
             for (int i = 0; i < n; i++) wrap(pos + 3 * (size_t)i, s.L);
             double k2 = kinetic2(n, vel, mass);
             T_prev = k2 / Nf / KB_REF;                    // src/integrator.cu:153-155

@@ -117,6 +117,9 @@ typedef struct {
     double P0;              /* bar */
     double tau_p;           /* fs */
     double compress;        /* isothermal compressibility, 1/bar */
+    int    nrigid;          /* rigid 3-site molecules (leapfrog only): SETTLE after each position update */
+    const int* rigid;       /* nrigid x 3 atom indices (O, H1, H2) */
+    double d_oh, d_hh;      /* their O-H and H-H distances (A) */
 } orc_md;
 
 /* Stats per step (length nsteps): ke, pe (lj+coul+bond+angle), temperature,
@@ -131,6 +134,24 @@ int orc_md_run(int n, double* pos, double* vel, const int* type, const double* c
                double* thermo_state /* [xi, v_xi] in/out, may be NULL */,
                double* volume /* per step: the box volume of the step's forces, may be NULL */,
                double* box_final /* [3] the box after the run, may be NULL */);
+
+/* ---- rigid 3-site water ----
+ * SETTLE (Miyamoto and Kollman, J. Comput. Chem. 13, 952 (1992)), restated from the
+ * paper: the reference's constraint code is uncompiled pseudo-code (src/settle.c,
+ * src/settle.cu:1-178, src/xshake.cu:15-82).  mol: nmol x 3 atom indices (O, H1,
+ * H2; the two H of equal mass); x0: positions before the step (satisfying the
+ * constraints); x1: after the unconstrained update, constrained in place.  The
+ * vectors from O to H are taken by minimum image and the corrections added to
+ * each atom's own coordinates, so no atom changes its periodic image. */
+void orc_settle(int nmol, const int* mol, const double* mass, const float* box3, double d_oh, double d_hh,
+                const double* x0, double* x1);
+/* SHAKE (Ryckaert, Ciccotti and Berendsen, J. Comput. Phys. 23, 327 (1977)) on the
+ * same three distances per molecule, iterated until |d^2 - r^2| < tol d^2 for all:
+ * the independent check of orc_settle (both give the positions whose corrections
+ * lie along the old bond vectors).  Returns the most iterations a molecule used,
+ * -1 if one did not converge within maxit. */
+int orc_shake(int nmol, const int* mol, const double* mass, const float* box3, double d_oh, double d_hh,
+              const double* x0, double* x1, double tol, int maxit);
 
 /* ---- force matching (config: NaCl, 7 parameters) ----
  * loss = mean over frames*atoms*3 of (F(theta) - Ftarget)^2 with charges
