@@ -121,6 +121,19 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
 /* bonds: nb x (i, j, type); angles: na x (i, center j, k, type); global 0-based
  * atom indices.  1-2 bonded pairs are excluded from pair forces (<= 4 per atom). */
 int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* angles);
+/* Rigid 3-site molecules (water), held by SETTLE (Miyamoto and Kollman,
+ * J. Comput. Chem. 13, 952 (1992)) after every leapfrog step.  The reference's
+ * constraint code (src/settle.c, src/settle.cu:1-178, src/xshake.cu:15-82) is
+ * uncompiled pseudo-code; this is the published method.  mol: nmol x (O, H1, H2)
+ * global 0-based atom indices (H1 and H2 of equal mass, all three in one
+ * replica), held at the O-H distance d_oh and the H-H distance d_hh.  The atoms
+ * of a molecule exclude each other's pair terms (with the 1-2 exclusions of
+ * md_set_topology, at most 4 per atom); give no bonds or angles among them.
+ * The temperature then counts 3N - 3 - 3 N_mol degrees of freedom per replica.
+ * Runs need the leapfrog integrator (NVE or Berendsen) and no pressure coupling.
+ * nmol = 0 removes the constraints.  Call after md_set_system and
+ * md_set_topology (which keeps them). */
+int md_set_rigid(md_engine* e, int nmol, const int* mol, float d_oh, float d_hh);
 
 /* pos (N x 3, required), vel (N x 3 or NULL = zero).  Builds the neighbor lists. */
 int md_upload_state(md_engine* e, const float* pos, const float* vel);

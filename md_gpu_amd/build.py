@@ -25,7 +25,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-Wall", "-Wno-unused-result"]
 
-SOURCES = ["nlist.hip", "step.hip", "persist.hip", "fm.hip", "engine.hip", "simulation.cpp"]
+SOURCES = ["nlist.hip", "step.hip", "persist.hip", "settle.hip", "fm.hip", "engine.hip", "simulation.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -49,6 +49,7 @@ extern "C" hipError_t md_launch_state_sums(const float4* vel, const float4* forc
 extern "C" hipError_t md_launch_persist(const PersistArgs& a, int lanes, size_t shmem, hipStream_t st, hipEvent_t ev0,
                                         hipEvent_t ev1);
 extern "C" hipError_t md_launch_persist_clear(unsigned* flags, hipStream_t st);
+extern "C" hipError_t md_launch_settle(const SettleArgs& a, hipStream_t st);
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream);
 
@@ -222,6 +223,12 @@ struct md_engine {
     DBuf<RunReport> report;        // run progress for the host
     DBuf<int> badcnt;              // list invariant counters (md_check_lists)
     DBuf<int4> excl, ang_list;
+    std::vector<int4> excl_bond;   // host copy of the 1-2 exclusions (md_set_topology)
+    int nrigid = 0;                // rigid 3-site molecules held by SETTLE (settle.hip)
+    float rig_oh = 0, rig_hh = 0;  // their O-H and H-H distances
+    std::vector<int> rig_h;        // host copy: nrigid x (O, H1, H2)
+    DBuf<int4> rig_mol;            // [nrigid] (O, H1, H2, 0)
+    DBuf<int> rig_of;              // [N] molecule of an O atom, -2 for an H atom, -1 unconstrained
     DBuf<uint32_t> bond_list;
     DBuf<int> nbond, nang;
     DBuf<RepParam> rep, snap_rep;
@@ -348,6 +355,52 @@ int upload_ljtab(md_engine* e, const float* lj1, const float* lj2) {
     make_ljtab(e, lj1, lj2, t.data());
     HIPCHECK(hipMemcpyAsync(e->ljtab.p, t.data(), 64 * sizeof(float4), hipMemcpyHostToDevice, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
+    return MD_OK;
+}
+
+// ---- exclusions and rigid molecules ------------------------------------
+// The device exclusion list: the 1-2 bonded partners of md_set_topology and the
+// other atoms of each rigid molecule (at most 4 per atom).
+int upload_exclusions(md_engine* e) {
+    const int N = e->N;
+    std::vector<int4> ex = e->excl_bond;
+    ex.resize(N, make_int4(-1, -1, -1, -1));
+    for (int m = 0; m < e->nrigid; m++) {
+        const int* a = &e->rig_h[3 * (size_t)m];
+        for (int u = 0; u < 3; u++)
+            for (int v = 0; v < 3; v++) {
+                if (u == v) continue;
+                int* sl = &ex[a[u]].x;
+                bool have = false;
+                int k = 0;
+                for (; k < 4 && sl[k] >= 0; k++) have |= sl[k] == a[v];
+                if (have) continue;
+                if (k == 4) return fail(MD_E_CAPACITY, "atom %d has more than 4 excluded partners", a[u]);
+                sl[k] = a[v];
+            }
+    }
+    bool any = false;
+    for (const int4& q : ex) any |= q.x >= 0;
+    HIPCHECK(hipMemcpy(e->excl.p, ex.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    e->has_excl = any;
+    return MD_OK;
+}
+
+// Temperature degrees of freedom: 3N - 3 per replica (src/integrator.cu:155),
+// less 3 per rigid molecule of the replica.
+int upload_dof(md_engine* e) {
+    std::vector<int> nmol(e->R, 0);
+    for (int m = 0; m < e->nrigid; m++) {
+        const int o = e->rig_h[3 * (size_t)m];
+        const int r = (int)(std::upper_bound(e->atom_off.begin(), e->atom_off.end(), o) - e->atom_off.begin()) - 1;
+        nmol[r]++;
+    }
+    for (int r = 0; r < e->R; r++) {
+        RepParam& p = e->rep_h[r];
+        p.nf = (float)(3.0 * e->natom[r] - 3.0 - 3.0 * nmol[r]);
+        p.Q = (double)p.nf * p.kT0 * (double)e->in.tau * e->in.tau;
+    }
+    HIPCHECK(hipMemcpy(e->rep.p, e->rep_h.data(), e->R * sizeof(RepParam), hipMemcpyHostToDevice));
     return MD_OK;
 }
 
@@ -559,11 +612,24 @@ int enqueue_rebuild(md_engine* e, TimedSpan* ts, bool commit = true) {
     return MD_OK;
 }
 
+SettleArgs settle_args(md_engine* e, int par) {
+    SettleArgs g{};
+    g.pos[0] = e->pos[0].p; g.pos[1] = e->pos[1].p; g.vel = e->vel.p; g.pos_build = e->pos_build.p;
+    g.ids = e->ids.p; g.tile_rep = e->tile_rep.p; g.rep = e->rep.p; g.st = e->rst.p; g.slot_of = e->slot_of.p;
+    g.rig_of = e->rig_of.p; g.rig_mol = e->rig_mol.p; g.part = e->part.p; g.err = e->err.p;
+    g.par = par; g.ntiles = e->ntiles(); g.dt = e->in.dt; g.skin2q = e->skin2q;
+    g.rc_hh = 0.5f * e->rig_hh;
+    g.h = (float)std::sqrt((double)e->rig_oh * e->rig_oh - 0.25 * (double)e->rig_hh * e->rig_hh);
+    return g;
+}
+
 int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool finalize = true) {
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) ev = ts->take();   // the launch records the kernel's own start and end
     HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->st, ev.first, ev.second));
     if (ts) ts->steps.push_back(ev);
+    // rigid molecules: SETTLE on the step's positions, before the finalize reads the slots
+    if (e->nrigid > 0 && a.mode == MODE_LF) HIPCHECK(md_launch_settle(settle_args(e, a.par), e->st));
     if (finalize) HIPCHECK(md_launch_step_finalize(a, e->lanes, e->st));
     return MD_OK;
 }
@@ -857,6 +923,9 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     }
     e->R = nrep;
     e->natom.assign(natom, natom + nrep);
+    e->nrigid = 0;   // a new system: no rigid molecules, no exclusions until given
+    e->rig_h.clear();
+    e->excl_bond.clear();
     e->box.assign(box, box + 3 * nrep);
     e->T0.assign(nrep, 0.f);
     if (T0) e->T0.assign(T0, T0 + nrep);
@@ -976,7 +1045,8 @@ int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* a
             al[(size_t)a * 4 + nan_[a]++] = ent[s];
         }
     }
-    HIPCHECK(hipMemcpy(e->excl.p, ex.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    e->excl_bond = ex;
+    CHECK(upload_exclusions(e));
     HIPCHECK(hipMemcpy(e->bond_list.p, bl.data(), bl.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->nbond.p, nbo.data(), N * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(e->ang_list.p, al.data(), al.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -984,7 +1054,47 @@ int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* a
     e->nb = nb;
     e->na = na;
     e->bonded = nb > 0 || na > 0;
-    e->has_excl = nb > 0;
+    e->drop_graphs();
+    if (e->has_state) CHECK(full_rebuild(e));
+    return MD_OK;
+}
+
+int md_set_rigid(md_engine* e, int nmol, const int* mol, float d_oh, float d_hh) {
+    if (!e || !e->has_sys) return fail(MD_E_STATE, "set the system first");
+    if (nmol < 0 || (nmol > 0 && !mol)) return fail(MD_E_INVALID, "bad molecule array");
+    if (nmol > 0 && !(d_oh > 0.0f && d_hh > 0.0f && d_hh < 2.0f * d_oh))
+        return fail(MD_E_INVALID, "rigid geometry needs 0 < d_hh < 2 d_oh (got %g, %g)", d_oh, d_hh);
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    const int N = e->N;
+    std::vector<int> of(N, -1);
+    for (int m = 0; m < nmol; m++) {
+        const int* a = mol + 3 * (size_t)m;
+        int rep_of[3];
+        for (int u = 0; u < 3; u++) {
+            if (a[u] < 0 || a[u] >= N) return fail(MD_E_INVALID, "molecule %d: atom %d out of range", m, a[u]);
+            if (of[a[u]] != -1) return fail(MD_E_INVALID, "atom %d is in two rigid molecules", a[u]);
+            of[a[u]] = u == 0 ? m : -2;
+            rep_of[u] = (int)(std::upper_bound(e->atom_off.begin(), e->atom_off.end(), a[u]) - e->atom_off.begin()) - 1;
+        }
+        if (rep_of[1] != rep_of[0] || rep_of[2] != rep_of[0])
+            return fail(MD_E_INVALID, "molecule %d spans two replicas", m);
+        if (e->mass_h[a[1]] != e->mass_h[a[2]]) return fail(MD_E_INVALID, "molecule %d: the two H masses differ", m);
+    }
+    e->nrigid = nmol;
+    e->rig_oh = d_oh;
+    e->rig_hh = d_hh;
+    e->rig_h.assign(mol, mol + 3 * (size_t)nmol);
+    if (nmol > 0) {
+        std::vector<int4> mh(nmol);
+        for (int m = 0; m < nmol; m++) mh[m] = make_int4(mol[3 * m], mol[3 * m + 1], mol[3 * m + 2], 0);
+        CHECK(e->rig_mol.alloc(nmol));
+        HIPCHECK(hipMemcpy(e->rig_mol.p, mh.data(), nmol * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    CHECK(e->rig_of.alloc(N));
+    HIPCHECK(hipMemcpy(e->rig_of.p, of.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    CHECK(upload_exclusions(e));
+    CHECK(upload_dof(e));
     e->drop_graphs();
     if (e->has_state) CHECK(full_rebuild(e));
     return MD_OK;
@@ -1126,8 +1236,10 @@ namespace {
 int enqueue_segment(md_engine* e, int energy_every, int blocks) {
     const int ce = std::max(1, e->check_every);
     for (int b = 0; b < blocks; b++) {
-        if (e->defer && lf_form(e) && !e->in.barostat) {   // the block's first step kernel opens the rebuilt lists
-            // (pressure coupling scales the box in the finalize kernel after every step)
+        if (e->defer && lf_form(e) && !e->in.barostat && e->nrigid == 0) {
+            // the block's first step kernel opens the rebuilt lists (pressure coupling scales the
+            // box in the finalize kernel after every step; SETTLE runs between a step kernel and
+            // its finalize)
             CHECK(enqueue_rebuild(e, nullptr, false));
             CHECK(enqueue_lf_block(e, energy_every, ce));
         } else {
@@ -1302,6 +1414,8 @@ int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     if (nsteps < 0) return fail(MD_E_INVALID, "nsteps < 0");
     if (nsteps == 0) return MD_OK;
     if (energy_every < 0) return fail(MD_E_INVALID, "energy_every < 0");
+    if (e->nrigid > 0 && (e->in.integrator != MD_LEAPFROG || e->in.barostat != MD_BARO_NONE))
+        return fail(MD_E_INVALID, "rigid molecules need the leapfrog integrator and no pressure coupling");
     HIPCHECK(hipSetDevice(e->dev));
     const bool vv = e->in.integrator == MD_VVERLET, fused = vv_fused(e);
     if (stats) CHECK(ensure_stats(e, nsteps));

@@ -60,7 +60,7 @@ struct RepParam {
     int cs_off;        // offset in cell_start (= cell_off + replica index)
     int slot_off, nslot, natom, atom_off;
     float T0;
-    float nf;          // 3N - 3
+    float nf;          // 3N - 3, less 3 per rigid molecule
     double kT0, Q;     // Nose-Hoover: kB T0, N_f kB T0 tau^2
     float Lb[3];       // box at the replica's last list build (pressure coupling scales L)
     float pad_;
@@ -258,6 +258,28 @@ struct KickArgs {
     float dt, skin2q;
     int apply_drift;      // 0: run end (v *= S_end only)
     int lfgate;           // the step kernel is in leapfrog form: a kicked replica stays counted in n_run
+};
+
+// SETTLE for rigid 3-site molecules (settle.hip), between a leapfrog step kernel
+// and its finalize.
+struct SettleArgs {
+    float4* pos[2];
+    float4* vel;
+    const float4* pos_build;
+    const int* ids;
+    const int* tile_rep;
+    const RepParam* rep;
+    const RepState* st;
+    const int* slot_of;
+    const int* rig_of;    // [N] molecule index of an O atom, -2 for an H atom, -1 unconstrained
+    const int4* rig_mol;  // [nmol] (O, H1, H2, 0) original atom indices
+    double* part;         // the step kernel's partial slots (KE and skin-trip planes rewritten)
+    int* err;
+    int par;              // parity of the step launch's slots
+    int ntiles;
+    float dt, skin2q;
+    float rc_hh;          // half the H-H distance
+    float h;              // height of the triangle: sqrt(d_OH^2 - rc_hh^2)
 };
 
 // Persistent kernel for small single systems (persist.hip): all steps of a run

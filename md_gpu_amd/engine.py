@@ -79,6 +79,7 @@ def lib():
             "md_set_forcefield": ([P, P], C.c_int), "md_set_integrator": ([P, P], C.c_int),
             "md_set_system": ([P, C.c_int, P, P, P, P, P, P], C.c_int),
             "md_set_topology": ([P, C.c_int, P, C.c_int, P], C.c_int),
+            "md_set_rigid": ([P, C.c_int, P, C.c_float, C.c_float], C.c_int),
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
@@ -107,7 +108,8 @@ def exported_symbols() -> List[str]:
             "md_run", "md_get_neighbors", "md_check_lists", "md_get_thresholds", "md_get_info", "md_set_launch",
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
-            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums", "md_persist_info"]
+            "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums", "md_persist_info",
+            "md_set_rigid"]
 
 
 def _check(rc: int):
@@ -197,9 +199,17 @@ class Engine:
         a = np.concatenate(angles).astype(np.int32) if angles else np.zeros((0, 4), np.int32)
         _check(lib().md_set_topology(self._h, b.shape[0], _ptr(np.ascontiguousarray(b)), a.shape[0],
                                      _ptr(np.ascontiguousarray(a))))
+        rig = [np.asarray(s.rigid, np.int32) + o for s, o in zip(systems, offs) if len(s.rigid)]
+        if rig:
+            self.set_rigid(np.concatenate(rig), *s0.rigid_d)
         pos = np.ascontiguousarray(np.concatenate([s.pos for s in systems]), np.float32)
         vel = np.ascontiguousarray(np.concatenate([s.vel for s in systems]), np.float32)
         self.upload(pos, vel)
+
+    def set_rigid(self, mol, d_oh: float, d_hh: float):
+        """Rigid 3-site molecules (md_set_rigid): mol (nmol, 3) global atom indices (O, H1, H2)."""
+        mol = np.ascontiguousarray(mol, np.int32).reshape(-1, 3)
+        _check(lib().md_set_rigid(self._h, mol.shape[0], _ptr(mol), float(d_oh), float(d_hh)))
 
     @classmethod
     def from_system(cls, systems, device: int = 0) -> "Engine":
