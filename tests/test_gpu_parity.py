@@ -147,7 +147,12 @@ def test_integrator_short_run(E, orc, name, mk, nsteps):
 
 
 def test_nve_energy_drift(E, orc):
-    """Argon, NVE velocity Verlet dt 2 fs, 1000 steps: GPU drift no worse than the oracle's."""
+    """Argon, NVE velocity Verlet dt 2 fs, 1000 steps: a loose sanity bound (the GPU
+    drift within twice the oracle's, half its fluctuation, or 1e-6 of the energy).
+    Over 1000 steps the drift is the size of the fluctuation, so this is no
+    statistical comparison; test_gpu_fullsize.py::test_nve_drift_10000_steps (and
+    the persistent kernel's test_persist_nve_drift_10000_steps) make that one:
+    |drift_gpu| <= |drift_oracle| + 2 sigma over 10,000 steps."""
     s = S.argon()
     say("nve drift argon 1000")
     e = E.Engine.from_system(s)
