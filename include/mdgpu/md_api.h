@@ -203,7 +203,8 @@ typedef struct md_launch_config {
     int grid;          /* unused (0): the step kernel runs one workgroup per 64-atom tile */
     int cell_div;      /* cell edge >= r_list / cell_div, 1..3 */
     int check_every;   /* steps queued between two (device-gated) list-rebuild chains */
-    int build_lanes;   /* unused (4 or 8 accepted) */
+    int build_lanes;   /* wavefronts per 64-atom tile in the list build: 4 (256 threads, more
+                          workgroups resident per CU) or 8 (512 threads); the same lists */
     int submit;        /* 0: replayed HIP graph segments, 1: direct kernel launches */
     int refresh;       /* 1: also rebuild a list after check_every steps, ahead of the skin trip */
     int build_kernel;  /* 0: the tile neighbor-set build (the only list build) */

@@ -258,7 +258,7 @@ struct md_engine {
     int jcap_max = 0;              // largest set the step kernel's LDS holds on this device
 
     long long step = 0;          // steps completed (all replicas advance together between calls)
-    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 8, submit = 0, refresh = 0, build_kernel = 0;
+    int lanes = 8, grid = 0, cell_div = 2, check_every = 8, build_lanes = 4, submit = 0, refresh = 0, build_kernel = 0;
     int defer = 1;                 // leapfrog: finalize each step inside the next step kernel of the block
     int vv_fuse = 1;               // NVE velocity Verlet in leapfrog form (one step kernel per step)
     int fm_fused = 1;              // force matching without bonded terms: one pass for loss and gradient
@@ -532,6 +532,7 @@ BuildArgs build_args(md_engine* e) {
     b.tile_rep = e->tile_rep.p; b.rep = e->rep.p; b.repw = e->rep.p;
     b.nslot = e->S; b.nrep = e->R; b.cap = e->cap; b.jcap = e->jcap; b.rl2 = e->rl2; b.has_excl = e->has_excl ? 1 : 0;
     b.row_g = e->lanes;
+    b.build_nw = e->build_lanes;
     b.lfgate = lf_form(e) ? 1 : 0;
     return b;
 }
@@ -1772,6 +1773,32 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     if (shapes.empty()) return fail(MD_E_CAPACITY, "no launch shape fits this system");
     std::sort(shapes.begin(), shapes.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
     shapes.resize(std::min<size_t>(shapes.size(), 3));
+    // the tile build's wavefronts per tile (the same lists: build time only), at the best shape
+    {
+        double bbest = 1e30;
+        for (int nw : {4, 8}) {
+            md_launch_config c = best;
+            c.cell_div = shapes[0].cell_div;
+            c.lanes = shapes[0].lanes;
+            c.build_lanes = nw;
+            c.persist = 0;
+            if (md_set_launch(e, &c) != MD_OK) continue;
+            CHECK(restore());
+            HIPCHECK(hipEventRecord(t0, e->st));
+            for (int k = 0; k < 3; k++) {
+                HIPCHECK(md_launch_force_trip(e->rst.p, e->gate.p, e->R, e->st));
+                CHECK(enqueue_rebuild(e, nullptr));
+            }
+            HIPCHECK(hipEventRecord(t1, e->st));
+            HIPCHECK(hipEventSynchronize(t1));
+            float bms = 0;
+            HIPCHECK(hipEventElapsedTime(&bms, t0, t1));
+            if (bms < bbest) {
+                bbest = bms;
+                best.build_lanes = nw;
+            }
+        }
+    }
     // stage 2, whole runs (best of two): shape x rebuild-check interval x age refresh x
     // submission mode x deferred finalize.  A trial times runs of nsteps steps each (the
     // length of the caller's md_run calls: a short run pays its first rebuild chain and

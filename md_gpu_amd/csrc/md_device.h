@@ -359,6 +359,7 @@ struct BuildArgs {
     float rl2;
     int has_excl;
     int row_g;            // lanes per atom of the step kernel (row layout, lpos16)
+    int build_nw;         // wavefronts per tile in the tile build (4 or 8; the same lists)
     int lfgate;           // the step kernel is in leapfrog form: n_run counts every valid replica short of
                           // its target, whatever its phase
 };

@@ -249,11 +249,13 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
     a.slot_of[myid] = f;
 }
 
-// Tile neighbor-set build: one workgroup of 8 wavefronts per 64-slot tile, 8
-// atoms per wavefront (held as wavefront-uniform values).
+// Tile neighbor-set build: one workgroup of NW wavefronts (4 or 8) per 64-slot
+// tile, 64 / NW atoms per wavefront (held as wavefront-uniform values).  The set
+// and rows do not depend on NW: four wavefronts (more workgroups resident per CU
+// to overlap the latency-bound candidate streaming) build the same lists as eight.
 //  1. The image cells within r_list of the tile's bounding box are listed in LDS
 //     (region order, two-pass block scan), in batches of kTab region cells.
-//  2. Their members are streamed 512 per chunk (a thread per candidate); those
+//  2. Their members are streamed 64 NW per chunk (a thread per candidate); those
 //     whose image lies r_list or farther from the box are dropped, the rest are
 //     appended to an LDS ring in candidate order (stable block compaction).
 //  3. Every 64 ring entries, each wavefront tests the 64 candidates (a lane each)
@@ -262,9 +264,9 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
 //     wavefronts' hit masks goes through LDS, one barrier per 64 candidates), and
 //     each atom's hits are appended to its row at count + (hits in lower lanes).
 // So set and rows follow candidate order and do not depend on scheduling.
-template <bool EXCL>
-__global__ __launch_bounds__(512) void k_nl_tbuild(BuildArgs a) {
-    constexpr int NT = 512, NW = 8, AW = 8, RING = 1024, PER = kTab / NT;
+template <bool EXCL, int NW>
+__global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
+    constexpr int NT = 64 * NW, AW = 64 / NW, RING = 1024, PER = kTab / NT;
     __shared__ int s_tstart[kTab];        // first slot of the cell
     __shared__ int s_tpref[kTab + 1];     // first candidate index of each table row; [ntab] = total
     __shared__ int s_tcode[kTab];         // image shift code of the cell
@@ -646,8 +648,13 @@ extern "C" hipError_t md_launch_rebuild(const BuildArgs& a, int commit, hipStrea
     hipLaunchKernelGGL(k_nl_scatter, dim3(nb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_nl_permute, dim3(nb), dim3(256), 0, st, a);
     const dim3 ntiles(a.nslot / 64);
-    if (a.has_excl) hipLaunchKernelGGL((k_nl_tbuild<true>), ntiles, dim3(512), 0, st, a);
-    else hipLaunchKernelGGL((k_nl_tbuild<false>), ntiles, dim3(512), 0, st, a);
+    if (a.build_nw == 8) {
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_tbuild<true, 8>), ntiles, dim3(512), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_tbuild<false, 8>), ntiles, dim3(512), 0, st, a);
+    } else {
+        if (a.has_excl) hipLaunchKernelGGL((k_nl_tbuild<true, 4>), ntiles, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_nl_tbuild<false, 4>), ntiles, dim3(256), 0, st, a);
+    }
     if (commit) hipLaunchKernelGGL(k_nl_commit, dim3((a.nrep + 3) / 4), dim3(256), 0, st, a);
     return hipGetLastError();
 }

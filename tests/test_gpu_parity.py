@@ -57,13 +57,13 @@ def oracle_forces(orc, s):
     return orc.forces(s)
 
 
-@pytest.mark.parametrize("lanes,cell_div", [(4, 1), (8, 2), (16, 3)])
+@pytest.mark.parametrize("lanes,cell_div,build", [(4, 1, 4), (8, 2, 8), (16, 3, 4)])
 @pytest.mark.parametrize("name", list(CASES))
-def test_neighbor_sets_bit_exact(E, orc, name, lanes, cell_div):
+def test_neighbor_sets_bit_exact(E, orc, name, lanes, cell_div, build):
     s = CASES[name]
-    say("neighbors", name, s.n, "lanes", lanes, "cell_div", cell_div)
+    say("neighbors", name, s.n, "lanes", lanes, "cell_div", cell_div, "build wavefronts", build)
     e = E.Engine.from_system(s)
-    e.set_launch(lanes=lanes, cell_div=cell_div)
+    e.set_launch(lanes=lanes, cell_div=cell_div, build_lanes=build)
     assert all(v == 0 for v in e.check_lists().values()), e.check_lists()
     pos, _, _ = e.download()
     rc2, rl2, sk = e.thresholds()
@@ -76,6 +76,30 @@ def test_neighbor_sets_bit_exact(E, orc, name, lanes, cell_div):
         got = np.sort(nbrs[i, :counts[i]])
         assert np.array_equal(got, on[i, :oc[i]]), f"atom {i}"
     e.close()
+
+
+@pytest.mark.parametrize("name", ["argon864", "nacl4096_wolf", "water1536"])
+def test_build_wavefronts_same_lists(E, name):
+    """The tile build with 4 or 8 wavefronts per tile writes the same sets and rows
+    (candidate order, not scheduling, fixes them): identical neighbor arrays, and
+    identical 200-step trajectories through in-run rebuilds."""
+    s = CASES[name]
+    say("build wavefronts", name, s.n)
+    out = []
+    for nw in (4, 8):
+        e = E.Engine.from_system(s)
+        e.set_launch(build_lanes=nw, persist=0)
+        counts, nbrs = e.neighbors()
+        st = e.run(200, energy_every=10)
+        pos, vel, _ = e.download()
+        out.append((counts, nbrs, st, pos, vel, e.info()["rebuilds"]))
+        e.close()
+    a, b = out
+    assert a[5] == b[5] and a[5] > 1
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for nm in a[2].dtype.names:
+        assert np.array_equal(a[2][nm], b[2][nm]), nm
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
 
 
 @pytest.mark.parametrize("name", list(CASES))
