@@ -90,6 +90,56 @@ __device__ __forceinline__ void walk_row_from(const float4* s_j, const uint4* __
     }
 }
 
+// walk_row_from for a workgroup whose lane groups hold a tile's atoms by
+// decreasing row length (the step kernel's rperm order): lane 0 of each
+// wavefront holds the wavefront's longest row, so in that row's last block no
+// lane of the wavefront has more entries than lane 0 has there.  The wavefront
+// computes only that many of the block's 8 entries per lane (rounded up to a
+// pair); the ones it leaves out are the dummy for every lane, whose pair is
+// beyond the cutoff, so the sums are the same bits as walk_row_from's.
+template <int G, class F>
+__device__ __forceinline__ void walk_row_sorted(const float4* s_j, const uint4* __restrict__ row4, int nn, int sub,
+                                                uint4 e0, uint4 e1, F&& f) {
+    const int nb = (int)(((uint32_t)nn + 8 * G - 1) / (8 * G));
+    const int nn0 = __builtin_amdgcn_readfirstlane(nn);
+    const int nb0 = (nn0 + 8 * G - 1) / (8 * G);
+    const int last0 = (nn0 - 8 * G * (nb0 - 1) + G - 1) / G;   // lane 0's entries in its last block
+    for (int b = 0; b < nb; b++) {
+        const uint4 e2 = row4[min(b + 2, nb - 1) * G + sub];
+        const uint32_t w[4] = {e0.x, e0.y, e0.z, e0.w};
+        auto at = [&](int u) {   // entries are byte offsets of the staged neighbors
+            return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_j) +
+                                                    ((u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xFFFFu)));
+        };
+        const int cnt = b < nb0 - 1 ? 8 : last0;   // wavefront-uniform
+        if (cnt > 6) {
+            float4 pj[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) pj[u] = at(u);
+#pragma unroll
+            for (int u = 0; u < 8; u++) f(pj[u]);
+        } else if (cnt > 4) {
+            float4 pj[6];
+#pragma unroll
+            for (int u = 0; u < 6; u++) pj[u] = at(u);
+#pragma unroll
+            for (int u = 0; u < 6; u++) f(pj[u]);
+        } else if (cnt > 2) {
+            float4 pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) pj[u] = at(u);
+#pragma unroll
+            for (int u = 0; u < 4; u++) f(pj[u]);
+        } else {
+            const float4 p0 = at(0), p1 = at(1);
+            f(p0);
+            f(p1);
+        }
+        e0 = e1;
+        e1 = e2;
+    }
+}
+
 template <int G, class F>
 __device__ __forceinline__ void walk_row(const float4* s_j, const uint4* __restrict__ row4, int nn, int sub, F&& f) {
     const int nb = (int)(((uint32_t)nn + 8 * G - 1) / (8 * G));

@@ -478,10 +478,11 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
             evi += (double)(ff * rsq);
         }
     };
+    // lane groups hold the tile's atoms by decreasing row length (rperm)
     if (ENMODE != 0 && en)
-        walk_row_from<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::true_type{}); });
+        walk_row_sorted<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::true_type{}); });
     else
-        walk_row_from<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::false_type{}); });
+        walk_row_sorted<G>(s_j, row4, nn, sub, q0, q1, [&](const float4& pj) { pair(pj, std::false_type{}); });
 
     // bonded terms (lanes of the atom's group share them)
     double ebo = 0.0, ean = 0.0, bvi = 0.0;
