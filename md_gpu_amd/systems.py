@@ -88,6 +88,9 @@ class System:
     # at the distances rigid_d = (d_OH, d_HH) by SETTLE after every step
     rigid: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
     rigid_d: tuple = (1.0, 1.6330)
+    # smooth particle-mesh Ewald for the reciprocal part of the damped (lkspace 2)
+    # Coulomb term: (K0, K1, K2, order), empty = none
+    pme: tuple = ()
 
     @property
     def n(self) -> int:
@@ -186,6 +189,32 @@ def argon(ncell: int = 6, seed: int = 1, rho_star: float = 0.8442, t_star: float
     return System("argon", box, wrap_centered(pos, L).astype(np.float32), vel.astype(np.float32), typ,
                   np.zeros(n, np.float32), ff, integrator="vverlet", thermostat="none", dt=2.0,
                   T0=t_star * eps_k)
+
+
+def fft_size(n: int) -> int:
+    """Smallest m >= n whose prime factors are 2, 3 and 5."""
+    m = max(int(n), 1)
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 1
+
+
+def with_pme(s: System, spacing: float = 1.0, order: int = 4, alpha: Optional[float] = None) -> System:
+    """s with Ewald electrostatics: the damped real-space term (lkspace 2,
+    src/forces.cu:97-110) with alpha = 3.09 / rc by default (the reference's
+    estimate, src/simulation.cu:93) plus the smooth-PME reciprocal part on a grid
+    of at most `spacing` A per point and B-splines of `order`."""
+    t = s.copy()
+    t.ff = dataclasses.replace(s.ff, coulomb=COUL_DAMPED, alpha=float(3.09 / s.ff.rc if alpha is None else alpha))
+    K = [fft_size(int(math.ceil(float(L) / spacing))) for L in s.box]
+    t.pme = (K[0], K[1], K[2], int(order))
+    t.name = s.name + "_pme"
+    return t
 
 
 def nacl_ff(alpha=0.2, rc=12.0, skin=1.0) -> ForceField:

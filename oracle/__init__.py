@@ -30,7 +30,11 @@ class OrcMD(C.Structure):
     _fields_ = [("integrator", C.c_int), ("thermostat", C.c_int), ("dt", C.c_double), ("T0", C.c_double),
                 ("tau", C.c_double), ("skin", C.c_double), ("barostat", C.c_int), ("P0", C.c_double),
                 ("tau_p", C.c_double), ("compress", C.c_double), ("nrigid", C.c_int), ("rigid", C.c_void_p),
-                ("d_oh", C.c_double), ("d_hh", C.c_double)]
+                ("d_oh", C.c_double), ("d_hh", C.c_double), ("pme_order", C.c_int), ("pme_K", C.c_int * 3)]
+
+
+class OrcPME(C.Structure):
+    _fields_ = [("K", C.c_int * 3), ("order", C.c_int), ("beta", C.c_double)]
 
 
 def build():
@@ -67,6 +71,16 @@ def lib():
         L.orc_shake.restype = C.c_int
         L.orc_fm_loss_grad.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
+        L.orc_fft3.argtypes = [P, P, C.c_int]
+        L.orc_fft3.restype = None
+        L.orc_bspline.argtypes = [C.c_int, C.c_double, P, P]
+        L.orc_bspline.restype = None
+        L.orc_pme_recip.argtypes = [P, C.c_int, P, P, P, C.c_double, P, P]
+        L.orc_ewald_recip.argtypes = [C.c_int, P, P, P, C.c_double, C.c_double, C.c_int, P, P]
+        L.orc_pme_excl.argtypes = [C.c_int, P, P, P, P, C.c_double, C.c_double, P, P]
+        L.orc_ewald_self.argtypes = [C.c_int, P, P, C.c_double, C.c_double, P]
+        for f in (L.orc_pme_recip, L.orc_ewald_recip, L.orc_pme_excl, L.orc_ewald_self):
+            f.restype = C.c_double
         _lib = L
     return _lib
 
@@ -169,12 +183,100 @@ def bonded_forces(sys_, pos=None):
 
 def forces(sys_, erfc_exact=0, cap=None):
     """Total FP64 force with canonical FP32 list/cutoff predicates.
-    Returns F (n,3), energies [lj, coul, bond, angle], virial (pair + bond)."""
+    Returns F (n,3), energies [lj, coul, bond, angle], virial (pair + bond).
+    With sys_.pme the Coulomb energy, forces and virial include the Ewald
+    reciprocal part, the excluded-pair correction and the self term."""
     rc2, rl2, _, _ = thresholds(sys_.ff.rc, sys_.ff.skin, sys_.box)
     counts, nb = nlist(sys_.pos, sys_.box, rl2, sys_.exclusions(), cap)
     Fp, ep, wp = pair_forces(sys_, counts, nb, rc2, erfc_exact)
     Fb, eb, wb = bonded_forces(sys_)
+    if getattr(sys_, "pme", ()):
+        F, e, w = ewald_long_range(sys_, sys_.pos)
+        Fp = Fp + F
+        ep = ep.copy()
+        ep[1] += e
+        wp = wp + w
     return Fp + Fb, np.concatenate([ep, eb]), wp + wb
+
+
+def _pme_par(K, order, beta):
+    o = OrcPME()
+    o.K[:] = [int(k) for k in K]
+    o.order = int(order)
+    o.beta = float(beta)
+    return o
+
+
+def pme_recip(x, q, box, K, order, beta, qfac):
+    """SPME reciprocal (energy, forces (n, 3), virial trace) in FP64."""
+    x = np.ascontiguousarray(x, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    F = np.zeros_like(x)
+    W = np.zeros(1)
+    E = lib().orc_pme_recip(C.byref(_pme_par(K, order, beta)), x.shape[0], _p(x), _p(q), _p(b), float(qfac),
+                            _p(F), _p(W))
+    return E, F, float(W[0])
+
+
+def ewald_recip(x, q, box, beta, qfac, kmax):
+    """Direct Ewald reciprocal sum over |m_d| <= kmax: (energy, forces, virial trace)."""
+    x = np.ascontiguousarray(x, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    F = np.zeros_like(x)
+    W = np.zeros(1)
+    E = lib().orc_ewald_recip(x.shape[0], _p(x), _p(q), _p(b), float(beta), float(qfac), int(kmax), _p(F), _p(W))
+    return E, F, float(W[0])
+
+
+def pme_excl(x, q, box, excl, beta, qfac):
+    """Excluded-pair correction -qq erf(beta r)/r: (energy, forces, virial)."""
+    x = np.ascontiguousarray(x, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    ex = np.ascontiguousarray(excl, np.int32)
+    F = np.zeros_like(x)
+    W = np.zeros(1)
+    E = lib().orc_pme_excl(x.shape[0], _p(x), _p(q), _p(b), _p(ex), float(beta), float(qfac), _p(F), _p(W))
+    return E, F, float(W[0])
+
+
+def ewald_self(q, box, beta, qfac):
+    """Self plus neutralizing-background term: (energy, virial)."""
+    q = np.ascontiguousarray(q, np.float64)
+    b = np.ascontiguousarray(box, np.float32)
+    W = np.zeros(1)
+    E = lib().orc_ewald_self(q.shape[0], _p(q), _p(b), float(beta), float(qfac), _p(W))
+    return E, float(W[0])
+
+
+def ewald_long_range(sys_, x):
+    """Everything the damped real-space sum leaves to the rest of Ewald for a system
+    with sys_.pme = (K0, K1, K2, order): reciprocal + excluded pairs + self."""
+    K0, K1, K2, order = sys_.pme
+    beta, qf = sys_.ff.alpha, sys_.ff.qfac
+    q = np.asarray(sys_.charge, np.float64)
+    e1, F1, w1 = pme_recip(x, q, sys_.box, (K0, K1, K2), order, beta, qf)
+    e2, F2, w2 = pme_excl(x, q, sys_.box, sys_.exclusions(), beta, qf)
+    e3, w3 = ewald_self(q, sys_.box, beta, qf)
+    return F1 + F2, e1 + e2 + e3, w1 + w2 + w3
+
+
+def fft3(a, sign=-1):
+    """Unnormalized 3D DFT of a complex array (the oracle's own FFT)."""
+    z = np.ascontiguousarray(a, np.complex128).copy()
+    K = np.ascontiguousarray(z.shape, np.int32)
+    lib().orc_fft3(_p(z), _p(K), int(sign))
+    return z
+
+
+def bspline(order, w):
+    """(M_p(w + j), M_p'(w + j)) for j = 0..p-1."""
+    th = np.zeros(order)
+    dth = np.zeros(order)
+    lib().orc_bspline(int(order), float(w), _p(th), _p(dth))
+    return th, dth
 
 
 def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
@@ -200,6 +302,10 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     md.nrigid = int(rig.shape[0])
     md.rigid = rig.ctypes.data if rig.shape[0] else None
     md.d_oh, md.d_hh = (float(x) for x in getattr(sys_, "rigid_d", (1.0, 1.0)))
+    pme = getattr(sys_, "pme", ())
+    if pme:
+        md.pme_K[:] = [int(k) for k in pme[:3]]
+        md.pme_order = int(pme[3])
     b = np.ascontiguousarray(sys_.bonds, np.int32)
     a = np.ascontiguousarray(sys_.angles, np.int32)
     ex = np.ascontiguousarray(sys_.exclusions(), np.int32)
@@ -210,8 +316,10 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     nreb = lib().orc_md_run(n, _p(p), _p(v), _p(typ), _p(q), _p(m), _p(box), C.byref(o), C.byref(md),
                             b.shape[0], _p(b), a.shape[0], _p(a), _p(ex), nsteps, _p(ke), _p(pe), _p(T), _p(H),
                             _p(ts), _p(vol), _p(box_end))
-    if nreb < 0:
+    if nreb == -1:
         raise ValueError("the oracle runs rigid molecules with leapfrog only")
+    if nreb == -2:
+        raise ValueError("PME needs the damped Coulomb term and a fixed box")
     return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb, volume=vol, box=box_end)
 
 

@@ -559,6 +559,7 @@ struct MDSys {
     const int* angles;
     const int* excl;
     double rl;
+    orc_pme pme{};   // order 0: no reciprocal part
     std::vector<int> counts, nbrs;
     int cap = 0;
     std::vector<double> pos_build;
@@ -682,6 +683,15 @@ double md_forces(MDSys& s, const double* pos, double* F, double* vir = nullptr) 
         }
         epot += 0.5 * ff->ak2[t] * (th - ff->at0[t]) * (th - ff->at0[t]);
     }
+    // Ewald reciprocal part, excluded-pair correction and self term (md_pme.cpp)
+    if (s.pme.order > 0) {
+        const float b3[3] = {(float)s.L[0], (float)s.L[1], (float)s.L[2]};
+        double w1 = 0.0, w2 = 0.0, w3 = 0.0;
+        epot += orc_pme_recip(&s.pme, n, pos, s.charge, b3, ff->qfac, F, &w1);
+        if (s.excl) epot += orc_pme_excl(n, pos, s.charge, b3, s.excl, s.pme.beta, ff->qfac, F, &w2);
+        epot += orc_ewald_self(n, s.charge, b3, s.pme.beta, ff->qfac, &w3);
+        w += w1 + w2 + w3;
+    }
     if (vir) *vir = w;
     return epot;
 }
@@ -716,6 +726,12 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
     // src/integrator.cu:155, less the three constraints of each rigid molecule
     const double Nf = 3.0 * n - 3.0 - 3.0 * (md->nrigid > 0 ? md->nrigid : 0);
     if (md->nrigid > 0 && md->integrator != ORC_INT_LEAPFROG) return -1;   // SETTLE with leapfrog only
+    if (md->pme_order > 0) {   // damped Coulomb, fixed box
+        if (ff->coulomb != ORC_COUL_DAMPED || md->barostat != ORC_BARO_NONE) return -2;
+        s.pme.order = md->pme_order;
+        for (int d = 0; d < 3; d++) s.pme.K[d] = md->pme_K[d];
+        s.pme.beta = ff->alpha;
+    }
     std::vector<double> x_old(md->nrigid > 0 ? 3 * (size_t)n : 0);
     std::vector<double> F(3 * (size_t)n);
     std::vector<double> im(n);

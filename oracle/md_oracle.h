@@ -120,6 +120,8 @@ typedef struct {
     int    nrigid;          /* rigid 3-site molecules (leapfrog only): SETTLE after each position update */
     const int* rigid;       /* nrigid x 3 atom indices (O, H1, H2) */
     double d_oh, d_hh;      /* their O-H and H-H distances (A) */
+    int    pme_order;       /* > 0 (damped Coulomb only): add the smooth-PME reciprocal part, the */
+    int    pme_K[3];        /* excluded-pair correction and the self term at beta = ff->alpha */
 } orc_md;
 
 /* Stats per step (length nsteps): ke, pe (lj+coul+bond+angle), temperature,
@@ -152,6 +154,39 @@ void orc_settle(int nmol, const int* mol, const double* mass, const float* box3,
  * -1 if one did not converge within maxit. */
 int orc_shake(int nmol, const int* mol, const double* mass, const float* box3, double d_oh, double d_hh,
               const double* x0, double* x1, double tol, int maxit);
+
+/* ---- Ewald reciprocal space (md_pme.cpp) ----
+ * Smooth particle-mesh Ewald (Essmann et al., J. Chem. Phys. 103, 8577 (1995)),
+ * restated from the paper: the reference vendors GROMACS' PME (src/pme.c) without
+ * building it, and its lkspace 2 path has only the damped real-space term
+ * (src/forces.cu:97-110).  Positions in the box centred at 0; u = K (x / L + 1/2). */
+#define ORC_PME_MAX_ORDER 12
+typedef struct {
+    int    K[3];            /* grid points per box edge */
+    int    order;           /* B-spline order p (2..12) */
+    double beta;            /* Ewald splitting parameter (1/A), the damped term's alpha */
+} orc_pme;
+/* Unnormalized 3D DFT in place of a K0 x K1 x K2 complex array (interleaved re, im;
+ * last index fastest); sign -1 forward, +1 backward. */
+void orc_fft3(double* data, const int* K, int sign);
+/* th[j] = M_p(w + j), dth[j] = M_p'(w + j), j = 0..p-1 (cardinal B-spline of order p). */
+void orc_bspline(int order, double w, double* th, double* dth);
+/* SPME reciprocal energy (kJ/mol); adds the forces to F (n x 3, may be NULL);
+ * *W = trace of the reciprocal virial (may be NULL). */
+double orc_pme_recip(const orc_pme* pm, int n, const double* x, const double* q, const float* box3,
+                     double qfac, double* F, double* W);
+/* The Ewald reciprocal sum over all m = (a/L0, b/L1, c/L2), |a|,|b|,|c| <= kmax, m != 0,
+ * by structure factors: the limit SPME converges to.  Same outputs. */
+double orc_ewald_recip(int n, const double* x, const double* q, const float* box3, double beta, double qfac,
+                       int kmax, double* F, double* W);
+/* Excluded pairs (excl: n x 4 partner indices, -1 empty, each pair listed from both
+ * atoms): -qfac qi qj erf(beta r)/r per pair at the minimum image, removing what the
+ * reciprocal sum adds for pairs the real-space sum leaves out.  Same outputs. */
+double orc_pme_excl(int n, const double* x, const double* q, const float* box3, const int* excl, double beta,
+                    double qfac, double* F, double* W);
+/* Self term -qfac beta/sqrt(pi) sum q^2 plus the neutralizing-background term
+ * -qfac pi (sum q)^2 / (2 V beta^2); *W = its virial (3 x the background term). */
+double orc_ewald_self(int n, const double* q, const float* box3, double beta, double qfac, double* W);
 
 /* ---- force matching (config: NaCl, 7 parameters) ----
  * loss = mean over frames*atoms*3 of (F(theta) - Ftarget)^2 with charges
