@@ -135,6 +135,21 @@ int md_set_topology(md_engine* e, int nb, const int* bonds, int na, const int* a
  * md_set_topology (which keeps them). */
 int md_set_rigid(md_engine* e, int nmol, const int* mol, float d_oh, float d_hh);
 
+/* Smooth particle-mesh Ewald (Essmann et al., J. Chem. Phys. 103, 8577 (1995))
+ * for the reciprocal-space part of the damped Coulomb term (MD_COUL_DAMPED, the
+ * reference's lkspace 2, src/forces.cu:97-110, src/simulation.cu:90-94), at the
+ * splitting parameter beta = the force field's alpha.  The reference reads kmax
+ * for it and computes no reciprocal part; its GROMACS PME (src/pme.c) is vendored,
+ * not built.  With PME on, each step adds the reciprocal forces (order-p
+ * B-splines on a grid[0] x grid[1] x grid[2] mesh, FFTs by hipFFT), removes
+ * -qfac qi qj erf(beta r)/r for every excluded pair, and adds the self and
+ * neutralizing-background terms to the Coulomb energy and the virial.
+ * order: 4, 6 or 8 (0 turns PME off); grid: points per box edge, >= order, at
+ * most 1024 (sizes with factors 2, 3, 5 transform fastest).  Runs need one
+ * replica, charged atoms, MD_COUL_DAMPED and no pressure coupling (checked at
+ * the next md_run / md_compute_forces).  Force matching has no PME term. */
+int md_set_pme(md_engine* e, const int* grid, int order);
+
 /* pos (N x 3, required), vel (N x 3 or NULL = zero).  Builds the neighbor lists. */
 int md_upload_state(md_engine* e, const float* pos, const float* vel);
 /* Any pointer may be NULL.  Arrays are N x 3 in the uploaded atom order. */

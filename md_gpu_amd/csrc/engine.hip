@@ -25,6 +25,8 @@
 #include <string>
 #include <vector>
 
+#include <hipfft/hipfft.h>
+
 #include "md_device.h"
 #include "mdgpu/md_api.h"
 
@@ -50,6 +52,7 @@ extern "C" hipError_t md_launch_persist(const PersistArgs& a, int lanes, size_t 
                                         hipEvent_t ev1);
 extern "C" hipError_t md_launch_persist_clear(unsigned* flags, hipStream_t st);
 extern "C" hipError_t md_launch_settle(const SettleArgs& a, hipStream_t st);
+extern "C" int md_launch_pme(const PmeArgs& a, hipfftHandle r2c, hipfftHandle c2r, hipStream_t st);
 extern "C" hipError_t md_launch_report(const RepState* st, int nrep, const int* tmax, int ntiles, const int* err,
                                        RunReport* out, hipStream_t stream);
 
@@ -273,6 +276,23 @@ struct md_engine {
     static constexpr double kQueueRatio0 = 1.15;
     double queue_ratio = kQueueRatio0;   // launched / completed steps, learned per launch configuration
     int fm_nframes = 0;
+    // smooth PME (pme.hip): requested grid and order (order 0: none); the buffers,
+    // tables and FFT plans are made for the current slots and box by pme_prepare
+    int pme_order = 0, pme_K[3] = {0, 0, 0};
+    bool pme_on = false;           // prepared: the step chain includes the PME kernels
+    bool vv_force_stale = false;   // the force array (velocity Verlet's first half kick) predates a PME change
+    int pme_S = 0;                 // slot count and box the buffers and tables were made for
+    float pme_L[3] = {0, 0, 0};
+    double pme_e0 = 0, pme_w0 = 0; // self + background energy and virial
+    int pme_nconv = 0, pme_ngather = 0;
+    DBuf<unsigned long long> pme_qacc;
+    DBuf<float> pme_qgrid;
+    DBuf<float2> pme_sgrid, pme_tab;
+    DBuf<float4> pme_frec;
+    DBuf<double> pme_part, pme_out;
+    DBuf<unsigned> pme_counter;
+    DBuf<char> pme_work;
+    hipfftHandle pme_r2c = nullptr, pme_c2r = nullptr;
     // segment graphs by key (check interval, energy interval, blocks); two
     // executable copies each, launched alternately, so the next replay can be
     // submitted while the previous one still runs
@@ -286,6 +306,7 @@ struct md_engine {
 
     ~md_engine() {
         drop_graphs();
+        drop_pme_plans();
         for (auto e : ev_pool) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -295,6 +316,11 @@ struct md_engine {
         graphs.clear();
     }
     int ntiles() const { return S / MD_TILE; }
+    void drop_pme_plans() {
+        if (pme_r2c) (void)hipfftDestroy(pme_r2c);
+        if (pme_c2r) (void)hipfftDestroy(pme_c2r);
+        pme_r2c = pme_c2r = nullptr;
+    }
 };
 
 namespace {
@@ -566,6 +592,8 @@ StepArgs step_args(md_engine* e, int mode, int energy_every) {
     a.baro_k = (float)((double)e->in.compress * e->in.dt / (double)(e->in.tau_p > 0 ? e->in.tau_p : 1.0f));
     a.rl = std::sqrt(e->rl2);
     a.rc = e->ff.rc;
+    a.frec = e->pme_on ? e->pme_frec.p : nullptr;
+    a.pme_e = e->pme_on ? e->pme_out.p : nullptr;
     for (int k = 0; k < 8; k++) {
         a.bk2[k] = e->ff.bond_k2[k]; a.br0[k] = e->ff.bond_r0[k];
         a.ak2[k] = e->ff.angle_k2[k]; a.at0[k] = e->ff.angle_t0[k];
@@ -624,7 +652,129 @@ SettleArgs settle_args(md_engine* e, int par) {
     return g;
 }
 
+// M_p(w + j), j = 0..p-1 (the recursion of pme.hip, in FP64)
+void bspline_host(int p, double w, double* th) {
+    double a[MD_PME_MAX_ORDER + 1] = {0};
+    a[0] = w;
+    a[1] = 1.0 - w;
+    for (int k = 2; k < p; k++) {
+        double b[MD_PME_MAX_ORDER + 1];
+        for (int j = 0; j <= k; j++) b[j] = ((w + j) * (j < k ? a[j] : 0.0) + (k + 1 - w - j) * (j > 0 ? a[j - 1] : 0.0)) / k;
+        for (int j = 0; j <= k; j++) a[j] = b[j];
+    }
+    for (int j = 0; j < p; j++) th[j] = a[j];
+}
+
+// Buffers, tables and FFT plans of the requested PME for the current slots and
+// box (once; again after md_set_system or md_set_pme).
+int pme_prepare(md_engine* e) {
+    if (e->pme_order == 0) {
+        e->pme_on = false;
+        return MD_OK;
+    }
+    if (e->pme_on) return MD_OK;
+    if (e->R != 1) return fail(MD_E_INVALID, "PME runs one replica per engine (this one has %d)", e->R);
+    if (e->coul_kernel != MD_COUL_DAMPED)
+        return fail(MD_E_INVALID, "PME needs the damped Coulomb term (lkspace 2) and charged atoms");
+    if (e->in.barostat != MD_BARO_NONE) return fail(MD_E_INVALID, "PME runs with a fixed box (no pressure coupling)");
+    if (!(e->ff.alpha > 0.0f)) return fail(MD_E_INVALID, "PME needs a positive splitting parameter alpha");
+    const int p = e->pme_order;
+    const int* K = e->pme_K;
+    for (int d = 0; d < 3; d++) {
+        if (K[d] < p) return fail(MD_E_INVALID, "PME grid edge %d has %d points, fewer than the order %d", d, K[d], p);
+        e->pme_L[d] = e->box[d];
+    }
+    e->drop_graphs();   // captured segments hold the previous buffers
+    e->drop_pme_plans();
+    const size_t n = (size_t)K[0] * K[1] * K[2], nh = (size_t)K[0] * K[1] * (K[2] / 2 + 1);
+    e->pme_nconv = (int)std::min<size_t>((nh + 255) / 256, 2048);
+    e->pme_ngather = (e->S + 255) / 256;
+    CHECK(e->pme_qacc.alloc(n));
+    CHECK(e->pme_qgrid.alloc(n));
+    CHECK(e->pme_sgrid.alloc(nh));
+    CHECK(e->pme_tab.alloc((size_t)K[0] + K[1] + K[2]));
+    CHECK(e->pme_frec.alloc(e->S));
+    CHECK(e->pme_part.alloc(2 * (size_t)(e->pme_nconv + e->pme_ngather)));
+    CHECK(e->pme_out.alloc(2));
+    CHECK(e->pme_counter.alloc(1));
+    // per axis: the signed reciprocal component m / L and |b(m)|^2 (Essmann et al. eq. 4.4)
+    const double pi = 3.14159265358979323846;
+    double mint[MD_PME_MAX_ORDER];
+    bspline_host(p, 0.0, mint);   // M_p(j)
+    std::vector<float2> tab;
+    for (int d = 0; d < 3; d++)
+        for (int m = 0; m < K[d]; m++) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k <= p - 2; k++) {
+                re += mint[k + 1] * std::cos(2.0 * pi * m * k / K[d]);
+                im += mint[k + 1] * std::sin(2.0 * pi * m * k / K[d]);
+            }
+            const int ms = m <= K[d] / 2 ? m : m - K[d];
+            tab.push_back(make_float2((float)(ms / (double)e->pme_L[d]), (float)(1.0 / (re * re + im * im))));
+        }
+    HIPCHECK(hipMemcpy(e->pme_tab.p, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice));
+    // self and neutralizing-background terms
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < e->N; i++) {
+        s1 += e->charge_h[i];
+        s2 += (double)e->charge_h[i] * e->charge_h[i];
+    }
+    const double beta = e->ff.alpha, V = (double)e->pme_L[0] * e->pme_L[1] * e->pme_L[2];
+    const double ebg = -(double)e->ff.qfac * pi * s1 * s1 / (2.0 * V * beta * beta);
+    e->pme_e0 = -(double)e->ff.qfac * beta / std::sqrt(pi) * s2 + ebg;
+    e->pme_w0 = 3.0 * ebg;
+    // FFT plans on the engine's stream with one work area allocated here, so a
+    // captured segment makes no allocation
+    size_t ws1 = 0, ws2 = 0;
+    auto fft_ok = [](hipfftResult r) { return r == HIPFFT_SUCCESS; };
+    if (!fft_ok(hipfftCreate(&e->pme_r2c)) || !fft_ok(hipfftCreate(&e->pme_c2r)))
+        return fail(MD_E_HIP, "hipfftCreate failed");
+    if (!fft_ok(hipfftSetAutoAllocation(e->pme_r2c, 0)) || !fft_ok(hipfftSetAutoAllocation(e->pme_c2r, 0)))
+        return fail(MD_E_HIP, "hipfftSetAutoAllocation failed");
+    if (!fft_ok(hipfftMakePlan3d(e->pme_r2c, K[0], K[1], K[2], HIPFFT_R2C, &ws1)) ||
+        !fft_ok(hipfftMakePlan3d(e->pme_c2r, K[0], K[1], K[2], HIPFFT_C2R, &ws2)))
+        return fail(MD_E_HIP, "hipFFT plan for a %d x %d x %d grid failed", K[0], K[1], K[2]);
+    CHECK(e->pme_work.alloc(std::max<size_t>(std::max(ws1, ws2), 16)));
+    if (!fft_ok(hipfftSetWorkArea(e->pme_r2c, e->pme_work.p)) || !fft_ok(hipfftSetWorkArea(e->pme_c2r, e->pme_work.p)) ||
+        !fft_ok(hipfftSetStream(e->pme_r2c, e->st)) || !fft_ok(hipfftSetStream(e->pme_c2r, e->st)))
+        return fail(MD_E_HIP, "hipFFT work area or stream set-up failed");
+    e->pme_S = e->S;
+    e->pme_on = true;
+    return MD_OK;
+}
+
+PmeArgs pme_args(md_engine* e, int mode) {
+    PmeArgs g{};
+    g.pos[0] = e->pos[0].p; g.pos[1] = e->pos[1].p; g.ids = e->ids.p; g.slot_of = e->slot_of.p; g.excl = e->excl.p;
+    g.st = e->rst.p; g.qacc = e->pme_qacc.p; g.qgrid = e->pme_qgrid.p; g.sgrid = e->pme_sgrid.p; g.tab = e->pme_tab.p;
+    g.frec = e->pme_frec.p; g.part = e->pme_part.p; g.counter = e->pme_counter.p; g.out = e->pme_out.p;
+    g.K0 = e->pme_K[0]; g.K1 = e->pme_K[1]; g.K2 = e->pme_K[2]; g.order = e->pme_order;
+    g.nslot = e->S; g.mode = mode; g.has_excl = e->has_excl ? 1 : 0;
+    g.nconv = e->pme_nconv; g.ngather = e->pme_ngather;
+    const double L[3] = {e->pme_L[0], e->pme_L[1], e->pme_L[2]};
+    g.L0 = e->pme_L[0]; g.L1 = e->pme_L[1]; g.L2 = e->pme_L[2];
+    g.Li0 = (float)(1.0 / L[0]); g.Li1 = (float)(1.0 / L[1]); g.Li2 = (float)(1.0 / L[2]);
+    g.hl0 = 0.5 * L[0]; g.hl1 = 0.5 * L[1]; g.hl2 = 0.5 * L[2];
+    g.kl0 = e->pme_K[0] / L[0]; g.kl1 = e->pme_K[1] / L[1]; g.kl2 = e->pme_K[2] / L[2];
+    const double beta = e->ff.alpha, pi = 3.14159265358979323846;
+    g.beta = (float)beta;
+    g.qfac = e->ff.qfac;
+    g.pi2_b2 = (float)(pi * pi / (beta * beta));
+    g.cpref = (float)((double)e->ff.qfac / (pi * L[0] * L[1] * L[2]));
+    g.e0 = e->pme_e0; g.w0 = e->pme_w0;
+    return g;
+}
+
+int enqueue_pme(md_engine* e, int mode) {
+    const int rc = md_launch_pme(pme_args(e, mode), e->pme_r2c, e->pme_c2r, e->st);
+    if (rc == 2) return fail(MD_E_HIP, "hipFFT execution failed");
+    if (rc != 0) return fail(MD_E_HIP, "PME kernel launch failed");
+    return MD_OK;
+}
+
 int enqueue_stepkernel(md_engine* e, const StepArgs& a, TimedSpan* ts, bool finalize = true) {
+    // long-range part first: the step kernel adds its forces and energy
+    if (e->pme_on) CHECK(enqueue_pme(e, a.mode));
     std::pair<hipEvent_t, hipEvent_t> ev{};
     if (ts) ev = ts->take();   // the launch records the kernel's own start and end
     HIPCHECK(md_launch_step(a, e->lanes, e->coul_kernel, e->st, ev.first, ev.second));
@@ -909,6 +1059,7 @@ int md_set_system(md_engine* e, int nrep, const int* natom, const float* box, co
     if (nrep < 1) return fail(MD_E_INVALID, "nrep must be >= 1");
     HIPCHECK(hipSetDevice(e->dev));
     HIPCHECK(hipStreamSynchronize(e->st));
+    e->pme_on = false;   // PME buffers, tables and plans are made again for the new system
     long long N = 0, S = 0;
     for (int r = 0; r < nrep; r++) {
         if (natom[r] < 2) return fail(MD_E_INVALID, "replica %d has %d atoms", r, natom[r]);
@@ -1101,6 +1252,25 @@ int md_set_rigid(md_engine* e, int nmol, const int* mol, float d_oh, float d_hh)
     return MD_OK;
 }
 
+int md_set_pme(md_engine* e, const int* grid, int order) {
+    if (!e) return fail(MD_E_INVALID, "null engine");
+    if (order != 0) {
+        if (order != 4 && order != 6 && order != 8) return fail(MD_E_INVALID, "PME order must be 4, 6 or 8 (got %d)", order);
+        if (!grid) return fail(MD_E_INVALID, "null grid");
+        for (int d = 0; d < 3; d++)
+            if (grid[d] < order || grid[d] > 1024)
+                return fail(MD_E_INVALID, "PME grid edge %d must be in [order, 1024] (got %d)", d, grid[d]);
+    }
+    HIPCHECK(hipSetDevice(e->dev));
+    HIPCHECK(hipStreamSynchronize(e->st));
+    e->pme_order = order;
+    for (int d = 0; d < 3; d++) e->pme_K[d] = order ? grid[d] : 0;
+    e->pme_on = false;
+    e->vv_force_stale = e->has_state;
+    e->drop_graphs();
+    return MD_OK;
+}
+
 int md_upload_state(md_engine* e, const float* pos, const float* vel) {
     CHECK(check_ready(e, false));
     if (!pos) return fail(MD_E_INVALID, "null positions");
@@ -1164,6 +1334,7 @@ int md_upload_state(md_engine* e, const float* pos, const float* vel) {
         a.stats = nullptr;
         CHECK(enqueue_stepkernel(e, a, nullptr));
         HIPCHECK(hipStreamSynchronize(e->st));
+        e->vv_force_stale = e->pme_order > 0 && !e->pme_on;   // md_run adds the PME part first
     }
     return MD_OK;
 }
@@ -1209,10 +1380,12 @@ int md_download_state(md_engine* e, float* pos, float* vel, float* force) {
 int md_compute_forces(md_engine* e, float* force, double* energy, double* virial) {
     CHECK(check_ready(e, true));
     HIPCHECK(hipSetDevice(e->dev));
+    CHECK(pme_prepare(e));
     CHECK(refresh_lists(e));
     StepArgs a = step_args(e, MODE_FORCE, 0);
     a.force_energy = 1;
     CHECK(enqueue_stepkernel(e, a, nullptr));
+    e->vv_force_stale = false;   // the force array now holds F(x) with the PME part as set
     std::vector<StatRec> rec(e->R);
     HIPCHECK(hipMemcpyAsync(rec.data(), e->stats.p, e->R * sizeof(StatRec), hipMemcpyDeviceToHost, e->st));
     HIPCHECK(hipStreamSynchronize(e->st));
@@ -1237,7 +1410,7 @@ namespace {
 int enqueue_segment(md_engine* e, int energy_every, int blocks) {
     const int ce = std::max(1, e->check_every);
     for (int b = 0; b < blocks; b++) {
-        if (e->defer && lf_form(e) && !e->in.barostat && e->nrigid == 0) {
+        if (e->defer && lf_form(e) && !e->in.barostat && e->nrigid == 0 && !e->pme_on) {
             // the block's first step kernel opens the rebuilt lists (pressure coupling scales the
             // box in the finalize kernel after every step; SETTLE runs between a step kernel and
             // its finalize)
@@ -1412,6 +1585,17 @@ extern "C" {
 
 int md_run(md_engine* e, int nsteps, int energy_every, md_step_stats* stats) {
     CHECK(check_ready(e, true));
+    CHECK(pme_prepare(e));
+    if (e->vv_force_stale && e->in.integrator == MD_VVERLET) {
+        // velocity Verlet's first half kick reads F(x0): with the PME part as now set
+        HIPCHECK(hipSetDevice(e->dev));
+        CHECK(refresh_lists(e));
+        StepArgs a = step_args(e, MODE_FORCE, 0);
+        a.stats = nullptr;
+        CHECK(enqueue_stepkernel(e, a, nullptr));
+        HIPCHECK(hipStreamSynchronize(e->st));
+    }
+    e->vv_force_stale = false;
     if (nsteps < 0) return fail(MD_E_INVALID, "nsteps < 0");
     if (nsteps == 0) return MD_OK;
     if (energy_every < 0) return fail(MD_E_INVALID, "energy_every < 0");
@@ -1718,6 +1902,7 @@ int md_timing_get(md_engine* e, double* step_ms, long long* step_n, double* reb_
 
 int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     CHECK(check_ready(e, true));
+    CHECK(pme_prepare(e));
     if (nsteps < 1) return fail(MD_E_INVALID, "nsteps must be >= 1");
     HIPCHECK(hipSetDevice(e->dev));
     const bool saved_timing = e->timing;
@@ -2022,6 +2207,7 @@ static int fm_evaluate(md_engine* e, const float* lj1, const float* lj2, float q
 
 static int fm_check(md_engine* e, const float* lj1, const float* lj2) {
     CHECK(check_ready(e, true));
+    if (e->pme_order) return fail(MD_E_INVALID, "force matching has no PME term: set the order to 0");
     if (!e->fm_nframes) return fail(MD_E_STATE, "call md_fm_set_frames first");
     if (!lj1 || !lj2) return fail(MD_E_INVALID, "null LJ tables");
     if (e->ff.ntype > 4) return fail(MD_E_CAPACITY, "force matching supports at most 4 atom types");

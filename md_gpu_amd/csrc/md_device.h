@@ -226,6 +226,48 @@ struct StepArgs {
     float p0;             // bar
     float baro_k;         // compress dt / tau_p, 1/bar
     float rl, rc;         // list radius and cutoff (the skin check under a scaled box)
+    // smooth PME (damped Coulomb kernel only): the reciprocal and excluded-pair force
+    // of each slot, added to the pair force, and the step's energy and virial of the
+    // long-range part, added to the replica's first tile; null without PME
+    const float4* frec;
+    const double* pme_e;
+};
+
+// Smooth particle-mesh Ewald for one replica (pme.hip), a chain of kernels and
+// two FFTs before each step kernel: spread the charges onto the grid (fixed
+// point, so the sum does not depend on the order of the atomic adds), convert,
+// transform, multiply by the influence function (energy and virial on the
+// way), transform back, gather the forces.
+#define MD_PME_MAX_ORDER 8
+#define MD_PME_QUNIT 4294967296.0   // fixed-point units per charge unit in the spread grid
+struct PmeArgs {
+    const float4* pos[2];       // current buffer pos[cur_buf(st[0])]
+    const int* ids;             // [S] original index or -1
+    const int* slot_of;         // [N]
+    const int4* excl;           // [N] excluded partners (-1 empty)
+    const RepState* st;         // the replica (one)
+    unsigned long long* qacc;   // [K0 K1 K2] charge grid in fixed point, zero between steps
+    float* qgrid;               // [K0 K1 K2] charges (transform input); dE/dQ (inverse output)
+    float2* sgrid;              // [K0 K1 (K2/2 + 1)] the transform
+    const float2* tab;          // [K0 + K1 + K2] per axis and index: (signed m / L, |b(m)|^2)
+    float4* frec;               // [S] reciprocal + excluded-pair force per slot
+    double* part;               // [nconv + ngather][2] energy and virial per workgroup
+    unsigned* counter;          // gather workgroups done (the last one resets it)
+    double* out;                // [2] energy and virial of the long-range part
+    int K0, K1, K2;
+    int nslot;
+    int mode;                   // MODE_*: which replicas step (the step kernel's gate)
+    int has_excl;
+    int nconv, ngather;         // workgroups of the influence and gather kernels
+    int order;                  // B-spline order (4, 6 or 8)
+    float L0, L1, L2, Li0, Li1, Li2;
+    double hl0, hl1, hl2;       // L / 2
+    double kl0, kl1, kl2;       // K / L
+    float beta;
+    float qfac;
+    float pi2_b2;               // pi^2 / beta^2
+    float cpref;                // qfac / (pi V)
+    double e0, w0;              // self + background energy and its virial (constants)
 };
 
 // Progress of a run for the host: one small copy per check instead of one per

@@ -1,0 +1,314 @@
+// pme.hip -- smooth particle-mesh Ewald (Essmann, Perera, Berkowitz, Darden, Lee,
+// Pedersen, J. Chem. Phys. 103, 8577 (1995)): the reciprocal-space part that the
+// damped Coulomb term (lkspace 2, src/forces.cu:97-110) leaves out, for one
+// replica.  The reference vendors GROMACS' PME (src/pme.c:829-1340) without
+// building it; this is the method from the paper, as the oracle's md_pme.cpp.
+//
+// Per step, before the step kernel (which adds frec to each atom's pair force and
+// the long-range energy and virial to the replica's first tile):
+//   k_pme_spread  one thread per slot: the order-p B-spline weights of the atom's
+//                 scaled coordinates u = K (x / L + 1/2) (in FP64, so the weights
+//                 carry only FP32 rounding), p^3 64-bit fixed-point atomic adds of
+//                 q th_x th_y th_z into the charge grid: integer sums, so the grid
+//                 does not depend on the order of the adds and runs repeat bit for bit
+//   k_pme_fix     fixed point -> float, clearing the fixed-point grid for the next step
+//   R2C FFT       (hipFFT, on the engine's stream, its work area allocated up front)
+//   k_pme_conv    multiply by the influence function C(m) = qfac exp(-pi^2 m^2/b^2)
+//                 B(m) / (pi V m^2), summing the energy 1/2 C |S|^2 and the traced
+//                 virial 1/2 C |S|^2 (1 - 2 pi^2 m^2 / b^2) over the half spectrum
+//   C2R FFT       -> dE/dQ on the grid
+//   k_pme_gather  one thread per slot: F = -q grad(sum_k dE/dQ(k) th th th), plus the
+//                 correction -qq erf(b r)/r of its excluded pairs; the last
+//                 workgroup to finish sums every workgroup's energy and virial in
+//                 a fixed order and adds the self and background constants.
+// Every kernel first reads the replica's state and does nothing for a step the
+// step kernel will not take (the same test as the step kernel), so the chain can
+// sit inside a device-gated graph segment.
+#include <hipfft/hipfft.h>
+
+#include "md_device.h"
+#include "md_reduce.h"
+
+namespace {
+
+constexpr int PT = 256;   // threads per workgroup
+
+__device__ __forceinline__ bool pme_go(const RepState& s, int mode) {
+    if (mode == MODE_FORCE) return !stale(s);
+    if (mode == MODE_LF) return !stale(s) && s.step < s.target;
+    return s.phase == 1 && !stale(s);
+}
+
+// th[j] = M_P(w + j), dth[j] = M_P'(w + j), j = 0..P-1, 0 <= w <= 1:
+// M_2(x) = 1 - |x - 1|, M_{k+1}(x) = [x M_k(x) + (k + 1 - x) M_k(x - 1)] / k,
+// M_k'(x) = M_{k-1}(x) - M_{k-1}(x - 1).
+template <int P, bool DER>
+__device__ __forceinline__ void bspline(float w, float* th, float* dth) {
+    float a[P];
+    a[0] = w;
+    a[1] = 1.0f - w;
+#pragma unroll
+    for (int j = 2; j < P; j++) a[j] = 0.0f;
+#pragma unroll
+    for (int k = 2; k < P; k++) {
+        if (DER && k == P - 1) {
+#pragma unroll
+            for (int j = 0; j < P; j++) dth[j] = (j < k ? a[j] : 0.0f) - (j > 0 ? a[j - 1] : 0.0f);
+        }
+        const float rk = 1.0f / (float)k;
+        float b[P];
+#pragma unroll
+        for (int j = 0; j <= k; j++)
+            b[j] = ((w + (float)j) * (j < k ? a[j] : 0.0f) + ((float)(k + 1 - j) - w) * (j > 0 ? a[j - 1] : 0.0f)) * rk;
+#pragma unroll
+        for (int j = 0; j <= k; j++) a[j] = b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < P; j++) th[j] = a[j];
+}
+
+// Grid index k0 = floor(u) mod K and the fraction w = u - floor(u) of
+// u = (x + L/2) K / L, in FP64.
+__device__ __forceinline__ void grid_coord(float x, double hl, double kl, int K, int& k0, float& w) {
+    double u = ((double)x + hl) * kl;
+    if (!(fabs(u) < 1e9)) u = 0.0;   // a non-finite coordinate (the step kernel reports it) stays in the grid
+    const double fl = floor(u);
+    w = (float)(u - fl);
+    int k = (int)((long long)fl % K);
+    k0 = k < 0 ? k + K : k;
+}
+
+// index of stencil point j (k0 - j, periodic)
+__device__ __forceinline__ int wrapk(int k, int K) { return k < 0 ? k + K : k; }
+
+template <int P>
+__global__ __launch_bounds__(PT) void k_pme_spread(const PmeArgs a) {
+    const RepState s = a.st[0];
+    if (!pme_go(s, a.mode)) return;   // grid-uniform
+    const int slot = blockIdx.x * PT + threadIdx.x;
+    if (slot >= a.nslot || a.ids[slot] < 0) return;
+    const float4 x = a.pos[cur_buf(s)][slot];
+    if (x.w == 0.0f) return;
+    int k0, k1, k2;
+    float w0, w1, w2;
+    grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
+    grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
+    grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+    float t0[P], t1[P], t2[P];
+    bspline<P, false>(w0, t0, nullptr);
+    bspline<P, false>(w1, t1, nullptr);
+    bspline<P, false>(w2, t2, nullptr);
+    int g2[P];
+#pragma unroll
+    for (int c = 0; c < P; c++) g2[c] = wrapk(k2 - c, a.K2);
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        const int g0 = wrapk(k0 - i, a.K0);
+        const float qa = x.w * t0[i];
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            unsigned long long* row = a.qacc + ((size_t)g0 * a.K1 + wrapk(k1 - j, a.K1)) * a.K2;
+            const float qab = qa * t1[j];
+#pragma unroll
+            for (int c = 0; c < P; c++) {
+                const long long v = __double2ll_rn((double)(qab * t2[c]) * MD_PME_QUNIT);
+                atomicAdd(row + g2[c], (unsigned long long)v);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(PT) void k_pme_fix(const PmeArgs a) {
+    if (!pme_go(a.st[0], a.mode)) return;
+    const size_t n = (size_t)a.K0 * a.K1 * a.K2;
+    for (size_t i = (size_t)blockIdx.x * PT + threadIdx.x; i < n; i += (size_t)gridDim.x * PT) {
+        const long long v = (long long)a.qacc[i];
+        a.qacc[i] = 0ull;
+        a.qgrid[i] = (float)((double)v * (1.0 / MD_PME_QUNIT));
+    }
+}
+
+// FP64 sum of (e, w) over the workgroup (fixed order) into part[k]
+__device__ __forceinline__ void block_pair_sum(double e, double w, double* part, int k) {
+    __shared__ double s_red[PT / 64][2];
+    e = wave_sum_d(e);
+    w = wave_sum_d(w);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_red[wave][0] = e;
+        s_red[wave][1] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double te = 0.0, tw = 0.0;
+        for (int v = 0; v < PT / 64; v++) {
+            te += s_red[v][0];
+            tw += s_red[v][1];
+        }
+        part[2 * (size_t)k] = te;
+        part[2 * (size_t)k + 1] = tw;
+    }
+}
+
+__global__ __launch_bounds__(PT) void k_pme_conv(const PmeArgs a) {
+    if (!pme_go(a.st[0], a.mode)) return;   // grid-uniform
+    const int H = a.K2 / 2 + 1;
+    const size_t n = (size_t)a.K0 * a.K1 * H;
+    const float2* t1 = a.tab + a.K0;
+    const float2* t2 = a.tab + a.K0 + a.K1;
+    double e = 0.0, w = 0.0;
+    for (size_t i = (size_t)blockIdx.x * PT + threadIdx.x; i < n; i += (size_t)gridDim.x * PT) {
+        const int i2 = (int)(i % H);
+        const size_t r = i / H;
+        const int i1 = (int)(r % a.K1), i0 = (int)(r / a.K1);
+        const float2 u0 = a.tab[i0], u1 = t1[i1], u2 = t2[i2];
+        const float mm = fmaf(u2.x, u2.x, fmaf(u1.x, u1.x, u0.x * u0.x));
+        const float2 S = a.sgrid[i];
+        float C = 0.0f;
+        if (i != 0) C = a.cpref * expf(-a.pi2_b2 * mm) / mm * (u0.y * u1.y * u2.y);
+        // the half spectrum: k2 = 0 and (K2 even) K2/2 stand for themselves, the
+        // others also for their conjugate partners
+        const double half = (i2 == 0 || 2 * i2 == a.K2) ? 0.5 : 1.0;
+        const double em = half * (double)C * ((double)S.x * S.x + (double)S.y * S.y);
+        e += em;
+        w += em * (1.0 - 2.0 * (double)a.pi2_b2 * (double)mm);
+        a.sgrid[i] = make_float2(S.x * C, S.y * C);
+    }
+    block_pair_sum(e, w, a.part, blockIdx.x);
+}
+
+template <int P>
+__global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
+    __shared__ bool s_last;
+    const RepState s = a.st[0];
+    if (!pme_go(s, a.mode)) return;   // grid-uniform
+    const int slot = blockIdx.x * PT + threadIdx.x;
+    double e = 0.0, w = 0.0;
+    const int id = slot < a.nslot ? a.ids[slot] : -1;
+    if (id >= 0) {
+        const float4* __restrict__ pin = a.pos[cur_buf(s)];
+        const float4 x = pin[slot];
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (x.w != 0.0f) {
+            int k0, k1, k2;
+            float w0, w1, w2;
+            grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
+            grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
+            grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+            float t0[P], t1[P], t2[P], d0[P], d1[P], d2[P];
+            bspline<P, true>(w0, t0, d0);
+            bspline<P, true>(w1, t1, d1);
+            bspline<P, true>(w2, t2, d2);
+            int g2[P];
+#pragma unroll
+            for (int c = 0; c < P; c++) g2[c] = wrapk(k2 - c, a.K2);
+            float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const int g0 = wrapk(k0 - i, a.K0);
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                    const float* row = a.qgrid + ((size_t)g0 * a.K1 + wrapk(k1 - j, a.K1)) * a.K2;
+                    float sz = 0.0f, sdz = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < P; c++) {
+                        const float ph = row[g2[c]];
+                        sz = fmaf(ph, t2[c], sz);
+                        sdz = fmaf(ph, d2[c], sdz);
+                    }
+                    gx = fmaf(d0[i] * t1[j], sz, gx);
+                    gy = fmaf(t0[i] * d1[j], sz, gy);
+                    gz = fmaf(t0[i] * t1[j], sdz, gz);
+                }
+            }
+            fx = -x.w * gx * (float)a.kl0;
+            fy = -x.w * gy * (float)a.kl1;
+            fz = -x.w * gz * (float)a.kl2;
+        }
+        if (a.has_excl && x.w != 0.0f) {
+            // excluded pairs: U = -qq erf(b r)/r, F_i = qq [c2 e^{-b^2 r^2}/r - erf(b r)/r^2] d/r
+            const int4 ex = a.excl[id];
+            const int pj[4] = {ex.x, ex.y, ex.z, ex.w};
+            const float c2 = a.beta * 1.1283791670955126f;   // 2 beta / sqrt(pi)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (pj[k] < 0) continue;
+                const float4 y = pin[a.slot_of[pj[k]]];
+                if (y.w == 0.0f) continue;
+                const float dx = minimg(x.x - y.x, a.L0, a.Li0), dy = minimg(x.y - y.y, a.L1, a.Li1),
+                            dz = minimg(x.z - y.z, a.L2, a.Li2);
+                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float r = sqrtf(r2), ir = 1.0f / r;
+                const float br = a.beta * r;
+                const float erfv = erff(br);
+                const float qq = a.qfac * x.w * y.w;
+                const float fr = qq * (c2 * expf(-br * br) - erfv * ir) * ir * ir;
+                fx = fmaf(fr, dx, fx);
+                fy = fmaf(fr, dy, fy);
+                fz = fmaf(fr, dz, fz);
+                // energy and virial in FP64: for neutral molecules these terms nearly
+                // cancel the self term, each far larger than their sum
+                const double rd = sqrt((double)r2), bd = (double)a.beta * rd;
+                const double qqd = (double)a.qfac * (double)x.w * (double)y.w, erfd = erf(bd);
+                e -= 0.5 * qqd * erfd / rd;   // each pair is listed from both atoms
+                w += 0.5 * qqd * (1.1283791670955126 * bd * exp(-bd * bd) - erfd) / rd;
+            }
+        }
+        a.frec[slot] = make_float4(fx, fy, fz, 0.0f);
+    }
+    block_pair_sum(e, w, a.part, a.nconv + blockIdx.x);
+    // the last workgroup to finish adds up every workgroup's pair in a fixed order
+    __threadfence();
+    if (threadIdx.x == 0) s_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    const int np = a.nconv + a.ngather;
+    double te = 0.0, tw = 0.0;
+    for (int k = threadIdx.x; k < np; k += PT) {
+        te += a.part[2 * (size_t)k];
+        tw += a.part[2 * (size_t)k + 1];
+    }
+    __syncthreads();   // s_red of block_pair_sum is reused
+    block_pair_sum(te, tw, a.out, 0);
+    if (threadIdx.x == 0) {
+        a.out[0] += a.e0;
+        a.out[1] += a.w0;
+        atomicExch(a.counter, 0u);
+    }
+}
+
+template <int P>
+void launch_spread(const PmeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_pme_spread<P>, dim3((a.nslot + PT - 1) / PT), dim3(PT), 0, st, a);
+}
+
+template <int P>
+void launch_gather(const PmeArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_pme_gather<P>, dim3(a.ngather), dim3(PT), 0, st, a);
+}
+
+}  // namespace
+
+// Queue the chain for one step.  Returns 0, or 1 (HIP launch error) or 2 (hipFFT error).
+extern "C" int md_launch_pme(const PmeArgs& a, hipfftHandle r2c, hipfftHandle c2r, hipStream_t st) {
+    switch (a.order) {
+    case 4: launch_spread<4>(a, st); break;
+    case 6: launch_spread<6>(a, st); break;
+    case 8: launch_spread<8>(a, st); break;
+    default: return 1;
+    }
+    const size_t n = (size_t)a.K0 * a.K1 * a.K2;
+    const int nfix = (int)std::min<size_t>((n + PT - 1) / PT, 4096);
+    hipLaunchKernelGGL(k_pme_fix, dim3(nfix), dim3(PT), 0, st, a);
+    if (hipGetLastError() != hipSuccess) return 1;
+    if (hipfftExecR2C(r2c, a.qgrid, reinterpret_cast<hipfftComplex*>(a.sgrid)) != HIPFFT_SUCCESS) return 2;
+    hipLaunchKernelGGL(k_pme_conv, dim3(a.nconv), dim3(PT), 0, st, a);
+    if (hipfftExecC2R(c2r, reinterpret_cast<hipfftComplex*>(a.sgrid), a.qgrid) != HIPFFT_SUCCESS) return 2;
+    switch (a.order) {
+    case 4: launch_gather<4>(a, st); break;
+    case 6: launch_gather<6>(a, st); break;
+    default: launch_gather<8>(a, st); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}

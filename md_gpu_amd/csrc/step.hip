@@ -324,10 +324,11 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
     const uint4* __restrict__ row4 = reinterpret_cast<const uint4*>(a.nl16 + (size_t)slot * a.cap);
     const uint4 q0 = row4[sub];
     const uint4 q1 = row4[(a.cap >= 16 * G ? G : 0) + sub];   // block 1 when the capacity holds one
-    float4 v_e = make_float4(0.f, 0.f, 0.f, 0.f), pb_e = v_e;
+    float4 v_e = make_float4(0.f, 0.f, 0.f, 0.f), pb_e = v_e, fr_e = v_e;
     if (sub == 0) {
         v_e = a.vel[slot];
         pb_e = a.pos_build[slot];
+        if (COUL == 2 && a.frec) fr_e = a.frec[slot];   // PME: long-range force of the atom
     }
     loads_issued();
     if (tid < 64) s_lj[tid] = lj_v;
@@ -548,6 +549,11 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
     double dke = 0.0;
     int trip = 0;
     if (active && sub == 0) {
+        if (COUL == 2 && a.frec) {
+            fx += fr_e.x;
+            fy += fr_e.y;
+            fz += fr_e.z;
+        }
         a.force[slot] = make_float4(fx, fy, fz, 0.0f);
         if (a.mode == MODE_LF) {
             float4 v = v_e;                 // loaded in the first round trip
@@ -629,6 +635,12 @@ __global__ __launch_bounds__(64 * G, COUL == 3 ? 5 : 4) void k_step(const StepAr
         double t = s_red[0][tid];
 #pragma unroll
         for (int w = 1; w < NW; w++) t += s_red[w][tid];
+        if (COUL == 2 && a.pme_e && ENMODE != 0 && en && leader) {
+            // PME: the replica's long-range terms (the slots hold sums over ordered
+            // pairs, halved by the finalize)
+            if (tid == ACC_COUL) t += 2.0 * a.pme_e[0];
+            if (tid == ACC_VIR) t += 2.0 * a.pme_e[1];
+        }
         a.part[((size_t)a.par * MD_NSTAT + tid) * a.ntiles + tile] = t;
     }
 }

@@ -80,6 +80,7 @@ def lib():
             "md_set_system": ([P, C.c_int, P, P, P, P, P, P], C.c_int),
             "md_set_topology": ([P, C.c_int, P, C.c_int, P], C.c_int),
             "md_set_rigid": ([P, C.c_int, P, C.c_float, C.c_float], C.c_int),
+            "md_set_pme": ([P, P, C.c_int], C.c_int),
             "md_upload_state": ([P, P, P], C.c_int), "md_download_state": ([P, P, P, P], C.c_int),
             "md_compute_forces": ([P, P, P, P], C.c_int), "md_run": ([P, C.c_int, C.c_int, P], C.c_int),
             "md_get_neighbors": ([P, C.c_int, P, P], C.c_int), "md_get_thresholds": ([P, P, P, P], C.c_int),
@@ -109,7 +110,7 @@ def exported_symbols() -> List[str]:
             "md_get_launch", "md_autotune",
             "md_timing_enable", "md_timing_get", "md_copy_bench", "md_rebuild_bench", "md_fm_set_frames", "md_fm_loss_grad",
             "md_set_list_capacity", "md_fm_loss_grad_noalloc", "md_get_box", "md_get_sums", "md_persist_info",
-            "md_set_rigid"]
+            "md_set_rigid", "md_set_pme"]
 
 
 def _check(rc: int):
@@ -202,6 +203,11 @@ class Engine:
         rig = [np.asarray(s.rigid, np.int32) + o for s, o in zip(systems, offs) if len(s.rigid)]
         if rig:
             self.set_rigid(np.concatenate(rig), *s0.rigid_d)
+        pme = getattr(s0, "pme", ())
+        if pme:
+            if len(systems) != 1:
+                raise MDError(-1, "PME runs one replica per engine")
+            self.set_pme(pme[:3], pme[3])
         pos = np.ascontiguousarray(np.concatenate([s.pos for s in systems]), np.float32)
         vel = np.ascontiguousarray(np.concatenate([s.vel for s in systems]), np.float32)
         self.upload(pos, vel)
@@ -210,6 +216,12 @@ class Engine:
         """Rigid 3-site molecules (md_set_rigid): mol (nmol, 3) global atom indices (O, H1, H2)."""
         mol = np.ascontiguousarray(mol, np.int32).reshape(-1, 3)
         _check(lib().md_set_rigid(self._h, mol.shape[0], _ptr(mol), float(d_oh), float(d_hh)))
+
+    def set_pme(self, grid, order: int):
+        """Smooth PME for the damped Coulomb term (md_set_pme): grid (3,) points per
+        box edge, B-spline order 4, 6 or 8; order 0 turns it off."""
+        g = np.ascontiguousarray(grid, np.int32).reshape(3)
+        _check(lib().md_set_pme(self._h, _ptr(g), int(order)))
 
     @classmethod
     def from_system(cls, systems, device: int = 0) -> "Engine":
