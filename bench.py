@@ -282,6 +282,30 @@ def extra_workloads(args, want_cpu, tune):
         "nacl32768_wolf_nvt_lf", nacl, lambda p: 65.0 * count_pairs(p, nacl.box, nacl.ff.rc)[0], 5000, 300, tune,
         nacl.dt,
         cpu_nacl if want_cpu else None, (lambda e, s: force_error(e, s)) if want_cpu else None)
+    # the same NaCl with Ewald electrostatics: damped real space (alpha 3.09 / rc)
+    # plus smooth PME, order 4 on a 96^3 grid (0.94 A)
+    nacl_pme = S.with_pme(nacl, spacing=1.0, order=4)
+
+    def cpu_nacl_pme(e, s):
+        pos, vel, _ = e.download()
+        t = s.copy()
+        t.pos, t.vel = pos, vel
+        n, sps, cores = oracle_cpu_steps(t, 12.0)
+        return {"value": sps, "unit": "timesteps/s", "cores": cores, "kind": "port",
+                "sample": f"{n} steps of the same state, FP64 oracle with FP64 SPME (OpenMP, {cpu_info()})"}
+    out["nacl32768_pme_nvt_lf"] = step_workload(
+        "nacl32768_pme_nvt_lf", nacl_pme, lambda p: 65.0 * count_pairs(p, nacl.box, nacl.ff.rc)[0], 2000, 200,
+        tune, nacl.dt, cpu_nacl_pme if want_cpu else None,
+        (lambda e, s: force_error(e, s)) if want_cpu else None)
+    # rigid water 65,536 molecules with SETTLE and PME (128^3 grid), leapfrog Berendsen 2 fs
+    wr = S.with_pme(S.water(rigid=True), spacing=1.0, order=4)
+
+    def f_wr(p):
+        n, noo = count_pairs(p, wr.box, wr.ff.rc, wr.type, lambda a, b: (a == 0) & (b == 0))
+        return 60.0 * n + 5.0 * noo
+    out["water196608_rigid_pme_nvt_lf"] = step_workload(
+        "water196608_rigid_pme_nvt_lf", wr, f_wr, 500, 100, tune, wr.dt,
+        (lambda e, s: cpu_small(e, s, 10.0)) if want_cpu else None)
     # argon 864, NVE velocity Verlet (the 1000-step case, timed over 5000 steps: a
     # step takes microseconds)
     ar = S.argon()

@@ -1984,8 +1984,8 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
             }
         }
     }
-    // stage 2, whole runs (best of two): shape x rebuild-check interval x age refresh x
-    // submission mode x deferred finalize.  A trial times runs of nsteps steps each (the
+    // stage 2, whole runs (best of two): launch mode (age refresh, submission, deferred
+    // finalize), then shape x rebuild-check interval.  A trial times runs of nsteps steps each (the
     // length of the caller's md_run calls: a short run pays its first rebuild chain and
     // waits for the slowest replica, which long runs amortise), at least 4 segments' worth.
     double best_run = 1e30;
@@ -2017,23 +2017,43 @@ int md_autotune(md_engine* e, int nsteps, double* best_ms) {
     };
     struct Trial { double ms; md_launch_config c; };
     std::vector<Trial> trials;
-    for (const Shape& sh : shapes)
-        for (int sub : {0, 1})
-            for (int rf : {0, 1})
-                for (int df : {1, 0})
-                for (int ce : {3, 4, 6, 8, 12, 16, 24, 32}) {
-                    if (df == 0 && !lf) continue;   // deferral applies to leapfrog-form steps only
-                    md_launch_config c = base;
-                    c.cell_div = sh.cell_div;
-                    c.lanes = sh.lanes;
-                    c.check_every = ce;
-                    c.submit = sub;
-                    c.refresh = rf;
-                    c.defer = df;
-                    double per = 1e30;
-                    CHECK(time_run(c, total, 2, per));
-                    trials.push_back({per, c});
-                }
+    // With PME a step launched after a skin trip (and skipped until the next
+    // rebuild chain) still runs its FFTs: short check intervals pay off there.
+    const std::vector<int> ces = e->pme_on ? std::vector<int>{1, 2, 3, 4, 6, 8}
+                                           : std::vector<int>{3, 4, 6, 8, 12, 16, 24, 32};
+    auto trial = [&](const Shape& sh, int sub, int rf, int df, int ce, std::vector<Trial>& into) -> int {
+        md_launch_config c = base;
+        c.cell_div = sh.cell_div;
+        c.lanes = sh.lanes;
+        c.check_every = ce;
+        c.submit = sub;
+        c.refresh = rf;
+        c.defer = df;
+        double per = 1e30;
+        CHECK(time_run(c, total, 2, per));
+        into.push_back({per, c});
+        return MD_OK;
+    };
+    // 2a: submission mode x age refresh x deferred finalize at the best shape and a
+    // middle check interval; the two best modes go on
+    std::vector<Trial> modes;
+    for (int sub : {0, 1})
+        for (int rf : {0, 1})
+            for (int df : {1, 0}) {
+                if (df == 0 && !lf) continue;              // deferral applies to leapfrog-form steps only
+                if (df == 1 && e->pme_on && lf) continue;  // PME steps finalize in their own kernel
+                CHECK(trial(shapes[0], sub, rf, df, ces[ces.size() / 2], modes));
+            }
+    std::sort(modes.begin(), modes.end(), [](const Trial& x, const Trial& y) { return x.ms < y.ms; });
+    trials = modes;
+    modes.resize(std::min<size_t>(modes.size(), 2));
+    // 2b: the check interval for the two best shapes and modes
+    for (size_t k = 0; k < std::min<size_t>(shapes.size(), 2); k++)
+        for (const Trial& m : modes)
+            for (int ce : ces) {
+                if (k == 0 && ce == ces[ces.size() / 2]) continue;   // timed in 2a
+                CHECK(trial(shapes[k], m.c.submit, m.c.refresh, m.c.defer, ce, trials));
+            }
     // the persistent kernel (small single systems), one trial per lanes per atom
     if (persist_eligible(e))
         for (int l : {4, 8, 16}) {

@@ -6,11 +6,12 @@
 //
 // Per step, before the step kernel (which adds frec to each atom's pair force and
 // the long-range energy and virial to the replica's first tile):
-//   k_pme_spread  one thread per slot: the order-p B-spline weights of the atom's
-//                 scaled coordinates u = K (x / L + 1/2) (in FP64, so the weights
-//                 carry only FP32 rounding), p^3 64-bit fixed-point atomic adds of
-//                 q th_x th_y th_z into the charge grid: integer sums, so the grid
-//                 does not depend on the order of the adds and runs repeat bit for bit
+//   k_pme_spread  the order-p B-spline weights of each atom's scaled coordinates
+//                 u = K (x / L + 1/2) (in FP64, so the weights carry only FP32
+//                 rounding), p^3 64-bit fixed-point atomic adds of q th_x th_y th_z
+//                 into the charge grid by a wavefront per atom: integer sums, so the
+//                 grid does not depend on the order of the adds and runs repeat bit
+//                 for bit
 //   k_pme_fix     fixed point -> float, clearing the fixed-point grid for the next step
 //   R2C FFT       (hipFFT, on the engine's stream, its work area allocated up front)
 //   k_pme_conv    multiply by the influence function C(m) = qfac exp(-pi^2 m^2/b^2)
@@ -81,39 +82,63 @@ __device__ __forceinline__ void grid_coord(float x, double hl, double kl, int K,
 // index of stencil point j (k0 - j, periodic)
 __device__ __forceinline__ int wrapk(int k, int K) { return k < 0 ? k + K : k; }
 
+// Lane l of a wavefront first computes the stencil of slot (wavefront's base + l);
+// then the wavefront walks its 64 atoms, its lanes taking the atom's P^3 grid
+// points with the z index fastest, so each atomic instruction touches P^2 runs of
+// P consecutive grid words instead of 64 scattered ones (atomics are combined
+// per cache line on the way to L2).
 template <int P>
 __global__ __launch_bounds__(PT) void k_pme_spread(const PmeArgs a) {
     const RepState s = a.st[0];
     if (!pme_go(s, a.mode)) return;   // grid-uniform
+    const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * PT + threadIdx.x;
-    if (slot >= a.nslot || a.ids[slot] < 0) return;
-    const float4 x = a.pos[cur_buf(s)][slot];
-    if (x.w == 0.0f) return;
-    int k0, k1, k2;
-    float w0, w1, w2;
-    grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
-    grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
-    grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+    float q = 0.0f;
+    int k0 = 0, k1 = 0, k2 = 0;
     float t0[P], t1[P], t2[P];
-    bspline<P, false>(w0, t0, nullptr);
-    bspline<P, false>(w1, t1, nullptr);
-    bspline<P, false>(w2, t2, nullptr);
-    int g2[P];
 #pragma unroll
-    for (int c = 0; c < P; c++) g2[c] = wrapk(k2 - c, a.K2);
+    for (int c = 0; c < P; c++) t0[c] = t1[c] = t2[c] = 0.0f;
+    if (slot < a.nslot && a.ids[slot] >= 0) {
+        const float4 x = a.pos[cur_buf(s)][slot];
+        q = x.w;
+        float w0, w1, w2;
+        grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
+        grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
+        grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+        bspline<P, false>(w0, t0, nullptr);
+        bspline<P, false>(w1, t1, nullptr);
+        bspline<P, false>(w2, t2, nullptr);
 #pragma unroll
-    for (int i = 0; i < P; i++) {
-        const int g0 = wrapk(k0 - i, a.K0);
-        const float qa = x.w * t0[i];
+        for (int c = 0; c < P; c++) t0[c] *= q;   // the charge rides on the x weights
+    }
+    unsigned long long todo = __ballot(q != 0.0f);
+    while (todo) {
+        const int t = __builtin_ctzll(todo);   // wavefront-uniform
+        todo &= todo - 1;
+        const int b0 = __builtin_amdgcn_readlane(k0, t), b1 = __builtin_amdgcn_readlane(k1, t),
+                  b2 = __builtin_amdgcn_readlane(k2, t);
+        float u0[P], u1[P], u2[P];
 #pragma unroll
-        for (int j = 0; j < P; j++) {
-            unsigned long long* row = a.qacc + ((size_t)g0 * a.K1 + wrapk(k1 - j, a.K1)) * a.K2;
-            const float qab = qa * t1[j];
+        for (int c = 0; c < P; c++) {
+            u0[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t0[c]), t));
+            u1[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t1[c]), t));
+            u2[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t2[c]), t));
+        }
 #pragma unroll
-            for (int c = 0; c < P; c++) {
-                const long long v = __double2ll_rn((double)(qab * t2[c]) * MD_PME_QUNIT);
-                atomicAdd(row + g2[c], (unsigned long long)v);
+        for (int p0 = 0; p0 < P * P * P; p0 += 64) {
+            const int p = p0 + lane;
+            if (P * P * P % 64 != 0 && p >= P * P * P) break;
+            const int i = p / (P * P), j = (p / P) % P, c = p % P;
+            float wx = u0[0], wy = u1[0], wz = u2[0];
+#pragma unroll
+            for (int m = 1; m < P; m++) {
+                wx = i == m ? u0[m] : wx;
+                wy = j == m ? u1[m] : wy;
+                wz = c == m ? u2[m] : wz;
             }
+            const size_t idx = ((size_t)wrapk(b0 - i, a.K0) * a.K1 + wrapk(b1 - j, a.K1)) * a.K2 + wrapk(b2 - c, a.K2);
+            const long long v = __double2ll_rn((double)(wx * wy * wz) * MD_PME_QUNIT);
+            atomicAdd(a.qacc + idx, (unsigned long long)v);
         }
     }
 }
