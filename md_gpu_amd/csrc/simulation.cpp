@@ -29,6 +29,8 @@ struct RunOptions {
     float tau = 100.0f;             // reference tco = 0.01 /fs (src/integrator.cu:146)
     float tau_p = 1000.0f;          // NPT: Berendsen pressure coupling time, fs
     float compress = 4.5e-5f;       // NPT: isothermal compressibility, 1/bar (water)
+    int pme_K[3] = {0, 0, 0};       // "pme: K0 K1 K2 order" (lkspace 2 only): smooth PME grid
+    int pme_order = 0;              // and B-spline order; 0 = the reference's real-space term only
 };
 std::mutex g_opt_mu;
 std::map<const integr*, RunOptions> g_opts;
@@ -308,6 +310,12 @@ simulation* SimulationInit(char* name_sim) {
                 fprintf(stderr, "md_gpu: %s: barostat needs tau_p > 0 and compressibility >= 0\n", file.c_str());
                 ok = false;
             }
+        } else if (strcmp(saux, "pme:") == 0 &&
+                   fscanf(fin, "%d %d %d %d", &opt.pme_K[0], &opt.pme_K[1], &opt.pme_K[2], &opt.pme_order) == 4) {
+            if (pars->lkspace != 2) {
+                fprintf(stderr, "md_gpu: %s: pme needs the Ewald real-space term (lkspace 2)\n", file.c_str());
+                ok = false;
+            }
         } else {
             fprintf(stderr, "md_gpu: %s: unknown key %s\n", file.c_str(), saux);
             ok = false;
@@ -412,6 +420,7 @@ int SimulationRun_CUDA(params* pars, integr* integ, ffield* fld, config* conf) {
         angles.push_back((int)fld->iang[a].z - 1); angles.push_back((int)fld->iang[a].w - 1);
     }
     if ((rc = md_set_topology(e, fld->nbonds, bonds.data(), fld->nangle, angles.data()))) return die(e, rc);
+    if (opt.pme_order > 0 && (rc = md_set_pme(e, opt.pme_K, opt.pme_order))) return die(e, rc);
     if ((rc = md_upload_state(e, pos.data(), vel.data()))) return die(e, rc);
     // small single Lennard-Jones systems run every step of an md_run call in one
     // launch (the engine ignores the option for any other system)

@@ -43,6 +43,8 @@ def write_inputs(s: System, name: str, timrun: int = 100, timprn: int = 10, eps_
         f.write(f"thermostat: {s.thermostat} {s.tau}\n")
         if ens == "npt":
             f.write(f"barostat: {s.barostat} {s.tau_p} {s.compress}\n")
+        if getattr(s, "pme", ()):
+            f.write("pme: {} {} {} {}\n".format(*s.pme))
     nt = ff.ntype
     counts = np.bincount(s.type, minlength=nt)
     qtype = [float(s.charge[s.type == t][0]) if counts[t] else 0.0 for t in range(nt)]
@@ -170,6 +172,7 @@ def read_system(name: str) -> System:
     p += 2
     integ, thermo, tau = "leapfrog", ("berendsen" if ens in ("nvt", "npt") else "none"), 100.0
     baro, tau_p, compress = ("berendsen" if ens == "npt" else "none"), 1000.0, 4.5e-5
+    pme = ()
     while p < len(tok):
         if tok[p] == "integrator:":
             integ = tok[p + 1]; p += 2
@@ -177,6 +180,8 @@ def read_system(name: str) -> System:
             thermo, tau = tok[p + 1], float(tok[p + 2]); p += 3
         elif tok[p] == "barostat:":
             baro, tau_p, compress = tok[p + 1], float(tok[p + 2]), float(tok[p + 3]); p += 4
+        elif tok[p] == "pme:":
+            pme = tuple(int(x) for x in tok[p + 1:p + 5]); p += 5
         else:
             raise ValueError(f"unknown key {tok[p]}")
     fld = read_fld(name + ".fld")
@@ -189,4 +194,4 @@ def read_system(name: str) -> System:
         vel = np.zeros_like(pos)
     return System(os.path.basename(name), box, pos, vel, typ, q, ff, bonds=fld["bonds"], angles=fld["angles"],
                   integrator=integ, thermostat=thermo if ens != "nve" else "none", dt=dt, T0=T0, tau=tau,
-                  barostat=baro if ens == "npt" else "none", P0=P0, tau_p=tau_p, compress=compress)
+                  barostat=baro if ens == "npt" else "none", P0=P0, tau_p=tau_p, compress=compress, pme=pme)

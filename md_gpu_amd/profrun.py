@@ -1,11 +1,12 @@
 """Short fixed-launch run of one benchmark workload for hardware-counter
 collection and kernel traces: molten NaCl (32,768 ions, the default), argon
 (864 atoms, NVE velocity Verlet), water (196,608 atoms, NVT velocity Verlet),
-the 512 batched Ar-Xe replicas, force matching (NaCl 4,096 ions x 64 frames,
+the same NaCl and rigid water (SETTLE) with smooth PME (order 4, about 1 A
+grid: 96^3 and 128^3), the 512 batched Ar-Xe replicas, force matching (NaCl 4,096 ions x 64 frames,
 --steps loss-and-gradient evaluations) or a plain 64 MiB device copy (the
 calibration of the HBM byte counters).
 
-    python -m md_gpu_amd.profrun [--system nacl|argon|water|arxe|fm|copy]
+    python -m md_gpu_amd.profrun [--system nacl|argon|water|arxe|nacl_pme|water_rigid_pme|fm|copy]
                                  [--launch k=v,...] [--steps N] [--timed]
 
 Counter collection serialises every dispatch, so this driver keeps the run
@@ -33,7 +34,7 @@ import numpy as np
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--launch", default="lanes=8,cell_div=2,check_every=8,refresh=1,defer=1,submit=1")
-    ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe", "fm", "copy"])
+    ap.add_argument("--system", default="nacl", choices=["nacl", "argon", "water", "arxe", "nacl_pme", "water_rigid_pme", "fm", "copy"])
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--timed", action="store_true")
@@ -48,7 +49,9 @@ def main() -> int:
         return 0
     if args.system == "fm":
         return fm_main(args, cfg)
-    build = {"nacl": S.nacl, "argon": S.argon, "water": S.water, "arxe": S.arxe_replicas}[args.system]
+    build = {"nacl": S.nacl, "argon": S.argon, "water": S.water, "arxe": S.arxe_replicas,
+             "nacl_pme": lambda: S.with_pme(S.nacl()),
+             "water_rigid_pme": lambda: S.with_pme(S.water(rigid=True))}[args.system]
     e = E.Engine.from_system(build())
     e.set_launch(**cfg)
     try:

@@ -170,3 +170,24 @@ def test_write_read_round_trip(tmp_path, mk):
     assert np.allclose(t.ff.lj1, s.ff.lj1, rtol=1e-6) and np.allclose(t.ff.lj2, s.ff.lj2, rtol=1e-6)
     assert t.integrator == s.integrator and t.thermostat == s.thermostat
     assert np.array_equal(t.bonds, s.bonds) and np.array_equal(t.angles, s.angles)
+
+
+def test_pme_key_round_trip_and_parser(tmp_path):
+    """The optional "pme: K0 K1 K2 order" key (lkspace 2 only): written, read back,
+    accepted by SimulationInit, refused after another lkspace."""
+    s = S.with_pme(S.nacl(nsites=6), spacing=1.0, order=6)
+    name = str(tmp_path / "pme")
+    fileio.write_inputs(s, name)
+    t = fileio.read_system(name)
+    assert t.pme == s.pme and t.ff.coulomb == S.COUL_DAMPED
+    assert t.ff.alpha == pytest.approx(3.09 / s.ff.rc)
+    init, fin = compat()
+    p = init(C.create_string_buffer(name.encode(), 512))
+    assert p
+    assert p.contents.pars.contents.lkspace == 2
+    w = S.nacl(nsites=6)
+    w.pme = s.pme   # Wolf (lkspace 1) with the key
+    bad = str(tmp_path / "bad")
+    fileio.write_inputs(w, bad)
+    assert not init(C.create_string_buffer(bad.encode(), 512))
+    fin(p)

@@ -32,11 +32,15 @@ def stat_lines(out):
     return rows
 
 
-@pytest.mark.parametrize("kind", ["nacl_lf_berendsen", "water_vv_nh", "argon_vv_nve"])
+@pytest.mark.parametrize("kind", ["nacl_lf_berendsen", "water_vv_nh", "argon_vv_nve", "nacl_pme_lf_berendsen"])
 def test_athom_matches_oracle(tmp_path, orc, kind):
-    """argon_vv_nve runs on the persistent small-system kernel (athom enables it)."""
+    """argon_vv_nve runs on the persistent small-system kernel (athom enables it);
+    nacl_pme_lf_berendsen adds the optional "pme: K0 K1 K2 order" key to lkspace 2."""
     if kind == "nacl_lf_berendsen":
         s = S.nacl(nsites=12)
+        timrun, timprn = 20, 5
+    elif kind == "nacl_pme_lf_berendsen":
+        s = S.with_pme(S.nacl(nsites=12), spacing=1.0, order=4)
         timrun, timprn = 20, 5
     elif kind == "argon_vv_nve":
         s = S.argon()
