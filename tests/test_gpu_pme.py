@@ -6,8 +6,8 @@ independence, finite differences and convergence to the direct Ewald sum).
   and uneven grids (odd and even edges, odd last edge), orders 4, 6 and 8;
 - rigid water (excluded intramolecular pairs) forces at 1,536 and 196,608 atoms,
   and 30 leapfrog steps with SETTLE against the oracle's trajectory;
-- NaCl leapfrog Berendsen and flexible water velocity Verlet Nose-Hoover (1-2
-  exclusions) trajectories;
+- NaCl leapfrog Berendsen, NaCl NVE velocity Verlet (with energy conservation)
+  and flexible water velocity Verlet Nose-Hoover (1-2 exclusions) trajectories;
 - runs repeat bit for bit (fixed-point charge spreading), and graph-captured
   segments equal direct submission (the FFTs inside captured graphs);
 - what PME refuses: several replicas, another Coulomb kind, pressure coupling,
@@ -171,6 +171,26 @@ def test_flexible_water_vv_nh_trajectory(E, orc):
     s = S.with_pme(perturbed(S.water(nmol=512), 0.02), spacing=1.0, order=4)
     say("pme flexible water vv nh 20 steps")
     run_vs_oracle(E, orc, s, 20)
+
+
+def test_nacl_nve_vv_trajectory_and_conservation(E, orc):
+    """NVE velocity Verlet runs in leapfrog form (one step kernel per step): 20 steps
+    against the oracle, then 400 more conserving the total energy within the
+    oracle's own bounds (drift < 0.2 %, span < 0.5 % of the kinetic energy)."""
+    s = dataclasses.replace(S.with_pme(S.nacl(nsites=12, T=600.0), spacing=1.0, order=4),
+                            thermostat="none", integrator="vverlet", dt=1.0)
+    say("pme nacl nve vv 20 + 400 steps")
+    run_vs_oracle(E, orc, s, 20)
+    e = E.Engine.from_system(s)
+    n = 400
+    st = e.run(n, energy_every=1)
+    H = st["conserved"][:, 0]
+    ke = st["ke"][:, 0].mean()
+    slope = np.polyfit(np.arange(n), H, 1)[0]
+    say("  drift", slope * n, "span", H.max() - H.min(), "ke", ke)
+    assert abs(slope) * n < 0.002 * ke
+    assert H.max() - H.min() < 0.005 * ke
+    e.close()
 
 
 def test_runs_repeat_and_graphs_equal_direct(E):
