@@ -756,6 +756,7 @@ PmeArgs pme_args(md_engine* e, int mode) {
     g.Li0 = (float)(1.0 / L[0]); g.Li1 = (float)(1.0 / L[1]); g.Li2 = (float)(1.0 / L[2]);
     g.hl0 = 0.5 * L[0]; g.hl1 = 0.5 * L[1]; g.hl2 = 0.5 * L[2];
     g.kl0 = e->pme_K[0] / L[0]; g.kl1 = e->pme_K[1] / L[1]; g.kl2 = e->pme_K[2] / L[2];
+    g.ik0 = 1.0 / e->pme_K[0]; g.ik1 = 1.0 / e->pme_K[1]; g.ik2 = 1.0 / e->pme_K[2];
     const double beta = e->ff.alpha, pi = 3.14159265358979323846;
     g.beta = (float)beta;
     g.qfac = e->ff.qfac;

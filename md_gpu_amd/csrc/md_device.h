@@ -263,6 +263,7 @@ struct PmeArgs {
     float L0, L1, L2, Li0, Li1, Li2;
     double hl0, hl1, hl2;       // L / 2
     double kl0, kl1, kl2;       // K / L
+    double ik0, ik1, ik2;       // 1 / K
     float beta;
     float qfac;
     float pi2_b2;               // pi^2 / beta^2

@@ -69,14 +69,17 @@ __device__ __forceinline__ void bspline(float w, float* th, float* dth) {
 }
 
 // Grid index k0 = floor(u) mod K and the fraction w = u - floor(u) of
-// u = (x + L/2) K / L, in FP64.
-__device__ __forceinline__ void grid_coord(float x, double hl, double kl, int K, int& k0, float& w) {
+// u = (x + L/2) K / L, in FP64: the modulo as fl - K floor(fl / K) with one
+// correction by K where the rounded quotient lands on the wrong integer (no
+// emulated 64-bit integer division).
+__device__ __forceinline__ void grid_coord(float x, double hl, double kl, double ik, int K, int& k0, float& w) {
     double u = ((double)x + hl) * kl;
     if (!(fabs(u) < 1e9)) u = 0.0;   // a non-finite coordinate (the step kernel reports it) stays in the grid
     const double fl = floor(u);
     w = (float)(u - fl);
-    int k = (int)((long long)fl % K);
-    k0 = k < 0 ? k + K : k;
+    double m = fl - (double)K * floor(fl * ik);
+    m = m >= (double)K ? m - (double)K : (m < 0.0 ? m + (double)K : m);
+    k0 = (int)m;
 }
 
 // index of stencil point j (k0 - j, periodic)
@@ -102,9 +105,9 @@ __global__ __launch_bounds__(PT) void k_pme_spread(const PmeArgs a) {
         const float4 x = a.pos[cur_buf(s)][slot];
         q = x.w;
         float w0, w1, w2;
-        grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
-        grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
-        grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+        grid_coord(x.x, a.hl0, a.kl0, a.ik0, a.K0, k0, w0);
+        grid_coord(x.y, a.hl1, a.kl1, a.ik1, a.K1, k1, w1);
+        grid_coord(x.z, a.hl2, a.kl2, a.ik2, a.K2, k2, w2);
         bspline<P, false>(w0, t0, nullptr);
         bspline<P, false>(w1, t1, nullptr);
         bspline<P, false>(w2, t2, nullptr);
@@ -217,9 +220,9 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
         if (x.w != 0.0f) {
             int k0, k1, k2;
             float w0, w1, w2;
-            grid_coord(x.x, a.hl0, a.kl0, a.K0, k0, w0);
-            grid_coord(x.y, a.hl1, a.kl1, a.K1, k1, w1);
-            grid_coord(x.z, a.hl2, a.kl2, a.K2, k2, w2);
+            grid_coord(x.x, a.hl0, a.kl0, a.ik0, a.K0, k0, w0);
+            grid_coord(x.y, a.hl1, a.kl1, a.ik1, a.K1, k1, w1);
+            grid_coord(x.z, a.hl2, a.kl2, a.ik2, a.K2, k2, w2);
             float t0[P], t1[P], t2[P], d0[P], d1[P], d2[P];
             bspline<P, true>(w0, t0, d0);
             bspline<P, true>(w1, t1, d1);
