@@ -290,7 +290,6 @@ struct md_engine {
     DBuf<float2> pme_sgrid, pme_tab;
     DBuf<float4> pme_frec;
     DBuf<double> pme_part, pme_out;
-    DBuf<unsigned> pme_counter;
     DBuf<char> pme_work;
     hipfftHandle pme_r2c = nullptr, pme_c2r = nullptr;
     // segment graphs by key (check interval, energy interval, blocks); two
@@ -688,7 +687,7 @@ int pme_prepare(md_engine* e) {
     e->drop_pme_plans();
     const size_t n = (size_t)K[0] * K[1] * K[2], nh = (size_t)K[0] * K[1] * (K[2] / 2 + 1);
     e->pme_nconv = (int)std::min<size_t>((nh + 255) / 256, 2048);
-    e->pme_ngather = (e->S + 255) / 256;
+    e->pme_ngather = (e->S + 63) / 64;   // 4 lanes per slot, 256 lanes per workgroup
     CHECK(e->pme_qacc.alloc(n));
     CHECK(e->pme_qgrid.alloc(n));
     CHECK(e->pme_sgrid.alloc(nh));
@@ -696,7 +695,6 @@ int pme_prepare(md_engine* e) {
     CHECK(e->pme_frec.alloc(e->S));
     CHECK(e->pme_part.alloc(2 * (size_t)(e->pme_nconv + e->pme_ngather)));
     CHECK(e->pme_out.alloc(2));
-    CHECK(e->pme_counter.alloc(1));
     // per axis: the signed reciprocal component m / L and |b(m)|^2 (Essmann et al. eq. 4.4)
     const double pi = 3.14159265358979323846;
     double mint[MD_PME_MAX_ORDER];
@@ -747,7 +745,7 @@ PmeArgs pme_args(md_engine* e, int mode) {
     PmeArgs g{};
     g.pos[0] = e->pos[0].p; g.pos[1] = e->pos[1].p; g.ids = e->ids.p; g.slot_of = e->slot_of.p; g.excl = e->excl.p;
     g.st = e->rst.p; g.qacc = e->pme_qacc.p; g.qgrid = e->pme_qgrid.p; g.sgrid = e->pme_sgrid.p; g.tab = e->pme_tab.p;
-    g.frec = e->pme_frec.p; g.part = e->pme_part.p; g.counter = e->pme_counter.p; g.out = e->pme_out.p;
+    g.frec = e->pme_frec.p; g.part = e->pme_part.p; g.out = e->pme_out.p;
     g.K0 = e->pme_K[0]; g.K1 = e->pme_K[1]; g.K2 = e->pme_K[2]; g.order = e->pme_order;
     g.nslot = e->S; g.mode = mode; g.has_excl = e->has_excl ? 1 : 0;
     g.nconv = e->pme_nconv; g.ngather = e->pme_ngather;

@@ -252,7 +252,6 @@ struct PmeArgs {
     const float2* tab;          // [K0 + K1 + K2] per axis and index: (signed m / L, |b(m)|^2)
     float4* frec;               // [S] reciprocal + excluded-pair force per slot
     double* part;               // [nconv + ngather][2] energy and virial per workgroup
-    unsigned* counter;          // gather workgroups done (the last one resets it)
     double* out;                // [2] energy and virial of the long-range part
     int K0, K1, K2;
     int nslot;

@@ -18,10 +18,10 @@
 //                 B(m) / (pi V m^2), summing the energy 1/2 C |S|^2 and the traced
 //                 virial 1/2 C |S|^2 (1 - 2 pi^2 m^2 / b^2) over the half spectrum
 //   C2R FFT       -> dE/dQ on the grid
-//   k_pme_gather  one thread per slot: F = -q grad(sum_k dE/dQ(k) th th th), plus the
-//                 correction -qq erf(b r)/r of its excluded pairs; the last
-//                 workgroup to finish sums every workgroup's energy and virial in
-//                 a fixed order and adds the self and background constants.
+//   k_pme_gather  four lanes per slot: F = -q grad(sum_k dE/dQ(k) th th th), plus the
+//                 correction -qq erf(b r)/r of its excluded pairs
+//   k_pme_sum     one workgroup: every workgroup's energy and virial in a fixed
+//                 order, plus the self and background constants.
 // Every kernel first reads the replica's state and does nothing for a step the
 // step kernel will not take (the same test as the step kernel), so the chain can
 // sit inside a device-gated graph segment.
@@ -205,67 +205,97 @@ __global__ __launch_bounds__(PT) void k_pme_conv(const PmeArgs a) {
     block_pair_sum(e, w, a.part, blockIdx.x);
 }
 
+// Four lanes (a quad) per atom, 64 atoms per workgroup, so the load latency is
+// spread over four times the wavefronts: lanes 0..2 of the quad compute the
+// B-spline weights and derivatives of one axis each (into LDS, once per atom);
+// then lane c of the quad reads the stencil points with z offsets c and c + 4
+// and the quad's partial gradients are summed by DPP.  Lane c also takes the
+// atom's c-th excluded partner.
 template <int P>
 __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
-    __shared__ bool s_last;
+    __shared__ float s_t[PT / 4][3][P], s_d[PT / 4][3][P];
+    __shared__ int s_k[PT / 4][3];
     const RepState s = a.st[0];
     if (!pme_go(s, a.mode)) return;   // grid-uniform
-    const int slot = blockIdx.x * PT + threadIdx.x;
+    const int sub = threadIdx.x & 3, la = threadIdx.x >> 2;   // lane in the quad, atom in the workgroup
+    const int slot = blockIdx.x * (PT / 4) + la;
     double e = 0.0, w = 0.0;
     const int id = slot < a.nslot ? a.ids[slot] : -1;
-    if (id >= 0) {
-        const float4* __restrict__ pin = a.pos[cur_buf(s)];
-        const float4 x = pin[slot];
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (x.w != 0.0f) {
-            int k0, k1, k2;
-            float w0, w1, w2;
-            grid_coord(x.x, a.hl0, a.kl0, a.ik0, a.K0, k0, w0);
-            grid_coord(x.y, a.hl1, a.kl1, a.ik1, a.K1, k1, w1);
-            grid_coord(x.z, a.hl2, a.kl2, a.ik2, a.K2, k2, w2);
-            float t0[P], t1[P], t2[P], d0[P], d1[P], d2[P];
-            bspline<P, true>(w0, t0, d0);
-            bspline<P, true>(w1, t1, d1);
-            bspline<P, true>(w2, t2, d2);
-            int g2[P];
+    const float4* __restrict__ pin = a.pos[cur_buf(s)];
+    const float4 x = id >= 0 ? pin[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (x.w != 0.0f && sub < 3) {   // this lane's axis
+        const float xc = sub == 0 ? x.x : (sub == 1 ? x.y : x.z);
+        const double hl = sub == 0 ? a.hl0 : (sub == 1 ? a.hl1 : a.hl2);
+        const double kl = sub == 0 ? a.kl0 : (sub == 1 ? a.kl1 : a.kl2);
+        const double ik = sub == 0 ? a.ik0 : (sub == 1 ? a.ik1 : a.ik2);
+        const int K = sub == 0 ? a.K0 : (sub == 1 ? a.K1 : a.K2);
+        int k0;
+        float w0, t[P], d[P];
+        grid_coord(xc, hl, kl, ik, K, k0, w0);
+        bspline<P, true>(w0, t, d);
+        s_k[la][sub] = k0;
 #pragma unroll
-            for (int c = 0; c < P; c++) g2[c] = wrapk(k2 - c, a.K2);
-            float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) {
-                const int g0 = wrapk(k0 - i, a.K0);
-#pragma unroll
-                for (int j = 0; j < P; j++) {
-                    const float* row = a.qgrid + ((size_t)g0 * a.K1 + wrapk(k1 - j, a.K1)) * a.K2;
-                    float sz = 0.0f, sdz = 0.0f;
-#pragma unroll
-                    for (int c = 0; c < P; c++) {
-                        const float ph = row[g2[c]];
-                        sz = fmaf(ph, t2[c], sz);
-                        sdz = fmaf(ph, d2[c], sdz);
-                    }
-                    gx = fmaf(d0[i] * t1[j], sz, gx);
-                    gy = fmaf(t0[i] * d1[j], sz, gy);
-                    gz = fmaf(t0[i] * t1[j], sdz, gz);
-                }
-            }
-            fx = -x.w * gx * (float)a.kl0;
-            fy = -x.w * gy * (float)a.kl1;
-            fz = -x.w * gz * (float)a.kl2;
+        for (int m = 0; m < P; m++) {
+            s_t[la][sub][m] = t[m];
+            s_d[la][sub][m] = d[m];
         }
-        if (a.has_excl && x.w != 0.0f) {
+    }
+    __syncthreads();
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (x.w != 0.0f) {
+        const int k0 = s_k[la][0], k1 = s_k[la][1], k2 = s_k[la][2];
+        float t0[P], t1[P], d0[P], d1[P];
+#pragma unroll
+        for (int m = 0; m < P; m++) {
+            t0[m] = s_t[la][0][m];
+            d0[m] = s_d[la][0][m];
+            t1[m] = s_t[la][1][m];
+            d1[m] = s_d[la][1][m];
+        }
+        // this lane's z points c = sub and sub + 4 (weight 0 where c >= P)
+        constexpr int NH = P > 4 ? 2 : 1;
+        float tz[NH], dz[NH];
+        int gz[NH];
+#pragma unroll
+        for (int h = 0; h < NH; h++) {
+            const int c = sub + 4 * h;
+            const bool in = c < P;
+            tz[h] = in ? s_t[la][2][in ? c : 0] : 0.0f;
+            dz[h] = in ? s_d[la][2][in ? c : 0] : 0.0f;
+            gz[h] = in ? wrapk(k2 - c, a.K2) : 0;
+        }
+        float gx = 0.0f, gy = 0.0f, gzz = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            const int g0 = wrapk(k0 - i, a.K0);
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                const float* row = a.qgrid + ((size_t)g0 * a.K1 + wrapk(k1 - j, a.K1)) * a.K2;
+                float sz = 0.0f, sdz = 0.0f;
+#pragma unroll
+                for (int h = 0; h < NH; h++) {
+                    const float ph = row[gz[h]];
+                    sz = fmaf(ph, tz[h], sz);
+                    sdz = fmaf(ph, dz[h], sdz);
+                }
+                gx = fmaf(d0[i] * t1[j], sz, gx);
+                gy = fmaf(t0[i] * d1[j], sz, gy);
+                gzz = fmaf(t0[i] * t1[j], sdz, gzz);
+            }
+        }
+        fx = -x.w * gx * (float)a.kl0;
+        fy = -x.w * gy * (float)a.kl1;
+        fz = -x.w * gzz * (float)a.kl2;
+        if (a.has_excl) {
             // excluded pairs: U = -qq erf(b r)/r, F_i = qq [c2 e^{-b^2 r^2}/r - erf(b r)/r^2] d/r
             const int4 ex = a.excl[id];
-            const int pj[4] = {ex.x, ex.y, ex.z, ex.w};
-            const float c2 = a.beta * 1.1283791670955126f;   // 2 beta / sqrt(pi)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (pj[k] < 0) continue;
-                const float4 y = pin[a.slot_of[pj[k]]];
-                if (y.w == 0.0f) continue;
+            const int pj = sub == 0 ? ex.x : (sub == 1 ? ex.y : (sub == 2 ? ex.z : ex.w));
+            const float4 y = pj >= 0 ? pin[a.slot_of[pj]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (y.w != 0.0f) {
+                const float c2 = a.beta * 1.1283791670955126f;   // 2 beta / sqrt(pi)
                 const float dx = minimg(x.x - y.x, a.L0, a.Li0), dy = minimg(x.y - y.y, a.L1, a.Li1),
-                            dz = minimg(x.z - y.z, a.L2, a.Li2);
-                const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            dzz = minimg(x.z - y.z, a.L2, a.Li2);
+                const float r2 = fmaf(dzz, dzz, fmaf(dy, dy, dx * dx));
                 const float r = sqrtf(r2), ir = 1.0f / r;
                 const float br = a.beta * r;
                 const float erfv = erff(br);
@@ -273,7 +303,7 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
                 const float fr = qq * (c2 * expf(-br * br) - erfv * ir) * ir * ir;
                 fx = fmaf(fr, dx, fx);
                 fy = fmaf(fr, dy, fy);
-                fz = fmaf(fr, dz, fz);
+                fz = fmaf(fr, dzz, fz);
                 // energy and virial in FP64: for neutral molecules these terms nearly
                 // cancel the self term, each far larger than their sum
                 const double rd = sqrt((double)r2), bd = (double)a.beta * rd;
@@ -282,27 +312,30 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
                 w += 0.5 * qqd * (1.1283791670955126 * bd * exp(-bd * bd) - erfd) / rd;
             }
         }
-        a.frec[slot] = make_float4(fx, fy, fz, 0.0f);
     }
+    fx = group_sum<4>(fx);
+    fy = group_sum<4>(fy);
+    fz = group_sum<4>(fz);
+    if (id >= 0 && sub == 0) a.frec[slot] = make_float4(fx, fy, fz, 0.0f);
     block_pair_sum(e, w, a.part, a.nconv + blockIdx.x);
-    // the last workgroup to finish adds up every workgroup's pair in a fixed order
-    __threadfence();
-    if (threadIdx.x == 0) s_last = atomicAdd(a.counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+}
+
+// Every workgroup's (energy, virial) pair of the influence and gather kernels in
+// a fixed order, plus the self and background constants.  One workgroup after
+// the gather: a device-scope fence and a shared counter in each of the gather's
+// thousands of workgroups cost more than this launch.
+__global__ __launch_bounds__(PT) void k_pme_sum(const PmeArgs a) {
+    if (!pme_go(a.st[0], a.mode)) return;
     const int np = a.nconv + a.ngather;
     double te = 0.0, tw = 0.0;
     for (int k = threadIdx.x; k < np; k += PT) {
         te += a.part[2 * (size_t)k];
         tw += a.part[2 * (size_t)k + 1];
     }
-    __syncthreads();   // s_red of block_pair_sum is reused
     block_pair_sum(te, tw, a.out, 0);
     if (threadIdx.x == 0) {
         a.out[0] += a.e0;
         a.out[1] += a.w0;
-        atomicExch(a.counter, 0u);
     }
 }
 
@@ -338,5 +371,6 @@ extern "C" int md_launch_pme(const PmeArgs& a, hipfftHandle r2c, hipfftHandle c2
     case 6: launch_gather<6>(a, st); break;
     default: launch_gather<8>(a, st); break;
     }
+    hipLaunchKernelGGL(k_pme_sum, dim3(1), dim3(PT), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
