@@ -146,8 +146,9 @@ int md_set_rigid(md_engine* e, int nmol, const int* mol, float d_oh, float d_hh)
  * neutralizing-background terms to the Coulomb energy and the virial.
  * order: 4, 6 or 8 (0 turns PME off); grid: points per box edge, >= order, at
  * most 1024 (sizes with factors 2, 3, 5 transform fastest).  Runs need one
- * replica, charged atoms, MD_COUL_DAMPED and no pressure coupling (checked at
- * the next md_run / md_compute_forces).  Force matching has no PME term. */
+ * replica, charged atoms and MD_COUL_DAMPED (checked at the next md_run /
+ * md_compute_forces); under pressure coupling the long-range part follows the
+ * box of each step.  Force matching has no PME term. */
 int md_set_pme(md_engine* e, const int* grid, int order);
 
 /* pos (N x 3, required), vel (N x 3 or NULL = zero).  Builds the neighbor lists. */

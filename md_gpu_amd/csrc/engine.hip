@@ -281,9 +281,8 @@ struct md_engine {
     int pme_order = 0, pme_K[3] = {0, 0, 0};
     bool pme_on = false;           // prepared: the step chain includes the PME kernels
     bool vv_force_stale = false;   // the force array (velocity Verlet's first half kick) predates a PME change
-    int pme_S = 0;                 // slot count and box the buffers and tables were made for
-    float pme_L[3] = {0, 0, 0};
-    double pme_e0 = 0, pme_w0 = 0; // self + background energy and virial
+    int pme_S = 0;                 // slot count the buffers were made for
+    double pme_eself = 0, pme_qsum2 = 0;   // self term; (sum q)^2 for the background term
     int pme_nconv = 0, pme_ngather = 0;
     DBuf<unsigned long long> pme_qacc;
     DBuf<float> pme_qgrid;
@@ -675,14 +674,11 @@ int pme_prepare(md_engine* e) {
     if (e->R != 1) return fail(MD_E_INVALID, "PME runs one replica per engine (this one has %d)", e->R);
     if (e->coul_kernel != MD_COUL_DAMPED)
         return fail(MD_E_INVALID, "PME needs the damped Coulomb term (lkspace 2) and charged atoms");
-    if (e->in.barostat != MD_BARO_NONE) return fail(MD_E_INVALID, "PME runs with a fixed box (no pressure coupling)");
     if (!(e->ff.alpha > 0.0f)) return fail(MD_E_INVALID, "PME needs a positive splitting parameter alpha");
     const int p = e->pme_order;
     const int* K = e->pme_K;
-    for (int d = 0; d < 3; d++) {
+    for (int d = 0; d < 3; d++)
         if (K[d] < p) return fail(MD_E_INVALID, "PME grid edge %d has %d points, fewer than the order %d", d, K[d], p);
-        e->pme_L[d] = e->box[d];
-    }
     e->drop_graphs();   // captured segments hold the previous buffers
     e->drop_pme_plans();
     const size_t n = (size_t)K[0] * K[1] * K[2], nh = (size_t)K[0] * K[1] * (K[2] / 2 + 1);
@@ -695,7 +691,8 @@ int pme_prepare(md_engine* e) {
     CHECK(e->pme_frec.alloc(e->S));
     CHECK(e->pme_part.alloc(2 * (size_t)(e->pme_nconv + e->pme_ngather)));
     CHECK(e->pme_out.alloc(2));
-    // per axis: the signed reciprocal component m / L and |b(m)|^2 (Essmann et al. eq. 4.4)
+    // per axis: the signed frequency m and |b(m)|^2 (Essmann et al. eq. 4.4); the
+    // kernels divide m by the box edge of the step
     const double pi = 3.14159265358979323846;
     double mint[MD_PME_MAX_ORDER];
     bspline_host(p, 0.0, mint);   // M_p(j)
@@ -708,7 +705,7 @@ int pme_prepare(md_engine* e) {
                 im += mint[k + 1] * std::sin(2.0 * pi * m * k / K[d]);
             }
             const int ms = m <= K[d] / 2 ? m : m - K[d];
-            tab.push_back(make_float2((float)(ms / (double)e->pme_L[d]), (float)(1.0 / (re * re + im * im))));
+            tab.push_back(make_float2((float)ms, (float)(1.0 / (re * re + im * im))));
         }
     HIPCHECK(hipMemcpy(e->pme_tab.p, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice));
     // self and neutralizing-background terms
@@ -717,10 +714,9 @@ int pme_prepare(md_engine* e) {
         s1 += e->charge_h[i];
         s2 += (double)e->charge_h[i] * e->charge_h[i];
     }
-    const double beta = e->ff.alpha, V = (double)e->pme_L[0] * e->pme_L[1] * e->pme_L[2];
-    const double ebg = -(double)e->ff.qfac * pi * s1 * s1 / (2.0 * V * beta * beta);
-    e->pme_e0 = -(double)e->ff.qfac * beta / std::sqrt(pi) * s2 + ebg;
-    e->pme_w0 = 3.0 * ebg;
+    const double beta = e->ff.alpha;
+    e->pme_eself = -(double)e->ff.qfac * beta / std::sqrt(pi) * s2;
+    e->pme_qsum2 = s1 * s1;
     // FFT plans on the engine's stream with one work area allocated here, so a
     // captured segment makes no allocation
     size_t ws1 = 0, ws2 = 0;
@@ -749,18 +745,13 @@ PmeArgs pme_args(md_engine* e, int mode) {
     g.K0 = e->pme_K[0]; g.K1 = e->pme_K[1]; g.K2 = e->pme_K[2]; g.order = e->pme_order;
     g.nslot = e->S; g.mode = mode; g.has_excl = e->has_excl ? 1 : 0;
     g.nconv = e->pme_nconv; g.ngather = e->pme_ngather;
-    const double L[3] = {e->pme_L[0], e->pme_L[1], e->pme_L[2]};
-    g.L0 = e->pme_L[0]; g.L1 = e->pme_L[1]; g.L2 = e->pme_L[2];
-    g.Li0 = (float)(1.0 / L[0]); g.Li1 = (float)(1.0 / L[1]); g.Li2 = (float)(1.0 / L[2]);
-    g.hl0 = 0.5 * L[0]; g.hl1 = 0.5 * L[1]; g.hl2 = 0.5 * L[2];
-    g.kl0 = e->pme_K[0] / L[0]; g.kl1 = e->pme_K[1] / L[1]; g.kl2 = e->pme_K[2] / L[2];
+    g.rep = e->rep.p;
     g.ik0 = 1.0 / e->pme_K[0]; g.ik1 = 1.0 / e->pme_K[1]; g.ik2 = 1.0 / e->pme_K[2];
     const double beta = e->ff.alpha, pi = 3.14159265358979323846;
     g.beta = (float)beta;
     g.qfac = e->ff.qfac;
     g.pi2_b2 = (float)(pi * pi / (beta * beta));
-    g.cpref = (float)((double)e->ff.qfac / (pi * L[0] * L[1] * L[2]));
-    g.e0 = e->pme_e0; g.w0 = e->pme_w0;
+    g.e_self = e->pme_eself; g.qsum2 = e->pme_qsum2;
     return g;
 }
 

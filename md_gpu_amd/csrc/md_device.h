@@ -249,7 +249,7 @@ struct PmeArgs {
     unsigned long long* qacc;   // [K0 K1 K2] charge grid in fixed point, zero between steps
     float* qgrid;               // [K0 K1 K2] charges (transform input); dE/dQ (inverse output)
     float2* sgrid;              // [K0 K1 (K2/2 + 1)] the transform
-    const float2* tab;          // [K0 + K1 + K2] per axis and index: (signed m / L, |b(m)|^2)
+    const float2* tab;          // [K0 + K1 + K2] per axis and index: (signed frequency m, |b(m)|^2)
     float4* frec;               // [S] reciprocal + excluded-pair force per slot
     double* part;               // [nconv + ngather][2] energy and virial per workgroup
     double* out;                // [2] energy and virial of the long-range part
@@ -259,15 +259,13 @@ struct PmeArgs {
     int has_excl;
     int nconv, ngather;         // workgroups of the influence and gather kernels
     int order;                  // B-spline order (4, 6 or 8)
-    float L0, L1, L2, Li0, Li1, Li2;
-    double hl0, hl1, hl2;       // L / 2
-    double kl0, kl1, kl2;       // K / L
+    const RepParam* rep;        // the replica's box, read every step (pressure coupling changes it)
     double ik0, ik1, ik2;       // 1 / K
     float beta;
     float qfac;
     float pi2_b2;               // pi^2 / beta^2
-    float cpref;                // qfac / (pi V)
-    double e0, w0;              // self + background energy and its virial (constants)
+    double e_self;              // -qfac beta / sqrt(pi) sum q^2
+    double qsum2;               // (sum q)^2: the neutralizing-background term -qfac pi qsum2 / (2 V beta^2)
 };
 
 // Progress of a run for the host: one small copy per check instead of one per

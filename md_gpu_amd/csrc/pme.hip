@@ -24,7 +24,8 @@
 //                 order, plus the self and background constants.
 // Every kernel first reads the replica's state and does nothing for a step the
 // step kernel will not take (the same test as the step kernel), so the chain can
-// sit inside a device-gated graph segment.
+// sit inside a device-gated graph segment, and reads the replica's box (pressure
+// coupling rescales it after each step).
 #include <hipfft/hipfft.h>
 
 #include "md_device.h"
@@ -85,6 +86,26 @@ __device__ __forceinline__ void grid_coord(float x, double hl, double kl, double
 // index of stencil point j (k0 - j, periodic)
 __device__ __forceinline__ int wrapk(int k, int K) { return k < 0 ? k + K : k; }
 
+// The replica's box as the PME kernels use it, read at each launch: pressure
+// coupling rescales it after every step.
+struct PmeBox {
+    float L[3], Li[3];
+    double hl[3], kl[3];   // L / 2 and K / L
+};
+__device__ __forceinline__ PmeBox pme_box(const PmeArgs& a) {
+    const RepParam& p = a.rep[0];
+    PmeBox b;
+    const int K[3] = {a.K0, a.K1, a.K2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        b.L[d] = p.L[d];
+        b.Li[d] = p.Linv[d];
+        b.hl[d] = 0.5 * (double)p.L[d];
+        b.kl[d] = (double)K[d] / (double)p.L[d];
+    }
+    return b;
+}
+
 // Lane l of a wavefront first computes the stencil of slot (wavefront's base + l);
 // then the wavefront walks its 64 atoms, its lanes taking the atom's P^3 grid
 // points with the z index fastest, so each atomic instruction touches P^2 runs of
@@ -102,12 +123,13 @@ __global__ __launch_bounds__(PT) void k_pme_spread(const PmeArgs a) {
 #pragma unroll
     for (int c = 0; c < P; c++) t0[c] = t1[c] = t2[c] = 0.0f;
     if (slot < a.nslot && a.ids[slot] >= 0) {
+        const PmeBox bx = pme_box(a);
         const float4 x = a.pos[cur_buf(s)][slot];
         q = x.w;
         float w0, w1, w2;
-        grid_coord(x.x, a.hl0, a.kl0, a.ik0, a.K0, k0, w0);
-        grid_coord(x.y, a.hl1, a.kl1, a.ik1, a.K1, k1, w1);
-        grid_coord(x.z, a.hl2, a.kl2, a.ik2, a.K2, k2, w2);
+        grid_coord(x.x, bx.hl[0], bx.kl[0], a.ik0, a.K0, k0, w0);
+        grid_coord(x.y, bx.hl[1], bx.kl[1], a.ik1, a.K1, k1, w1);
+        grid_coord(x.z, bx.hl[2], bx.kl[2], a.ik2, a.K2, k2, w2);
         bspline<P, false>(w0, t0, nullptr);
         bspline<P, false>(w1, t1, nullptr);
         bspline<P, false>(w2, t2, nullptr);
@@ -184,16 +206,19 @@ __global__ __launch_bounds__(PT) void k_pme_conv(const PmeArgs a) {
     const size_t n = (size_t)a.K0 * a.K1 * H;
     const float2* t1 = a.tab + a.K0;
     const float2* t2 = a.tab + a.K0 + a.K1;
+    const PmeBox bx = pme_box(a);
+    const float cpref = (float)((double)a.qfac / (3.14159265358979323846 * (double)bx.L[0] * bx.L[1] * bx.L[2]));
     double e = 0.0, w = 0.0;
     for (size_t i = (size_t)blockIdx.x * PT + threadIdx.x; i < n; i += (size_t)gridDim.x * PT) {
         const int i2 = (int)(i % H);
         const size_t r = i / H;
         const int i1 = (int)(r % a.K1), i0 = (int)(r / a.K1);
         const float2 u0 = a.tab[i0], u1 = t1[i1], u2 = t2[i2];
-        const float mm = fmaf(u2.x, u2.x, fmaf(u1.x, u1.x, u0.x * u0.x));
+        const float m0 = u0.x * bx.Li[0], m1 = u1.x * bx.Li[1], m2 = u2.x * bx.Li[2];   // reciprocal vector
+        const float mm = fmaf(m2, m2, fmaf(m1, m1, m0 * m0));
         const float2 S = a.sgrid[i];
         float C = 0.0f;
-        if (i != 0) C = a.cpref * expf(-a.pi2_b2 * mm) / mm * (u0.y * u1.y * u2.y);
+        if (i != 0) C = cpref * expf(-a.pi2_b2 * mm) / mm * (u0.y * u1.y * u2.y);
         // the half spectrum: k2 = 0 and (K2 even) K2/2 stand for themselves, the
         // others also for their conjugate partners
         const double half = (i2 == 0 || 2 * i2 == a.K2) ? 0.5 : 1.0;
@@ -223,10 +248,10 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
     const int id = slot < a.nslot ? a.ids[slot] : -1;
     const float4* __restrict__ pin = a.pos[cur_buf(s)];
     const float4 x = id >= 0 ? pin[slot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const PmeBox bx = pme_box(a);
     if (x.w != 0.0f && sub < 3) {   // this lane's axis
         const float xc = sub == 0 ? x.x : (sub == 1 ? x.y : x.z);
-        const double hl = sub == 0 ? a.hl0 : (sub == 1 ? a.hl1 : a.hl2);
-        const double kl = sub == 0 ? a.kl0 : (sub == 1 ? a.kl1 : a.kl2);
+        const double hl = bx.hl[sub], kl = bx.kl[sub];
         const double ik = sub == 0 ? a.ik0 : (sub == 1 ? a.ik1 : a.ik2);
         const int K = sub == 0 ? a.K0 : (sub == 1 ? a.K1 : a.K2);
         int k0;
@@ -283,9 +308,9 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
                 gzz = fmaf(t0[i] * t1[j], sdz, gzz);
             }
         }
-        fx = -x.w * gx * (float)a.kl0;
-        fy = -x.w * gy * (float)a.kl1;
-        fz = -x.w * gzz * (float)a.kl2;
+        fx = -x.w * gx * (float)bx.kl[0];
+        fy = -x.w * gy * (float)bx.kl[1];
+        fz = -x.w * gzz * (float)bx.kl[2];
         if (a.has_excl) {
             // excluded pairs: U = -qq erf(b r)/r, F_i = qq [c2 e^{-b^2 r^2}/r - erf(b r)/r^2] d/r
             const int4 ex = a.excl[id];
@@ -293,8 +318,8 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
             const float4 y = pj >= 0 ? pin[a.slot_of[pj]] : make_float4(0.f, 0.f, 0.f, 0.f);
             if (y.w != 0.0f) {
                 const float c2 = a.beta * 1.1283791670955126f;   // 2 beta / sqrt(pi)
-                const float dx = minimg(x.x - y.x, a.L0, a.Li0), dy = minimg(x.y - y.y, a.L1, a.Li1),
-                            dzz = minimg(x.z - y.z, a.L2, a.Li2);
+                const float dx = minimg(x.x - y.x, bx.L[0], bx.Li[0]), dy = minimg(x.y - y.y, bx.L[1], bx.Li[1]),
+                            dzz = minimg(x.z - y.z, bx.L[2], bx.Li[2]);
                 const float r2 = fmaf(dzz, dzz, fmaf(dy, dy, dx * dx));
                 const float r = sqrtf(r2), ir = 1.0f / r;
                 const float br = a.beta * r;
@@ -334,8 +359,13 @@ __global__ __launch_bounds__(PT) void k_pme_sum(const PmeArgs a) {
     }
     block_pair_sum(te, tw, a.out, 0);
     if (threadIdx.x == 0) {
-        a.out[0] += a.e0;
-        a.out[1] += a.w0;
+        // self term, and the neutralizing background at the current volume (its
+        // virial -L dE/dL is three times itself)
+        const RepParam& p = a.rep[0];
+        const double V = (double)p.L[0] * p.L[1] * p.L[2], b = (double)a.beta;
+        const double ebg = -(double)a.qfac * 3.14159265358979323846 * a.qsum2 / (2.0 * V * b * b);
+        a.out[0] += a.e_self + ebg;
+        a.out[1] += 3.0 * ebg;
     }
 }
 

@@ -319,7 +319,7 @@ def md_run(sys_, nsteps, pos=None, vel=None, thermo_state=None, skin=None):
     if nreb == -1:
         raise ValueError("the oracle runs rigid molecules with leapfrog only")
     if nreb == -2:
-        raise ValueError("PME needs the damped Coulomb term and a fixed box")
+        raise ValueError("PME needs the damped Coulomb term")
     return dict(ke=ke, pe=pe, temp=T, conserved=H, pos=p, vel=v, thermo=ts, rebuilds=nreb, volume=vol, box=box_end)
 
 

@@ -726,8 +726,8 @@ extern "C" int orc_md_run(int n, double* pos, double* vel, const int* type, cons
     // src/integrator.cu:155, less the three constraints of each rigid molecule
     const double Nf = 3.0 * n - 3.0 - 3.0 * (md->nrigid > 0 ? md->nrigid : 0);
     if (md->nrigid > 0 && md->integrator != ORC_INT_LEAPFROG) return -1;   // SETTLE with leapfrog only
-    if (md->pme_order > 0) {   // damped Coulomb, fixed box
-        if (ff->coulomb != ORC_COUL_DAMPED || md->barostat != ORC_BARO_NONE) return -2;
+    if (md->pme_order > 0) {   // damped Coulomb (the reciprocal part uses the box of each step)
+        if (ff->coulomb != ORC_COUL_DAMPED) return -2;
         s.pme.order = md->pme_order;
         for (int d = 0; d < 3; d++) s.pme.K[d] = md->pme_K[d];
         s.pme.beta = ff->alpha;

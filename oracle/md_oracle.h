@@ -121,7 +121,8 @@ typedef struct {
     const int* rigid;       /* nrigid x 3 atom indices (O, H1, H2) */
     double d_oh, d_hh;      /* their O-H and H-H distances (A) */
     int    pme_order;       /* > 0 (damped Coulomb only): add the smooth-PME reciprocal part, the */
-    int    pme_K[3];        /* excluded-pair correction and the self term at beta = ff->alpha */
+    int    pme_K[3];        /* excluded-pair correction and the self term at beta = ff->alpha, in
+                               the box of each step (pressure coupling rescales it) */
 } orc_md;
 
 /* Stats per step (length nsteps): ke, pe (lj+coul+bond+angle), temperature,
