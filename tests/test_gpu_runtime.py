@@ -257,17 +257,23 @@ def test_invalid_calls_raise_and_leave_the_engine_usable(E):
     e.close()
 
 
-@pytest.mark.parametrize("kind", ["nacl_lf", "nacl_vv_nve", "water_vv_nh"])
+@pytest.mark.parametrize("kind", ["nacl_lf", "nacl_vv_nve", "water_vv_nh", "nacl_pme_lf", "water_pme_vv_nh"])
 def test_list_overflow_rolls_back_and_replays(E, kind):
     """A list capacity below the rows' length (the reference's nmaxlist too
     small, which it does not guard) makes the run roll back to its start, grow
     the capacity and replay: the result equals a run that never overflowed.
     Leapfrog, fused NVE velocity Verlet and velocity Verlet + Nose-Hoover (the
-    velocity-Verlet runs also restore the forces their first half kick reads)."""
+    velocity-Verlet runs also restore the forces their first half kick reads),
+    also with smooth PME (its grids are recomputed every step; the restored
+    forces include the long-range part)."""
     if kind == "nacl_lf":
         s = S.nacl(nsites=16)
     elif kind == "nacl_vv_nve":
         s = dataclasses.replace(S.nacl(nsites=16), integrator="vverlet", thermostat="none")
+    elif kind == "nacl_pme_lf":
+        s = S.with_pme(S.nacl(nsites=16))
+    elif kind == "water_pme_vv_nh":
+        s = S.with_pme(S.water(nmol=512))
     else:
         s = S.water(nmol=512)                 # rows hold ~560 entries
     say("list overflow replay", kind, s.n)
