@@ -3,7 +3,8 @@ FP64 oracle's SPME (oracle/md_pme.cpp, pinned by the Madelung constant, beta
 independence, finite differences and convergence to the direct Ewald sum).
 
 - forces, Coulomb energy and virial for NaCl (4,096 and 32,768 ions), on cubic
-  and uneven grids (odd and even edges, odd last edge), orders 4, 6 and 8;
+  and uneven grids (odd and even edges, odd last edge), orders 4, 6 and 8, and
+  in an orthorhombic box;
 - rigid water (excluded intramolecular pairs) forces at 1,536 and 196,608 atoms,
   and 30 leapfrog steps with SETTLE against the oracle's trajectory;
 - NaCl leapfrog Berendsen, NaCl NVE velocity Verlet (with energy conservation)
@@ -89,6 +90,20 @@ def check_forces(E, orc, s, tol=FORCE_TOL):
 def test_forces_nacl(E, orc, grid, order):
     s = nacl_pme(grid=grid, order=order)
     say("pme forces nacl4096", s.pme)
+    check_forces(E, orc, s)
+
+
+def test_forces_orthorhombic_box(E, orc):
+    """Box edges 49.6, 45.1 and 42.9 A (the rock-salt lattice stretched per axis)
+    on a 50 x 45 x 44 grid: every per-axis scale (grid coordinates, reciprocal
+    vectors, minimum images) against the oracle."""
+    s = perturbed(S.nacl(nsites=16), 0.15)
+    f = np.array([1.10, 1.0, 0.95])
+    s.box = (s.box.astype(np.float64) * f).astype(np.float32)
+    s.pos = S.wrap_centered(s.pos.astype(np.float64) * f, s.box.astype(np.float64)).astype(np.float32)
+    s = S.with_pme(s, spacing=1.0, order=4)
+    s.pme = (50, 45, 44, 4)
+    say("pme forces orthorhombic", s.box, s.pme)
     check_forces(E, orc, s)
 
 
