@@ -21,7 +21,8 @@
 //   k_pme_gather  four lanes per slot: F = -q grad(sum_k dE/dQ(k) th th th), plus the
 //                 correction -qq erf(b r)/r of its excluded pairs
 //   k_pme_sum     one workgroup: every workgroup's energy and virial in a fixed
-//                 order, plus the self and background constants.
+//                 order, plus the self term and the neutralizing-background term
+//                 at the step's volume.
 // Every kernel first reads the replica's state and does nothing for a step the
 // step kernel will not take (the same test as the step kernel), so the chain can
 // sit inside a device-gated graph segment, and reads the replica's box (pressure
