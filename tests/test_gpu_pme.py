@@ -135,9 +135,20 @@ def test_long_range_part_is_large(E, orc):
     e.close()
 
 
-def test_forces_rigid_water(E, orc):
+@pytest.mark.parametrize("order", [4, 6])
+def test_forces_rigid_water(E, orc, order):
     s = water_pme()
+    s.pme = (*s.pme[:3], order)
     say("pme forces rigid water", s.n, s.pme)
+    check_forces(E, orc, s)
+
+
+@pytest.mark.parametrize("alpha", [0.2, 0.35])
+def test_forces_nacl_splitting_parameter(E, orc, alpha):
+    """Other real/reciprocal splits than alpha = 3.09 / rc: the damped table, the
+    influence function and the self term all follow alpha."""
+    s = S.with_pme(perturbed(S.nacl(nsites=16), 0.2), spacing=0.8 if alpha > 0.3 else 1.0, order=4, alpha=alpha)
+    say("pme forces nacl alpha", alpha, s.pme)
     check_forces(E, orc, s)
 
 
