@@ -355,6 +355,9 @@ __global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
         const int jid = EXCL ? s_cid[q] : 0;
         unsigned long long m[AW];
         unsigned long long any = 0;
+        // each condition is balloted on its own and the masks are combined as
+        // scalars: a compare then writes its lane mask directly
+        const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
 #pragma unroll
         for (int i = 0; i < AW; i++) {
             // (xi - xj) - k L: the canonical minimum-image displacement for this image
@@ -362,13 +365,15 @@ __global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
             const float dy = (ayr[i] - cx.y) + sv.y;
             const float dz = (azr[i] - cx.z) + sv.z;
             const float rsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            bool ok = valid & (rsq < rl2) & (js != slot0 + wave * AW + i);
+            unsigned long long mi = vmask & __builtin_amdgcn_ballot_w64(rsq < rl2) &
+                                    __builtin_amdgcn_ballot_w64(js != slot0 + wave * AW + i);
             if (EXCL) {
                 const int4 ex = s_ex[wave * AW + i];
-                ok = ok & (jid != ex.x) & (jid != ex.y) & (jid != ex.z) & (jid != ex.w);
+                mi &= __builtin_amdgcn_ballot_w64(jid != ex.x) & __builtin_amdgcn_ballot_w64(jid != ex.y) &
+                      __builtin_amdgcn_ballot_w64(jid != ex.z) & __builtin_amdgcn_ballot_w64(jid != ex.w);
             }
-            m[i] = __builtin_amdgcn_ballot_w64(ok);
-            any |= m[i];
+            m[i] = mi;
+            any |= mi;
         }
         const int par = nproc & 1;
         if (lane == 0) s_any[par][wave] = any;
@@ -387,11 +392,18 @@ __global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
 #pragma unroll
         for (int i = 0; i < AW; i++) {
             if (m[i]) {
-                const bool hit = (m[i] >> lane) & 1ull;
                 const uint32_t e = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[i] >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m[i], (uint32_t)cnt[i]));
-                if (hit && e < (uint32_t)cap) wrow[(uint32_t)(i * cap) + lpos16_u(e, gs)] = lis;
-                cnt[i] += __popcll(m[i]);
+                const int nh = __popcll(m[i]);
+                // the hit lanes are the mask itself (no per-lane bit test); the
+                // capacity test per lane only where this batch can cross it
+                if (cnt[i] + nh <= cap) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(m[i])) wrow[(uint32_t)(i * cap) + lpos16_u(e, gs)] = lis;
+                } else {
+                    if (__builtin_amdgcn_inverse_ballot_w64(m[i]) && e < (uint32_t)cap)
+                        wrow[(uint32_t)(i * cap) + lpos16_u(e, gs)] = lis;
+                }
+                cnt[i] += nh;
             }
         }
         jn += __popcll(any);
