@@ -420,70 +420,101 @@ __global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
         const Axis az = make_axis(lz, hz, rlp, p.halfL[2], p.cinv[2], p.cedge[2], p.nc[2]);
         const int nreg = ax.nr * ay.nr * az.nr;
         const int cell_off = p.cell_off, cs_off = p.cs_off, ncx = p.nc[0], ncy = p.nc[1];
-        // region cell c -> (cell start, member count, shift code); count 0 when beyond r_list or empty
-        auto probe = [=](int c, int& cs, int& code) -> int {
+        // region cell c -> its cell's linear index and image shift code; -1 when the
+        // image cell lies r_list or farther from the box
+        auto locate = [=](int c, int& code) -> int {
             int kx, ky, kz, wx, wy, wz;
             const int cyz = floor_div(c, ax.nr, ax.inv_nr), cz = floor_div(cyz, ay.nr, ay.inv_nr);
             const float ddx = ax.gap(ax.lo + (c - cyz * ax.nr), kx, wx);
             const float ddy = ay.gap(ay.lo + (cyz - cz * ay.nr), ky, wy);
             const float ddz = az.gap(az.lo + cz, kz, wz);
-            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return 0;
-            const int rank = a.cell_rank[cell_off + wx + ncx * (wy + ncy * wz)];
-            cs = a.cell_start[cs_off + rank];
+            if (ddx * ddx + ddy * ddy + ddz * ddz >= rlp2) return -1;
             code = shift_code(kx, ky, kz);
-            return a.cell_start[cs_off + rank + 1] - cs;
+            return wx + ncx * (wy + ncy * wz);
         };
         for (int b0 = 0; b0 < nreg; b0 += kTab) {
             // ---- table of non-empty image cells within r_list of the box (region order);
             //      pass 1 counts, pass 2 writes
             const int c0 = b0 + tid * PER, c1 = min(min(c0 + PER, nreg), b0 + kTab);
+            // the thread's PER region cells: geometry first, then all their rank
+            // loads, then all their bounds loads (two round trips, not 2 PER)
+            int pcell[PER], pcode[PER], prank[PER], pcs[PER], pn[PER];
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                pcode[u] = 0;
+                pcell[u] = c0 + u < c1 ? locate(c0 + u, pcode[u]) : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) prank[u] = pcell[u] >= 0 ? a.cell_rank[cell_off + pcell[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                pcs[u] = 0;
+                pn[u] = 0;
+                if (pcell[u] >= 0) {
+                    pcs[u] = a.cell_start[cs_off + prank[u]];
+                    pn[u] = a.cell_start[cs_off + prank[u] + 1] - pcs[u];
+                }
+            }
             int n_keep = 0, csum = 0;
-            for (int c = c0; c < c1; c++) {
-                int cs, cd;
-                const int n = probe(c, cs, cd);
-                n_keep += n > 0;
-                csum += n;
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                n_keep += pn[u] > 0;
+                csum += pn[u];
             }
             int ntab, T;
             const int off = block_scan<NT>(n_keep, s_w, ntab);
             const int coff = block_scan<NT>(csum, s_w, T);
-            for (int c = c0, k = off, run = coff; c < c1; c++) {
-                int cs, cd;
-                const int n = probe(c, cs, cd);
-                if (n > 0) {
-                    s_tstart[k] = cs;
+#pragma unroll
+            for (int u = 0, k = off, run = coff; u < PER; u++) {
+                if (pn[u] > 0) {
+                    s_tstart[k] = pcs[u];
                     s_tpref[k] = run;
-                    s_tcode[k] = cd;
+                    s_tcode[k] = pcode[u];
                     k++;
-                    run += n;
+                    run += pn[u];
                 }
             }
             if (tid == 0) s_tpref[ntab] = T;
             __syncthreads();
-            // ---- stream the candidates, NT per chunk, into the ring (stable compaction)
+            // ---- stream the candidates, NT per chunk, into the ring (stable compaction).
+            //      The next chunk's members are located and their loads issued before
+            //      this chunk is tested, so the loads are in flight during the tests.
+            auto find = [&](int q, int& code) -> int {
+                int lo_ = 0, hi_ = ntab - 1;   // last table row with pref <= q
+                while (lo_ < hi_) {
+                    const int mid = (lo_ + hi_ + 1) >> 1;
+                    if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
+                }
+                code = s_tcode[lo_];
+                return s_tstart[lo_] + (q - s_tpref[lo_]);
+            };
+            int nj = -1, ncode = 0, nty = 0, nid = 0;
+            float4 nx = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (tid < T) {
+                nj = find(tid, ncode);
+                nx = a.pos_s[nj];
+                nty = a.typ_s[nj];
+                if (EXCL) nid = a.ids_s[nj];
+            }
             for (int base = 0; base < T; base += NT) {
-                const int q = base + tid;
+                const int j = nj, code = ncode, ty = nty, cid = nid;
+                float4 cx = nx;
+                nj = -1;
+                if (base + NT + tid < T) {
+                    nj = find(base + NT + tid, ncode);
+                    nx = a.pos_s[nj];
+                    nty = a.typ_s[nj];
+                    if (EXCL) nid = a.ids_s[nj];
+                }
                 bool keep = false;
-                float4 cx = make_float4(0.f, 0.f, 0.f, 0.f);
-                int cid = 0;
-                if (q < T) {
-                    int lo_ = 0, hi_ = ntab - 1;   // last table row with pref <= q
-                    while (lo_ < hi_) {
-                        const int mid = (lo_ + hi_ + 1) >> 1;
-                        if (s_tpref[mid] <= q) lo_ = mid; else hi_ = mid - 1;
-                    }
-                    const int j = s_tstart[lo_] + (q - s_tpref[lo_]);
-                    const int code = s_tcode[lo_];
-                    cx = a.pos_s[j];
-                    if (EXCL) cid = a.ids_s[j];
+                if (j >= 0) {
                     const float4 sv = s_shv[code];
                     const float ix = cx.x - sv.x, iy = cx.y - sv.y, iz = cx.z - sv.z;   // image position
                     const float gx = fmaxf(0.0f, fmaxf(lx - ix, ix - hx));
                     const float gy = fmaxf(0.0f, fmaxf(ly - iy, iy - hy));
                     const float gz = fmaxf(0.0f, fmaxf(lz - iz, iz - hz));
                     keep = gx * gx + gy * gy + gz * gz < rlp2;
-                    cx.w = __uint_as_float((uint32_t)j | ((uint32_t)code << MD_SLOTBITS) |
-                                           ((uint32_t)a.typ_s[j] << MD_JBITS));
+                    cx.w = __uint_as_float((uint32_t)j | ((uint32_t)code << MD_SLOTBITS) | ((uint32_t)ty << MD_JBITS));
                 }
                 int kept;
                 const int at = block_scan<NT>(keep ? 1 : 0, s_w, kept);
