@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_nl_permute(BuildArgs a) {
 //     each atom's hits are appended to its row at count + (hits in lower lanes).
 // So set and rows follow candidate order and do not depend on scheduling.
 template <bool EXCL, int NW>
-__global__ __launch_bounds__(64 * NW) void k_nl_tbuild(BuildArgs a) {
+__global__ __launch_bounds__(64 * NW, (NW == 4 && !EXCL) ? 5 : 1) void k_nl_tbuild(BuildArgs a) {
     constexpr int NT = 64 * NW, AW = 64 / NW, RING = 1024, PER = kTab / NT;
     __shared__ int s_tstart[kTab];        // first slot of the cell
     __shared__ int s_tpref[kTab + 1];     // first candidate index of each table row; [ntab] = total
