@@ -252,7 +252,9 @@ __global__ __launch_bounds__(PT) void k_pme_gather(const PmeArgs a) {
     const PmeBox bx = pme_box(a);
     if (x.w != 0.0f && sub < 3) {   // this lane's axis
         const float xc = sub == 0 ? x.x : (sub == 1 ? x.y : x.z);
-        const double hl = bx.hl[sub], kl = bx.kl[sub];
+        // selected, not indexed by the lane's axis: an indexed private array lives in scratch
+        const double hl = sub == 0 ? bx.hl[0] : (sub == 1 ? bx.hl[1] : bx.hl[2]);
+        const double kl = sub == 0 ? bx.kl[0] : (sub == 1 ? bx.kl[1] : bx.kl[2]);
         const double ik = sub == 0 ? a.ik0 : (sub == 1 ? a.ik1 : a.ik2);
         const int K = sub == 0 ? a.K0 : (sub == 1 ? a.K1 : a.K2);
         int k0;
